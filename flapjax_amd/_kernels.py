@@ -1,0 +1,344 @@
+"""Tensor-level wrappers of the C-ABI (device tensors in, device tensors out).
+
+Every function validates shapes on the host, allocates its outputs and any
+workspace through torch's caching allocator, and enqueues the HIP kernels on
+torch's current stream.  Nothing here computes on the host.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib as L
+
+F64 = torch.float64
+
+
+def _dev():
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def _c64(t, shape_tail=None, name="array"):
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{name} must be a torch.Tensor")
+    if not t.is_cuda:
+        raise ValueError(f"{name} must be a device tensor")
+    if t.dtype != F64:
+        raise TypeError(f"{name} must be float64, got {t.dtype}")
+    if shape_tail is not None and tuple(t.shape[-len(shape_tail):]) != tuple(shape_tail):
+        raise ValueError(f"{name} must end in shape {shape_tail}, got {tuple(t.shape)}")
+    return t.contiguous()
+
+
+def flow_desc(flow):
+    """fj_flowfield from a flapjax_amd.flowfields object."""
+    return flow.descriptor()
+
+
+# ---------------------------------------------------------------------------
+# K1
+# ---------------------------------------------------------------------------
+def n_segments(m: int, n: int) -> int:
+    return m * (n + 1) + (m + 1) * n
+
+
+def segments_from_grid(zeta, gamma, kernel_kind, eps, out=None, mirror=None):
+    """Filament table of one vertex grid (m+1, n+1, 3) with ring strengths (m, n)."""
+    zeta = _c64(zeta, (3,), "zeta")
+    if zeta.dim() != 3:
+        raise ValueError("zeta must be (m+1, n+1, 3)")
+    m, n = zeta.shape[0] - 1, zeta.shape[1] - 1
+    ns = n_segments(m, n)
+    if gamma is not None:
+        gamma = _c64(gamma, None, "gamma")
+        if tuple(gamma.shape) != (m, n):
+            raise ValueError(f"gamma must be ({m}, {n}), got {tuple(gamma.shape)}")
+    if out is None:
+        out = torch.empty((ns, L.SEG_STRIDE), dtype=F64, device=zeta.device)
+    if ns == 0:
+        return out
+    mp = mn = None
+    keep = []
+    if mirror is not None:
+        mp, a = L.host3(mirror[0])
+        mn, b = L.host3(mirror[1])
+        keep += [a, b]
+    rc = L.lib().fj_segments_from_grid(L.ptr(zeta), L.ptr(gamma) if gamma is not None else None, m, n, mp, mn,
+                                       int(kernel_kind), float(eps), L.ptr(out), L.stream())
+    L.check(rc, "fj_segments_from_grid")
+    return out
+
+
+def segments_from_endpoints(ends, gamma, kernel_kind, eps):
+    """Filament table from end points (S, 2, 3) and circulations (S,)."""
+    ends = _c64(ends, (2, 3), "filaments").reshape(-1, 2, 3)
+    ns = ends.shape[0]
+    if gamma is not None:
+        gamma = _c64(gamma, None, "gamma").reshape(-1)
+        if gamma.shape[0] != ns:
+            raise ValueError("one circulation per filament expected")
+    out = torch.empty((ns, L.SEG_STRIDE), dtype=F64, device=ends.device)
+    if ns:
+        rc = L.lib().fj_segments_from_endpoints(L.ptr(ends), L.ptr(gamma) if gamma is not None else None, ns,
+                                                int(kernel_kind), float(eps), L.ptr(out), L.stream())
+        L.check(rc, "fj_segments_from_endpoints")
+    return out
+
+
+def pair_sum(seg, targets, kernel_kind, eps, rcut, precision=L.PREC_FP64, mode=L.OUT_STORE, out=None, flow=None,
+             time=0.0, dt=0.0):
+    """Induced velocity of the filament table `seg` at `targets` (T, 3)."""
+    targets = _c64(targets, (3,), "targets")
+    flat = targets.reshape(-1, 3)
+    nt = flat.shape[0]
+    ns = 0 if seg is None else seg.shape[0]
+    if out is None:
+        if mode == L.OUT_ADD:
+            raise ValueError("OUT_ADD needs an output tensor")
+        out = torch.empty_like(flat)
+    if nt == 0:
+        return out
+    ws_bytes = L.lib().fj_pair_sum_workspace_bytes(nt, ns, int(precision))
+    ws = torch.empty(ws_bytes // 8 + 1, dtype=F64, device=flat.device)
+    fd = flow_desc(flow) if flow is not None else None
+    rc = L.lib().fj_pair_sum(L.ptr(seg) if ns else None, ns, L.ptr(flat), nt, L.ptr(out), int(kernel_kind),
+                             float(eps), float(rcut), int(precision), int(mode),
+                             ctypes.byref(fd) if fd is not None else None, float(time), float(dt), L.ptr(ws),
+                             L.stream())
+    L.check(rc, "fj_pair_sum")
+    return out
+
+
+# ---------------------------------------------------------------------------
+# K2
+# ---------------------------------------------------------------------------
+def aic_assemble(c, nc, zeta, kernel_kind, eps, rcut, out, col0=0, zeta_w=None, mirror=None, accumulate=False):
+    """Write the (T, m*n) block of grid `zeta` into `out` (T, ld) at column col0."""
+    c = _c64(c, (3,), "c").reshape(-1, 3)
+    nc = _c64(nc, (3,), "normals").reshape(-1, 3)
+    zeta = _c64(zeta, (3,), "zeta")
+    m, n = zeta.shape[0] - 1, zeta.shape[1] - 1
+    m_star = 0
+    if zeta_w is not None:
+        zeta_w = _c64(zeta_w, (3,), "zeta_w")
+        if zeta_w.shape[1] != n + 1:
+            raise ValueError("wake grid must have the bound grid's spanwise size")
+        m_star = zeta_w.shape[0] - 1
+    if out.dtype != F64 or not out.is_contiguous() or out.dim() != 2 or out.shape[0] != c.shape[0]:
+        raise ValueError("AIC output must be a contiguous float64 (T, ld) tensor")
+    mp = mn = None
+    keep = []
+    if mirror is not None:
+        mp, a = L.host3(mirror[0])
+        mn, b = L.host3(mirror[1])
+        keep += [a, b]
+    rc = L.lib().fj_aic_assemble(L.ptr(c), L.ptr(nc), c.shape[0], L.ptr(zeta), m, n,
+                                 L.ptr(zeta_w) if zeta_w is not None else None, m_star, mp, mn, int(kernel_kind),
+                                 float(eps), float(rcut), L.ptr(out), out.shape[1], int(col0), int(accumulate),
+                                 L.stream())
+    L.check(rc, "fj_aic_assemble")
+    return out
+
+
+# ---------------------------------------------------------------------------
+# K3
+# ---------------------------------------------------------------------------
+def lu_factor(A, overwrite=False):
+    """LU with partial pivoting (FP64).  Returns (LU, piv) with piv (2n,) int32:
+    LAPACK pivots then the composed permutation."""
+    A = _c64(A, None, "A")
+    n = A.shape[0]
+    if A.dim() != 2 or A.shape[1] != n:
+        raise ValueError("A must be square")
+    LU = A if overwrite else A.clone()
+    piv = torch.empty(2 * n, dtype=torch.int32, device=A.device)
+    info = torch.zeros(1, dtype=torch.int32, device=A.device)
+    rc = L.lib().fj_lu_factor(L.ptr(LU), n, n, L.ptr(piv), L.ptr(info), None, L.stream())
+    L.check(rc, "fj_lu_factor")
+    return LU, piv, info
+
+
+def lu_factor_f32(A32):
+    if A32.dtype != torch.float32 or not A32.is_cuda:
+        raise TypeError("A32 must be a float32 device tensor")
+    A32 = A32.contiguous()
+    n = A32.shape[0]
+    piv = torch.empty(2 * n, dtype=torch.int32, device=A32.device)
+    info = torch.zeros(1, dtype=torch.int32, device=A32.device)
+    rc = L.lib().fj_lu_factor_f32(L.ptr(A32), n, n, L.ptr(piv), L.ptr(info), None, L.stream())
+    L.check(rc, "fj_lu_factor_f32")
+    return A32, piv, info
+
+
+def lu_solve(LU, piv, b, trans=False):
+    """Solve with the factor; b (n,) or (n, nrhs) -> new tensor."""
+    n = LU.shape[0]
+    b = _c64(b, None, "b")
+    x = b.reshape(n, -1).t().contiguous().clone() if b.dim() == 2 else b.reshape(1, n).clone()
+    rc = L.lib().fj_lu_solve(L.ptr(LU), n, n, L.ptr(piv), L.ptr(x), x.shape[0], int(bool(trans)), L.stream())
+    L.check(rc, "fj_lu_solve")
+    return x.t().contiguous() if b.dim() == 2 else x.reshape(n)
+
+
+def solve_refined(A, LU32, piv, b, max_iter=8, tol=None):
+    """FP64-refined solution from an FP32 factor.  Returns (x, iters tensor)."""
+    A = _c64(A, None, "A")
+    n = A.shape[0]
+    b = _c64(b, None, "b").reshape(n)
+    if tol is None:
+        tol = 2.0 * np.finfo(np.float64).eps
+    x = torch.empty_like(b)
+    iters = torch.zeros(1, dtype=torch.int32, device=A.device)
+    ws = torch.empty(L.lib().fj_refine_workspace_bytes(n) // 8 + 1, dtype=F64, device=A.device)
+    rc = L.lib().fj_solve_refined(L.ptr(A), n, L.ptr(LU32), n, L.ptr(piv), n, L.ptr(b), L.ptr(x), int(max_iter),
+                                  float(tol), L.ptr(iters), L.ptr(ws), L.stream())
+    L.check(rc, "fj_solve_refined")
+    return x, iters
+
+
+# ---------------------------------------------------------------------------
+# K4 - K8
+# ---------------------------------------------------------------------------
+def grid_from_hg(hg, hg_dot, dof, zeta0, zeta=None, zeta_dot=None):
+    hg = _c64(hg, (4, 4), "hg")
+    zeta0 = _c64(zeta0, (3,), "zeta0")
+    m, n = zeta0.shape[0] - 1, zeta0.shape[1] - 1
+    if dof.dtype != torch.int32 or dof.numel() != n + 1:
+        raise ValueError("dof mapping must be int32 (n+1,)")
+    if zeta is None:
+        zeta = torch.empty_like(zeta0)
+    if hg_dot is not None:
+        hg_dot = _c64(hg_dot, (4, 4), "hg_dot")
+        if zeta_dot is None:
+            zeta_dot = torch.empty_like(zeta0)
+    rc = L.lib().fj_grid_from_hg(L.ptr(hg), L.ptr(hg_dot) if hg_dot is not None else None, L.ptr(dof),
+                                 L.ptr(zeta0), m, n, L.ptr(zeta), L.ptr(zeta_dot) if hg_dot is not None else None,
+                                 L.stream())
+    L.check(rc, "fj_grid_from_hg")
+    return zeta, (zeta_dot if hg_dot is not None else None)
+
+
+def colloc_normals(zeta, zeta_dot=None, want_c=True, want_n=True):
+    zeta = _c64(zeta, (3,), "zeta")
+    m, n = zeta.shape[0] - 1, zeta.shape[1] - 1
+    c = torch.empty((m, n, 3), dtype=F64, device=zeta.device) if want_c else None
+    nc = torch.empty((m, n, 3), dtype=F64, device=zeta.device) if want_n else None
+    cd = torch.empty((m, n, 3), dtype=F64, device=zeta.device) if zeta_dot is not None else None
+    rc = L.lib().fj_colloc_normals(L.ptr(zeta), L.ptr(zeta_dot) if zeta_dot is not None else None, m, n,
+                                   L.ptr(c) if c is not None else None, L.ptr(nc) if nc is not None else None,
+                                   L.ptr(cd) if cd is not None else None, L.stream())
+    L.check(rc, "fj_colloc_normals")
+    return c, nc, cd
+
+
+def filament_midpoints(zeta):
+    zeta = _c64(zeta, (3,), "zeta")
+    m, n = zeta.shape[0] - 1, zeta.shape[1] - 1
+    mid = torch.empty((n_segments(m, n), 3), dtype=F64, device=zeta.device)
+    rc = L.lib().fj_filament_midpoints(L.ptr(zeta), m, n, L.ptr(mid), L.stream())
+    L.check(rc, "fj_filament_midpoints")
+    return mid
+
+
+def rhs(c, nc, c_dot, v_w, flow, time, out=None):
+    c = _c64(c, (3,), "c").reshape(-1, 3)
+    nc = _c64(nc, (3,), "nc").reshape(-1, 3)
+    nt = c.shape[0]
+    if out is None:
+        out = torch.empty(nt, dtype=F64, device=c.device)
+    fd = flow_desc(flow)
+    rc = L.lib().fj_rhs(L.ptr(c), L.ptr(nc), L.ptr(c_dot.reshape(-1, 3).contiguous()) if c_dot is not None else None,
+                        L.ptr(v_w.reshape(-1, 3).contiguous()) if v_w is not None else None, nt, ctypes.byref(fd),
+                        float(time), L.ptr(out), L.stream())
+    L.check(rc, "fj_rhs")
+    return out
+
+
+def wake_shift(gamma_b_nm1, gamma_w_nm1, zeta_b_n, m_star, gamma_w_n, zeta_w_n):
+    m, n = gamma_b_nm1.shape
+    rc = L.lib().fj_wake_shift(L.ptr(gamma_b_nm1), L.ptr(gamma_w_nm1) if m_star else None, L.ptr(zeta_b_n), m, n,
+                               m_star, L.ptr(gamma_w_n) if m_star else None, L.ptr(zeta_w_n), L.stream())
+    L.check(rc, "fj_wake_shift")
+
+
+def wake_convect(zeta_w_nm1, vel, flow, time, dt, zeta_w_n):
+    m_star, n = zeta_w_nm1.shape[0] - 1, zeta_w_nm1.shape[1] - 1
+    fd = flow_desc(flow) if flow is not None else None
+    rc = L.lib().fj_wake_convect(L.ptr(zeta_w_nm1), L.ptr(vel) if vel is not None else None, m_star, n,
+                                 ctypes.byref(fd) if fd is not None else None, float(time), float(dt),
+                                 L.ptr(zeta_w_n), L.stream())
+    L.check(rc, "fj_wake_convect")
+
+
+def wake_initialise(zeta_b, s, direction, zeta_w=None):
+    zeta_b = _c64(zeta_b, (3,), "zeta_b")
+    m, n = zeta_b.shape[0] - 1, zeta_b.shape[1] - 1
+    s = _c64(s, None, "s")
+    m_star = s.shape[0] - 1
+    if zeta_w is None:
+        zeta_w = torch.empty((m_star + 1, n + 1, 3), dtype=F64, device=zeta_b.device)
+    dp, keep = L.host3(direction)
+    rc = L.lib().fj_wake_initialise(L.ptr(zeta_b), m, n, L.ptr(s), m_star, dp, L.ptr(zeta_w), L.stream())
+    L.check(rc, "fj_wake_initialise")
+    return zeta_w
+
+
+def static_wake_gamma(gamma_b, rows, out=None):
+    m, n = gamma_b.shape
+    if out is None:
+        out = torch.empty((rows, n), dtype=F64, device=gamma_b.device)
+    rc = L.lib().fj_static_wake_gamma(L.ptr(gamma_b), m, n, rows, L.ptr(out) if rows * n else None, L.stream())
+    L.check(rc, "fj_static_wake_gamma")
+    return out
+
+
+def steady_forces(zeta, zeta_dot, gamma_b, gamma_w0, v_mid, flow, time, rho, out=None):
+    zeta = _c64(zeta, (3,), "zeta")
+    m, n = zeta.shape[0] - 1, zeta.shape[1] - 1
+    if out is None:
+        out = torch.empty_like(zeta)
+    fd = flow_desc(flow) if flow is not None else None
+    rc = L.lib().fj_steady_forces(L.ptr(zeta), L.ptr(zeta_dot) if zeta_dot is not None else None, L.ptr(gamma_b),
+                                  L.ptr(gamma_w0) if gamma_w0 is not None else None, L.ptr(v_mid), m, n,
+                                  ctypes.byref(fd) if fd is not None else None, float(time), float(rho), L.ptr(out),
+                                  L.stream())
+    L.check(rc, "fj_steady_forces")
+    return out
+
+
+def gamma_dot_unsteady(gamma_n, gamma_nm1, gamma_dot_nm1, nc, dt, relax, rho, gamma_dot=None, f_uns=None):
+    m, n = gamma_n.shape
+    if gamma_dot is None:
+        gamma_dot = torch.empty_like(gamma_n)
+    if f_uns is None and nc is not None:
+        f_uns = torch.empty((m + 1, n + 1, 3), dtype=F64, device=gamma_n.device)
+    rc = L.lib().fj_gamma_dot_unsteady(L.ptr(gamma_n), L.ptr(gamma_nm1), L.ptr(gamma_dot_nm1),
+                                       L.ptr(nc) if nc is not None else None, m, n, float(dt), float(relax),
+                                       float(rho), L.ptr(gamma_dot), L.ptr(f_uns) if f_uns is not None else None,
+                                       L.stream())
+    L.check(rc, "fj_gamma_dot_unsteady")
+    return gamma_dot, f_uns
+
+
+def strip_alpha(zeta, f_steady, flow, time, rho, out=None):
+    m, n = zeta.shape[0] - 1, zeta.shape[1] - 1
+    if out is None:
+        out = torch.empty(n, dtype=F64, device=zeta.device)
+    fd = flow_desc(flow)
+    rc = L.lib().fj_strip_alpha(L.ptr(zeta), L.ptr(f_steady), m, n, ctypes.byref(fd), float(time), float(rho),
+                                L.ptr(out), L.stream())
+    L.check(rc, "fj_strip_alpha")
+    return out
+
+
+def flow_eval(flow, x, time):
+    x = _c64(x, (3,), "x")
+    flat = x.reshape(-1, 3)
+    out = torch.empty_like(flat)
+    fd = flow_desc(flow)
+    rc = L.lib().fj_flow_eval(ctypes.byref(fd), L.ptr(flat), flat.shape[0], float(time), L.ptr(out), L.stream())
+    L.check(rc, "fj_flow_eval")
+    return out.reshape(x.shape)

@@ -1,0 +1,148 @@
+"""ctypes binding of libflapjax_amd.so (the C-ABI in include/flapjax_amd.h).
+
+The library is built in-tree by ``python -m flapjax_amd.build``.  There is no
+fallback: if the library is missing or no GPU is visible, every compute call
+raises.  Tensors cross the boundary as raw device pointers; the stream is
+torch's current HIP stream.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from ctypes import POINTER, c_double, c_int, c_int32, c_size_t, c_void_p
+from pathlib import Path
+
+import torch
+
+LIB_PATH = Path(__file__).resolve().parent / "libflapjax_amd.so"
+
+FJ_OK = 0
+FJ_ERR_ARG = -1
+FJ_ERR_LAUNCH = -2
+FJ_ERR_ALLOC = -3
+FJ_ERR_SINGULAR = -4
+FJ_ERR_NOCONV = -5
+
+KERNEL_EPSILON, KERNEL_CUTOFF, KERNEL_PLAIN = 0, 1, 2
+PREC_FP64, PREC_MIXED = 0, 1
+OUT_STORE, OUT_ADD, OUT_CONVECT = 0, 1, 2
+FLOW_CONSTANT, FLOW_ONE_MINUS_COSINE = 0, 1
+SEG_STRIDE = 16
+
+_ERRS = {
+    FJ_ERR_ARG: "invalid argument",
+    FJ_ERR_LAUNCH: "kernel launch failed",
+    FJ_ERR_ALLOC: "workspace allocation failed",
+    FJ_ERR_SINGULAR: "singular matrix",
+    FJ_ERR_NOCONV: "iterative refinement did not converge",
+}
+
+
+class FlapjaxLibError(RuntimeError):
+    pass
+
+
+class FjFlowfield(ctypes.Structure):
+    _fields_ = [
+        ("kind", c_int),
+        ("relative_motion", c_int),
+        ("u_inf", c_double * 3),
+        ("gust_length", c_double),
+        ("gust_amplitude", c_double),
+        ("gust_travel_dir", c_double * 3),
+        ("gust_amplitude_dir", c_double * 3),
+        ("gust_x0", c_double * 3),
+    ]
+
+
+_SIGS = {
+    "fj_version": ([], ctypes.c_char_p),
+    "fj_pair_sum_workspace_bytes": ([c_int, c_int, c_int], c_size_t),
+    "fj_segments_from_grid": ([c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_int, c_double, c_void_p, c_void_p], c_int),
+    "fj_pair_sum": ([c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, c_double, c_double, c_int, c_int,
+                     POINTER(FjFlowfield), c_double, c_double, c_void_p, c_void_p], c_int),
+    "fj_aic_assemble": ([c_void_p, c_void_p, c_int, c_void_p, c_int, c_int, c_void_p, c_int, c_void_p, c_void_p,
+                         c_int, c_double, c_double, c_void_p, c_int, c_int, c_int, c_void_p], c_int),
+    "fj_lu_workspace_bytes": ([c_int], c_size_t),
+    "fj_lu_factor": ([c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p], c_int),
+    "fj_lu_solve": ([c_void_p, c_int, c_int, c_void_p, c_void_p, c_int, c_int, c_void_p], c_int),
+    "fj_lu_factor_f32": ([c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p], c_int),
+    "fj_refine_workspace_bytes": ([c_int], c_size_t),
+    "fj_solve_refined": ([c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p, c_int, c_double,
+                          c_void_p, c_void_p, c_void_p], c_int),
+    "fj_grid_from_hg": ([c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p], c_int),
+    "fj_colloc_normals": ([c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p], c_int),
+    "fj_filament_midpoints": ([c_void_p, c_int, c_int, c_void_p, c_void_p], c_int),
+    "fj_rhs": ([c_void_p, c_void_p, c_void_p, c_void_p, c_int, POINTER(FjFlowfield), c_double, c_void_p, c_void_p], c_int),
+    "fj_wake_shift": ([c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p], c_int),
+    "fj_wake_convect": ([c_void_p, c_void_p, c_int, c_int, POINTER(FjFlowfield), c_double, c_double, c_void_p,
+                         c_void_p], c_int),
+    "fj_wake_initialise": ([c_void_p, c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p], c_int),
+    "fj_static_wake_gamma": ([c_void_p, c_int, c_int, c_int, c_void_p, c_void_p], c_int),
+    "fj_segments_from_endpoints": ([c_void_p, c_void_p, c_int, c_int, c_double, c_void_p, c_void_p], c_int),
+    "fj_steady_forces": ([c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, POINTER(FjFlowfield),
+                          c_double, c_double, c_void_p, c_void_p], c_int),
+    "fj_gamma_dot_unsteady": ([c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_double, c_double, c_double,
+                               c_void_p, c_void_p, c_void_p], c_int),
+    "fj_strip_alpha": ([c_void_p, c_void_p, c_int, c_int, POINTER(FjFlowfield), c_double, c_double, c_void_p,
+                        c_void_p], c_int),
+    "fj_flow_eval": ([POINTER(FjFlowfield), c_void_p, c_int, c_double, c_void_p, c_void_p], c_int),
+}
+
+_lib = None
+
+
+def exported_symbols():
+    """Names the library must export (every entry point of the header)."""
+    return list(_SIGS)
+
+
+def load(require_gpu: bool = False):
+    """Load the library (once).  Raises if it is not built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise FlapjaxLibError(
+                f"{LIB_PATH} is not built; run `python -m flapjax_amd.build` (there is no CPU fallback)"
+            )
+        lib = ctypes.CDLL(str(LIB_PATH))
+        for name, (args, res) in _SIGS.items():
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = lib
+    if require_gpu and not torch.cuda.is_available():
+        raise FlapjaxLibError("flapjax_amd needs a ROCm GPU; none is visible (there is no CPU fallback)")
+    return _lib
+
+
+def lib():
+    return load(require_gpu=True)
+
+
+def check(rc: int, what: str) -> None:
+    if rc != FJ_OK:
+        raise FlapjaxLibError(f"{what}: {_ERRS.get(rc, 'error')} (status {rc})")
+
+
+def ptr(t):
+    """Device pointer of a tensor (or None)."""
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise ValueError("flapjax_amd expects device tensors")
+    if not t.is_contiguous():
+        raise ValueError("flapjax_amd expects contiguous tensors")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def host3(v):
+    """Host pointer to 3 doubles (or None)."""
+    if v is None:
+        return None
+    arr = (c_double * 3)(*[float(x) for x in v])
+    return ctypes.cast(arr, c_void_p), arr

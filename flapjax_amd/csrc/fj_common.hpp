@@ -1,0 +1,145 @@
+// Shared device-side definitions for the flapjax_amd UVLM hot path (gfx950).
+//
+// Data layout in HBM (all FP64 unless noted):
+//   vertex grids   (m+1, n+1, 3) row-major, xyz innermost (chordwise index first)
+//   ring strengths (m, n) row-major
+//   segment table  FJ_SEG_STRIDE doubles per de-duplicated filament (see below)
+//   targets        (T, 3) AoS
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/flapjax_amd.h"
+
+// One vortex filament, 16 doubles = 128 bytes, read by whole waves through the
+// scalar unit (every lane of a wave evaluates the same filament at once):
+//   [0..2] y0   start vertex          [3..5] y1   end vertex
+//   [6..8] r0 = y1 - y0               [9]   aux  (epsilon kernel: eps*|r0|^4,
+//                                                 cutoff kernel: 1/|r0|^2)
+//   [10]  g    filament circulation   [11]  |r0|^2
+//   [12..15] FP32 copies for the mixed form: r0x, r0y, r0z, aux, g, 0, 0, 0
+#define FJ_SEG_STRIDE 16
+
+#define FJ_INV_4PI 0.079577471545947667884441881686257181017229822870228
+
+struct FjFlow {
+  int kind;             // FJ_FLOW_CONSTANT or FJ_FLOW_ONE_MINUS_COSINE
+  int relative_motion;  // 1: the air moves (u_inf added), 0: the body moves
+  double u_inf[3];
+  double gust_length, gust_amplitude;
+  double travel_dir[3], amp_dir[3], x0[3];
+};
+
+static inline FjFlow fj_flow_from_desc(const fj_flowfield* f) {
+  FjFlow o;
+  o.kind = f ? f->kind : FJ_FLOW_CONSTANT;
+  o.relative_motion = f ? f->relative_motion : 1;
+  for (int k = 0; k < 3; ++k) {
+    o.u_inf[k] = f ? f->u_inf[k] : 0.0;
+    o.travel_dir[k] = f ? f->gust_travel_dir[k] : 0.0;
+    o.amp_dir[k] = f ? f->gust_amplitude_dir[k] : 0.0;
+    o.x0[k] = f ? f->gust_x0[k] : 0.0;
+  }
+  o.gust_length = f ? f->gust_length : 1.0;
+  o.gust_amplitude = f ? f->gust_amplitude : 0.0;
+  return o;
+}
+
+// Background flow at (x, t): aero/flowfields.py:88-97 (Constant) and
+// aero/flowfields.py:158-181 (OneMinusCosine, 0.5*A*(1-cos(pi*s/L)) on 0<s<2L).
+__device__ __forceinline__ void fj_flow_eval(const FjFlow& f, double x, double y, double z,
+                                             double t, double* u) {
+  u[0] = 0.0; u[1] = 0.0; u[2] = 0.0;
+  if (f.kind == FJ_FLOW_ONE_MINUS_COSINE) {
+    double rx = x - f.x0[0], ry = y - f.x0[1], rz = z - f.x0[2];
+    if (f.relative_motion) {
+      rx -= f.u_inf[0] * t; ry -= f.u_inf[1] * t; rz -= f.u_inf[2] * t;
+    }
+    const double s = rx * f.travel_dir[0] + ry * f.travel_dir[1] + rz * f.travel_dir[2];
+    if (s > 0.0 && s < 2.0 * f.gust_length) {
+      const double a = f.gust_amplitude * 0.5 * (1.0 - cos(M_PI * s / f.gust_length));
+      u[0] = f.amp_dir[0] * a; u[1] = f.amp_dir[1] * a; u[2] = f.amp_dir[2] * a;
+    }
+  }
+  if (f.relative_motion) {
+    u[0] += f.u_inf[0]; u[1] += f.u_inf[1]; u[2] += f.u_inf[2];
+  }
+}
+
+struct FjKernelParams {
+  double eps;   // regularisation (utils/constants.py:16, EPSILON = 1e-7)
+  double eps2;  // eps * eps
+  double rcut;  // cutoff radius (utils/constants.py:19, R_CUTOFF = 1e-4)
+};
+
+// FP64 1/sqrt(x) for positive normal x: hardware estimate (about 2^-23
+// relative) refined by one third-order step y(1 + e/2 + 3e^2/8), e = 1 - x y^2,
+// which leaves an error of a few ulp.  No special-value fix-up: callers only
+// pass arguments >= eps^2 > 0 (or +0 for the un-regularised kernel, where the
+// reference itself yields inf/nan).
+__device__ __forceinline__ double fj_rsqrt(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  const double e = fma(-x * y, y, 1.0);
+  return fma(y * e, fma(e, 0.375, 0.5), y);
+}
+
+// Influence of one filament scaled by its circulation g, WITHOUT the 1/(4 pi)
+// factor (callers apply it once per target):
+//   FJ_KERNEL_EPSILON: aero/utils.py:606-623
+//     K = c (r0 . d) / (|c|^2 + eps |r0|^4),  c = r1 x r2,
+//     d = r1 / sqrt(|r1|^2 + eps^2) - r2 / sqrt(|r2|^2 + eps^2)
+//   FJ_KERNEL_CUTOFF : aero/utils.py:626-648 (0 within rcut of the line,
+//                      else c (r0 . d) / |c|^2 with the same regularised d)
+//   FJ_KERNEL_PLAIN  : aero/utils.py:580-592 (un-regularised d, / |c|^2)
+// The division by D = |c|^2 (+ eps |r0|^4) is folded into the two reciprocal
+// square roots: (r0 . d) / D = r0 . (r1 / sqrt(s1 D^2) - r2 / sqrt(s2 D^2)),
+// which is the same quantity up to rounding and saves the FP64 divide.
+template <int KIND>
+__device__ __forceinline__ void fj_bs_eval(double x, double y, double z, const double* s,
+                                           const FjKernelParams& kp, double& vx, double& vy,
+                                           double& vz) {
+  const double r1x = x - s[0], r1y = y - s[1], r1z = z - s[2];
+  const double r2x = x - s[3], r2y = y - s[4], r2z = z - s[5];
+  const double r0x = s[6], r0y = s[7], r0z = s[8];
+  const double cx = r1y * r2z - r1z * r2y;
+  const double cy = r1z * r2x - r1x * r2z;
+  const double cz = r1x * r2y - r1y * r2x;
+  const double e2 = (KIND == FJ_KERNEL_PLAIN) ? 0.0 : kp.eps2;
+  const double s1 = r1x * r1x + r1y * r1y + r1z * r1z + e2;
+  const double s2 = r2x * r2x + r2y * r2y + r2z * r2z + e2;
+  double D = cx * cx + cy * cy + cz * cz;
+  if (KIND == FJ_KERNEL_EPSILON) D += s[9];
+  const double D2 = D * D;
+  const double w1 = fj_rsqrt(s1 * D2);
+  const double w2 = fj_rsqrt(s2 * D2);
+  const double dx = r1x * w1 - r2x * w2;
+  const double dy = r1y * w1 - r2y * w2;
+  const double dz = r1z * w1 - r2z * w2;
+  double q = s[10] * (r0x * dx + r0y * dy + r0z * dz);
+  if (KIND == FJ_KERNEL_CUTOFF) {
+    // distance from x to the filament line: |r1 - (r0.r1 / |r0|^2) r0|
+    const double sm = (r0x * r1x + r0y * r1y + r0z * r1z) * s[9];
+    const double px = r1x - sm * r0x, py = r1y - sm * r0y, pz = r1z - sm * r0z;
+    const double rad2 = px * px + py * py + pz * pz;
+    q = (rad2 > kp.rcut * kp.rcut) ? q : 0.0;
+  }
+  vx = cx * q; vy = cy * q; vz = cz * q;
+}
+
+// Launch plan of a pair sum (fj_biot_savart.hip)
+struct FjPairPlan {
+  int tpl;        // targets per lane
+  int tiles;      // target tiles (grid.x)
+  int chunks;     // filament chunks (grid.y)
+  int chunk_len;  // filaments per chunk
+  int ld;         // padded target count of the partial buffer
+};
+
+// Launch-shape helpers
+static inline int fj_cdiv(long long a, long long b) { return (int)((a + b - 1) / b); }
+
+#define FJ_CHECK_LAUNCH()                         \
+  do {                                            \
+    hipError_t e_ = hipGetLastError();            \
+    if (e_ != hipSuccess) return FJ_ERR_LAUNCH;   \
+  } while (0)

@@ -1,0 +1,239 @@
+"""Segment kernels, grid stencils, forces and wake convection
+(aero/utils.py of the reference), on the HIP path.
+
+The reference's ``KernelFunction`` plug-in (aero/utils.py:19) becomes a
+``SegmentKernel`` descriptor: the three kernels the reference ships map to a
+kernel kind the HIP code evaluates in bulk.  Calling a descriptor on one
+(target, filament) pair evaluates that pair on the device, so the reference's
+kernel unit tests read the same here.  Any other Python callable has no
+device implementation and raises ``NotImplementedError``.
+"""
+
+from __future__ import annotations
+
+from collections.abc import Callable, Sequence
+
+import numpy as np
+import torch
+
+from . import _kernels as K
+from . import _lib as L
+
+# utils/constants.py:13-19
+HORSESHOE_LENGTH = 1e3
+EPSILON = 1e-7
+R_CUTOFF = 1e-4
+
+F64 = torch.float64
+
+
+class SegmentKernel:
+    """A Biot-Savart segment kernel the device path implements."""
+
+    def __init__(self, kind: int, name: str, eps: float = EPSILON, rcut: float = R_CUTOFF):
+        self.kind = int(kind)
+        self.name = name
+        self.eps = float(eps)
+        self.rcut = float(rcut)
+
+    def __repr__(self):
+        return f"SegmentKernel({self.name}, eps={self.eps:g}, rcut={self.rcut:g})"
+
+    def __call__(self, x: torch.Tensor, y: torch.Tensor) -> torch.Tensor:
+        """Influence per unit circulation at x (3,) of the filament y (2, 3)."""
+        x = torch.as_tensor(x, dtype=F64, device=_device()).reshape(1, 3)
+        y = torch.as_tensor(y, dtype=F64, device=_device()).reshape(1, 2, 3)
+        seg = K.segments_from_endpoints(y, None, self.kind, self.eps)
+        return K.pair_sum(seg, x, self.kind, self.eps, self.rcut).reshape(3)
+
+
+# aero/utils.py:580-648
+biot_savart = SegmentKernel(L.KERNEL_PLAIN, "biot_savart")
+biot_savart_epsilon = SegmentKernel(L.KERNEL_EPSILON, "biot_savart_epsilon")
+biot_savart_cutoff = SegmentKernel(L.KERNEL_CUTOFF, "biot_savart_cutoff")
+
+KernelFunction = SegmentKernel
+
+
+def as_segment_kernel(kernel) -> SegmentKernel:
+    if kernel is None:
+        return biot_savart_epsilon
+    if isinstance(kernel, SegmentKernel):
+        return kernel
+    raise NotImplementedError(
+        "only the reference's biot_savart / biot_savart_epsilon / biot_savart_cutoff kernels have a "
+        f"device implementation (got {kernel!r})"
+    )
+
+
+def _device():
+    if not torch.cuda.is_available():
+        raise L.FlapjaxLibError("flapjax_amd needs a ROCm GPU; none is visible (there is no CPU fallback)")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def to_device(x) -> torch.Tensor:
+    if isinstance(x, torch.Tensor):
+        return x.to(device=_device(), dtype=F64).contiguous()
+    return torch.as_tensor(np.asarray(x, dtype=np.float64), device=_device()).contiguous()
+
+
+# ---------------------------------------------------------------------------
+# grids (aero/utils.py:23-36, 87-123, 651-663)
+# ---------------------------------------------------------------------------
+def make_rectangular_grid(m: int, n: int, chord: float, ea: float) -> torch.Tensor:
+    """aero/utils.py:23-36 (host-built geometry, moved to the device)."""
+    grid = np.zeros((m + 1, n + 1, 3))
+    grid[..., 0] = (np.linspace(0.0, chord, m + 1) - ea * chord)[:, None]
+    return to_device(grid)
+
+
+def compute_surf_c(zeta: torch.Tensor) -> torch.Tensor:
+    """Collocation points, aero/utils.py:87-93."""
+    return K.colloc_normals(to_device(zeta), want_n=False)[0]
+
+
+def compute_surf_nc(zeta: torch.Tensor) -> torch.Tensor:
+    """Panel normals of length 2 * area, aero/utils.py:96-105."""
+    return K.colloc_normals(to_device(zeta), want_c=False)[1]
+
+
+def compute_c(zetas):
+    return [compute_surf_c(z) for z in zetas]
+
+
+def compute_nc(zetas):
+    return [compute_surf_nc(z) for z in zetas]
+
+
+def mirror_grid(zeta: torch.Tensor, mirror_point, mirror_normal) -> torch.Tensor:
+    """aero/utils.py:651-663 (geometry set-up helper)."""
+    zeta = to_device(zeta)
+    p = to_device(mirror_point)
+    nrm = to_device(mirror_normal)
+    dn = ((zeta - p) * nrm).sum(-1, keepdim=True)
+    return zeta - 2.0 * dn * nrm
+
+
+# ---------------------------------------------------------------------------
+# velocity closures the fused device path recognises
+# ---------------------------------------------------------------------------
+class FlowAt:
+    """x -> U(x, t): the freestream closure of aero/uvlm.py:1045-1046."""
+
+    def __init__(self, flowfield, t):
+        self.flowfield = flowfield
+        self.t = float(t)
+
+    def __call__(self, x):
+        return self.flowfield.vmap_call(x, self.t)
+
+
+class InducedVelocity:
+    """x -> U(x, t) + sum of filament influences: the closures of
+    aero/uvlm.py:917-929 (free-wake convection) and :1023-1033 (forces).
+    ``segments`` maps (kernel kind, eps, rcut) -> filament table
+    (aic.segment_tables)."""
+
+    def __init__(self, flowfield, t, segments, kernel: SegmentKernel, precision=L.PREC_FP64):
+        self.flowfield = flowfield
+        self.t = float(t)
+        self.segments = segments
+        self.kernel = kernel
+        self.precision = precision
+
+    def induced(self, x, out=None, mode=L.OUT_STORE, flow=None, dt=0.0):
+        x = to_device(x)
+        flat = x.reshape(-1, 3)
+        if mode == L.OUT_CONVECT and len(self.segments) > 1:
+            raise NotImplementedError("fused convection needs one kernel kind for all surfaces")
+        first = True
+        for (kind, eps, rcut), seg in self.segments.items():
+            m = mode if first else L.OUT_ADD
+            out = K.pair_sum(seg, flat, kind, eps, rcut, self.precision, m, out=out, flow=flow, time=self.t, dt=dt)
+            first = False
+        if first:
+            out = K.pair_sum(None, flat, self.kernel.kind, self.kernel.eps, self.kernel.rcut, self.precision, mode,
+                             out=out, flow=flow, time=self.t, dt=dt)
+        return out
+
+    def __call__(self, x):
+        x = to_device(x)
+        v = self.induced(x)
+        return (v + self.flowfield.vmap_call(x.reshape(-1, 3), self.t)).reshape(x.shape)
+
+
+# ---------------------------------------------------------------------------
+# forces (aero/utils.py:126-273)
+# ---------------------------------------------------------------------------
+def calculate_steady_forcing(zeta_b, zeta_dot_b, gamma_b, gamma_w, rho, v_func: Callable, v_inputs=None):
+    """aero/utils.py:126-223.  ``v_func`` gives the total velocity at points;
+    an ``InducedVelocity`` closure is evaluated by the fused kernels."""
+    if v_inputs is not None:
+        raise NotImplementedError("v_inputs (linearised UVLM perturbations) are not on the device path")
+    out = []
+    zdots = zeta_dot_b if zeta_dot_b is not None else [None] * len(zeta_b)
+    for zb, zd, gb, gw in zip(zeta_b, zdots, gamma_b, gamma_w):
+        zb = to_device(zb)
+        mid = K.filament_midpoints(zb)
+        gw0 = to_device(gw)[0].contiguous() if gw.shape[0] > 0 else None
+        if isinstance(v_func, InducedVelocity):
+            v = v_func.induced(mid)
+            f = K.steady_forces(zb, to_device(zd) if zd is not None else None, to_device(gb), gw0, v,
+                                v_func.flowfield, v_func.t, rho)
+        else:
+            v = to_device(v_func(mid)).reshape(-1, 3).contiguous()
+            f = K.steady_forces(zb, to_device(zd) if zd is not None else None, to_device(gb), gw0, v, None, 0.0, rho)
+        out.append(f)
+    return out
+
+
+def strip_alpha(zeta_b, f_steady, v_func: FlowAt, rho):
+    """aero/utils.py:226-273 (v_func must be the freestream closure)."""
+    if not isinstance(v_func, FlowAt):
+        raise NotImplementedError("strip_alpha evaluates the freestream on the device; pass utils.FlowAt")
+    return [K.strip_alpha(to_device(z), to_device(f), v_func.flowfield, v_func.t, rho) for z, f in zip(zeta_b, f_steady)]
+
+
+# ---------------------------------------------------------------------------
+# wake convection (aero/utils.py:400-577)
+# ---------------------------------------------------------------------------
+def propagate_surf_wake(gamma_b_nm1, gamma_w_nm1, zeta_b_n, zeta_w_nm1, delta_w, v_func, dt, frozen_wake,
+                        linearise_variable_wake=False):
+    """aero/utils.py:400-500 (uniform discretisation)."""
+    if delta_w is not None:
+        raise NotImplementedError("variable wake discretisation (delta_w) is not on the device path")
+    gamma_b_nm1 = to_device(gamma_b_nm1)
+    gamma_w_nm1 = to_device(gamma_w_nm1)
+    zeta_b_n = to_device(zeta_b_n)
+    zeta_w_nm1 = to_device(zeta_w_nm1)
+    m_star = zeta_w_nm1.shape[0] - 1
+    zeta_w_n = torch.empty_like(zeta_w_nm1)
+    gamma_w_n = torch.empty_like(gamma_w_nm1)
+    K.wake_shift(gamma_b_nm1, gamma_w_nm1, zeta_b_n, m_star, gamma_w_n, zeta_w_n)
+    if m_star > 0:
+        base = zeta_w_nm1[:-1].reshape(-1, 3)
+        if isinstance(v_func, InducedVelocity):
+            v_func.induced(base, out=zeta_w_n[1:].reshape(-1, 3), mode=L.OUT_CONVECT, flow=v_func.flowfield, dt=dt)
+        elif isinstance(v_func, FlowAt):
+            K.wake_convect(zeta_w_nm1, None, v_func.flowfield, v_func.t, dt, zeta_w_n)
+        else:
+            vel = to_device(v_func(zeta_w_nm1[:-1])).reshape(-1, 3).contiguous()
+            K.wake_convect(zeta_w_nm1, vel, None, 0.0, dt, zeta_w_n)
+    if frozen_wake:
+        return None, gamma_w_n
+    return zeta_w_n, gamma_w_n
+
+
+def propagate_wake(gamma_b_nm1, gamma_w_nm1, zeta_b_n, zeta_w_nm1, delta_w: Sequence, v_func, dt, frozen_wake,
+                   linearise_variable_wake=False):
+    """aero/utils.py:531-577."""
+    zs = None if frozen_wake else []
+    gs = []
+    for i in range(len(gamma_b_nm1)):
+        z, g = propagate_surf_wake(gamma_b_nm1[i], gamma_w_nm1[i], zeta_b_n[i], zeta_w_nm1[i], delta_w[i], v_func, dt,
+                                   frozen_wake, linearise_variable_wake)
+        if zs is not None:
+            zs.append(z)
+        gs.append(g)
+    return zs, gs

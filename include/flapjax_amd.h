@@ -1,0 +1,241 @@
+/* flapjax_amd: C-ABI of the MI355X-native UVLM aerodynamic hot path.
+ *
+ * Every entry point takes caller-owned DEVICE pointers (FP64 unless stated),
+ * plain sizes, and an explicit HIP stream (hipStream_t passed as void*; NULL
+ * is the default stream).  Work is enqueued asynchronously on that stream.
+ * The return value is a status code: 0 on success, a negative FJ_ERR_* value
+ * otherwise (nothing is enqueued when an argument check fails).
+ *
+ * Layouts (the reference's, aero/uvlm.py:623-640 and aero/aic.py):
+ *   vertex grid      (m+1, n+1, 3) row-major, xyz innermost
+ *   ring strengths   (m, n) row-major
+ *   points / vectors (T, 3)
+ *   AIC              (rows, ld) row-major, rows = targets, columns = rings
+ *   segment table    FJ_SEG_STRIDE doubles per filament (see fj_segments_from_grid)
+ *
+ * Reference paths below are relative to src/flapjax/ of ben-l-p/flapjax v1.2.1.
+ */
+#ifndef FLAPJAX_AMD_H
+#define FLAPJAX_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define FJ_OK 0
+#define FJ_ERR_ARG (-1)      /* bad size / null pointer / unknown enum */
+#define FJ_ERR_LAUNCH (-2)   /* a kernel launch failed */
+#define FJ_ERR_ALLOC (-3)    /* workspace allocation failed */
+#define FJ_ERR_SINGULAR (-4) /* zero pivot met in the LU factorisation */
+#define FJ_ERR_NOCONV (-5)   /* iterative refinement did not converge */
+
+/* Biot-Savart kernel kinds: the three KernelFunction implementations the
+ * reference ships (aero/utils.py:19 type, :580-648 implementations). */
+#define FJ_KERNEL_EPSILON 0 /* biot_savart_epsilon, aero/utils.py:606-623 (default, aero/uvlm.py:280-285) */
+#define FJ_KERNEL_CUTOFF 1  /* biot_savart_cutoff,  aero/utils.py:626-648 */
+#define FJ_KERNEL_PLAIN 2   /* biot_savart,         aero/utils.py:580-592 */
+
+/* Arithmetic forms of the pair sums */
+#define FJ_PREC_FP64 0  /* plain FP64 drop-in */
+#define FJ_PREC_MIXED 1 /* FP32 far field, FP64 near field, FP64 accumulation */
+
+/* Epilogues of a pair sum (what is done with v at target t) */
+#define FJ_OUT_STORE 0   /* out[t] = v */
+#define FJ_OUT_ADD 1     /* out[t] += v */
+#define FJ_OUT_CONVECT 2 /* out[t] = x[t] + dt * (U(x[t], time) + v): wake convection,
+                            aero/utils.py:445-450 with the free-wake closure aero/uvlm.py:917-929 */
+
+/* Flow-field kinds, aero/flowfields.py */
+#define FJ_FLOW_CONSTANT 0          /* Constant, aero/flowfields.py:88-97 */
+#define FJ_FLOW_ONE_MINUS_COSINE 1  /* OneMinusCosine, aero/flowfields.py:101-200 */
+
+#define FJ_SEG_STRIDE_DOUBLES 16
+
+typedef struct fj_flowfield {
+  int kind;
+  int relative_motion;
+  double u_inf[3];
+  double gust_length;
+  double gust_amplitude;
+  double gust_travel_dir[3];    /* unit */
+  double gust_amplitude_dir[3]; /* unit */
+  double gust_x0[3];
+} fj_flowfield;
+
+/* ---- library ----------------------------------------------------------- */
+
+/* Version string of the library build. */
+const char* fj_version(void);
+
+/* Number of bytes of device workspace fj_pair_sum needs for n_tgt targets and
+ * n_seg segments (caller allocates; 256-byte alignment). */
+size_t fj_pair_sum_workspace_bytes(int n_tgt, int n_seg, int precision);
+
+/* ---- K1: de-duplicated filaments and pair sums --------------------------- */
+
+/* Unique filaments of one vertex grid with their effective circulations,
+ * replacing the segment construction of compute_v_ind (aero/aic.py:277-287):
+ * chordwise filaments (m, n+1) first, then spanwise (m+1, n), each row-major.
+ * gamma may be NULL (all circulations zero; geometry only).
+ * If mirror_point/mirror_normal (HOST pointers, 3 doubles each, normal unit)
+ * are non-NULL the grid is first reflected (aero/utils.py:651-663) and the
+ * circulations negated, which reproduces the subtraction at aero/aic.py:293-308.
+ * seg_out receives (m*(n+1) + (m+1)*n) records of FJ_SEG_STRIDE_DOUBLES. */
+int fj_segments_from_grid(const double* zeta, const double* gamma, int m, int n,
+                          const double* mirror_point, const double* mirror_normal,
+                          int kernel, double eps, double* seg_out, void* stream);
+
+/* Filament table from end points ends (n_seg, 2, 3) and circulations gamma
+ * (n_seg; NULL = unit), the filament-grid input of v_ind_vmap
+ * (aero/aic.py:208-245). */
+int fj_segments_from_endpoints(const double* ends, const double* gamma, int n_seg, int kernel,
+                               double eps, double* seg_out, void* stream);
+
+/* v(x_t) = sum_s K(x_t; seg_s) for n_tgt targets against n_seg filaments:
+ * the contraction done by v_ind_vmap (aero/aic.py:208-245) and summed by
+ * compute_v_ind (aero/aic.py:248-310).  The epilogue `mode` decides what is
+ * written to out (n_tgt, 3); for FJ_OUT_CONVECT, `flow`, `time` and `dt`
+ * give U(x, time) and the step; otherwise they are ignored (flow may be NULL).
+ * workspace: fj_pair_sum_workspace_bytes(n_tgt, n_seg, precision) bytes. */
+int fj_pair_sum(const double* seg, int n_seg, const double* targets, int n_tgt, double* out,
+                int kernel, double eps, double rcut, int precision, int mode,
+                const fj_flowfield* flow, double time, double dt, void* workspace,
+                void* stream);
+
+/* ---- K2: AIC assembly ----------------------------------------------------- */
+
+/* Normal-projected ring influence of the grid zeta (m+1, n+1, 3) at n_tgt
+ * collocation points with (non-unit) normals: compute_aic_grid
+ * (aero/aic.py:11-63), row t, column i*n + j:
+ *   A = -(M[i,j+1] - M[i,j]) + (N[i+1,j] - N[i,j]),  M, N = K . n.
+ * Writes A_out[t*lda + col0 + i*n + j] (accumulate=0) or adds to it
+ * (accumulate=1).  sign = +1 or -1 multiplies the block (mirror images are
+ * subtracted, aero/aic.py:101-110).
+ * If zeta_w is non-NULL (m_star >= 1 wake rows, (m_star+1, n+1, 3)), the
+ * summed influence of each wake column is added onto the trailing-edge row of
+ * the bound block (add_wake_influence, aero/aic.py:190-205; static solves). */
+int fj_aic_assemble(const double* colloc, const double* normals, int n_tgt, const double* zeta,
+                    int m, int n, const double* zeta_w, int m_star,
+                    const double* mirror_point, const double* mirror_normal, int kernel,
+                    double eps, double rcut, double* A_out, int lda, int col0, int accumulate,
+                    void* stream);
+
+/* ---- K3: circulation solve -------------------------------------------------
+ * Replaces jnp.linalg.solve(aic, -v_bc_n) at aero/uvlm.py:985 (and :1634). */
+
+/* Bytes of workspace fj_lu_factor needs for an n x n matrix. */
+size_t fj_lu_workspace_bytes(int n);
+
+/* In-place LU with partial pivoting of a row-major n x n FP64 matrix
+ * (P A = L U, unit-lower L).  ipiv (device, n int32) receives 0-based pivot
+ * rows in LAPACK order.  info (device, 1 int32) receives 0 or 1 + the first
+ * zero-pivot column. */
+int fj_lu_factor(double* A, int n, int lda, int32_t* ipiv, int32_t* info, void* workspace,
+                 void* stream);
+
+/* Solve (trans = 0) A x = b or (trans = 1) A^T x = b in place in b (n, nrhs
+ * column-major: right-hand side r at b + r*n) from fj_lu_factor's output. */
+int fj_lu_solve(const double* LU, int n, int lda, const int32_t* ipiv, double* b, int nrhs,
+                int trans, void* stream);
+
+/* FP32 LU factor (row-major, in place) for the refined form. */
+int fj_lu_factor_f32(float* A, int n, int lda, int32_t* ipiv, int32_t* info, void* workspace,
+                     void* stream);
+
+/* x <- solve(A, b) by FP32 LU (LU32, ipiv from fj_lu_factor_f32) refined in
+ * FP64: r = b - A x (FP64 A), x += LU32^{-1} r, for at most max_iter sweeps or
+ * until |r|_inf <= tol * (|A|_inf |x|_inf + |b|_inf).  iters (device int32)
+ * receives the sweeps done (negative if not converged).  workspace:
+ * fj_refine_workspace_bytes(n). */
+size_t fj_refine_workspace_bytes(int n);
+int fj_solve_refined(const double* A, int lda, const float* LU32, int ldlu, const int32_t* ipiv,
+                     int n, const double* b, double* x, int max_iter, double tol,
+                     int32_t* iters, void* workspace, void* stream);
+
+/* ---- K4-K8: stencils, convection, forces ------------------------------- */
+
+/* zeta = R_j zeta0_ij + x_j and, if hg_dot/zeta_dot are non-NULL,
+ * zeta_dot = W_j zeta0_ij + v_j (hg_to_zeta_b / hg_dot_to_zeta_b_dot,
+ * aero/uvlm.py:537-621, se3.vect_product algebra/se3.py:209-216).  hg and
+ * hg_dot are (n_nodes, 4, 4) row-major; dof_map (device, n+1 int32) selects
+ * the beam node of each spanwise station. */
+int fj_grid_from_hg(const double* hg, const double* hg_dot, const int32_t* dof_map,
+                    const double* zeta0, int m, int n, double* zeta, double* zeta_dot,
+                    void* stream);
+
+/* Collocation points (4-vertex average, aero/utils.py:87-93), normals
+ * (diag1 x diag2, aero/utils.py:96-105) and, if zeta_dot/c_dot non-NULL,
+ * collocation velocities (aero/uvlm.py:888-893).  Any output may be NULL. */
+int fj_colloc_normals(const double* zeta, const double* zeta_dot, int m, int n, double* c,
+                      double* nc, double* c_dot, void* stream);
+
+/* Filament midpoints of a vertex grid: chordwise (m, n+1) then spanwise
+ * (m+1, n), the targets of calculate_steady_forcing (aero/utils.py:162-164). */
+int fj_filament_midpoints(const double* zeta, int m, int n, double* mid, void* stream);
+
+/* rhs[t] = -(U(c_t, time) - c_dot_t + v_w_t) . n_t  (aero/uvlm.py:961-985);
+ * c_dot and v_w may be NULL (static solve). */
+int fj_rhs(const double* c, const double* nc, const double* c_dot, const double* v_w, int n_tgt,
+           const fj_flowfield* flow, double time, double* rhs, void* stream);
+
+/* Uniform-discretisation wake shift (aero/utils.py:427-442): gamma_w_n =
+ * [gamma_b_nm1[-1]; gamma_w_nm1[:-1]] and zeta_w_n[0] = zeta_b_n[-1] (TE
+ * row).  The convected rows 1..m_star of zeta_w_n come from fj_pair_sum with
+ * FJ_OUT_CONVECT (or fj_wake_convect for a prescribed wake). */
+int fj_wake_shift(const double* gamma_b_nm1, const double* gamma_w_nm1, const double* zeta_b_n,
+                  int m, int n, int m_star, double* gamma_w_n, double* zeta_w_n, void* stream);
+
+/* Convection of rows 0..m_star-1 of zeta_w_nm1 into rows 1..m_star of
+ * zeta_w_n: x + dt * vel (vel (m_star*(n+1), 3) given, e.g. from a caller's
+ * v_func) or, if vel is NULL, x + dt * U(x, time) (prescribed wake),
+ * aero/utils.py:445-450. */
+int fj_wake_convect(const double* zeta_w_nm1, const double* vel, int m_star, int n,
+                    const fj_flowfield* flow, double time, double dt, double* zeta_w_n,
+                    void* stream);
+
+/* Initial flat wake TE + s_i * dir (initialise_wake, aero/uvlm.py:662-689):
+ * s (device, m_star+1) = [0, cumsum(delta_w)], dir (HOST, 3) = unit U_inf. */
+int fj_wake_initialise(const double* zeta_b, int m, int n, const double* s, int m_star,
+                       const double* dir, double* zeta_w, void* stream);
+
+/* Static wake circulation: every one of `rows` wake rows = TE row of gamma_b
+ * (aero/uvlm.py:995-1012). */
+int fj_static_wake_gamma(const double* gamma_b, int m, int n, int rows, double* gamma_w,
+                         void* stream);
+
+/* Background flow at n_pts points (FlowField.vmap_call, aero/flowfields.py:49-60). */
+int fj_flow_eval(const fj_flowfield* flow, const double* x, int n_pts, double time, double* out,
+                 void* stream);
+
+/* Steady vertex forces (calculate_steady_forcing, aero/utils.py:126-223)
+ * from the velocity v_mid at the filament midpoints (layout of
+ * fj_filament_midpoints): f = rho * Gamma_fil * (v_rel x r), split to the
+ * filament ends, v_rel = U(mid, time) + v_mid - zeta_dot(mid).  With flow NULL,
+ * v_mid is taken as the total velocity (a caller's v_func).  gamma_w0 (n) is
+ * the first wake row or NULL.  zeta_dot may be NULL (static). */
+int fj_steady_forces(const double* zeta, const double* zeta_dot, const double* gamma_b,
+                     const double* gamma_w0, const double* v_mid, int m, int n,
+                     const fj_flowfield* flow, double time, double rho, double* f_out,
+                     void* stream);
+
+/* Filtered circulation rate and unsteady vertex force (aero/uvlm.py:691-715,
+ * :1086-1103): gdot = g*(G_n - G_nm1)/dt + (1-g)*gdot_nm1,
+ * f = split_to_vertex(rho * gdot * nc). */
+int fj_gamma_dot_unsteady(const double* gamma_n, const double* gamma_nm1,
+                          const double* gamma_dot_nm1, const double* nc, int m, int n, double dt,
+                          double relax, double rho, double* gamma_dot_n, double* f_unsteady,
+                          void* stream);
+
+/* Per-strip effective angle of attack (strip_alpha, aero/utils.py:226-273). */
+int fj_strip_alpha(const double* zeta, const double* f_steady, int m, int n,
+                   const fj_flowfield* flow, double time, double rho, double* alpha,
+                   void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FLAPJAX_AMD_H */
