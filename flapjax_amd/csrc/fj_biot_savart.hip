@@ -300,8 +300,6 @@ extern "C" int fj_pair_sum(const double* seg, int n_seg, const double* targets, 
   const FjPairPlan p = fj_pair_plan(n_tgt, n_seg);
   double* part = (double*)workspace;
   if (precision == FJ_PREC_MIXED) {
-    extern int fj_pair_partial_mixed(const FjPairPlan*, const double*, int, const double*, int,
-                                     double*, int, const FjKernelParams*, hipStream_t);
     const int rc = fj_pair_partial_mixed(&p, seg, n_seg, targets, n_tgt, part, kernel, &kp, st);
     if (rc != FJ_OK) return rc;
   } else {

@@ -135,6 +135,11 @@ struct FjPairPlan {
   int ld;         // padded target count of the partial buffer
 };
 
+// mixed-precision partial sums (fj_biot_savart_mixed.hip)
+int fj_pair_partial_mixed(const FjPairPlan* p, const double* seg, int n_seg, const double* tgt,
+                          int n_tgt, double* part, int kernel, const FjKernelParams* kp,
+                          hipStream_t st);
+
 // Launch-shape helpers
 static inline int fj_cdiv(long long a, long long b) { return (int)((a + b - 1) / b); }
 

@@ -5,11 +5,11 @@
 //    registers, each thread owning up to R rows; per column: register argmax,
 //    wave shuffles + LDS for the workgroup argmax, the pivot row broadcast
 //    through LDS, row exchange between the two owner threads, scale and
-//    rank-1 update in registers.  Then, one thread per trailing column: the
-//    panel's row exchanges (laswp) and the unit-lower triangular solve giving
-//    U12 (trsm), with L11 staged in LDS.  The left columns get their row
-//    exchanges in the same pass.  The composed row permutation is kept in
+//    rank-1 update in registers.  The composed row permutation is kept in
 //    ipiv[n .. 2n) so the solves gather Pb in one parallel step.
+//  * swap/trsm kernel, one thread per column outside the panel: the panel's
+//    row exchanges (laswp) and, right of the panel, the unit-lower triangular
+//    solve giving U12 (trsm) with L11 staged in LDS.
 //  * trailing update A22 -= L21 U12 on the matrix cores
 //    (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32), 64x64 tiles per
 //    workgroup, K = jb staged in LDS.
@@ -42,7 +42,6 @@ __global__ __launch_bounds__(FJ_LU_THREADS) void k_lu_panel(T* __restrict__ A, i
   __shared__ int s_idx[FJ_LU_WAVES];
   __shared__ T s_piv[2][JB];
   __shared__ T s_swp[2][JB];
-  __shared__ T s_L11[JB][JB + 1];
   __shared__ int s_ipiv[JB];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
@@ -59,7 +58,10 @@ __global__ __launch_bounds__(FJ_LU_THREADS) void k_lu_panel(T* __restrict__ A, i
       a[r][c] = (lr < rows && c < jb) ? A[(size_t)(k + lr) * lda + k + c] : T(0);
   }
 
-  for (int jj = 0; jj < jb; ++jj) {
+  // fully unrolled so that every register index below is a compile-time constant
+#pragma unroll
+  for (int jj = 0; jj < JB; ++jj) {
+    if (jj >= jb) break;  // uniform: jb is the same for the whole workgroup
     const int buf = jj & 1;
     // argmax |a[lr][jj]| over local rows lr >= jj
     double best = -1.0;
@@ -125,7 +127,7 @@ __global__ __launch_bounds__(FJ_LU_THREADS) void k_lu_panel(T* __restrict__ A, i
   }
   __syncthreads();
 
-  // write the panel back; rows < jb also go to LDS as L11
+  // write the panel back
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int lr = tid + r * FJ_LU_THREADS;
@@ -133,10 +135,6 @@ __global__ __launch_bounds__(FJ_LU_THREADS) void k_lu_panel(T* __restrict__ A, i
 #pragma unroll
       for (int c = 0; c < JB; ++c)
         if (c < jb) A[(size_t)(k + lr) * lda + k + c] = a[r][c];
-      if (lr < jb) {
-#pragma unroll
-        for (int c = 0; c < JB; ++c) s_L11[lr][c] = a[r][c];
-      }
     }
   }
   if (tid < jb) ipiv[k + tid] = s_ipiv[tid];
@@ -160,33 +158,42 @@ __global__ __launch_bounds__(FJ_LU_THREADS) void k_lu_panel(T* __restrict__ A, i
     __syncthreads();
     if (q >= 0) perm[q] = val;
   }
+}
 
-  // columns outside the panel: row exchanges, then U12 = L11^{-1} A12 on the right
-  for (int c = tid; c < n; c += FJ_LU_THREADS) {
-    if (c >= k && c < k + jb) continue;
-    for (int jj = 0; jj < jb; ++jj) {
-      const int p = s_ipiv[jj];
-      if (p != k + jj) {
-        const T t0 = A[(size_t)(k + jj) * lda + c];
-        A[(size_t)(k + jj) * lda + c] = A[(size_t)p * lda + c];
-        A[(size_t)p * lda + c] = t0;
-      }
+// Columns outside the panel, one thread each: the panel's row exchanges
+// (laswp), then U12 = L11^{-1} A12 for the columns right of the panel (trsm).
+template <typename T, int JB>
+__global__ __launch_bounds__(256) void k_lu_swap_trsm(T* __restrict__ A, int n, int lda, int k,
+                                                      int jb, const int32_t* __restrict__ ipiv) {
+  __shared__ T s_L11[JB][JB + 1];
+  __shared__ T s_u[JB][256];
+  __shared__ int s_ipiv[JB];
+  const int tid = threadIdx.x;
+  for (int e = tid; e < JB * JB; e += blockDim.x) {
+    const int r = e / JB, q = e % JB;
+    s_L11[r][q] = (r < jb && q < jb) ? A[(size_t)(k + r) * lda + k + q] : T(0);
+  }
+  if (tid < JB) s_ipiv[tid] = (tid < jb) ? ipiv[k + tid] : k + tid;
+  __syncthreads();
+  int c = blockIdx.x * blockDim.x + tid;
+  if (c >= n - jb) return;
+  if (c >= k) c += jb;  // skip the panel columns
+  for (int jj = 0; jj < jb; ++jj) {
+    const int p = s_ipiv[jj];
+    if (p != k + jj) {
+      const T t0 = A[(size_t)(k + jj) * lda + c];
+      A[(size_t)(k + jj) * lda + c] = A[(size_t)p * lda + c];
+      A[(size_t)p * lda + c] = t0;
     }
-    if (c >= k + jb) {
-      T u[JB];
-#pragma unroll
-      for (int r = 0; r < JB; ++r) u[r] = (r < jb) ? A[(size_t)(k + r) * lda + c] : T(0);
-#pragma unroll
-      for (int r = 1; r < JB; ++r) {
-        T s = u[r];
-#pragma unroll
-        for (int q = 0; q < JB; ++q)
-          if (q < r) s -= s_L11[r][q] * u[q];
-        u[r] = s;
-      }
-#pragma unroll
-      for (int r = 1; r < JB; ++r)
-        if (r < jb) A[(size_t)(k + r) * lda + c] = u[r];
+  }
+  if (c >= k + jb) {
+    // forward substitution with the column staged in this thread's LDS slot
+    for (int r = 0; r < jb; ++r) s_u[r][tid] = A[(size_t)(k + r) * lda + c];
+    for (int r = 1; r < jb; ++r) {
+      T acc = s_u[r][tid];
+      for (int q = 0; q < r; ++q) acc -= s_L11[r][q] * s_u[q][tid];
+      s_u[r][tid] = acc;
+      A[(size_t)(k + r) * lda + c] = acc;
     }
   }
 }
@@ -279,6 +286,11 @@ static int fj_lu_factor_t(T* A, int n, int lda, int32_t* ipiv, int32_t* info, hi
     hipLaunchKernelGGL((k_lu_panel<T, R, JB>), dim3(1), dim3(FJ_LU_THREADS), 0, st, A, n, lda, k,
                        ipiv, info);
     FJ_CHECK_LAUNCH();
+    if (n - jb > 0) {
+      hipLaunchKernelGGL((k_lu_swap_trsm<T, JB>), dim3(fj_cdiv(n - jb, 256)), dim3(256), 0, st, A, n,
+                         lda, k, jb, ipiv);
+      FJ_CHECK_LAUNCH();
+    }
     const int rest = n - k - jb;
     if (rest > 0) {
       const dim3 g(fj_cdiv(rest, FJ_GT), fj_cdiv(rest, FJ_GT));
@@ -295,7 +307,7 @@ static int fj_lu_dispatch(T* A, int n, int lda, int32_t* ipiv, int32_t* info, hi
   if (n <= 1024) return fj_lu_factor_t<T, 1, 32>(A, n, lda, ipiv, info, st);
   if (n <= 2048) return fj_lu_factor_t<T, 2, 16>(A, n, lda, ipiv, info, st);
   if (n <= 4096) return fj_lu_factor_t<T, 4, 16>(A, n, lda, ipiv, info, st);
-  if (n <= 8192) return fj_lu_factor_t<T, 8, 8>(A, n, lda, ipiv, info, st);
+  if (n <= 8192) return fj_lu_factor_t<T, 8, 4>(A, n, lda, ipiv, info, st);
   return FJ_ERR_ARG;
 }
 

@@ -1,0 +1,432 @@
+"""UVLM orchestration on the device (aero/uvlm.py:166-1406 of the reference).
+
+Same class name, constructor keywords and entry points as the reference
+(``set_design_variables``, ``solve_static``, ``solve_prescribed_dynamic``,
+``case_solve``, ``base_solve_from_grid``).  Every step runs as a fixed
+sequence of HIP kernels on torch's current stream with all state resident in
+HBM; the host only enqueues work.
+
+Per dynamic step (aero/uvlm.py:912-1103):
+  1. bound grids at n and n-1 and grid velocities from the beam poses (K8);
+  2. collocation points / normals / velocities into packed buffers (K5);
+  3. free-wake rollup: filament table of [bound(n-1), wake(n-1)] and one fused
+     pair-sum + convection launch per surface (K1 + K4), wake circulation shift;
+  4. bound AIC (K2) and right-hand side with the new wake's upwash (K1);
+  5. circulation solve (K3);
+  6. filament-midpoint velocities and steady vertex forces (K1 + K6), strip
+     alpha, circulation rate and unsteady force (K7).
+
+Arithmetic forms (``precision``):
+  * "fp64"  - the plain drop-in: FP64 everywhere, AIC re-assembled and
+              re-factored every step as the reference does;
+  * "mixed" - FP32 far field / FP64 near field for the rollup and force pair
+              sums, FP32 LU with FP64 iterative refinement against the FP64
+              AIC (residual b - A x in FP64).  The AIC and the wake upwash in
+              the right-hand side stay FP64, so the circulation solve is
+              FP64-accurate for the given wake.
+"""
+
+from __future__ import annotations
+
+from collections.abc import Sequence
+from warnings import warn
+
+import numpy as np
+import torch
+
+from . import _kernels as K
+from . import _lib as L
+from .aic import segment_tables
+from .data_structures import FIELDS, AeroCase, GridDiscretisation, field_shapes
+from .linalg import RefinedSolver
+from .utils import HORSESHOE_LENGTH, InducedVelocity, as_segment_kernel, to_device
+
+F64 = torch.float64
+PRECISIONS = {"fp64": L.PREC_FP64, "mixed": L.PREC_MIXED}
+
+
+class UVLM:
+    """aero/uvlm.py:166-339 (constructor checks follow the reference)."""
+
+    def __init__(self, grid_shapes, dof_mapping, variable_wake_disc: bool = False, mirror_point=None,
+                 mirror_normal=None, kernel=None, grid_func=None, free_wake: bool = False,
+                 gamma_dot_relaxation: float = 0.7, include_unsteady_force: bool = True, batch_size=64,
+                 polars=None, polar_circulation_scale: float = 0.0, precision: str = "fp64",
+                 aic_reuse: bool | None = None):
+        if isinstance(dof_mapping, (np.ndarray, torch.Tensor)):
+            dof_list = [dof_mapping]
+        elif isinstance(dof_mapping, Sequence):
+            dof_list = list(dof_mapping)
+        else:
+            raise TypeError("Invalid dof mapping type")
+        self.n_surf = len(grid_shapes)
+        gds = []
+        for g in grid_shapes:
+            if isinstance(g, GridDiscretisation):
+                gds.append(g)
+            elif isinstance(g, Sequence):
+                if len(g) != 3:
+                    raise ValueError("Grid shape tuple must have exactly three elements (m, n, m_star)")
+                gds.append(GridDiscretisation(*[int(v) for v in g]))
+            else:
+                raise TypeError("Grid shape must be either a Sequence of three integers or a GridDiscretisation instance")
+        self.grid_disc = tuple(gds)
+        if len(dof_list) != self.n_surf:
+            raise ValueError(f"Expected {self.n_surf} DOF mapping arrays, got {len(dof_list)}")
+        self._dof_host = []
+        for i, d in enumerate(dof_list):
+            d = np.asarray(d.detach().cpu() if isinstance(d, torch.Tensor) else d)
+            if not np.issubdtype(d.dtype, np.integer):
+                raise TypeError("dof_mapping must be an integer array")
+            if d.shape != (self.grid_disc[i].n + 1,):
+                raise ValueError(f"dof_mapping of surface {i} must have shape ({self.grid_disc[i].n + 1},)")
+            self._dof_host.append(d.astype(np.int32))
+        if grid_func is not None:
+            raise NotImplementedError("control-surface grid functions are not on the device path")
+        if polars is not None and any(p is not None for p in polars):
+            raise NotImplementedError("polar corrections are not on the device path")
+        if not 0.0 <= polar_circulation_scale <= 1.0:
+            raise ValueError(f"polar_circulation_scale must be in [0, 1], got {polar_circulation_scale}.")
+        if (mirror_point is None) != (mirror_normal is None):
+            raise ValueError("Both mirror_point and mirror_normal must be provided to apply mirroring, or both must "
+                             "be None to apply no mirroring.")
+        if precision not in PRECISIONS:
+            raise ValueError(f"precision must be one of {tuple(PRECISIONS)}")
+        self.kernel = as_segment_kernel(kernel)
+        self.kernels_b = [self.kernel] * self.n_surf
+        self.kernels_w = [self.kernel] * self.n_surf
+        self.variable_wake_disc = variable_wake_disc
+        self.free_wake = bool(free_wake)
+        self.gamma_dot_relaxation = float(gamma_dot_relaxation)
+        self.include_unsteady_force = include_unsteady_force
+        self.batch_size = batch_size
+        self.precision = precision
+        self._prec = PRECISIONS[precision]
+        self.aic_reuse = (precision == "mixed") if aic_reuse is None else bool(aic_reuse)
+        if mirror_point is None:
+            self.mirror = None
+            self.mirror_point = self.mirror_normal = None
+        else:
+            mp = np.asarray(mirror_point.detach().cpu() if isinstance(mirror_point, torch.Tensor) else mirror_point,
+                            dtype=np.float64)
+            mn = np.asarray(mirror_normal.detach().cpu() if isinstance(mirror_normal, torch.Tensor) else mirror_normal,
+                            dtype=np.float64)
+            mn = mn / np.linalg.norm(mn)
+            self.mirror = (mp, mn)
+            self.mirror_point, self.mirror_normal = mp, mn
+        self.n_bound_panels = tuple(gd.m * gd.n for gd in self.grid_disc)
+        self.n_wake_panels = tuple(gd.m_star * gd.n for gd in self.grid_disc)
+        self.n_b = sum(self.n_bound_panels)
+        self._b_off = np.concatenate(([0], np.cumsum(self.n_bound_panels))).astype(int)
+        nseg = [K.n_segments(gd.m, gd.n) for gd in self.grid_disc]
+        self._s_off = np.concatenate(([0], np.cumsum(nseg))).astype(int)
+        self.flowfield = None
+        self._dt = None
+        self.zeta_b0 = None
+        self.hg_ref = None
+        self._solver = RefinedSolver(reuse=self.aic_reuse) if precision == "mixed" else None
+        self._aic = None
+
+    # ------------------------------------------------------------------ setup
+    @property
+    def dt(self):
+        if self._dt is None:
+            raise ValueError("Time step length dt has not been set.")
+        return self._dt
+
+    def set_design_variables(self, dt, flowfield, zeta_b0, hg0, delta_w=None, reference_cs_angles=None):
+        """aero/uvlm.py:387-472."""
+        if isinstance(zeta_b0, (np.ndarray, torch.Tensor)):
+            z0 = [zeta_b0]
+        elif isinstance(zeta_b0, Sequence):
+            z0 = list(zeta_b0)
+        else:
+            raise TypeError("Invalid zeta_b0 type")
+        if len(z0) != self.n_surf:
+            raise ValueError(f"Expected {self.n_surf} aerodynamic grid coordinate arrays, got {len(z0)}")
+        self.zeta_b0 = []
+        for i, z in enumerate(z0):
+            z = to_device(z)
+            gd = self.grid_disc[i]
+            if tuple(z.shape) != (gd.m + 1, gd.n + 1, 3):
+                raise ValueError(f"zeta_b0 must have shape {(gd.m + 1, gd.n + 1, 3)}, got {tuple(z.shape)}")
+            self.zeta_b0.append(z)
+        if delta_w is not None and not (isinstance(delta_w, Sequence) and all(d is None for d in delta_w)):
+            raise NotImplementedError("variable wake discretisation (delta_w) is not on the device path")
+        if reference_cs_angles:
+            raise NotImplementedError("control surfaces are not on the device path")
+        hg0 = to_device(hg0)
+        if hg0.dim() != 3 or tuple(hg0.shape[1:]) != (4, 4):
+            raise ValueError("hg0 must have shape (n_nodes, 4, 4)")
+        self.hg_ref = hg0
+        self._dof = [torch.as_tensor(d, device=hg0.device) for d in self._dof_host]
+        if not isinstance(dt, (float, int, np.floating)):
+            if isinstance(dt, (np.ndarray, torch.Tensor)) and np.ndim(dt) == 0:
+                dt = float(dt)
+            else:
+                raise TypeError("dt must be either a float or an Array scalar")
+        self._dt = float(dt)
+        self.flowfield = flowfield
+        self.zeta_b_ref = self.hg_to_zeta_b(hg0)
+        self.zeta_w_ref = self.initialise_wake(self.zeta_b_ref)
+
+    def hg_to_zeta_b(self, hg_n, out=None):
+        """aero/uvlm.py:537-560."""
+        hg_n = to_device(hg_n)
+        res = []
+        for i in range(self.n_surf):
+            z, _ = K.grid_from_hg(hg_n, None, self._dof[i], self.zeta_b0[i], zeta=None if out is None else out[i])
+            res.append(z)
+        return res
+
+    def hg_dot_to_zeta_b_dot(self, hg_n, hg_dot_n, out=None):
+        """aero/uvlm.py:562-621 (no control surfaces): hg_dot @ [zeta0; 1]."""
+        hg_dot_n = to_device(hg_dot_n)
+        res = []
+        for i in range(self.n_surf):
+            z, _ = K.grid_from_hg(hg_dot_n, None, self._dof[i], self.zeta_b0[i], zeta=None if out is None else out[i])
+            res.append(z)
+        return res
+
+    def _wake_s(self, i):
+        gd = self.grid_disc[i]
+        dw = np.ones(gd.m_star) * self.dt * self.flowfield.u_inf_mag
+        return to_device(np.concatenate((np.zeros(1), np.cumsum(dw))))
+
+    def initialise_wake(self, zeta_b=None, out=None):
+        """aero/uvlm.py:662-689 (uniform discretisation)."""
+        zeta_b = self.zeta_b_ref if zeta_b is None else zeta_b
+        return [K.wake_initialise(to_device(zeta_b[i]), self._wake_s(i), self.flowfield.u_inf_dir,
+                                  zeta_w=None if out is None else out[i]) for i in range(self.n_surf)]
+
+    def _horseshoe_wake(self, zeta_b, i):
+        """aero/uvlm.py:642-660."""
+        gd = self.grid_disc[i]
+        if gd.m_star == 0:
+            warn("Horseshoe wake requested but m_star == 0, skipping.")
+            return to_device(zeta_b)[-1:].contiguous()
+        s = to_device(np.array([0.0, HORSESHOE_LENGTH]))
+        return K.wake_initialise(to_device(zeta_b), s, self.flowfield.u_inf_dir)
+
+    # ---------------------------------------------------------------- kernels
+    def _packed(self, per_surface_shape_fn):
+        return [per_surface_shape_fn(gd) for gd in self.grid_disc]
+
+    def _stencils(self, zeta_b_n, zeta_b_dot_n, c_out=None, nc_out=None):
+        """Collocation points and normals of all surfaces into packed (N_b, 3)
+        buffers (views per surface), plus collocation velocities."""
+        dev = zeta_b_n[0].device
+        c_all = torch.empty((self.n_b, 3), dtype=F64, device=dev)
+        nc_all = torch.empty((self.n_b, 3), dtype=F64, device=dev)
+        cd_all = torch.empty((self.n_b, 3), dtype=F64, device=dev) if zeta_b_dot_n is not None else None
+        for i, gd in enumerate(self.grid_disc):
+            a, b = self._b_off[i], self._b_off[i + 1]
+            zd = zeta_b_dot_n[i] if zeta_b_dot_n is not None else None
+            L.check(L.lib().fj_colloc_normals(L.ptr(zeta_b_n[i]), L.ptr(zd) if zd is not None else None, gd.m, gd.n,
+                                              L.ptr(c_all[a:b]), L.ptr(nc_all[a:b]),
+                                              L.ptr(cd_all[a:b]) if cd_all is not None else None, L.stream()),
+                    "fj_colloc_normals")
+        return c_all, nc_all, cd_all
+
+    def _assemble_aic(self, c_all, nc_all, zeta_b_n, zeta_w_n=None):
+        dev = c_all.device
+        if self._aic is None or self._aic.shape[0] != self.n_b:
+            self._aic = torch.empty((self.n_b, self.n_b), dtype=F64, device=dev)
+        A = self._aic
+        kern = self.kernel
+        for i in range(self.n_surf):  # target surface
+            a, b = self._b_off[i], self._b_off[i + 1]
+            blk = A[a:b]
+            for j in range(self.n_surf):  # source surface
+                K.aic_assemble(c_all[a:b], nc_all[a:b], zeta_b_n[j], kern.kind, kern.eps, kern.rcut, blk,
+                               int(self._b_off[j]), zeta_w=None if zeta_w_n is None else zeta_w_n[j],
+                               mirror=self.mirror)
+        return A
+
+    def _solve(self, A, rhs):
+        if self._solver is None:
+            LU, piv, info = K.lu_factor(A, overwrite=True)
+            return K.lu_solve(LU, piv, rhs)
+        # convergence of the reused FP32 factor is checked on the host (one
+        # sync) after every refactorisation and every 16th solve after that
+        self._n_solves = getattr(self, "_n_solves", 0) + 1
+        check = self._solver.factorisations == 0 or self._n_solves % 16 == 1
+        return self._solver.solve(A, rhs, check=check)
+
+    def _tables(self, zetas, gammas):
+        return segment_tables(zetas, gammas, [self.kernel] * len(zetas), self.mirror)
+
+    # ----------------------------------------------------------------- solve
+    def base_solve_from_grid(self, q_nm1, t_n, zeta_b_n, zeta_b_nm1, zeta_b_dot_n, static, horseshoe, out=None):
+        """aero/uvlm.py:824-1117.  Returns (c, nc, gamma_b, gamma_w, gamma_b_dot,
+        zeta_b, zeta_w, zeta_b_dot, f_steady, f_unsteady, alpha).  ``out``
+        (optional) is a snapshot dict of preallocated views to write into."""
+        if not (0.0 < self.gamma_dot_relaxation <= 1.0):
+            warn("Gamma_dot relaxation factor not in (0, 1]")
+        if not static and horseshoe:
+            warn("Horseshoe wake not compatible with non-static solve. Overriding horseshoe to False.")
+            horseshoe = False
+        if not static and q_nm1 is None:
+            raise ValueError("q_nm1 needs to be specified for dynamic solve")
+        ff = self.flowfield
+        t_n = float(t_n)
+        dev = zeta_b_n[0].device
+        o = out if out is not None else {}
+        c_all, nc_all, cd_all = self._stencils(zeta_b_n, zeta_b_dot_n)
+        c_n = [c_all[self._b_off[i]:self._b_off[i + 1]].reshape(gd.m, gd.n, 3) for i, gd in enumerate(self.grid_disc)]
+        nc_n = [nc_all[self._b_off[i]:self._b_off[i + 1]].reshape(gd.m, gd.n, 3)
+                for i, gd in enumerate(self.grid_disc)]
+
+        # ---- wake
+        if static:
+            if horseshoe:
+                zeta_w_n = [self._horseshoe_wake(zeta_b_n[i], i) for i in range(self.n_surf)]
+            else:
+                zeta_w_n = self.initialise_wake(zeta_b_n, out=o.get("zeta_w"))
+            gamma_w_n = None
+        else:
+            zeta_w_n = o.get("zeta_w") or [torch.empty_like(z) for z in q_nm1["zeta_w"]]
+            gamma_w_n = o.get("gamma_w") or [torch.empty_like(g) for g in q_nm1["gamma_w"]]
+            tables_nm1 = None
+            if self.free_wake:
+                tables_nm1 = self._tables(list(zeta_b_nm1) + list(q_nm1["zeta_w"]),
+                                          list(q_nm1["gamma_b"]) + list(q_nm1["gamma_w"]))
+            for i, gd in enumerate(self.grid_disc):
+                K.wake_shift(q_nm1["gamma_b"][i], q_nm1["gamma_w"][i], zeta_b_n[i], gd.m_star, gamma_w_n[i],
+                             zeta_w_n[i])
+                if gd.m_star == 0:
+                    continue
+                if self.free_wake:
+                    zw = q_nm1["zeta_w"][i]
+                    base = zw[:-1].reshape(-1, 3)
+                    dst = zeta_w_n[i][1:].reshape(-1, 3)
+                    (kind, eps, rcut), seg = next(iter(tables_nm1.items()))
+                    if len(tables_nm1) != 1:
+                        raise NotImplementedError("one kernel kind expected")
+                    K.pair_sum(seg, base, kind, eps, rcut, self._prec, L.OUT_CONVECT, out=dst, flow=ff, time=t_n,
+                               dt=self.dt)
+                else:
+                    K.wake_convect(q_nm1["zeta_w"][i], None, ff, t_n, self.dt, zeta_w_n[i])
+
+        # ---- system
+        A = self._assemble_aic(c_all, nc_all, zeta_b_n, zeta_w_n if static else None)
+        v_w = None
+        if not static:
+            tables_w = self._tables(zeta_w_n, gamma_w_n)
+            if tables_w:
+                v_w = None
+                for (kind, eps, rcut), seg in tables_w.items():
+                    v_w = K.pair_sum(seg, c_all, kind, eps, rcut, L.PREC_FP64,
+                                     L.OUT_STORE if v_w is None else L.OUT_ADD, out=v_w)
+        rhs = K.rhs(c_all, nc_all, cd_all if not static else None, v_w, ff, t_n)
+        gvec = self._solve(A, rhs)
+        gamma_b_n = []
+        for i, gd in enumerate(self.grid_disc):
+            g = gvec[self._b_off[i]:self._b_off[i + 1]].reshape(gd.m, gd.n)
+            if out is not None:
+                out["gamma_b"][i].copy_(g)
+                g = out["gamma_b"][i]
+            gamma_b_n.append(g)
+        if static:
+            gamma_w_n = []
+            for i, gd in enumerate(self.grid_disc):
+                rows = 1 if (horseshoe and gd.m_star != 0) else gd.m_star
+                gw = K.static_wake_gamma(gamma_b_n[i], rows)
+                gamma_w_n.append(gw)
+
+        # ---- forces
+        mids = torch.empty((int(self._s_off[-1]), 3), dtype=F64, device=dev)
+        for i, gd in enumerate(self.grid_disc):
+            L.check(L.lib().fj_filament_midpoints(L.ptr(zeta_b_n[i]), gd.m, gd.n,
+                                                  L.ptr(mids[self._s_off[i]:self._s_off[i + 1]]), L.stream()),
+                    "fj_filament_midpoints")
+        tables_n = self._tables(list(zeta_b_n) + list(zeta_w_n), list(gamma_b_n) + list(gamma_w_n))
+        v_mid = InducedVelocity(ff, t_n, tables_n, self.kernel, self._prec).induced(mids)
+        f_steady, alpha = [], []
+        for i, gd in enumerate(self.grid_disc):
+            gw0 = gamma_w_n[i][0].contiguous() if gamma_w_n[i].shape[0] > 0 else None
+            zd = zeta_b_dot_n[i] if zeta_b_dot_n is not None else None
+            fs = K.steady_forces(zeta_b_n[i], zd, gamma_b_n[i], gw0, v_mid[self._s_off[i]:self._s_off[i + 1]], ff,
+                                 t_n, ff.rho, out=None if out is None else out["f_steady"][i])
+            f_steady.append(fs)
+            alpha.append(K.strip_alpha(zeta_b_n[i], fs, ff, t_n, ff.rho,
+                                       out=None if out is None else out["alpha"][i]))
+        if static:
+            gamma_b_dot_n, f_unsteady = None, None
+        else:
+            gamma_b_dot_n, f_unsteady = [], []
+            for i in range(self.n_surf):
+                gd_, fu = K.gamma_dot_unsteady(gamma_b_n[i], q_nm1["gamma_b"][i], q_nm1["gamma_b_dot"][i], nc_n[i],
+                                               self.dt, self.gamma_dot_relaxation, ff.rho,
+                                               gamma_dot=None if out is None else out["gamma_b_dot"][i],
+                                               f_uns=None if out is None else out["f_unsteady"][i])
+                gamma_b_dot_n.append(gd_)
+                f_unsteady.append(fu)
+        return (c_n, nc_n, gamma_b_n, gamma_w_n, gamma_b_dot_n, zeta_b_n, zeta_w_n, zeta_b_dot_n, f_steady,
+                f_unsteady, alpha)
+
+    def _store(self, case, i_ts, res, static, horseshoe=False):
+        names = ("c", "nc", "gamma_b", "gamma_w", "gamma_b_dot", "zeta_b", "zeta_w", "zeta_b_dot", "f_steady",
+                 "f_unsteady", "alpha")
+        for name, vals in zip(names, res):
+            if vals is None:
+                continue
+            if static and name in ("gamma_b_dot", "zeta_b_dot", "f_unsteady"):
+                continue
+            for dst, src in zip(getattr(case, name), vals):
+                if dst[i_ts].data_ptr() != src.data_ptr():
+                    if name == "zeta_w" and horseshoe:
+                        continue
+                    dst[i_ts].copy_(src.expand_as(dst[i_ts]) if name == "gamma_w" else src)
+
+    def case_solve(self, case, i_ts, hg_n, hg_nm1, hg_dot_n, static, horseshoe, cs_ang_n=None, cs_ang_nm1=None,
+                   cs_vel_n=None):
+        """aero/uvlm.py:1119-1215 (in place in ``case``)."""
+        if cs_ang_n or cs_ang_nm1 or cs_vel_n:
+            raise NotImplementedError("control surfaces are not on the device path")
+        q_nm1 = None
+        if not static:
+            q_nm1 = {k: [a[i_ts - 1] for a in getattr(case, k)] for k in ("gamma_b", "gamma_w", "gamma_b_dot", "zeta_w")}
+            case.t[i_ts] = case.t[i_ts - 1] + self.dt if i_ts else 0.0
+        snap = case.snapshot(i_ts)
+        zeta_b_n = self.hg_to_zeta_b(hg_n if hg_n is not None else self.hg_ref, out=snap["zeta_b"])
+        zeta_b_dot_n = zeta_b_nm1 = None
+        if hg_dot_n is not None:
+            zeta_b_dot_n = self.hg_dot_to_zeta_b_dot(hg_n, hg_dot_n, out=snap["zeta_b_dot"])
+            zeta_b_nm1 = self.hg_to_zeta_b(hg_nm1)
+        out = None if static else snap
+        res = self.base_solve_from_grid(q_nm1, case.t[i_ts], zeta_b_n, zeta_b_nm1, zeta_b_dot_n, static, horseshoe,
+                                        out=out)
+        self._store(case, i_ts, res, static, horseshoe)
+        if horseshoe:
+            self.initialise_wake(zeta_b_n, out=snap["zeta_w"])
+        return case
+
+    def solve_static(self, hg=None, t=0.0, horseshoe=False, cs_ang=None):
+        """aero/uvlm.py:1286-1333."""
+        if cs_ang:
+            raise NotImplementedError("control surfaces are not on the device path")
+        if horseshoe and any(gd.m_star == 0 for gd in self.grid_disc):
+            warn("Horseshoe wake requested but m_star == 0 for at least one surface, skipping horseshoe wake.")
+        case = AeroCase.zeros(self.grid_disc, 1, self.hg_ref.device)
+        case.t[0] = float(t)
+        return self.case_solve(case, 0, hg, None, None, True, horseshoe)
+
+    def solve_prescribed_dynamic(self, init_case, hg_t, hg_dot_t, cs_ang_t=None, cs_vel_t=None, callback=None):
+        """aero/uvlm.py:1335-1406: one device step per time step."""
+        if cs_ang_t or cs_vel_t:
+            raise NotImplementedError("control surfaces are not on the device path")
+        hg_t = to_device(hg_t)
+        hg_dot_t = to_device(hg_dot_t)
+        if hg_t.dim() != 4 or tuple(hg_t.shape[2:]) != (4, 4):
+            raise ValueError("hg_t must have shape (n_tstep, n_nodes, 4, 4)")
+        if hg_t.shape != hg_dot_t.shape:
+            raise ValueError(f"hg_dot_n must have the same shape as hg_n, got {tuple(hg_dot_t.shape)} vs "
+                             f"{tuple(hg_t.shape)}")
+        n_tstep = hg_t.shape[0]
+        case = init_case.to_dynamic(0, n_tstep)
+        for i in range(1, n_tstep):
+            self.case_solve(case, i, hg_t[i], hg_t[i - 1], hg_dot_t[i], False, False)
+            if callback is not None:
+                callback(i, case)
+        return case

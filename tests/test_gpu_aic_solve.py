@@ -1,0 +1,123 @@
+"""GPU parity of AIC assembly (K2) and the circulation solve (K3)."""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import uvlm_oracle as O
+from tests.conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _t(x, dev):
+    return torch.as_tensor(np.asarray(x, dtype=np.float64), device=dev)
+
+
+def _wing(m, n, m_star, alpha_deg=5.0, span=10.0):
+    zeta0 = O.make_rectangular_grid(m, n, 1.0, 0.25)
+    hg = O.rigid_wing_hg(n, span, np.deg2rad(alpha_deg))
+    zb = O.vect_product(hg, zeta0)
+    s = np.concatenate(([0.0], np.cumsum(np.ones(m_star) * (1.0 / m))))
+    zw = zb[-1][None] + np.outer(s, [1.0, 0.0, 0.0])[:, None, :]
+    return zb, zw
+
+
+@pytest.mark.parametrize("kernel", ["epsilon", "cutoff"])
+def test_aic_static_with_wake_lumping(dev, kernel):
+    from flapjax_amd.aic import compute_aic_solve
+    from flapjax_amd.utils import biot_savart_cutoff, biot_savart_epsilon
+
+    kobj = {"epsilon": biot_savart_epsilon, "cutoff": biot_savart_cutoff}[kernel]
+    zb, zw = _wing(8, 32, 20)
+    c, nc = O.compute_surf_c(zb), O.compute_surf_nc(zb)
+    ref = O.compute_aic_solve([c], [nc], [zb], [zw], kernel, kernel)
+    A = compute_aic_solve([_t(c, dev)], [_t(nc, dev)], [_t(zb, dev)], [_t(zw, dev)], [kobj], [kobj]).cpu().numpy()
+    assert A.shape == (256, 256)
+    assert rel_err(A, ref) < 1e-12
+    # dynamic (bound only), with a mirror plane
+    mp, mn = np.zeros(3), np.array([0.0, 1.0, 0.0])
+    ref_b = O.compute_aic_solve([c], [nc], [zb], None, kernel, kernel, mp, mn)
+    A_b = compute_aic_solve([_t(c, dev)], [_t(nc, dev)], [_t(zb, dev)], None, [kobj], None, mirror_point=mp,
+                            mirror_normal=mn).cpu().numpy()
+    assert rel_err(A_b, ref_b) < 1e-12
+
+
+def test_aic_times_gamma_equals_normal_velocity(dev):
+    """A @ gamma == v_ind(c) . n (closed relation between K2 and K1)."""
+    from flapjax_amd.aic import compute_aic_solve, compute_v_ind
+
+    zb, _ = _wing(6, 20, 0)
+    c, nc = O.compute_surf_c(zb), O.compute_surf_nc(zb)
+    A = compute_aic_solve([_t(c, dev)], [_t(nc, dev)], [_t(zb, dev)], None, None, None)
+    g = np.random.default_rng(1).normal(size=(6, 20))
+    v = compute_v_ind(_t(c, dev), [_t(zb, dev)], [_t(g, dev)], None)
+    lhs = (A @ _t(g.ravel(), dev)).cpu().numpy()
+    rhs = (v * _t(nc, dev)).sum(-1).reshape(-1).cpu().numpy()
+    assert rel_err(lhs, rhs) < 1e-12
+
+
+@pytest.mark.parametrize("n", [1, 7, 33, 64, 100, 257, 1024, 1500, 2100])
+def test_lu_solve_random(dev, n):
+    from flapjax_amd import _kernels as K
+
+    rng = np.random.default_rng(n)
+    A = rng.normal(size=(n, n)) + np.eye(n) * 0.5
+    b = rng.normal(size=n)
+    LU, piv, info = K.lu_factor(_t(A, dev))
+    assert int(info.item()) == 0
+    x = K.lu_solve(LU, piv, _t(b, dev)).cpu().numpy()
+    ref = np.linalg.solve(A, b)
+    # backward error of a stable LU, and agreement with LAPACK
+    res = np.linalg.norm(A @ x - b, np.inf) / (np.linalg.norm(A, np.inf) * np.linalg.norm(x, np.inf))
+    assert res < 50 * n * np.finfo(float).eps
+    assert rel_err(x, ref) < 1e-9
+    # the transposed solve (adjoint)
+    xt = K.lu_solve(LU, piv, _t(b, dev), trans=True).cpu().numpy()
+    assert rel_err(xt, np.linalg.solve(A.T, b)) < 1e-9
+    # the permutation and factors reproduce P A = L U
+    if n <= 257:
+        LUh = LU.cpu().numpy()
+        perm = piv.cpu().numpy()[n:]
+        Lm = np.tril(LUh, -1) + np.eye(n)
+        Um = np.triu(LUh)
+        assert rel_err(Lm @ Um, A[perm]) < 1e-13
+
+
+def test_lu_pivots_match_lapack_order(dev):
+    from flapjax_amd import _kernels as K
+    import scipy.linalg
+
+    rng = np.random.default_rng(7)
+    A = rng.normal(size=(96, 96))
+    LU, piv, _ = K.lu_factor(_t(A, dev))
+    lu_ref, piv_ref = scipy.linalg.lu_factor(A)
+    assert np.array_equal(piv.cpu().numpy()[:96], piv_ref)
+    assert rel_err(LU.cpu().numpy(), lu_ref) < 1e-12
+
+
+def test_singular_reports_info(dev):
+    from flapjax_amd import _kernels as K
+
+    A = np.ones((40, 40))
+    _, _, info = K.lu_factor(_t(A, dev))
+    assert int(info.item()) == 2
+
+
+@pytest.mark.parametrize("n", [256, 1024])
+def test_refined_solver_on_aic(dev, n):
+    """FP32 LU + FP64 refinement reaches the FP64 solution on a real AIC."""
+    from flapjax_amd.aic import compute_aic_solve
+    from flapjax_amd.linalg import RefinedSolver, solve
+
+    m = 8 if n == 256 else 16
+    zb, zw = _wing(m, n // m, 20)
+    c, nc = O.compute_surf_c(zb), O.compute_surf_nc(zb)
+    A = compute_aic_solve([_t(c, dev)], [_t(nc, dev)], [_t(zb, dev)], [_t(zw, dev)], None, None)
+    b = _t(-(np.array([1.0, 0.0, 0.0]) * nc).sum(-1).ravel(), dev)
+    x64 = solve(A, b).cpu().numpy()
+    rs = RefinedSolver()
+    xr = rs.solve(A, b).cpu().numpy()
+    print(f"refined solve n={n}: sweeps={rs.last_iters}, rel err vs FP64 LU={rel_err(xr, x64):.2e}")
+    assert rel_err(xr, x64) < 1e-12
+    assert rs.last_iters >= 1
