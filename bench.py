@@ -1,0 +1,248 @@
+"""Benchmark of the UVLM hot path on one MI355X.
+
+Workload: unsteady rigid-wing UVLM with free wake -- 16 x 64 bound panels
+(chord 1, span 10), 200 wake rows, pitch-plunge motion (mean alpha 5 deg,
+pitch 3 deg, plunge 0.05 chord, reduced frequency 0.2), dt = chord / (16 U),
+full wake-on-wake rollup every step.  A "step" is one UVLM time step on the
+device (rollup + wake convection, AIC, circulation solve, forces).
+
+Metric: Biot-Savart pair interactions per second over the whole step
+(4.52e8 de-duplicated (target, filament) pairs per step), for the plain FP64
+drop-in (``value``) and the mixed form (FP32 far field + FP64 near field,
+FP32 LU + FP64 refinement).  Also reported: per-step p50/p99 latency, cases/s
+for a whole 200-step case, the roofline of the dominant kernel (the rollup
+pair-sum launch, 13,000 x 27,992 pairs, 57 FLOP per pair, FP64 VALU bound),
+and the CPU baseline (the oracle's C restatement of the same pair sum, OpenMP
+over all host threads, on a bounded sample).
+
+    python bench.py [--gpus 1] [--steps 40] [--warmup 5]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+M, N, M_STAR, SPAN = 16, 64, 200, 10.0
+N_TSTEP = 200
+ALPHA0, PITCH, PLUNGE, K_RED = 5.0, 3.0, 0.05, 0.2
+F_PAIR = 57  # algorithmic FLOP per pair interaction
+PEAK_FP64 = 78.6  # TFLOP/s, MI355X FP64 vector (datasheet)
+
+
+def n_seg(m, n):
+    return m * (n + 1) + (m + 1) * n
+
+
+def pairs_per_step(m=M, n=N, m_star=M_STAR):
+    s_b, s_w = n_seg(m, n), n_seg(m_star, n)
+    rollup = m_star * (n + 1) * (s_b + s_w)
+    forces = s_b * (s_b + s_w)
+    upwash = m * n * s_w
+    aic = m * n * s_b
+    return rollup + forces + upwash + aic, rollup
+
+
+def pitch_plunge(n, span, n_tstep, dt):
+    """hg (n_tstep, n+1, 4, 4) and hg_dot (skew rate convention of the
+    reference's prescribed-motion tests) for the pitch-plunge motion."""
+    omega = 2.0 * K_RED  # k = omega c / (2 U), c = U = 1
+    t = np.arange(n_tstep) * dt
+    th = np.deg2rad(ALPHA0) + np.deg2rad(PITCH) * np.sin(omega * t)
+    thd = np.deg2rad(PITCH) * omega * np.cos(omega * t)
+    h = PLUNGE * np.sin(omega * t)
+    hd = PLUNGE * omega * np.cos(omega * t)
+    hg = np.zeros((n_tstep, n + 1, 4, 4))
+    c, s = np.cos(th), np.sin(th)
+    hg[:, :, 0, 0] = c[:, None]
+    hg[:, :, 0, 2] = s[:, None]
+    hg[:, :, 1, 1] = 1.0
+    hg[:, :, 2, 0] = -s[:, None]
+    hg[:, :, 2, 2] = c[:, None]
+    hg[:, :, 1, 3] = np.linspace(0.0, span, n + 1)[None, :]
+    hg[:, :, 2, 3] = h[:, None]
+    hg[:, :, 3, 3] = 1.0
+    hgd = np.zeros_like(hg)
+    hgd[:, :, 0, 2] = thd[:, None]
+    hgd[:, :, 2, 0] = -thd[:, None]
+    hgd[:, :, 2, 3] = hd[:, None]
+    return hg, hgd
+
+
+def make_case(precision):
+    import flapjax_amd as fj
+    from flapjax_amd.uvlm import UVLM
+    from flapjax_amd.utils import make_rectangular_grid
+
+    dt = 1.0 / M
+    hg, hgd = pitch_plunge(N, SPAN, N_TSTEP, dt)
+    uv = UVLM([(M, N, M_STAR)], np.arange(N + 1), free_wake=True, precision=precision)
+    uv.set_design_variables(dt, fj.Constant(np.array([1.0, 0.0, 0.0]), 1.225, True),
+                            make_rectangular_grid(M, N, 1.0, 0.25), hg[0])
+    return uv, hg, hgd
+
+
+def time_form(precision, steps, warmup):
+    import torch
+
+    uv, hg_np, hgd_np = make_case(precision)
+    from flapjax_amd.utils import to_device
+
+    hg, hgd = to_device(hg_np), to_device(hgd_np)
+    static = uv.solve_static(hg=hg[0])
+    case = static.to_dynamic(0, N_TSTEP)
+    i = 1
+
+    def step():
+        nonlocal case, i
+        if i >= N_TSTEP:
+            case = static.to_dynamic(0, N_TSTEP)
+            i = 1
+        uv.case_solve(case, i, hg[i], hg[i - 1], hgd[i], False, False)
+        i += 1
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+    t0 = time.perf_counter()
+    ev[0].record()
+    for k in range(steps):
+        step()
+        ev[k + 1].record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    lat = np.array([ev[k].elapsed_time(ev[k + 1]) for k in range(steps)])
+    # whole case (static + 199 dynamic steps) end to end
+    torch.cuda.synchronize()
+    tc = time.perf_counter()
+    s2 = uv.solve_static(hg=hg[0])
+    c2 = s2.to_dynamic(0, N_TSTEP)
+    for j in range(1, N_TSTEP):
+        uv.case_solve(c2, j, hg[j], hg[j - 1], hgd[j], False, False)
+    torch.cuda.synchronize()
+    case_s = time.perf_counter() - tc
+    return {"wall_s": wall, "ms_per_step": 1e3 * wall / steps, "p50_ms": float(np.percentile(lat, 50)),
+            "p99_ms": float(np.percentile(lat, 99)), "case_s": case_s}
+
+
+def rollup_roofline(reps=20):
+    """Time the dominant launch alone (rollup pair sum, 13,000 x 27,992) with
+    HIP events on the stream it runs on."""
+    import torch
+
+    from flapjax_amd import _kernels as K
+    from flapjax_amd import _lib as L
+    from flapjax_amd.utils import to_device
+
+    rng = np.random.default_rng(0)
+    uv, hg_np, _ = make_case("fp64")
+    zb = uv.zeta_b_ref[0]
+    zw = uv.zeta_w_ref[0].clone()
+    zw += to_device(rng.normal(size=zw.shape) * 0.01 * np.array([0.0, 0.0, 1.0]))
+    gb = to_device(rng.uniform(-0.1, 0.1, (M, N)))
+    gw = to_device(rng.uniform(-0.1, 0.1, (M_STAR, N)))
+    seg = torch.cat([K.segments_from_grid(zb, gb, L.KERNEL_EPSILON, 1e-7),
+                     K.segments_from_grid(zw, gw, L.KERNEL_EPSILON, 1e-7)])
+    tgt = zw[:-1].reshape(-1, 3).contiguous()
+    out = torch.empty_like(tgt)
+    res = {}
+    for name, prec in (("fp64", L.PREC_FP64), ("mixed", L.PREC_MIXED)):
+        for _ in range(3):
+            K.pair_sum(seg, tgt, L.KERNEL_EPSILON, 1e-7, 1e-4, prec, out=out)
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record()
+        for _ in range(reps):
+            K.pair_sum(seg, tgt, L.KERNEL_EPSILON, 1e-7, 1e-4, prec, out=out)
+        e.record()
+        torch.cuda.synchronize()
+        ms = s.elapsed_time(e) / reps
+        pairs = tgt.shape[0] * seg.shape[0]
+        res[name] = {"ms": ms, "pairs": pairs, "gpairs_s": pairs / ms / 1e6,
+                     "tflops": pairs * F_PAIR / (ms * 1e-3) / 1e12}
+    return res, (tgt.cpu().numpy(), seg.cpu().numpy())
+
+
+def cpu_baseline(sample, min_seconds=10.0):
+    """The oracle's C restatement of the FP64 pair sum, all host threads."""
+    from oracle import c_oracle
+
+    tgt, seg = sample
+    y0, y1, g = seg[:, 0:3], seg[:, 3:6], seg[:, 10]
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    t0 = time.perf_counter()
+    reps = 0
+    while True:
+        c_oracle.pair_sum_eps(tgt, y0, y1, g)
+        reps += 1
+        el = time.perf_counter() - t0
+        if el >= min_seconds or reps >= 50:
+            break
+    pairs = reps * tgt.shape[0] * y0.shape[0]
+    return {"value": pairs / el / 1e9, "unit": "Gpairs/s", "cores": threads, "kind": "port",
+            "sample": f"{reps} x rollup pair sum of one step ({tgt.shape[0]} targets x {y0.shape[0]} filaments), "
+                      f"FP64 C (gcc -O3 -fopenmp), {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the UVLM path here runs on a single MI355X "
+              "(multi-GPU sharding is a design note only); run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    if not torch.cuda.is_available():
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    pairs, rollup_pairs = pairs_per_step()
+    f64 = time_form("fp64", args.steps, args.warmup)
+    mix = time_form("mixed", args.steps, args.warmup)
+    roof, sample = rollup_roofline()
+    cpu = None if args.no_cpu else cpu_baseline(sample)
+    value = pairs * args.steps / f64["wall_s"] / 1e9
+    value_mixed = pairs * args.steps / mix["wall_s"] / 1e9
+    line = {
+        "metric": "Biot-Savart Gpairs/s (UVLM free-wake step, plain FP64 drop-in)",
+        "value": round(value, 3),
+        "unit": "Gpairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(f64["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded pitch-plunge rigid wing)",
+        "config": {"workload": "unsteady rigid-wing UVLM, free wake: 16x64 bound panels, 200 wake rows, "
+                               "pitch-plunge k=0.2, dt=c/(16U)", "pairs_per_step": pairs,
+                   "parallelism": "single GPU"},
+        "latency_ms": {"p50": round(f64["p50_ms"], 4), "p99": round(f64["p99_ms"], 4)},
+        "cases_per_s": round(1.0 / f64["case_s"], 3),
+        "mixed": {"value": round(value_mixed, 3), "unit": "Gpairs/s", "dtype": "f32+f64",
+                  "ms_per_step": round(mix["ms_per_step"], 4),
+                  "latency_ms": {"p50": round(mix["p50_ms"], 4), "p99": round(mix["p99_ms"], 4)},
+                  "cases_per_s": round(1.0 / mix["case_s"], 3), "speedup_vs_fp64": round(value_mixed / value, 3)},
+        "roofline": {"bound": "valu", "achieved": round(roof["fp64"]["tflops"], 3), "peak": PEAK_FP64,
+                     "unit": "TFLOP/s", "frac": round(roof["fp64"]["tflops"] / PEAK_FP64, 4), "traffic": None,
+                     "kernel": "k_pair_partial_f64 + k_pair_reduce (rollup pair sum)",
+                     "launch_ms": round(roof["fp64"]["ms"], 4), "pairs_per_launch": roof["fp64"]["pairs"],
+                     "mixed_launch_ms": round(roof["mixed"]["ms"], 4)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

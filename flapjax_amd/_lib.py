@@ -86,6 +86,8 @@ _SIGS = {
                                c_void_p, c_void_p, c_void_p], c_int),
     "fj_strip_alpha": ([c_void_p, c_void_p, c_int, c_int, POINTER(FjFlowfield), c_double, c_double, c_void_p,
                         c_void_p], c_int),
+    "fj_pair_sum_backward": ([c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, c_double, c_double, c_void_p,
+                              c_int, c_void_p, c_void_p], c_int),
     "fj_flow_eval": ([POINTER(FjFlowfield), c_void_p, c_int, c_double, c_void_p, c_void_p], c_int),
 }
 

@@ -1,0 +1,193 @@
+"""Differentiation form: hand-written adjoint kernels behind torch.autograd.
+
+* ``PairSum``     - v = sum_s g_s K(x; y0_s, y1_s); backward runs the
+                    adjoint Biot-Savart kernels (fj_pair_sum_backward) for
+                    dL/dx, dL/dy0, dL/dy1, dL/dg.
+* ``LinearSolve`` - x = A^{-1} b by the device LU; backward solves the
+                    transposed system with the same factors (fj_lu_solve,
+                    trans=1): dL/db = A^{-T} xbar, dL/dA = -(A^{-T} xbar) x^T.
+* ``AICSolve``    - Gamma = A(geometry)^{-1} b without ever differentiating
+                    the assembled matrix: in reverse mode the geometry
+                    gradient of -lambda^T A Gamma is the adjoint of the pair
+                    sum v_ind(c; zeta, Gamma) . n weighted by lambda, so it
+                    reuses PairSum's adjoint kernels.
+
+These replace what the reference obtains from jax.vjp of its residuals
+(aero/uvlm.py:1529-2047, coupled/gradients/coupled.py:540-629).
+"""
+
+from __future__ import annotations
+
+import torch
+
+from . import _kernels as K
+from . import _lib as L
+from .utils import SegmentKernel, as_segment_kernel
+
+F64 = torch.float64
+
+
+class PairSum(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, targets, y0, y1, g, kernel: SegmentKernel):
+        kern = as_segment_kernel(kernel)
+        tflat = targets.detach().reshape(-1, 3).contiguous()
+        ends = torch.stack((y0.detach().reshape(-1, 3), y1.detach().reshape(-1, 3)), dim=1).contiguous()
+        seg = K.segments_from_endpoints(ends, g.detach().reshape(-1).contiguous(), kern.kind, kern.eps)
+        v = K.pair_sum(seg, tflat, kern.kind, kern.eps, kern.rcut)
+        ctx.save_for_backward(tflat, seg)
+        ctx.kern = kern
+        ctx.shapes = (targets.shape, y0.shape, y1.shape, g.shape)
+        return v.reshape(targets.shape)
+
+    @staticmethod
+    def backward(ctx, gv):
+        tflat, seg = ctx.saved_tensors
+        kern = ctx.kern
+        ts, y0s, y1s, gs = ctx.shapes
+        gv = gv.reshape(-1, 3).contiguous().to(F64)
+        need_x = ctx.needs_input_grad[0]
+        need_s = any(ctx.needs_input_grad[1:4])
+        gx = torch.empty_like(tflat) if need_x else None
+        gseg = torch.empty((seg.shape[0], 7), dtype=F64, device=seg.device) if need_s else None
+        rc = L.lib().fj_pair_sum_backward(L.ptr(seg), seg.shape[0], L.ptr(tflat), tflat.shape[0], L.ptr(gv),
+                                          kern.kind, kern.eps, kern.rcut, L.ptr(gx) if need_x else None, 0,
+                                          L.ptr(gseg) if need_s else None, L.stream())
+        L.check(rc, "fj_pair_sum_backward")
+        g_t = gx.reshape(ts) if need_x else None
+        g0 = gseg[:, 0:3].reshape(y0s) if need_s else None
+        g1 = gseg[:, 3:6].reshape(y1s) if need_s else None
+        gg = gseg[:, 6].reshape(gs) if need_s else None
+        return g_t, g0, g1, gg, None
+
+
+def pair_sum(targets, y0, y1, g, kernel=None):
+    return PairSum.apply(targets, y0, y1, g, kernel)
+
+
+def grid_filaments(zeta, gamma):
+    """Differentiable de-duplicated filaments of a vertex grid (aero/aic.py:277-287)."""
+    gm = torch.diff(torch.nn.functional.pad(gamma, (1, 1, 0, 0)), dim=1)
+    gn = -torch.diff(torch.nn.functional.pad(gamma, (0, 0, 1, 1)), dim=0)
+    y0 = torch.cat((zeta[:-1].reshape(-1, 3), zeta[:, :-1].reshape(-1, 3)))
+    y1 = torch.cat((zeta[1:].reshape(-1, 3), zeta[:, 1:].reshape(-1, 3)))
+    return y0, y1, torch.cat((gm.reshape(-1), gn.reshape(-1)))
+
+
+def mirror_grid(zeta, mirror):
+    p = torch.as_tensor(mirror[0], dtype=F64, device=zeta.device)
+    nrm = torch.as_tensor(mirror[1], dtype=F64, device=zeta.device)
+    return zeta - 2.0 * ((zeta - p) * nrm).sum(-1, keepdim=True) * nrm
+
+
+def v_ind(targets, zetas, gammas, kernel=None, mirror=None):
+    """Differentiable compute_v_ind (aero/aic.py:248-310) on PairSum."""
+    ys0, ys1, gs = [], [], []
+    for z, g in zip(zetas, gammas):
+        if z.numel() == 0 or g.numel() == 0:
+            continue
+        y0, y1, ge = grid_filaments(z, g)
+        ys0.append(y0), ys1.append(y1), gs.append(ge)
+        if mirror is not None:
+            zm = mirror_grid(z, mirror)
+            y0, y1, ge = grid_filaments(zm, g)
+            ys0.append(y0), ys1.append(y1), gs.append(-ge)
+    if not gs:
+        return torch.zeros_like(targets)
+    return pair_sum(targets, torch.cat(ys0), torch.cat(ys1), torch.cat(gs), kernel)
+
+
+class LinearSolve(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, A, b):
+        LU, piv, _ = K.lu_factor(A.detach().contiguous())
+        x = K.lu_solve(LU, piv, b.detach().contiguous())
+        ctx.save_for_backward(LU, piv, x)
+        return x
+
+    @staticmethod
+    def backward(ctx, gx):
+        LU, piv, x = ctx.saved_tensors
+        lam = K.lu_solve(LU, piv, gx.contiguous().to(F64), trans=True)
+        gA = -torch.outer(lam, x) if ctx.needs_input_grad[0] else None
+        return gA, lam
+
+
+def linear_solve(A, b):
+    return LinearSolve.apply(A, b)
+
+
+def _aic_action(c, nc, zetas_b, zetas_w, gamma_vec, shapes, kernel, mirror):
+    """(A Gamma)_t = n_t . v_ind(c_t; bound rings with Gamma [+ static wake
+    rings carrying the TE row of Gamma]) -- the action of the assembled AIC
+    (aero/aic.py:137-205) written with differentiable pair sums."""
+    gammas, zs = [], []
+    off = 0
+    for i, (m, n) in enumerate(shapes):
+        g = gamma_vec[off:off + m * n].reshape(m, n)
+        off += m * n
+        zs.append(zetas_b[i])
+        gammas.append(g)
+        if zetas_w is not None and zetas_w[i].shape[0] > 1:
+            zs.append(zetas_w[i])
+            gammas.append(g[-1:].expand(zetas_w[i].shape[0] - 1, n))
+    v = v_ind(c, zs, gammas, kernel, mirror)
+    return (v * nc).sum(-1)
+
+
+class AICSolve(torch.autograd.Function):
+    """Gamma = A^{-1} b where A = AIC(c, nc, zeta_b[, zeta_w]) is assembled on
+    the device; differentiable in b and in the geometry."""
+
+    @staticmethod
+    def forward(ctx, b, c, nc, kernel, mirror, shapes, has_wake, *grids):
+        from .aic import compute_aic_solve
+
+        nb = len(shapes)
+        zb = [z.detach() for z in grids[:nb]]
+        zw = [z.detach() for z in grids[nb:]] if has_wake else None
+        kern = as_segment_kernel(kernel)
+        # one collocation block per surface (packed c / nc split by panel count)
+        cs, ns, off = [], [], 0
+        for (m, n) in shapes:
+            cs.append(c.detach()[off:off + m * n].reshape(m, n, 3))
+            ns.append(nc.detach()[off:off + m * n].reshape(m, n, 3))
+            off += m * n
+        mp = mn = None
+        if mirror is not None:
+            mp, mn = mirror
+        A = compute_aic_solve(cs, ns, zb, zw, [kern] * nb, [kern] * nb if zw is not None else None,
+                              mirror_point=mp, mirror_normal=mn)
+        LU, piv, _ = K.lu_factor(A, overwrite=True)
+        x = K.lu_solve(LU, piv, b.detach().contiguous())
+        ctx.save_for_backward(LU, piv, x, c, nc, *grids)
+        ctx.meta = (kern, mirror, shapes, has_wake, nb)
+        return x
+
+    @staticmethod
+    def backward(ctx, gx):
+        LU, piv, x, c, nc, *grids = ctx.saved_tensors
+        kern, mirror, shapes, has_wake, nb = ctx.meta
+        lam = K.lu_solve(LU, piv, gx.contiguous().to(F64), trans=True)
+        need = ctx.needs_input_grad
+        geo_inputs = [c, nc] + list(grids)
+        geo_need = [need[1], need[2]] + list(need[7:])
+        geo_grads = [None] * len(geo_inputs)
+        if any(geo_need):
+            with torch.enable_grad():
+                leaves = [t.detach().requires_grad_(True) for t in geo_inputs]
+                zb = leaves[2:2 + nb]
+                zw = leaves[2 + nb:] if has_wake else None
+                s = (lam * _aic_action(leaves[0], leaves[1], zb, zw, x, shapes, kern, mirror)).sum()
+                req = [lf for lf, nd in zip(leaves, geo_need) if nd]
+                gr = torch.autograd.grad(-s, req, allow_unused=True)
+            it = iter(gr)
+            geo_grads = [next(it) if nd else None for nd in geo_need]
+        return (lam, geo_grads[0], geo_grads[1], None, None, None, None, *geo_grads[2:])
+
+
+def aic_solve(b, c, nc, zetas_b, zetas_w=None, kernel=None, mirror=None):
+    """Differentiable circulation solve; c, nc packed (N_b, 3)."""
+    shapes = tuple((z.shape[0] - 1, z.shape[1] - 1) for z in zetas_b)
+    grids = list(zetas_b) + (list(zetas_w) if zetas_w is not None else [])
+    return AICSolve.apply(b, c, nc, kernel, mirror, shapes, zetas_w is not None, *grids)

@@ -106,6 +106,16 @@ int fj_pair_sum(const double* seg, int n_seg, const double* targets, int n_tgt, 
                 const fj_flowfield* flow, double time, double dt, void* workspace,
                 void* stream);
 
+/* Reverse mode of fj_pair_sum (FJ_OUT_STORE) for hand-written adjoints:
+ * with upstream weights grad_v (n_tgt, 3),
+ *   grad_targets (n_tgt, 3)   = dL/dx          (written, or added if accumulate)
+ *   grad_seg     (n_seg, 7)   = [dL/dy0 (3), dL/dy1 (3), dL/dg]  (written)
+ * Either output may be NULL.  No counterpart in the reference (which gets
+ * this from jax.vjp of compute_v_ind, aero/aic.py:248-310). */
+int fj_pair_sum_backward(const double* seg, int n_seg, const double* targets, int n_tgt,
+                         const double* grad_v, int kernel, double eps, double rcut,
+                         double* grad_targets, int accumulate, double* grad_seg, void* stream);
+
 /* ---- K2: AIC assembly ----------------------------------------------------- */
 
 /* Normal-projected ring influence of the grid zeta (m+1, n+1, 3) at n_tgt

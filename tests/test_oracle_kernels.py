@@ -175,3 +175,16 @@ def test_linearity_in_gamma():
     v = O.compute_v_ind(x, [zeta], [2.0 * g1 - 3.0 * g2], "epsilon")
     v_ref = 2.0 * O.compute_v_ind(x, [zeta], [g1], "epsilon") - 3.0 * O.compute_v_ind(x, [zeta], [g2], "epsilon")
     assert np.allclose(v, v_ref, rtol=1e-12, atol=1e-15)
+
+
+def test_c_restatement_matches_numpy_oracle():
+    from oracle import c_oracle
+
+    rng = np.random.default_rng(9)
+    x = rng.normal(size=(300, 3))
+    y0 = rng.normal(size=(400, 3))
+    y1 = y0 + 0.1 * rng.normal(size=(400, 3))
+    g = rng.normal(size=400)
+    a = c_oracle.pair_sum_eps(x, y0, y1, g)
+    b = O.pair_sum(x, y0, y1, g)
+    assert np.max(np.abs(a - b)) <= 1e-13 * np.max(np.abs(b))
