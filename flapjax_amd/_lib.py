@@ -88,6 +88,11 @@ _SIGS = {
                         c_void_p], c_int),
     "fj_pair_sum_backward": ([c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, c_double, c_double, c_void_p,
                               c_int, c_void_p, c_void_p], c_int),
+    "fj_batched_task_bytes": ([c_int], c_size_t),
+    "fj_batched_workspace_bytes": ([c_int, c_void_p, c_void_p], c_size_t),
+    "fj_pair_sum_batched": ([c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                             c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_double, c_double, c_void_p, c_void_p,
+                             c_void_p, c_void_p], c_int),
     "fj_flow_eval": ([POINTER(FjFlowfield), c_void_p, c_int, c_double, c_void_p, c_void_p], c_int),
 }
 

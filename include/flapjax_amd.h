@@ -116,6 +116,24 @@ int fj_pair_sum_backward(const double* seg, int n_seg, const double* targets, in
                          const double* grad_v, int kernel, double eps, double rcut,
                          double* grad_targets, int accumulate, double* grad_seg, void* stream);
 
+/* Fused pair sums of many independent cases in one launch (serving loop).
+ * Per task i (HOST arrays of n_tasks): targets rows [tgt_off, tgt_off+n_tgt)
+ * of `targets`, filaments rows [seg_off, seg_off+n_seg) of `seg`, results to
+ * rows [out_off, out_off+n_tgt) of `out` with epilogue mode[i] (dt, time,
+ * flows used by FJ_OUT_CONVECT; may be NULL otherwise).  Tasks must be sorted
+ * by tgt_off with disjoint target ranges.  task_table: device buffer of
+ * fj_batched_task_bytes(n_tasks); host_staging: host buffer of the same size
+ * (free to reuse once the call returns); workspace:
+ * fj_batched_workspace_bytes(...).  No counterpart in the reference (which
+ * vmaps/pmaps whole cases, tests/coupled/dynamic/test_parallel_cantilever.py). */
+size_t fj_batched_task_bytes(int n_tasks);
+size_t fj_batched_workspace_bytes(int n_tasks, const int* n_tgt, const int* n_seg);
+int fj_pair_sum_batched(int n_tasks, const int* tgt_off, const int* n_tgt, const int* seg_off,
+                        const int* n_seg, const int* out_off, const int* mode, const double* dt,
+                        const double* time, const fj_flowfield* flows, const double* seg,
+                        const double* targets, double* out, int kernel, double eps, double rcut,
+                        void* task_table, void* host_staging, void* workspace, void* stream);
+
 /* ---- K2: AIC assembly ----------------------------------------------------- */
 
 /* Normal-projected ring influence of the grid zeta (m+1, n+1, 3) at n_tgt

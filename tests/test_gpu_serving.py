@@ -1,0 +1,93 @@
+"""Serving form: continuous batching keeps every case's result.
+
+* each case served in a mix equals the same case served alone (1e-12);
+* a served case equals the plain UVLM drop-in run of that case (1e-10 on the
+  lift history, circulation and wake);
+* the fused multi-case pair sum equals per-case pair sums.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from tests.conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _stream():
+    from flapjax_amd.serving import CaseRequest
+
+    specs = [(4, 8, 12, 6), (8, 16, 9, 8), (4, 16, 15, 5), (8, 8, 7, 7), (6, 12, 11, 4)]
+    return [CaseRequest(i, m, n, st, ms, 2.0 + i, 0.1 + 0.05 * i, arrival=0.002 * i)
+            for i, (m, n, st, ms) in enumerate(specs)]
+
+
+def test_batched_pair_sum_equals_single(dev):
+    from flapjax_amd import _kernels as K
+    from flapjax_amd import _lib as L
+    from flapjax_amd.serving import ServingEngine
+
+    rng = np.random.default_rng(0)
+    eng = ServingEngine()
+    sizes = [(70, 100), (0, 30), (130, 0), (64, 257), (1, 1)]
+    segs, tgts, tasks = [], [], []
+    s_off = t_off = 0
+    for nt, ns in sizes:
+        y0 = rng.normal(size=(ns, 3))
+        y1 = y0 + 0.2 * rng.normal(size=(ns, 3))
+        g = rng.normal(size=ns)
+        ends = torch.as_tensor(np.stack((y0, y1), 1), device=dev)
+        segs.append(K.segments_from_endpoints(ends, torch.as_tensor(g, device=dev), L.KERNEL_EPSILON, 1e-7))
+        tgts.append(torch.as_tensor(rng.normal(size=(nt, 3)), device=dev))
+        tasks.append((t_off, nt, s_off, ns, t_off, L.OUT_STORE, 0.0, 0.0))
+        s_off += ns
+        t_off += nt
+    tgt_all = torch.cat(tgts)
+    out = torch.full_like(tgt_all, np.nan)
+    eng._batched(tasks, torch.cat(segs), tgt_all, out, L.PREC_FP64)
+    for (nt, ns), seg, tg, task in zip(sizes, segs, tgts, tasks):
+        ref = K.pair_sum(seg if ns else None, tg, L.KERNEL_EPSILON, 1e-7, 1e-4) if nt else tg
+        got = out[task[0]:task[0] + nt]
+        assert torch.allclose(got, ref, rtol=1e-13, atol=1e-15)
+
+
+def test_mix_equals_alone_and_equals_uvlm(dev):
+    from flapjax_amd import Constant
+    from flapjax_amd.serving import ServingEngine, motion
+    from flapjax_amd.uvlm import UVLM
+
+    reqs = _stream()
+    mix, stats = ServingEngine().run(reqs)
+    assert stats["cases"] == len(reqs)
+    by_id = {r.case_id: r for r in mix}
+    for req in reqs:
+        alone, _ = ServingEngine().run([req])
+        a, b = alone[0], by_id[req.case_id]
+        assert rel_err(b.lift, a.lift) < 1e-12
+        assert rel_err(b.gamma_b_final, a.gamma_b_final) < 1e-12
+        # the plain drop-in UVLM on the same case
+        hg, hgd = motion(req)
+        uv = UVLM([(req.m, req.n, req.m_star)], np.arange(req.n + 1), free_wake=True)
+        z0 = np.zeros((req.m + 1, req.n + 1, 3))
+        z0[..., 0] = (np.linspace(0.0, 1.0, req.m + 1) - 0.25)[:, None]
+        uv.set_design_variables(1.0 / req.m, Constant(np.array([1.0, 0, 0]), 1.225, True), z0, hg[0])
+        d = uv.solve_prescribed_dynamic(uv.solve_static(hg=hg[0]), hg, hgd)
+        lift = d.f_steady[0][..., 2].sum(dim=(1, 2)).cpu().numpy()
+        assert rel_err(b.lift[1:], lift[1:]) < 1e-10
+        assert rel_err(b.gamma_b_final, d.gamma_b[0][-1].cpu().numpy()) < 1e-10
+        assert rel_err(b.zeta_w_final, d.zeta_w[0][-1].cpu().numpy()) < 1e-10
+    print("serving stats:", stats)
+
+
+def test_admission_by_footprint(dev):
+    from flapjax_amd.serving import ServingEngine, footprint_bytes
+
+    reqs = _stream()
+    budget = max(footprint_bytes(r) for r in reqs) + 1
+    res, stats = ServingEngine(hbm_budget_bytes=budget).run(reqs)
+    assert stats["cases"] == len(reqs)
+    # with room for about one case at a time, starts are serialised
+    starts = sorted(r.start for r in res)
+    finishes = sorted(r.finish for r in res)
+    assert all(s >= f - 1e-9 for s, f in zip(starts[1:], finishes[:-1]))
