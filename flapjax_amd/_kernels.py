@@ -183,18 +183,21 @@ def lu_solve(LU, piv, b, trans=False):
     return x.t().contiguous() if b.dim() == 2 else x.reshape(n)
 
 
-def solve_refined(A, LU32, piv, b, max_iter=8, tol=None):
-    """FP64-refined solution from an FP32 factor.  Returns (x, iters tensor)."""
+def solve_refined(A, LU, piv, b, max_iter=8, tol=None, out=None):
+    """FP64-refined solution of A x = b from the factor LU (FP32 or FP64) of
+    A or of a nearby matrix.  Returns (x, iters tensor: sweeps, <0 if not
+    converged)."""
     A = _c64(A, None, "A")
     n = A.shape[0]
     b = _c64(b, None, "b").reshape(n)
     if tol is None:
         tol = 2.0 * np.finfo(np.float64).eps
-    x = torch.empty_like(b)
+    x = torch.empty_like(b) if out is None else out
     iters = torch.zeros(1, dtype=torch.int32, device=A.device)
     ws = torch.empty(L.lib().fj_refine_workspace_bytes(n) // 8 + 1, dtype=F64, device=A.device)
-    rc = L.lib().fj_solve_refined(L.ptr(A), n, L.ptr(LU32), n, L.ptr(piv), n, L.ptr(b), L.ptr(x), int(max_iter),
-                                  float(tol), L.ptr(iters), L.ptr(ws), L.stream())
+    fn = L.lib().fj_solve_refined if LU.dtype == torch.float32 else L.lib().fj_solve_refined_f64
+    rc = fn(L.ptr(A), n, L.ptr(LU), n, L.ptr(piv), n, L.ptr(b), L.ptr(x), int(max_iter), float(tol), L.ptr(iters),
+            L.ptr(ws), L.stream())
     L.check(rc, "fj_solve_refined")
     return x, iters
 

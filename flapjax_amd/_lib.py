@@ -70,6 +70,8 @@ _SIGS = {
     "fj_refine_workspace_bytes": ([c_int], c_size_t),
     "fj_solve_refined": ([c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p, c_int, c_double,
                           c_void_p, c_void_p, c_void_p], c_int),
+    "fj_solve_refined_f64": ([c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p, c_int,
+                              c_double, c_void_p, c_void_p, c_void_p], c_int),
     "fj_grid_from_hg": ([c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p], c_int),
     "fj_colloc_normals": ([c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p], c_int),
     "fj_filament_midpoints": ([c_void_p, c_int, c_int, c_void_p, c_void_p], c_int),

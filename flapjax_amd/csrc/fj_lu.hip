@@ -480,8 +480,9 @@ __global__ void k_residual(const double* __restrict__ A, int lda, int n,
 }
 
 // single workgroup: norms, convergence test, correction solve, update
+template <typename T>
 __global__ __launch_bounds__(FJ_LU_THREADS) void k_refine_step(
-    const float* __restrict__ LU, int ldlu, const int32_t* __restrict__ ipiv, int n,
+    const T* __restrict__ LU, int ldlu, const int32_t* __restrict__ ipiv, int n,
     const double* __restrict__ A_rownorm_max, const double* __restrict__ b,
     const double* __restrict__ r, double* __restrict__ x, double tol, int32_t* __restrict__ flags,
     int last) {
@@ -519,8 +520,8 @@ __global__ __launch_bounds__(FJ_LU_THREADS) void k_refine_step(
   const int32_t* perm = ipiv + n;
   for (int i = tid; i < n; i += blockDim.x) xs[i] = r[perm[i]];
   __syncthreads();
-  fj_tri_forward<float, double>(LU, n, ldlu, xs, true, false);
-  fj_tri_backward<float, double>(LU, n, ldlu, xs, false, false);
+  fj_tri_forward<T, double>(LU, n, ldlu, xs, true, false);
+  fj_tri_backward<T, double>(LU, n, ldlu, xs, false, false);
   for (int i = tid; i < n; i += blockDim.x) x[i] += xs[i];
   if (tid == 0) flags[1] += 1;
 }
@@ -559,31 +560,48 @@ extern "C" size_t fj_refine_workspace_bytes(int n) {
   return ((size_t)(n + 8) * sizeof(double) + 64 + 255) / 256 * 256;
 }
 
-extern "C" int fj_solve_refined(const double* A, int lda, const float* LU32, int ldlu,
-                                const int32_t* ipiv, int n, const double* b, double* x,
-                                int max_iter, double tol, int32_t* iters, void* workspace,
-                                void* stream) {
-  if (n <= 0 || n > FJ_SOLVE_MAXN || lda < n || ldlu < n || !A || !LU32 || !ipiv || !b || !x ||
+template <typename T>
+static int fj_solve_refined_t(const double* A, int lda, const T* LU, int ldlu,
+                              const int32_t* ipiv, int n, const double* b, double* x, int max_iter,
+                              double tol, int32_t* iters, void* workspace, int x_is_guess,
+                              hipStream_t st) {
+  if (n <= 0 || n > FJ_SOLVE_MAXN || lda < n || ldlu < n || !A || !LU || !ipiv || !b || !x ||
       !iters || !workspace || max_iter < 0)
     return FJ_ERR_ARG;
-  hipStream_t st = (hipStream_t)stream;
   double* r = (double*)workspace;
   double* anorm = r + n;
   int32_t* flags = (int32_t*)(anorm + 4);
   hipLaunchKernelGGL(k_refine_init, dim3(1), dim3(1), 0, st, flags);
   hipLaunchKernelGGL(k_rownorm, dim3(1), dim3(FJ_LU_THREADS), 0, st, A, lda, n, anorm);
-  // x0 = LU32^{-1} b
-  hipLaunchKernelGGL(k_lu_solve<float>, dim3(1), dim3(FJ_LU_THREADS), 0, st, LU32, n, ldlu, ipiv,
-                     b, x, 0);
+  // x0 = LU^{-1} b, unless the caller's x is the starting guess
+  if (!x_is_guess)
+    hipLaunchKernelGGL(k_lu_solve<T>, dim3(1), dim3(FJ_LU_THREADS), 0, st, LU, n, ldlu, ipiv, b, x,
+                       0);
   FJ_CHECK_LAUNCH();
   for (int it = 0; it <= max_iter; ++it) {
     hipLaunchKernelGGL(k_residual, dim3(fj_cdiv(n, 4)), dim3(256), 0, st, A, lda, n, b, x, r,
                        flags);
-    hipLaunchKernelGGL(k_refine_step, dim3(1), dim3(FJ_LU_THREADS), 0, st, LU32, ldlu, ipiv, n,
+    hipLaunchKernelGGL(k_refine_step<T>, dim3(1), dim3(FJ_LU_THREADS), 0, st, LU, ldlu, ipiv, n,
                        anorm, b, r, x, tol, flags, it == max_iter ? 1 : 0);
     FJ_CHECK_LAUNCH();
   }
   hipLaunchKernelGGL(k_refine_finish, dim3(1), dim3(1), 0, st, flags, iters);
   FJ_CHECK_LAUNCH();
   return FJ_OK;
+}
+
+extern "C" int fj_solve_refined(const double* A, int lda, const float* LU32, int ldlu,
+                                const int32_t* ipiv, int n, const double* b, double* x,
+                                int max_iter, double tol, int32_t* iters, void* workspace,
+                                void* stream) {
+  return fj_solve_refined_t<float>(A, lda, LU32, ldlu, ipiv, n, b, x, max_iter, tol, iters,
+                                   workspace, 0, (hipStream_t)stream);
+}
+
+extern "C" int fj_solve_refined_f64(const double* A, int lda, const double* LU, int ldlu,
+                                    const int32_t* ipiv, int n, const double* b, double* x,
+                                    int max_iter, double tol, int32_t* iters, void* workspace,
+                                    void* stream) {
+  return fj_solve_refined_t<double>(A, lda, LU, ldlu, ipiv, n, b, x, max_iter, tol, iters,
+                                    workspace, 0, (hipStream_t)stream);
 }

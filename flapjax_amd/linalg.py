@@ -2,11 +2,12 @@
 
 Two forms:
   * ``solve``           FP64 LU with partial pivoting + triangular solves;
-  * ``RefinedSolver``   FP32 LU on the matrix cores, FP64 iterative refinement
-                        against the FP64 matrix (residual b - A x in FP64);
-                        the FP32 factor can be reused across steps while the
-                        matrix stays close (rigid motion), refactoring when
-                        refinement stops converging.
+  * ``RefinedSolver``   LU (FP32 on the matrix cores by default, or FP64)
+                        with FP64 iterative refinement against the FP64
+                        matrix (residual b - A x in FP64); the factor can be
+                        reused across steps while the matrix stays close
+                        (rigid motion), refactoring when refinement stops
+                        converging.
 """
 
 from __future__ import annotations
@@ -39,18 +40,22 @@ def solve(A, b):
 class RefinedSolver:
     """FP32-factor + FP64-refinement solver with optional factor reuse."""
 
-    def __init__(self, max_iter: int = 10, tol: float | None = None, reuse: bool = True):
+    def __init__(self, max_iter: int = 10, tol: float | None = None, reuse: bool = True,
+                 factor_dtype=torch.float32):
         self.max_iter = max_iter
         self.tol = tol
         self.reuse = reuse
+        self.factor_dtype = factor_dtype
         self._lu32 = None
         self._piv = None
         self.factorisations = 0
         self.last_iters = None
 
     def factor(self, A):
-        A32 = A.to(torch.float32)
-        self._lu32, self._piv, _ = K.lu_factor_f32(A32)
+        if self.factor_dtype == torch.float32:
+            self._lu32, self._piv, _ = K.lu_factor_f32(A.to(torch.float32))
+        else:
+            self._lu32, self._piv, _ = K.lu_factor(A)
         self.factorisations += 1
 
     def solve(self, A, b, check: bool = True):

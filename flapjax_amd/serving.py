@@ -282,16 +282,10 @@ class ServingEngine:
             rhs = K.rhs(c, nc, cd, vw_all[off:off + nb], fl, i * sl.dt)
             off += nb
             x = sl.gb.reshape(-1)
-            if self.precision == "mixed":
-                rc = L.lib().fj_solve_refined(L.ptr(sl.A), nb, L.ptr(sl.lu), nb, L.ptr(sl.piv), nb, L.ptr(rhs),
-                                              L.ptr(x), 10, 2.0 * np.finfo(float).eps, L.ptr(sl.iters), L.ptr(sl.ws),
-                                              L.stream())
-                L.check(rc, "fj_solve_refined")
-            else:
-                x.copy_(K.lu_solve(sl.lu, sl.piv, rhs))
-                # one FP64 refinement sweep against the current AIC
-                r = rhs - sl.A @ x
-                x.add_(K.lu_solve(sl.lu, sl.piv, r))
+            fn = L.lib().fj_solve_refined if self.precision == "mixed" else L.lib().fj_solve_refined_f64
+            rc = fn(L.ptr(sl.A), nb, L.ptr(sl.lu), nb, L.ptr(sl.piv), nb, L.ptr(rhs), L.ptr(x), 10,
+                    2.0 * np.finfo(float).eps, L.ptr(sl.iters), L.ptr(sl.ws), L.stream())
+            L.check(rc, "fj_solve_refined")
         # (3) forces: velocities at the filament midpoints (fused)
         seg_parts, tgt_parts, tasks = [], [], []
         s_off = t_off = 0

@@ -18,7 +18,10 @@ Per dynamic step (aero/uvlm.py:912-1103):
 
 Arithmetic forms (``precision``):
   * "fp64"  - the plain drop-in: FP64 everywhere, AIC re-assembled and
-              re-factored every step as the reference does;
+              re-factored every step as the reference does (with
+              ``aic_reuse=True`` the FP64 factor is kept across steps and
+              every step's system is solved by FP64 refinement against the
+              freshly assembled AIC instead);
   * "mixed" - FP32 far field / FP64 near field for the rollup and force pair
               sums, FP32 LU with FP64 iterative refinement against the FP64
               AIC (residual b - A x in FP64).  The AIC and the wake upwash in
@@ -124,7 +127,12 @@ class UVLM:
         self._dt = None
         self.zeta_b0 = None
         self.hg_ref = None
-        self._solver = RefinedSolver(reuse=self.aic_reuse) if precision == "mixed" else None
+        if precision == "mixed":
+            self._solver = RefinedSolver(reuse=self.aic_reuse)
+        elif self.aic_reuse:
+            self._solver = RefinedSolver(reuse=True, factor_dtype=torch.float64)
+        else:
+            self._solver = None
         self._aic = None
 
     # ------------------------------------------------------------------ setup

@@ -184,6 +184,13 @@ int fj_solve_refined(const double* A, int lda, const float* LU32, int ldlu, cons
                      int n, const double* b, double* x, int max_iter, double tol,
                      int32_t* iters, void* workspace, void* stream);
 
+/* Same with an FP64 factor of a nearby matrix (a factor kept across time
+ * steps while the AIC changes only by rounding, e.g. rigid motion): the
+ * result solves the CURRENT A to FP64 accuracy. */
+int fj_solve_refined_f64(const double* A, int lda, const double* LU, int ldlu,
+                         const int32_t* ipiv, int n, const double* b, double* x, int max_iter,
+                         double tol, int32_t* iters, void* workspace, void* stream);
+
 /* ---- K4-K8: stencils, convection, forces ------------------------------- */
 
 /* zeta = R_j zeta0_ij + x_j and, if hg_dot/zeta_dot are non-NULL,
