@@ -345,3 +345,20 @@ def flow_eval(flow, x, time):
     rc = L.lib().fj_flow_eval(ctypes.byref(fd), L.ptr(flat), flat.shape[0], float(time), L.ptr(out), L.stream())
     L.check(rc, "fj_flow_eval")
     return out.reshape(x.shape)
+
+
+def wake_rediscretise(zeta_pre, gamma_pre, s_disc, zeta_w=None, gamma_w=None):
+    """Arc-length re-sampling of the convected wake (variable discretisation)."""
+    m_star = zeta_pre.shape[0] - 2
+    n = zeta_pre.shape[1] - 1
+    dev = zeta_pre.device
+    if zeta_w is None:
+        zeta_w = torch.empty((m_star + 1, n + 1, 3), dtype=F64, device=dev)
+    if gamma_w is None:
+        gamma_w = torch.empty((m_star, n), dtype=F64, device=dev)
+    ws = torch.empty(L.lib().fj_wake_rediscretise_workspace_bytes(m_star, n) // 8 + 1, dtype=F64, device=dev)
+    rc = L.lib().fj_wake_rediscretise(L.ptr(zeta_pre), L.ptr(gamma_pre) if m_star > 0 else None, m_star, n,
+                                      L.ptr(s_disc), L.ptr(zeta_w), L.ptr(gamma_w) if m_star > 0 else None,
+                                      L.ptr(ws), L.stream())
+    L.check(rc, "fj_wake_rediscretise")
+    return zeta_w, gamma_w

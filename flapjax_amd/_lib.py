@@ -80,6 +80,9 @@ _SIGS = {
     "fj_wake_convect": ([c_void_p, c_void_p, c_int, c_int, POINTER(FjFlowfield), c_double, c_double, c_void_p,
                          c_void_p], c_int),
     "fj_wake_initialise": ([c_void_p, c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p], c_int),
+    "fj_wake_rediscretise_workspace_bytes": ([c_int, c_int], c_size_t),
+    "fj_wake_rediscretise": ([c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
+                             c_int),
     "fj_static_wake_gamma": ([c_void_p, c_int, c_int, c_int, c_void_p, c_void_p], c_int),
     "fj_segments_from_endpoints": ([c_void_p, c_void_p, c_int, c_int, c_double, c_void_p, c_void_p], c_int),
     "fj_steady_forces": ([c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, POINTER(FjFlowfield),

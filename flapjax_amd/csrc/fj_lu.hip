@@ -209,12 +209,17 @@ __global__ void k_perm_init(int32_t* __restrict__ ipiv, int n, int32_t* __restri
 template <typename T>
 struct FjMfma;
 
+// Operands of both 16x16x4 instructions: lane l holds A[l % 16][l / 16] and
+// B[l / 16][l % 16].  Output element v of lane l is column l % 16 and row
+//   FP64: 4 v + l / 16   (four one-row groups)
+//   FP32: 4 (l / 16) + v (one four-row group)
 template <>
 struct FjMfma<double> {
   typedef double __attribute__((ext_vector_type(4))) acc_t;
   static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int row(int lk, int v) { return 4 * v + lk; }
 };
 template <>
 struct FjMfma<float> {
@@ -222,6 +227,7 @@ struct FjMfma<float> {
   static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int row(int lk, int v) { return 4 * lk + v; }
 };
 
 #define FJ_GT 64  // C tile edge
@@ -271,7 +277,7 @@ __global__ __launch_bounds__(256) void k_lu_update(T* __restrict__ A, int n, int
     for (int b = 0; b < 2; ++b)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int i = r0 + wr + a * 16 + 4 * lk + v;
+        const int i = r0 + wr + a * 16 + FjMfma<T>::row(lk, v);
         const int j = c0 + wc + b * 16 + li;
         if (i < n && j < n) A[(size_t)i * lda + j] -= acc[a][b][v];
       }

@@ -430,3 +430,91 @@ extern "C" int fj_strip_alpha(const double* zeta, const double* f_steady, int m,
 }
 
 extern "C" const char* fj_version(void) { return "flapjax_amd 0.1.0 (gfx950)"; }
+
+// ---- variable wake discretisation (aero/utils.py:452-491) --------------------
+// Per spanwise column: arc length along the m_pre = m_star+2 convected points,
+// linear interpolation (clamped at the ends, as jnp.interp) onto the target
+// stations s_disc = [0, cumsum(delta_w)] for the vertices and onto their
+// midpoints for the circulations; circulation stations are the arc length
+// averaged over the two neighbouring columns and rows
+// (neighbour_average(s, (0, 1)), algebra/array_utils.py:106-135).
+__device__ __forceinline__ double fj_interp(double x, const double* xp, const double* fp, int np,
+                                            int fstride) {
+  if (x <= xp[0]) return fp[0];
+  if (x >= xp[np - 1]) return fp[(size_t)(np - 1) * fstride];
+  int lo = 0, hi = np - 1;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (xp[mid] <= x) lo = mid; else hi = mid;
+  }
+  const double t = (x - xp[lo]) / (xp[hi] - xp[lo]);
+  const double f0 = fp[(size_t)lo * fstride], f1 = fp[(size_t)hi * fstride];
+  return f0 + t * (f1 - f0);
+}
+
+// arc length s (m_pre, n+1) of the pre-discretisation wake zeta_pre (m_pre, n+1, 3)
+__global__ void k_wake_arclength(const double* __restrict__ zp, int m_pre, int n,
+                                 double* __restrict__ s) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j > n) return;
+  double acc = 0.0;
+  s[j] = 0.0;
+  for (int i = 1; i < m_pre; ++i) {
+    const double* a = zp + 3 * ((size_t)(i - 1) * (n + 1) + j);
+    const double* b = zp + 3 * ((size_t)i * (n + 1) + j);
+    const double dx = b[0] - a[0], dy = b[1] - a[1], dz = b[2] - a[2];
+    acc += sqrt(dx * dx + dy * dy + dz * dz);
+    s[(size_t)i * (n + 1) + j] = acc;
+  }
+}
+
+__global__ void k_wake_rediscretise(const double* __restrict__ zp, const double* __restrict__ gp,
+                                    const double* __restrict__ s, int m_pre, int n,
+                                    const double* __restrict__ s_disc, int m_star,
+                                    double* __restrict__ scratch, double* __restrict__ zw,
+                                    double* __restrict__ gw) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j > n) return;
+  // column j of the arc length, contiguous copy for the search
+  double* xs = scratch + (size_t)j * 2 * m_pre;
+  for (int i = 0; i < m_pre; ++i) xs[i] = s[(size_t)i * (n + 1) + j];
+  for (int i = 0; i <= m_star; ++i)
+    for (int k = 0; k < 3; ++k)
+      zw[3 * ((size_t)i * (n + 1) + j) + k] =
+          fj_interp(s_disc[i], xs, zp + 3 * (size_t)j + k, m_pre, 3 * (n + 1));
+  if (j < n && m_star > 0) {
+    // circulation stations: s averaged along the rows, then the two columns
+    double* xg = xs + m_pre;
+    for (int i = 0; i < m_pre - 1; ++i) {
+      const double a0 = 0.5 * (s[(size_t)i * (n + 1) + j] + s[(size_t)(i + 1) * (n + 1) + j]);
+      const double a1 = 0.5 * (s[(size_t)i * (n + 1) + j + 1] + s[(size_t)(i + 1) * (n + 1) + j + 1]);
+      xg[i] = 0.5 * (a0 + a1);
+    }
+    for (int i = 0; i < m_star; ++i) {
+      const double sd = 0.5 * (s_disc[i] + s_disc[i + 1]);
+      gw[(size_t)i * n + j] = fj_interp(sd, xg, gp + j, m_pre - 1, n);
+    }
+  }
+}
+
+extern "C" size_t fj_wake_rediscretise_workspace_bytes(int m_star, int n) {
+  const size_t m_pre = (size_t)m_star + 2;
+  return (m_pre * (n + 1) + (size_t)(n + 1) * 2 * m_pre) * sizeof(double) + 256;
+}
+
+extern "C" int fj_wake_rediscretise(const double* zeta_pre, const double* gamma_pre, int m_star,
+                                    int n, const double* s_disc, double* zeta_w, double* gamma_w,
+                                    void* workspace, void* stream) {
+  if (m_star < 0 || n < 0 || !zeta_pre || !s_disc || !zeta_w || !workspace) return FJ_ERR_ARG;
+  if (m_star > 0 && (!gamma_pre || !gamma_w)) return FJ_ERR_ARG;
+  const int m_pre = m_star + 2;
+  double* s = (double*)workspace;
+  double* scratch = s + (size_t)m_pre * (n + 1);
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_wake_arclength, dim3(fj_cdiv(n + 1, 64)), dim3(64), 0, st, zeta_pre, m_pre, n,
+                     s);
+  hipLaunchKernelGGL(k_wake_rediscretise, dim3(fj_cdiv(n + 1, 64)), dim3(64), 0, st, zeta_pre,
+                     gamma_pre, s, m_pre, n, s_disc, m_star, scratch, zeta_w, gamma_w);
+  FJ_CHECK_LAUNCH();
+  return FJ_OK;
+}

@@ -322,6 +322,7 @@ class ServingEngine:
         t0 = time_source()
         ticks = 0
         pairs = 0
+        peak = 0
         while pending or active:
             now = time_source() - t0
             # admission: arrived, slot free, footprint fits
@@ -334,6 +335,7 @@ class ServingEngine:
                 sl.start = time_source() - t0
                 sl.fp = fp
                 used += fp
+                peak = max(peak, used)
                 active.append(sl)
             if not active:
                 wait = pending[0].arrival - (time_source() - t0)
@@ -363,7 +365,7 @@ class ServingEngine:
         lat = np.array([r.latency for r in results])
         stats = {"cases": len(results), "makespan_s": makespan, "cases_per_s": len(results) / makespan,
                  "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)), "ticks": ticks,
-                 "pairs": pairs, "gpairs_per_s": pairs / makespan / 1e9}
+                 "pairs": pairs, "gpairs_per_s": pairs / makespan / 1e9, "peak_bytes": peak}
         return results, stats
 
 

@@ -200,9 +200,10 @@ def strip_alpha(zeta_b, f_steady, v_func: FlowAt, rho):
 # ---------------------------------------------------------------------------
 def propagate_surf_wake(gamma_b_nm1, gamma_w_nm1, zeta_b_n, zeta_w_nm1, delta_w, v_func, dt, frozen_wake,
                         linearise_variable_wake=False):
-    """aero/utils.py:400-500 (uniform discretisation)."""
+    """aero/utils.py:400-500 (uniform and variable discretisation)."""
     if delta_w is not None:
-        raise NotImplementedError("variable wake discretisation (delta_w) is not on the device path")
+        return _propagate_variable(to_device(gamma_b_nm1), to_device(gamma_w_nm1), to_device(zeta_b_n),
+                                   to_device(zeta_w_nm1), delta_w, v_func, dt, frozen_wake)
     gamma_b_nm1 = to_device(gamma_b_nm1)
     gamma_w_nm1 = to_device(gamma_w_nm1)
     zeta_b_n = to_device(zeta_b_n)
@@ -220,6 +221,48 @@ def propagate_surf_wake(gamma_b_nm1, gamma_w_nm1, zeta_b_n, zeta_w_nm1, delta_w,
         else:
             vel = to_device(v_func(zeta_w_nm1[:-1])).reshape(-1, 3).contiguous()
             K.wake_convect(zeta_w_nm1, vel, None, 0.0, dt, zeta_w_n)
+    if frozen_wake:
+        return None, gamma_w_n
+    return zeta_w_n, gamma_w_n
+
+
+def _convect_rows(zeta_rows, v_func, dt, dst):
+    """dst = zeta_rows + dt * v_func(zeta_rows) on the device (fused for the
+    package's velocity closures)."""
+    flat = zeta_rows.reshape(-1, 3)
+    out = dst.reshape(-1, 3)
+    if isinstance(v_func, InducedVelocity):
+        v_func.induced(flat, out=out, mode=L.OUT_CONVECT, flow=v_func.flowfield, dt=dt)
+    else:
+        # fj_wake_convect maps rows 0..r-1 of a (r+1)-row source onto rows
+        # 1..r of a separate destination
+        n_rows, ncol = zeta_rows.shape[0], zeta_rows.shape[1] - 1
+        src = torch.empty((n_rows + 1, ncol + 1, 3), dtype=F64, device=zeta_rows.device)
+        src[:-1] = zeta_rows
+        res = torch.empty_like(src)
+        if isinstance(v_func, FlowAt):
+            K.wake_convect(src, None, v_func.flowfield, v_func.t, dt, res)
+        else:
+            vel = to_device(v_func(zeta_rows)).reshape(-1, 3).contiguous()
+            K.wake_convect(src, vel, None, 0.0, dt, res)
+        out.copy_(res[1:].reshape(-1, 3))
+
+
+def _propagate_variable(gamma_b_nm1, gamma_w_nm1, zeta_b_n, zeta_w_nm1, delta_w, v_func, dt, frozen_wake):
+    """Variable discretisation (aero/utils.py:431-491): all m_star+1 rows are
+    convected, then every column is re-sampled by arc length."""
+    m_star, n = zeta_w_nm1.shape[0] - 1, zeta_w_nm1.shape[1] - 1
+    dw = np.asarray(delta_w.detach().cpu() if isinstance(delta_w, torch.Tensor) else delta_w, dtype=np.float64)
+    if dw.shape != (m_star,):
+        raise ValueError(f"delta_w must have shape ({m_star},)")
+    s_disc = to_device(np.concatenate((np.zeros(1), np.cumsum(dw))))
+    zeta_pre = torch.empty((m_star + 2, n + 1, 3), dtype=F64, device=zeta_w_nm1.device)
+    zeta_pre[0] = zeta_b_n[-1]
+    _convect_rows(zeta_w_nm1, v_func, dt, zeta_pre[1:])
+    gamma_pre = torch.cat((gamma_b_nm1[-1:], gamma_w_nm1), dim=0).contiguous()
+    zeta_w_n, gamma_w_n = K.wake_rediscretise(zeta_pre, gamma_pre, s_disc)
+    if m_star == 0:
+        gamma_w_n = torch.zeros_like(gamma_w_nm1)
     if frozen_wake:
         return None, gamma_w_n
     return zeta_w_n, gamma_w_n

@@ -159,8 +159,23 @@ class UVLM:
             if tuple(z.shape) != (gd.m + 1, gd.n + 1, 3):
                 raise ValueError(f"zeta_b0 must have shape {(gd.m + 1, gd.n + 1, 3)}, got {tuple(z.shape)}")
             self.zeta_b0.append(z)
-        if delta_w is not None and not (isinstance(delta_w, Sequence) and all(d is None for d in delta_w)):
-            raise NotImplementedError("variable wake discretisation (delta_w) is not on the device path")
+        if delta_w is None:
+            dws = [None] * self.n_surf
+        elif isinstance(delta_w, (np.ndarray, torch.Tensor)):
+            dws = [delta_w]
+        elif isinstance(delta_w, Sequence):
+            dws = list(delta_w)
+        else:
+            raise TypeError("Invalid delta_w type")
+        self.delta_w = []
+        for i, dw in enumerate(dws):
+            if dw is None:
+                self.delta_w.append(None)
+                continue
+            dw = np.asarray(dw.detach().cpu() if isinstance(dw, torch.Tensor) else dw, dtype=np.float64)
+            if dw.shape != (self.grid_disc[i].m_star,):
+                raise ValueError(f"delta_w must have shape ({self.grid_disc[i].m_star},)")
+            self.delta_w.append(dw)
         if reference_cs_angles:
             raise NotImplementedError("control surfaces are not on the device path")
         hg0 = to_device(hg0)
@@ -198,7 +213,8 @@ class UVLM:
 
     def _wake_s(self, i):
         gd = self.grid_disc[i]
-        dw = np.ones(gd.m_star) * self.dt * self.flowfield.u_inf_mag
+        dw = self.delta_w[i] if getattr(self, "delta_w", None) and self.delta_w[i] is not None else \
+            np.ones(gd.m_star) * self.dt * self.flowfield.u_inf_mag
         return to_device(np.concatenate((np.zeros(1), np.cumsum(dw))))
 
     def initialise_wake(self, zeta_b=None, out=None):
@@ -300,6 +316,17 @@ class UVLM:
                 tables_nm1 = self._tables(list(zeta_b_nm1) + list(q_nm1["zeta_w"]),
                                           list(q_nm1["gamma_b"]) + list(q_nm1["gamma_w"]))
             for i, gd in enumerate(self.grid_disc):
+                if self.delta_w[i] is not None:
+                    # variable discretisation (aero/utils.py:431-491)
+                    from .utils import FlowAt, _propagate_variable
+
+                    vf = InducedVelocity(ff, t_n, tables_nm1, self.kernel, self._prec) if self.free_wake \
+                        else FlowAt(ff, t_n)
+                    zn, gn = _propagate_variable(q_nm1["gamma_b"][i], q_nm1["gamma_w"][i], zeta_b_n[i],
+                                                 q_nm1["zeta_w"][i], self.delta_w[i], vf, self.dt, False)
+                    zeta_w_n[i].copy_(zn)
+                    gamma_w_n[i].copy_(gn)
+                    continue
                 K.wake_shift(q_nm1["gamma_b"][i], q_nm1["gamma_w"][i], zeta_b_n[i], gd.m_star, gamma_w_n[i],
                              zeta_w_n[i])
                 if gd.m_star == 0:

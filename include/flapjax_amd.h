@@ -232,6 +232,17 @@ int fj_wake_convect(const double* zeta_w_nm1, const double* vel, int m_star, int
                     const fj_flowfield* flow, double time, double dt, double* zeta_w_n,
                     void* stream);
 
+/* Variable wake discretisation (aero/utils.py:452-491): zeta_pre
+ * (m_star+2, n+1, 3) is [TE row; all m_star+1 convected rows], gamma_pre
+ * (m_star+1, n) is [TE circulation; all m_star wake rows].  Each column is
+ * re-sampled by arc length onto s_disc (device, m_star+1) = [0,
+ * cumsum(delta_w)] (vertices) and onto its midpoints (circulations).
+ * workspace: fj_wake_rediscretise_workspace_bytes(m_star, n). */
+size_t fj_wake_rediscretise_workspace_bytes(int m_star, int n);
+int fj_wake_rediscretise(const double* zeta_pre, const double* gamma_pre, int m_star, int n,
+                         const double* s_disc, double* zeta_w, double* gamma_w, void* workspace,
+                         void* stream);
+
 /* Initial flat wake TE + s_i * dir (initialise_wake, aero/uvlm.py:662-689):
  * s (device, m_star+1) = [0, cumsum(delta_w)], dir (HOST, 3) = unit U_inf. */
 int fj_wake_initialise(const double* zeta_b, int m, int n, const double* s, int m_star,

@@ -84,10 +84,11 @@ def test_admission_by_footprint(dev):
     from flapjax_amd.serving import ServingEngine, footprint_bytes
 
     reqs = _stream()
-    budget = max(footprint_bytes(r) for r in reqs) + 1
+    fps = sorted(footprint_bytes(r) for r in reqs)
+    budget = fps[-1] + fps[0] // 2  # never room for two of the larger cases
     res, stats = ServingEngine(hbm_budget_bytes=budget).run(reqs)
     assert stats["cases"] == len(reqs)
-    # with room for about one case at a time, starts are serialised
-    starts = sorted(r.start for r in res)
-    finishes = sorted(r.finish for r in res)
-    assert all(s >= f - 1e-9 for s, f in zip(starts[1:], finishes[:-1]))
+    assert stats["peak_bytes"] <= budget
+    # a generous budget runs cases concurrently
+    res2, stats2 = ServingEngine().run(reqs)
+    assert stats2["peak_bytes"] > fps[-1]

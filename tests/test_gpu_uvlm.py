@@ -111,3 +111,32 @@ def test_mixed_precision_error_is_reported(dev):
             for k in ("gamma_b", "f_steady", "zeta_w")}
     print("mixed-form errors vs FP64 oracle:", errs)
     assert errs["gamma_b"] < 1e-6 and errs["f_steady"] < 1e-5 and errs["zeta_w"] < 1e-6
+
+
+@pytest.mark.parametrize("free_wake", [False, True])
+def test_variable_wake_discretisation(dev, free_wake):
+    """delta_w path (aero/utils.py:452-491) vs the oracle, heaving wing as in
+    the reference's tests/aero/test_variable_wake_disc.py."""
+    import flapjax_amd as fj
+    from flapjax_amd.uvlm import UVLM
+
+    m, n, m_star, steps = 4, 8, 10, 8
+    dt = 1.0 / (m * 10.0)
+    zeta0 = O.make_rectangular_grid(m, n, 1.0, 0.0)
+    hg = O.rigid_wing_hg(n, 5.0, 0.0)
+    t = np.arange(steps) * dt
+    w = 2 * np.pi * 3.0
+    hg_t = np.stack([hg] * steps)
+    hg_t[:, :, 2, 3] += (0.1 * 0.5 * (1 - np.cos(w * t)))[:, None]
+    hgd_t = np.zeros_like(hg_t)
+    hgd_t[:, :, 2, 3] = (0.1 * w * 0.5 * np.sin(w * t))[:, None]
+    delta_w = np.linspace(1.0, 1.6, m_star) * dt * 10.0
+    orc = O.UVLMOracle([(m, n, m_star)], np.arange(n + 1), free_wake=free_wake, gamma_dot_relaxation=1.0)
+    orc.set_design_variables(dt, O.Constant(np.array([10.0, 0, 0]), 1.225), zeta0, hg, delta_w=[delta_w])
+    gpu = UVLM([(m, n, m_star)], np.arange(n + 1), free_wake=free_wake, gamma_dot_relaxation=1.0,
+               variable_wake_disc=True)
+    gpu.set_design_variables(dt, fj.Constant(np.array([10.0, 0, 0]), 1.225, True), zeta0, hg, delta_w=[delta_w])
+    do = orc.solve_prescribed_dynamic(orc.solve_static(), hg_t, hgd_t)
+    dg = gpu.solve_prescribed_dynamic(gpu.solve_static(), hg_t, hgd_t)
+    _cmp(dg, do, ("gamma_b", "gamma_w", "f_steady"), TOL)
+    _cmp(dg, do, ("zeta_w",), TOL_WAKE)
