@@ -74,23 +74,23 @@ def pitch_plunge(n, span, n_tstep, dt):
     return hg, hgd
 
 
-def make_case(precision):
+def make_case(precision, aic_reuse=True):
     import flapjax_amd as fj
     from flapjax_amd.uvlm import UVLM
     from flapjax_amd.utils import make_rectangular_grid
 
     dt = 1.0 / M
     hg, hgd = pitch_plunge(N, SPAN, N_TSTEP, dt)
-    uv = UVLM([(M, N, M_STAR)], np.arange(N + 1), free_wake=True, precision=precision)
+    uv = UVLM([(M, N, M_STAR)], np.arange(N + 1), free_wake=True, precision=precision, aic_reuse=aic_reuse)
     uv.set_design_variables(dt, fj.Constant(np.array([1.0, 0.0, 0.0]), 1.225, True),
                             make_rectangular_grid(M, N, 1.0, 0.25), hg[0])
     return uv, hg, hgd
 
 
-def time_form(precision, steps, warmup):
+def time_form(precision, steps, warmup, aic_reuse=True, whole_case=True):
     import torch
 
-    uv, hg_np, hgd_np = make_case(precision)
+    uv, hg_np, hgd_np = make_case(precision, aic_reuse)
     from flapjax_amd.utils import to_device
 
     hg, hgd = to_device(hg_np), to_device(hgd_np)
@@ -118,6 +118,9 @@ def time_form(precision, steps, warmup):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     lat = np.array([ev[k].elapsed_time(ev[k + 1]) for k in range(steps)])
+    if not whole_case:
+        return {"wall_s": wall, "ms_per_step": 1e3 * wall / steps, "p50_ms": float(np.percentile(lat, 50)),
+                "p99_ms": float(np.percentile(lat, 99)), "case_s": float("nan")}
     # whole case (static + 199 dynamic steps) end to end
     torch.cuda.synchronize()
     tc = time.perf_counter()
@@ -208,6 +211,7 @@ def main():
     pairs, rollup_pairs = pairs_per_step()
     f64 = time_form("fp64", args.steps, args.warmup)
     mix = time_form("mixed", args.steps, args.warmup)
+    lit = time_form("fp64", max(3, args.steps // 4), 1, aic_reuse=False, whole_case=False)
     roof, sample = rollup_roofline()
     cpu = None if args.no_cpu else cpu_baseline(sample)
     value = pairs * args.steps / f64["wall_s"] / 1e9
@@ -230,6 +234,8 @@ def main():
                    "parallelism": "single GPU"},
         "latency_ms": {"p50": round(f64["p50_ms"], 4), "p99": round(f64["p99_ms"], 4)},
         "cases_per_s": round(1.0 / f64["case_s"], 3),
+        "fp64_refactor_each_step": {"ms_per_step": round(lit["ms_per_step"], 4),
+                                    "value": round(pairs / lit["ms_per_step"] / 1e6, 3)},
         "mixed": {"value": round(value_mixed, 3), "unit": "Gpairs/s", "dtype": "f32+f64",
                   "ms_per_step": round(mix["ms_per_step"], 4),
                   "latency_ms": {"p50": round(mix["p50_ms"], 4), "p99": round(mix["p99_ms"], 4)},

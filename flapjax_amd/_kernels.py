@@ -146,58 +146,76 @@ def aic_assemble(c, nc, zeta, kernel_kind, eps, rcut, out, col0=0, zeta_w=None, 
 # ---------------------------------------------------------------------------
 # K3
 # ---------------------------------------------------------------------------
-def lu_factor(A, overwrite=False):
-    """LU with partial pivoting (FP64).  Returns (LU, piv) with piv (2n,) int32:
-    LAPACK pivots then the composed permutation."""
-    A = _c64(A, None, "A")
+class Factor(tuple):
+    """(lu, piv, aux, info) of fj_lu_factor / fj_lu_factor_f32: the in-place
+    factor, 2n int32 pivots + permutation, diagonal-block inverses, status."""
+
+    __slots__ = ()
+
+    def __new__(cls, lu, piv, aux, info):
+        return tuple.__new__(cls, (lu, piv, aux, info))
+
+    lu = property(lambda self: self[0])
+    piv = property(lambda self: self[1])
+    aux = property(lambda self: self[2])
+    info = property(lambda self: self[3])
+
+
+def _factor(A, fn, dtype):
     n = A.shape[0]
     if A.dim() != 2 or A.shape[1] != n:
         raise ValueError("A must be square")
-    LU = A if overwrite else A.clone()
     piv = torch.empty(2 * n, dtype=torch.int32, device=A.device)
     info = torch.zeros(1, dtype=torch.int32, device=A.device)
-    rc = L.lib().fj_lu_factor(L.ptr(LU), n, n, L.ptr(piv), L.ptr(info), None, L.stream())
-    L.check(rc, "fj_lu_factor")
-    return LU, piv, info
+    aux = torch.empty(L.lib().fj_lu_aux_bytes(n) // 8 + 1, dtype=F64, device=A.device)
+    rc = fn(L.ptr(A), n, n, L.ptr(piv), L.ptr(info), L.ptr(aux), L.stream())
+    L.check(rc, "LU factorisation")
+    return Factor(A, piv, aux, info)
+
+
+def lu_factor(A, overwrite=False):
+    """LU with partial pivoting (FP64) -> Factor."""
+    A = _c64(A, None, "A")
+    return _factor(A if overwrite else A.clone(), L.lib().fj_lu_factor, F64)
 
 
 def lu_factor_f32(A32):
+    """FP32 LU (matrix cores) -> Factor."""
     if A32.dtype != torch.float32 or not A32.is_cuda:
         raise TypeError("A32 must be a float32 device tensor")
-    A32 = A32.contiguous()
-    n = A32.shape[0]
-    piv = torch.empty(2 * n, dtype=torch.int32, device=A32.device)
-    info = torch.zeros(1, dtype=torch.int32, device=A32.device)
-    rc = L.lib().fj_lu_factor_f32(L.ptr(A32), n, n, L.ptr(piv), L.ptr(info), None, L.stream())
-    L.check(rc, "fj_lu_factor_f32")
-    return A32, piv, info
+    return _factor(A32.contiguous(), L.lib().fj_lu_factor_f32, torch.float32)
 
 
-def lu_solve(LU, piv, b, trans=False):
-    """Solve with the factor; b (n,) or (n, nrhs) -> new tensor."""
+def lu_solve(fac, b, trans=False):
+    """Solve with an FP64 factor; b (n,) or (n, nrhs) -> new tensor."""
+    LU, piv, aux = fac[0], fac[1], fac[2]
+    if LU.dtype != F64:
+        raise TypeError("lu_solve takes an FP64 factor (use solve_refined for FP32 factors)")
     n = LU.shape[0]
     b = _c64(b, None, "b")
     x = b.reshape(n, -1).t().contiguous().clone() if b.dim() == 2 else b.reshape(1, n).clone()
-    rc = L.lib().fj_lu_solve(L.ptr(LU), n, n, L.ptr(piv), L.ptr(x), x.shape[0], int(bool(trans)), L.stream())
+    rc = L.lib().fj_lu_solve(L.ptr(LU), n, n, L.ptr(piv), L.ptr(aux), L.ptr(x), x.shape[0], int(bool(trans)),
+                             L.stream())
     L.check(rc, "fj_lu_solve")
     return x.t().contiguous() if b.dim() == 2 else x.reshape(n)
 
 
-def solve_refined(A, LU, piv, b, max_iter=8, tol=None, out=None):
-    """FP64-refined solution of A x = b from the factor LU (FP32 or FP64) of
-    A or of a nearby matrix.  Returns (x, iters tensor: sweeps, <0 if not
-    converged)."""
+def solve_refined(A, fac, b, max_iter=8, tol=None, out=None):
+    """FP64-refined solution of A x = b from the factor (FP32 or FP64) of A or
+    of a nearby matrix.  Returns (x, iters tensor: corrections made, negative
+    if the tolerance was not reached)."""
     A = _c64(A, None, "A")
+    LU, piv, aux = fac[0], fac[1], fac[2]
     n = A.shape[0]
     b = _c64(b, None, "b").reshape(n)
     if tol is None:
-        tol = 2.0 * np.finfo(np.float64).eps
+        tol = 4.0 * np.finfo(np.float64).eps
     x = torch.empty_like(b) if out is None else out
     iters = torch.zeros(1, dtype=torch.int32, device=A.device)
     ws = torch.empty(L.lib().fj_refine_workspace_bytes(n) // 8 + 1, dtype=F64, device=A.device)
     fn = L.lib().fj_solve_refined if LU.dtype == torch.float32 else L.lib().fj_solve_refined_f64
-    rc = fn(L.ptr(A), n, L.ptr(LU), n, L.ptr(piv), n, L.ptr(b), L.ptr(x), int(max_iter), float(tol), L.ptr(iters),
-            L.ptr(ws), L.stream())
+    rc = fn(L.ptr(A), n, L.ptr(LU), n, L.ptr(piv), L.ptr(aux), n, L.ptr(b), L.ptr(x), int(max_iter), float(tol),
+            L.ptr(iters), L.ptr(ws), L.stream())
     L.check(rc, "fj_solve_refined")
     return x, iters
 

@@ -63,15 +63,15 @@ _SIGS = {
                      POINTER(FjFlowfield), c_double, c_double, c_void_p, c_void_p], c_int),
     "fj_aic_assemble": ([c_void_p, c_void_p, c_int, c_void_p, c_int, c_int, c_void_p, c_int, c_void_p, c_void_p,
                          c_int, c_double, c_double, c_void_p, c_int, c_int, c_int, c_void_p], c_int),
-    "fj_lu_workspace_bytes": ([c_int], c_size_t),
+    "fj_lu_aux_bytes": ([c_int], c_size_t),
     "fj_lu_factor": ([c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p], c_int),
-    "fj_lu_solve": ([c_void_p, c_int, c_int, c_void_p, c_void_p, c_int, c_int, c_void_p], c_int),
+    "fj_lu_solve": ([c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p], c_int),
     "fj_lu_factor_f32": ([c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p], c_int),
     "fj_refine_workspace_bytes": ([c_int], c_size_t),
-    "fj_solve_refined": ([c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p, c_int, c_double,
-                          c_void_p, c_void_p, c_void_p], c_int),
-    "fj_solve_refined_f64": ([c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p, c_int,
-                              c_double, c_void_p, c_void_p, c_void_p], c_int),
+    "fj_solve_refined": ([c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int,
+                          c_double, c_void_p, c_void_p, c_void_p], c_int),
+    "fj_solve_refined_f64": ([c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p,
+                              c_int, c_double, c_void_p, c_void_p, c_void_p], c_int),
     "fj_grid_from_hg": ([c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p], c_int),
     "fj_colloc_normals": ([c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p], c_int),
     "fj_filament_midpoints": ([c_void_p, c_int, c_int, c_void_p, c_void_p], c_int),

@@ -100,15 +100,15 @@ def v_ind(targets, zetas, gammas, kernel=None, mirror=None):
 class LinearSolve(torch.autograd.Function):
     @staticmethod
     def forward(ctx, A, b):
-        LU, piv, _ = K.lu_factor(A.detach().contiguous())
-        x = K.lu_solve(LU, piv, b.detach().contiguous())
-        ctx.save_for_backward(LU, piv, x)
+        fac = K.lu_factor(A.detach().contiguous())
+        x = K.lu_solve(fac, b.detach().contiguous())
+        ctx.save_for_backward(fac.lu, fac.piv, fac.aux, x)
         return x
 
     @staticmethod
     def backward(ctx, gx):
-        LU, piv, x = ctx.saved_tensors
-        lam = K.lu_solve(LU, piv, gx.contiguous().to(F64), trans=True)
+        LU, piv, aux, x = ctx.saved_tensors
+        lam = K.lu_solve((LU, piv, aux), gx.contiguous().to(F64), trans=True)
         gA = -torch.outer(lam, x) if ctx.needs_input_grad[0] else None
         return gA, lam
 
@@ -158,17 +158,17 @@ class AICSolve(torch.autograd.Function):
             mp, mn = mirror
         A = compute_aic_solve(cs, ns, zb, zw, [kern] * nb, [kern] * nb if zw is not None else None,
                               mirror_point=mp, mirror_normal=mn)
-        LU, piv, _ = K.lu_factor(A, overwrite=True)
-        x = K.lu_solve(LU, piv, b.detach().contiguous())
-        ctx.save_for_backward(LU, piv, x, c, nc, *grids)
+        fac = K.lu_factor(A, overwrite=True)
+        x = K.lu_solve(fac, b.detach().contiguous())
+        ctx.save_for_backward(fac.lu, fac.piv, fac.aux, x, c, nc, *grids)
         ctx.meta = (kern, mirror, shapes, has_wake, nb)
         return x
 
     @staticmethod
     def backward(ctx, gx):
-        LU, piv, x, c, nc, *grids = ctx.saved_tensors
+        LU, piv, aux, x, c, nc, *grids = ctx.saved_tensors
         kern, mirror, shapes, has_wake, nb = ctx.meta
-        lam = K.lu_solve(LU, piv, gx.contiguous().to(F64), trans=True)
+        lam = K.lu_solve((LU, piv, aux), gx.contiguous().to(F64), trans=True)
         need = ctx.needs_input_grad
         geo_inputs = [c, nc] + list(grids)
         geo_need = [need[1], need[2]] + list(need[7:])

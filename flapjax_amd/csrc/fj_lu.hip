@@ -20,26 +20,26 @@
 
 #define FJ_LU_THREADS 1024
 #define FJ_LU_WAVES (FJ_LU_THREADS / 64)
+#define FJ_PANEL_THREADS 512
+#define FJ_PANEL_WAVES (FJ_PANEL_THREADS / 64)
 
 template <typename T>
 __device__ __forceinline__ T fj_abs(T x) { return x < T(0) ? -x : x; }
 
-// workgroup argmax of (value, index), smallest index among equal values
-__device__ __forceinline__ void fj_argmax_pair(double& v, int& idx) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    const double ov = __shfl_xor(v, off);
-    const int oi = __shfl_xor(idx, off);
-    if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
-  }
+// Pivot search key: |a| as the high bits (non-negative doubles order like
+// their bit patterns) with the low 13 mantissa bits replaced by 8191 - row,
+// so one unsigned max picks the largest |a| and, among values equal to about
+// 2^-39 relative, the smallest row.  Rows < 8192.
+__device__ __forceinline__ unsigned long long fj_pivot_key(double a, int row) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(a < 0.0 ? -a : a);
+  return (b & ~0x1FFFull) | (unsigned long long)(8191 - row);
 }
 
 template <typename T, int R, int JB>
-__global__ __launch_bounds__(FJ_LU_THREADS) void k_lu_panel(T* __restrict__ A, int n, int lda,
-                                                             int k, int32_t* __restrict__ ipiv,
-                                                             int32_t* __restrict__ info) {
-  __shared__ double s_val[FJ_LU_WAVES];
-  __shared__ int s_idx[FJ_LU_WAVES];
+__global__ __launch_bounds__(FJ_PANEL_THREADS) void k_lu_panel(T* __restrict__ A, int n, int lda,
+                                                               int k, int32_t* __restrict__ ipiv,
+                                                               int32_t* __restrict__ info) {
+  __shared__ unsigned long long s_key[FJ_PANEL_WAVES];
   __shared__ T s_piv[2][JB];
   __shared__ T s_swp[2][JB];
   __shared__ int s_ipiv[JB];
@@ -48,11 +48,11 @@ __global__ __launch_bounds__(FJ_LU_THREADS) void k_lu_panel(T* __restrict__ A, i
   const int jb = min(JB, n - k);
   const int rows = n - k;
 
-  // load the panel: thread tid owns local rows tid + r * 1024
+  // load the panel: thread tid owns local rows tid + r * FJ_PANEL_THREADS
   T a[R][JB];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    const int lr = tid + r * FJ_LU_THREADS;
+    const int lr = tid + r * FJ_PANEL_THREADS;
 #pragma unroll
     for (int c = 0; c < JB; ++c)
       a[r][c] = (lr < rows && c < jb) ? A[(size_t)(k + lr) * lda + k + c] : T(0);
@@ -61,76 +61,76 @@ __global__ __launch_bounds__(FJ_LU_THREADS) void k_lu_panel(T* __restrict__ A, i
   // fully unrolled so that every register index below is a compile-time constant
 #pragma unroll
   for (int jj = 0; jj < JB; ++jj) {
-    if (jj >= jb) break;  // uniform: jb is the same for the whole workgroup
-    const int buf = jj & 1;
-    // argmax |a[lr][jj]| over local rows lr >= jj
-    double best = -1.0;
-    int bidx = 0x7fffffff;
+    if (jj < jb) {  // uniform: jb is the same for the whole workgroup
+      const int buf = jj & 1;
+      unsigned long long key = 0ull;
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const int lr = tid + r * FJ_LU_THREADS;
-      if (lr >= jj && lr < rows) {
-        const double v = (double)fj_abs(a[r][jj]);
-        if (v > best) { best = v; bidx = lr; }
+      for (int r = 0; r < R; ++r) {
+        const int lr = tid + r * FJ_PANEL_THREADS;
+        if (lr >= jj && lr < rows) {
+          const unsigned long long kk = fj_pivot_key((double)a[r][jj], lr);
+          key = kk > key ? kk : key;
+        }
       }
-    }
-    fj_argmax_pair(best, bidx);
-    if (lane == 0) { s_val[wave] = best; s_idx[wave] = bidx; }
-    __syncthreads();
-    best = s_val[0]; bidx = s_idx[0];
 #pragma unroll
-    for (int w = 1; w < FJ_LU_WAVES; ++w) {
-      const double v = s_val[w];
-      const int i = s_idx[w];
-      if (v > best || (v == best && i < bidx)) { best = v; bidx = i; }
-    }
-    const int p = bidx;  // local pivot row
-    // owners publish the pivot row and the current row jj
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const int lr = tid + r * FJ_LU_THREADS;
-      if (lr == p) {
-#pragma unroll
-        for (int c = 0; c < JB; ++c) s_piv[buf][c] = a[r][c];
+      for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(key, off);
+        key = o > key ? o : key;
       }
-      if (lr == jj) {
+      if (lane == 0) s_key[wave] = key;
+      __syncthreads();
+      key = s_key[0];
 #pragma unroll
-        for (int c = 0; c < JB; ++c) s_swp[buf][c] = a[r][c];
+      for (int w = 1; w < FJ_PANEL_WAVES; ++w) key = s_key[w] > key ? s_key[w] : key;
+      const int p = 8191 - (int)(key & 0x1FFFull);  // local pivot row
+      // owners publish the pivot row and the current row jj
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int lr = tid + r * FJ_PANEL_THREADS;
+        if (lr == p) {
+#pragma unroll
+          for (int c = 0; c < JB; ++c) s_piv[buf][c] = a[r][c];
+        }
+        if (lr == jj) {
+#pragma unroll
+          for (int c = 0; c < JB; ++c) s_swp[buf][c] = a[r][c];
+        }
       }
-    }
-    if (tid == 0) {
-      s_ipiv[jj] = k + p;
-      if (best == 0.0 && *info == 0) *info = k + jj + 1;
-    }
-    __syncthreads();
-    const T piv = s_piv[buf][jj];
-    const T inv = (piv != T(0)) ? T(1) / piv : T(0);
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const int lr = tid + r * FJ_LU_THREADS;
-      if (lr == jj) {
-#pragma unroll
-        for (int c = 0; c < JB; ++c) a[r][c] = s_piv[buf][c];
-      } else if (lr == p) {
-#pragma unroll
-        for (int c = 0; c < JB; ++c) a[r][c] = s_swp[buf][c];
+      if (tid == 0) {
+        s_ipiv[jj] = k + p;
+        if ((key >> 13) == 0ull && *info == 0) *info = k + jj + 1;  // zero column
       }
-      if (lr > jj && lr < rows) {
-        const T l = a[r][jj] * inv;
-        a[r][jj] = l;
+      __syncthreads();
+      const T piv = s_piv[buf][jj];
+      const T inv = (piv != T(0)) ? T(1) / piv : T(0);
 #pragma unroll
-        for (int c = 0; c < JB; ++c)
-          if (c > jj) a[r][c] -= l * s_piv[buf][c];
+      for (int r = 0; r < R; ++r) {
+        const int lr = tid + r * FJ_PANEL_THREADS;
+        if (lr == jj) {
+#pragma unroll
+          for (int c = 0; c < JB; ++c) a[r][c] = s_piv[buf][c];
+        } else if (lr == p) {
+#pragma unroll
+          for (int c = 0; c < JB; ++c) a[r][c] = s_swp[buf][c];
+        }
+        if (lr > jj && lr < rows) {
+          const T l = a[r][jj] * inv;
+          a[r][jj] = l;
+#pragma unroll
+          for (int c = 0; c < JB; ++c)
+            if (c > jj) a[r][c] -= l * s_piv[buf][c];
+        }
       }
+      // the next column writes the other LDS buffers (s_key is protected by
+      // the second barrier of this column)
     }
-    // the next column writes the other LDS buffer; one barrier per column
   }
   __syncthreads();
 
   // write the panel back
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    const int lr = tid + r * FJ_LU_THREADS;
+    const int lr = tid + r * FJ_PANEL_THREADS;
     if (lr < rows) {
 #pragma unroll
       for (int c = 0; c < JB; ++c)
@@ -283,13 +283,19 @@ __global__ __launch_bounds__(256) void k_lu_update(T* __restrict__ A, int n, int
       }
 }
 
+template <typename T>
+__global__ __launch_bounds__(64) void k_diag_inverses(const T* __restrict__ LU, int n, int lda,
+                                                      T* __restrict__ aux);
+__host__ __device__ static inline int fj_nblk(int n);
+
 template <typename T, int R, int JB>
-static int fj_lu_factor_t(T* A, int n, int lda, int32_t* ipiv, int32_t* info, hipStream_t st) {
+static int fj_lu_factor_t(T* A, int n, int lda, int32_t* ipiv, int32_t* info, T* aux,
+                          hipStream_t st) {
   hipLaunchKernelGGL(k_perm_init, dim3(fj_cdiv(n, 256)), dim3(256), 0, st, ipiv, n, info);
   FJ_CHECK_LAUNCH();
   for (int k = 0; k < n; k += JB) {
     const int jb = (n - k < JB) ? n - k : JB;
-    hipLaunchKernelGGL((k_lu_panel<T, R, JB>), dim3(1), dim3(FJ_LU_THREADS), 0, st, A, n, lda, k,
+    hipLaunchKernelGGL((k_lu_panel<T, R, JB>), dim3(1), dim3(FJ_PANEL_THREADS), 0, st, A, n, lda, k,
                        ipiv, info);
     FJ_CHECK_LAUNCH();
     if (n - jb > 0) {
@@ -304,172 +310,193 @@ static int fj_lu_factor_t(T* A, int n, int lda, int32_t* ipiv, int32_t* info, hi
       FJ_CHECK_LAUNCH();
     }
   }
+  hipLaunchKernelGGL(k_diag_inverses<T>, dim3(fj_nblk(n), 2), dim3(64), 0, st, A, n, lda, aux);
+  FJ_CHECK_LAUNCH();
   return FJ_OK;
 }
 
 template <typename T>
-static int fj_lu_dispatch(T* A, int n, int lda, int32_t* ipiv, int32_t* info, hipStream_t st) {
-  if (n <= 0 || lda < n || !A || !ipiv || !info) return FJ_ERR_ARG;
-  if (n <= 1024) return fj_lu_factor_t<T, 1, 32>(A, n, lda, ipiv, info, st);
-  if (n <= 2048) return fj_lu_factor_t<T, 2, 16>(A, n, lda, ipiv, info, st);
-  if (n <= 4096) return fj_lu_factor_t<T, 4, 16>(A, n, lda, ipiv, info, st);
-  if (n <= 8192) return fj_lu_factor_t<T, 8, 4>(A, n, lda, ipiv, info, st);
-  return FJ_ERR_ARG;
+static int fj_lu_dispatch(T* A, int n, int lda, int32_t* ipiv, int32_t* info, T* aux,
+                          hipStream_t st) {
+  if (n <= 0 || n > 8192 || lda < n || !A || !ipiv || !info || !aux) return FJ_ERR_ARG;
+  if (n <= 512) return fj_lu_factor_t<T, 1, 32>(A, n, lda, ipiv, info, aux, st);
+  if (n <= 1024) return fj_lu_factor_t<T, 2, 32>(A, n, lda, ipiv, info, aux, st);
+  if (n <= 2048) return fj_lu_factor_t<T, 4, 16>(A, n, lda, ipiv, info, aux, st);
+  if (n <= 4096) return fj_lu_factor_t<T, 8, 8>(A, n, lda, ipiv, info, aux, st);
+  return fj_lu_factor_t<T, 16, 4>(A, n, lda, ipiv, info, aux, st);
 }
 
-extern "C" size_t fj_lu_workspace_bytes(int n) {
-  (void)n;
-  return 256;
+extern "C" int fj_lu_factor(double* A, int n, int lda, int32_t* ipiv, int32_t* info, void* aux,
+                            void* stream) {
+  return fj_lu_dispatch<double>(A, n, lda, ipiv, info, (double*)aux, (hipStream_t)stream);
 }
 
-extern "C" int fj_lu_factor(double* A, int n, int lda, int32_t* ipiv, int32_t* info,
-                            void* workspace, void* stream) {
-  (void)workspace;
-  return fj_lu_dispatch<double>(A, n, lda, ipiv, info, (hipStream_t)stream);
-}
-
-extern "C" int fj_lu_factor_f32(float* A, int n, int lda, int32_t* ipiv, int32_t* info,
-                                void* workspace, void* stream) {
-  (void)workspace;
-  return fj_lu_dispatch<float>(A, n, lda, ipiv, info, (hipStream_t)stream);
+extern "C" int fj_lu_factor_f32(float* A, int n, int lda, int32_t* ipiv, int32_t* info, void* aux,
+                                void* stream) {
+  return fj_lu_dispatch<float>(A, n, lda, ipiv, info, (float*)aux, (hipStream_t)stream);
 }
 
 // ---- triangular solves -------------------------------------------------------
+// The factor carries an auxiliary array: the inverses of the 64x64 diagonal
+// blocks of L (unit lower) and U (upper), written by fj_lu_factor.  A solve is
+// then a sequence of 64-row blocks, each a 64x64 mat-vec with the block
+// inverse (one wave, lanes over rows) followed by the update of the remaining
+// rows (one thread per row, a contiguous 64-element row segment each); x
+// stays in LDS.  The transposed solve uses the transposed block inverses and
+// reads the factor column-wise (coalesced across threads).
 #define FJ_TS_B 64
-
-// x (LDS, n values) : solve with L (unit lower, trans=0) then U, or with U^T
-// then L^T (trans=1).  Element accessors read the row-major factor.
-template <typename T, typename X>
-__device__ void fj_tri_forward(const T* __restrict__ LU, int n, int lda, X* x, bool unit,
-                               bool transposed) {
-  // forward substitution with a lower factor M: M = L (trans=0) or U^T (trans=1)
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  __shared__ X diag[FJ_TS_B][FJ_TS_B + 1];
-  for (int b0 = 0; b0 < n; b0 += FJ_TS_B) {
-    const int bs = min(FJ_TS_B, n - b0);
-    // stage the diagonal block M[b0+i][b0+q]
-    for (int e = tid; e < FJ_TS_B * FJ_TS_B; e += blockDim.x) {
-      const int i = e / FJ_TS_B, q = e % FJ_TS_B;
-      X v = 0;
-      if (i < bs && q < bs)
-        v = transposed ? (X)LU[(size_t)(b0 + q) * lda + b0 + i] : (X)LU[(size_t)(b0 + i) * lda + b0 + q];
-      diag[i][q] = v;
-    }
-    __syncthreads();
-    if (wave == 0) {
-      X xi = (lane < bs) ? x[b0 + lane] : X(0);
-      for (int r = 0; r < bs; ++r) {
-        X xr = __shfl(xi, r);
-        if (!unit) {
-          xr = xr / diag[r][r];
-          if (lane == r) xi = xr;
-        }
-        if (lane > r && lane < bs) xi -= diag[lane][r] * xr;
-      }
-      if (lane < bs) x[b0 + lane] = xi;
-    }
-    __syncthreads();
-    // rows below the block: x[i] -= sum_q M[i][b0+q] x[b0+q]
-    for (int i = b0 + bs + wave; i < n; i += blockDim.x / 64) {
-      X s = 0;
-      if (lane < bs) {
-        const X mv = transposed ? (X)LU[(size_t)(b0 + lane) * lda + i] : (X)LU[(size_t)i * lda + b0 + lane];
-        s = mv * x[b0 + lane];
-      }
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-      if (lane == 0) x[i] -= s;
-    }
-    __syncthreads();
-  }
-}
-
-template <typename T, typename X>
-__device__ void fj_tri_backward(const T* __restrict__ LU, int n, int lda, X* x, bool unit,
-                                bool transposed) {
-  // backward substitution with an upper factor M: M = U (trans=0) or L^T (trans=1)
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  __shared__ X diag[FJ_TS_B][FJ_TS_B + 1];
-  const int nblk = (n + FJ_TS_B - 1) / FJ_TS_B;
-  for (int bi = nblk - 1; bi >= 0; --bi) {
-    const int b0 = bi * FJ_TS_B;
-    const int bs = min(FJ_TS_B, n - b0);
-    for (int e = tid; e < FJ_TS_B * FJ_TS_B; e += blockDim.x) {
-      const int i = e / FJ_TS_B, q = e % FJ_TS_B;
-      X v = 0;
-      if (i < bs && q < bs)
-        v = transposed ? (X)LU[(size_t)(b0 + q) * lda + b0 + i] : (X)LU[(size_t)(b0 + i) * lda + b0 + q];
-      diag[i][q] = v;
-    }
-    __syncthreads();
-    if (wave == 0) {
-      X xi = (lane < bs) ? x[b0 + lane] : X(0);
-      for (int r = bs - 1; r >= 0; --r) {
-        X xr = __shfl(xi, r);
-        if (!unit) {
-          xr = xr / diag[r][r];
-          if (lane == r) xi = xr;
-        }
-        if (lane < r) xi -= diag[lane][r] * xr;
-      }
-      if (lane < bs) x[b0 + lane] = xi;
-    }
-    __syncthreads();
-    for (int i = wave; i < b0; i += blockDim.x / 64) {
-      X s = 0;
-      if (lane < bs) {
-        const X mv = transposed ? (X)LU[(size_t)(b0 + lane) * lda + i] : (X)LU[(size_t)i * lda + b0 + lane];
-        s = mv * x[b0 + lane];
-      }
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-      if (lane == 0) x[i] -= s;
-    }
-    __syncthreads();
-  }
-}
-
 #define FJ_SOLVE_MAXN 8192
 
-// b (FP64, n) -> x (FP64, n) through the factor of type T; x may alias b.
+__host__ __device__ static inline int fj_nblk(int n) { return (n + FJ_TS_B - 1) / FJ_TS_B; }
+
+extern "C" size_t fj_lu_aux_bytes(int n) {
+  return (size_t)2 * fj_nblk(n) * FJ_TS_B * FJ_TS_B * sizeof(double) + 256;
+}
+
+// aux layout (type T): Linv blocks then Uinv blocks, each (nblk, 64, 64) row-major
 template <typename T>
-__global__ __launch_bounds__(FJ_LU_THREADS) void k_lu_solve(const T* __restrict__ LU, int n,
-                                                             int lda,
-                                                             const int32_t* __restrict__ ipiv,
-                                                             const double* b, double* x,
-                                                             int trans) {
-  __shared__ double xs[FJ_SOLVE_MAXN];
+__global__ __launch_bounds__(64) void k_diag_inverses(const T* __restrict__ LU, int n, int lda,
+                                                      T* __restrict__ aux) {
+  __shared__ double sb[FJ_TS_B][FJ_TS_B + 1];
+  __shared__ double sy[FJ_TS_B][FJ_TS_B + 1];
+  const int b = blockIdx.x, which = blockIdx.y;  // 0: L, 1: U
+  const int nblk = fj_nblk(n);
+  const int b0 = b * FJ_TS_B, bs = min(FJ_TS_B, n - b0);
+  const int j = threadIdx.x;  // column of the inverse
+  for (int r = 0; r < FJ_TS_B; ++r) sb[r][j] = (r < bs && j < bs) ? (double)LU[(size_t)(b0 + r) * lda + b0 + j] : 0.0;
+  __syncthreads();
+  if (which == 0) {
+    // unit lower: y[r] = delta_rj - sum_{q<r} L[r][q] y[q]
+    for (int r = 0; r < FJ_TS_B; ++r) {
+      double acc = (r == j) ? 1.0 : 0.0;
+      for (int q = 0; q < r; ++q) acc -= sb[r][q] * sy[q][j];
+      sy[r][j] = (r < bs && j < bs) ? acc : 0.0;
+    }
+  } else {
+    for (int r = FJ_TS_B - 1; r >= 0; --r) {
+      double acc = (r == j) ? 1.0 : 0.0;
+      for (int q = r + 1; q < FJ_TS_B; ++q) acc -= sb[r][q] * sy[q][j];
+      sy[r][j] = (r < bs && j < bs) ? acc / sb[r][r] : 0.0;
+    }
+  }
+  T* o = aux + ((size_t)which * nblk + b) * FJ_TS_B * FJ_TS_B;
+  for (int r = 0; r < FJ_TS_B; ++r) o[r * FJ_TS_B + j] = (T)sy[r][j];
+}
+
+// forward sweep with a lower-triangular operator M (M = L, or U^T when trans)
+template <typename T>
+__device__ void fj_sweep_forward(const T* __restrict__ LU, const T* __restrict__ inv, int n,
+                                 int lda, double* x, bool trans) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nblk = fj_nblk(n);
+  for (int b = 0; b < nblk; ++b) {
+    const int b0 = b * FJ_TS_B, bs = min(FJ_TS_B, n - b0);
+    const T* Bi = inv + (size_t)b * FJ_TS_B * FJ_TS_B;
+    if (wave == 0) {
+      double acc = 0.0;
+      if (lane < bs)
+        for (int q = 0; q < bs; ++q)
+          acc += (double)(trans ? Bi[q * FJ_TS_B + lane] : Bi[lane * FJ_TS_B + q]) * x[b0 + q];
+      if (lane < bs) x[b0 + lane] = acc;  // all lanes read before any writes (one wave)
+    }
+    __syncthreads();
+    for (int i = b0 + bs + tid; i < n; i += blockDim.x) {
+      double acc = 0.0;
+      if (!trans) {
+        const T* row = LU + (size_t)i * lda + b0;
+        for (int q = 0; q < bs; ++q) acc += (double)row[q] * x[b0 + q];
+      } else {
+        for (int q = 0; q < bs; ++q) acc += (double)LU[(size_t)(b0 + q) * lda + i] * x[b0 + q];
+      }
+      x[i] -= acc;
+    }
+    __syncthreads();
+  }
+}
+
+// backward sweep with an upper-triangular operator M (M = U, or L^T when trans)
+template <typename T>
+__device__ void fj_sweep_backward(const T* __restrict__ LU, const T* __restrict__ inv, int n,
+                                  int lda, double* x, bool trans) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nblk = fj_nblk(n);
+  for (int b = nblk - 1; b >= 0; --b) {
+    const int b0 = b * FJ_TS_B, bs = min(FJ_TS_B, n - b0);
+    const T* Bi = inv + (size_t)b * FJ_TS_B * FJ_TS_B;
+    if (wave == 0) {
+      double acc = 0.0;
+      if (lane < bs)
+        for (int q = 0; q < bs; ++q)
+          acc += (double)(trans ? Bi[q * FJ_TS_B + lane] : Bi[lane * FJ_TS_B + q]) * x[b0 + q];
+      if (lane < bs) x[b0 + lane] = acc;
+    }
+    __syncthreads();
+    for (int i = tid; i < b0; i += blockDim.x) {
+      double acc = 0.0;
+      if (!trans) {
+        const T* row = LU + (size_t)i * lda + b0;
+        for (int q = 0; q < bs; ++q) acc += (double)row[q] * x[b0 + q];
+      } else {
+        for (int q = 0; q < bs; ++q) acc += (double)LU[(size_t)(b0 + q) * lda + i] * x[b0 + q];
+      }
+      x[i] -= acc;
+    }
+    __syncthreads();
+  }
+}
+
+// x <- solve(b) with the factor; x may alias b
+template <typename T>
+__device__ void fj_lu_solve_block(const T* __restrict__ LU, int n, int lda,
+                                  const int32_t* __restrict__ ipiv, const T* __restrict__ aux,
+                                  const double* b, double* x, int trans, double* xs) {
+  const int nblk = fj_nblk(n);
+  const T* Linv = aux;
+  const T* Uinv = aux + (size_t)nblk * FJ_TS_B * FJ_TS_B;
   const int32_t* perm = ipiv + n;
   if (!trans) {
     for (int i = threadIdx.x; i < n; i += blockDim.x) xs[i] = b[perm[i]];
     __syncthreads();
-    fj_tri_forward<T, double>(LU, n, lda, xs, true, false);
-    fj_tri_backward<T, double>(LU, n, lda, xs, false, false);
+    fj_sweep_forward<T>(LU, Linv, n, lda, xs, false);
+    fj_sweep_backward<T>(LU, Uinv, n, lda, xs, false);
     for (int i = threadIdx.x; i < n; i += blockDim.x) x[i] = xs[i];
   } else {
     for (int i = threadIdx.x; i < n; i += blockDim.x) xs[i] = b[i];
     __syncthreads();
-    fj_tri_forward<T, double>(LU, n, lda, xs, false, true);
-    fj_tri_backward<T, double>(LU, n, lda, xs, true, true);
-    __syncthreads();
+    fj_sweep_forward<T>(LU, Uinv, n, lda, xs, true);
+    fj_sweep_backward<T>(LU, Linv, n, lda, xs, true);
     for (int i = threadIdx.x; i < n; i += blockDim.x) x[perm[i]] = xs[i];
   }
+  __syncthreads();
 }
 
-extern "C" int fj_lu_solve(const double* LU, int n, int lda, const int32_t* ipiv, double* b,
-                           int nrhs, int trans, void* stream) {
-  if (n <= 0 || n > FJ_SOLVE_MAXN || lda < n || nrhs < 0 || !LU || !ipiv || !b) return FJ_ERR_ARG;
+template <typename T>
+__global__ __launch_bounds__(FJ_LU_THREADS) void k_lu_solve(const T* __restrict__ LU, int n,
+                                                             int lda,
+                                                             const int32_t* __restrict__ ipiv,
+                                                             const T* __restrict__ aux,
+                                                             const double* b, double* x,
+                                                             int trans) {
+  __shared__ double xs[FJ_SOLVE_MAXN];
+  fj_lu_solve_block<T>(LU, n, lda, ipiv, aux, b, x, trans, xs);
+}
+
+extern "C" int fj_lu_solve(const double* LU, int n, int lda, const int32_t* ipiv, const void* aux,
+                           double* b, int nrhs, int trans, void* stream) {
+  if (n <= 0 || n > FJ_SOLVE_MAXN || lda < n || nrhs < 0 || !LU || !ipiv || !aux || !b)
+    return FJ_ERR_ARG;
   for (int r = 0; r < nrhs; ++r) {
     hipLaunchKernelGGL(k_lu_solve<double>, dim3(1), dim3(FJ_LU_THREADS), 0, (hipStream_t)stream,
-                       LU, n, lda, ipiv, b + (size_t)r * n, b + (size_t)r * n, trans);
+                       LU, n, lda, ipiv, (const double*)aux, b + (size_t)r * n, b + (size_t)r * n,
+                       trans);
     FJ_CHECK_LAUNCH();
   }
   return FJ_OK;
 }
 
-// ---- FP64 iterative refinement around an FP32 factor -----------------------
-// r = b - A x (FP64); one wave per row, lanes over columns (coalesced rows).
-// flags[0]: converged, flags[1]: sweeps done; amax[0]: |r|_inf scratch.
+// ---- FP64 iterative refinement around a kept factor -------------------------
+// r = b - A x (FP64), one wave per row, lanes over columns (coalesced rows).
+// flags[0]: 0 running, 1 converged, -1 gave up; flags[1]: corrections done.
 __global__ void k_residual(const double* __restrict__ A, int lda, int n,
                            const double* __restrict__ b, const double* __restrict__ x,
                            double* __restrict__ r, const int32_t* __restrict__ flags) {
@@ -485,71 +512,38 @@ __global__ void k_residual(const double* __restrict__ A, int lda, int n,
   if (lane == 0) r[row] = b[row] - s;
 }
 
-// single workgroup: norms, convergence test, correction solve, update
+// one workgroup: correction d = LU^{-1} r, x += d; converged when
+// |d|_inf <= tol |x|_inf (the correction no longer changes x at FP64 level)
 template <typename T>
 __global__ __launch_bounds__(FJ_LU_THREADS) void k_refine_step(
-    const T* __restrict__ LU, int ldlu, const int32_t* __restrict__ ipiv, int n,
-    const double* __restrict__ A_rownorm_max, const double* __restrict__ b,
-    const double* __restrict__ r, double* __restrict__ x, double tol, int32_t* __restrict__ flags,
-    int last) {
+    const T* __restrict__ LU, int ldlu, const int32_t* __restrict__ ipiv,
+    const T* __restrict__ aux, int n, const double* __restrict__ r, double* __restrict__ x,
+    double tol, int32_t* __restrict__ flags, int last) {
   __shared__ double xs[FJ_SOLVE_MAXN];
-  __shared__ double red[3][FJ_LU_WAVES];
+  __shared__ double red[2][FJ_LU_WAVES];
   if (flags[0]) return;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  double rm = 0.0, xm = 0.0, bm = 0.0;
+  fj_lu_solve_block<T>(LU, n, ldlu, ipiv, aux, r, xs, 0, xs);
+  double dm = 0.0, xm = 0.0;
   for (int i = tid; i < n; i += blockDim.x) {
-    rm = fmax(rm, fabs(r[i]));
-    xm = fmax(xm, fabs(x[i]));
-    bm = fmax(bm, fabs(b[i]));
+    const double xn = x[i] + xs[i];
+    x[i] = xn;
+    dm = fmax(dm, fabs(xs[i]));
+    xm = fmax(xm, fabs(xn));
   }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
-    rm = fmax(rm, __shfl_xor(rm, off));
+    dm = fmax(dm, __shfl_xor(dm, off));
     xm = fmax(xm, __shfl_xor(xm, off));
-    bm = fmax(bm, __shfl_xor(bm, off));
   }
-  if (lane == 0) { red[0][wave] = rm; red[1][wave] = xm; red[2][wave] = bm; }
+  if (lane == 0) { red[0][wave] = dm; red[1][wave] = xm; }
   __syncthreads();
-  rm = red[0][0]; xm = red[1][0]; bm = red[2][0];
-  for (int w = 1; w < FJ_LU_WAVES; ++w) {
-    rm = fmax(rm, red[0][w]); xm = fmax(xm, red[1][w]); bm = fmax(bm, red[2][w]);
-  }
-  __syncthreads();
-  if (rm <= tol * (A_rownorm_max[0] * xm + bm)) {
-    if (tid == 0) flags[0] = 1;
-    return;
-  }
-  if (last) {
-    if (tid == 0) flags[0] = -1;
-    return;
-  }
-  const int32_t* perm = ipiv + n;
-  for (int i = tid; i < n; i += blockDim.x) xs[i] = r[perm[i]];
-  __syncthreads();
-  fj_tri_forward<T, double>(LU, n, ldlu, xs, true, false);
-  fj_tri_backward<T, double>(LU, n, ldlu, xs, false, false);
-  for (int i = tid; i < n; i += blockDim.x) x[i] += xs[i];
-  if (tid == 0) flags[1] += 1;
-}
-
-// |A|_inf (max absolute row sum) for the stopping test
-__global__ void k_rownorm(const double* __restrict__ A, int lda, int n, double* __restrict__ out) {
-  __shared__ double red[FJ_LU_WAVES];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  double best = 0.0;
-  for (int row = wave; row < n; row += FJ_LU_WAVES) {
-    double s = 0.0;
-    for (int c = lane; c < n; c += 64) s += fabs(A[(size_t)row * lda + c]);
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-    best = fmax(best, s);
-  }
-  if (lane == 0) red[wave] = best;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double m = red[0];
-    for (int w = 1; w < FJ_LU_WAVES; ++w) m = fmax(m, red[w]);
-    out[0] = m;
+  if (tid == 0) {
+    dm = red[0][0]; xm = red[1][0];
+    for (int w = 1; w < FJ_LU_WAVES; ++w) { dm = fmax(dm, red[0][w]); xm = fmax(xm, red[1][w]); }
+    flags[1] += 1;
+    if (dm <= tol * xm) flags[0] = 1;
+    else if (last) flags[0] = -1;
   }
 }
 
@@ -559,7 +553,17 @@ __global__ void k_refine_init(int32_t* flags) {
 }
 
 __global__ void k_refine_finish(const int32_t* flags, int32_t* iters) {
-  iters[0] = (flags[0] == 1) ? flags[1] : -flags[1] - 1;
+  iters[0] = (flags[0] == 1) ? flags[1] : -flags[1];
+}
+
+template <typename T>
+__global__ __launch_bounds__(FJ_LU_THREADS) void k_first_solve(const T* __restrict__ LU, int n,
+                                                                int lda,
+                                                                const int32_t* __restrict__ ipiv,
+                                                                const T* __restrict__ aux,
+                                                                const double* b, double* x) {
+  __shared__ double xs[FJ_SOLVE_MAXN];
+  fj_lu_solve_block<T>(LU, n, lda, ipiv, aux, b, x, 0, xs);
 }
 
 extern "C" size_t fj_refine_workspace_bytes(int n) {
@@ -568,27 +572,23 @@ extern "C" size_t fj_refine_workspace_bytes(int n) {
 
 template <typename T>
 static int fj_solve_refined_t(const double* A, int lda, const T* LU, int ldlu,
-                              const int32_t* ipiv, int n, const double* b, double* x, int max_iter,
-                              double tol, int32_t* iters, void* workspace, int x_is_guess,
-                              hipStream_t st) {
-  if (n <= 0 || n > FJ_SOLVE_MAXN || lda < n || ldlu < n || !A || !LU || !ipiv || !b || !x ||
-      !iters || !workspace || max_iter < 0)
+                              const int32_t* ipiv, const T* aux, int n, const double* b,
+                              double* x, int max_iter, double tol, int32_t* iters,
+                              void* workspace, hipStream_t st) {
+  if (n <= 0 || n > FJ_SOLVE_MAXN || lda < n || ldlu < n || !A || !LU || !ipiv || !aux || !b ||
+      !x || !iters || !workspace || max_iter < 1)
     return FJ_ERR_ARG;
   double* r = (double*)workspace;
-  double* anorm = r + n;
-  int32_t* flags = (int32_t*)(anorm + 4);
+  int32_t* flags = (int32_t*)(r + n + 4);
   hipLaunchKernelGGL(k_refine_init, dim3(1), dim3(1), 0, st, flags);
-  hipLaunchKernelGGL(k_rownorm, dim3(1), dim3(FJ_LU_THREADS), 0, st, A, lda, n, anorm);
-  // x0 = LU^{-1} b, unless the caller's x is the starting guess
-  if (!x_is_guess)
-    hipLaunchKernelGGL(k_lu_solve<T>, dim3(1), dim3(FJ_LU_THREADS), 0, st, LU, n, ldlu, ipiv, b, x,
-                       0);
+  hipLaunchKernelGGL(k_first_solve<T>, dim3(1), dim3(FJ_LU_THREADS), 0, st, LU, n, ldlu, ipiv, aux,
+                     b, x);
   FJ_CHECK_LAUNCH();
-  for (int it = 0; it <= max_iter; ++it) {
+  for (int it = 0; it < max_iter; ++it) {
     hipLaunchKernelGGL(k_residual, dim3(fj_cdiv(n, 4)), dim3(256), 0, st, A, lda, n, b, x, r,
                        flags);
-    hipLaunchKernelGGL(k_refine_step<T>, dim3(1), dim3(FJ_LU_THREADS), 0, st, LU, ldlu, ipiv, n,
-                       anorm, b, r, x, tol, flags, it == max_iter ? 1 : 0);
+    hipLaunchKernelGGL(k_refine_step<T>, dim3(1), dim3(FJ_LU_THREADS), 0, st, LU, ldlu, ipiv, aux,
+                       n, r, x, tol, flags, it == max_iter - 1 ? 1 : 0);
     FJ_CHECK_LAUNCH();
   }
   hipLaunchKernelGGL(k_refine_finish, dim3(1), dim3(1), 0, st, flags, iters);
@@ -597,17 +597,17 @@ static int fj_solve_refined_t(const double* A, int lda, const T* LU, int ldlu,
 }
 
 extern "C" int fj_solve_refined(const double* A, int lda, const float* LU32, int ldlu,
-                                const int32_t* ipiv, int n, const double* b, double* x,
-                                int max_iter, double tol, int32_t* iters, void* workspace,
-                                void* stream) {
-  return fj_solve_refined_t<float>(A, lda, LU32, ldlu, ipiv, n, b, x, max_iter, tol, iters,
-                                   workspace, 0, (hipStream_t)stream);
+                                const int32_t* ipiv, const void* aux, int n, const double* b,
+                                double* x, int max_iter, double tol, int32_t* iters,
+                                void* workspace, void* stream) {
+  return fj_solve_refined_t<float>(A, lda, LU32, ldlu, ipiv, (const float*)aux, n, b, x, max_iter,
+                                   tol, iters, workspace, (hipStream_t)stream);
 }
 
 extern "C" int fj_solve_refined_f64(const double* A, int lda, const double* LU, int ldlu,
-                                    const int32_t* ipiv, int n, const double* b, double* x,
-                                    int max_iter, double tol, int32_t* iters, void* workspace,
-                                    void* stream) {
-  return fj_solve_refined_t<double>(A, lda, LU, ldlu, ipiv, n, b, x, max_iter, tol, iters,
-                                    workspace, 0, (hipStream_t)stream);
+                                    const int32_t* ipiv, const void* aux, int n, const double* b,
+                                    double* x, int max_iter, double tol, int32_t* iters,
+                                    void* workspace, void* stream) {
+  return fj_solve_refined_t<double>(A, lda, LU, ldlu, ipiv, (const double*)aux, n, b, x,
+                                    max_iter, tol, iters, workspace, (hipStream_t)stream);
 }

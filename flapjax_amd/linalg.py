@@ -20,21 +20,17 @@ from .utils import to_device
 
 
 def lu_factor(A):
-    """(LU, piv): in-place style FP64 factor of a copy of A."""
-    LU, piv, info = K.lu_factor(to_device(A))
-    return LU, piv
+    """FP64 factor of a copy of A (K.Factor: lu, piv, aux, info)."""
+    return K.lu_factor(to_device(A))
 
 
-def lu_solve(lu_piv, b, trans=False):
-    LU, piv = lu_piv
-    return K.lu_solve(LU, piv, to_device(b), trans=trans)
+def lu_solve(fac, b, trans=False):
+    return K.lu_solve(fac, to_device(b), trans=trans)
 
 
 def solve(A, b):
     """x = A^{-1} b in FP64 on the device."""
-    A = to_device(A)
-    LU, piv, info = K.lu_factor(A)
-    return K.lu_solve(LU, piv, to_device(b))
+    return K.lu_solve(K.lu_factor(to_device(A)), to_device(b))
 
 
 class RefinedSolver:
@@ -46,30 +42,29 @@ class RefinedSolver:
         self.tol = tol
         self.reuse = reuse
         self.factor_dtype = factor_dtype
-        self._lu32 = None
-        self._piv = None
+        self._fac = None
         self.factorisations = 0
         self.last_iters = None
 
     def factor(self, A):
         if self.factor_dtype == torch.float32:
-            self._lu32, self._piv, _ = K.lu_factor_f32(A.to(torch.float32))
+            self._fac = K.lu_factor_f32(A.to(torch.float32))
         else:
-            self._lu32, self._piv, _ = K.lu_factor(A)
+            self._fac = K.lu_factor(A)
         self.factorisations += 1
 
     def solve(self, A, b, check: bool = True):
         A = to_device(A)
         b = to_device(b)
-        if self._lu32 is None or not self.reuse or self._lu32.shape != A.shape:
+        if self._fac is None or not self.reuse or self._fac.lu.shape != A.shape:
             self.factor(A)
-        x, iters = K.solve_refined(A, self._lu32, self._piv, b, self.max_iter, self.tol)
+        x, iters = K.solve_refined(A, self._fac, b, self.max_iter, self.tol)
         if check:
             it = int(iters.item())
             if it < 0 and self.reuse:
                 # the stored factor is too far from A: refactor and retry once
                 self.factor(A)
-                x, iters = K.solve_refined(A, self._lu32, self._piv, b, self.max_iter, self.tol)
+                x, iters = K.solve_refined(A, self._fac, b, self.max_iter, self.tol)
                 it = int(iters.item())
             if it < 0:
                 raise L.FlapjaxLibError(f"refinement did not converge in {self.max_iter} sweeps")

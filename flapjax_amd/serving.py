@@ -151,17 +151,16 @@ class _Slot:
         c, nc, _ = K.colloc_normals(self.zb)
         K.aic_assemble(c, nc, self.zb, kernel_kind, EPSILON, R_CUTOFF, self.A, 0, zeta_w=self.zw)
         rhs = K.rhs(c, nc, None, None, flow, 0.0)
-        LU, piv, _ = K.lu_factor(self.A)
-        self.gb = K.lu_solve(LU, piv, rhs).reshape(m, n).contiguous()
+        self.gb = K.lu_solve(K.lu_factor(self.A), rhs).reshape(m, n).contiguous()
         K.static_wake_gamma(self.gb, ms, out=self.gw)
         self.gb_prev = self.gb.clone()
         # the factor kept for the dynamic (bound-only) system
         c, nc, _ = K.colloc_normals(self.zb)
         K.aic_assemble(c, nc, self.zb, kernel_kind, EPSILON, R_CUTOFF, self.A, 0)
         if precision == "mixed":
-            self.lu, self.piv, _ = K.lu_factor_f32(self.A.to(torch.float32))
+            self.fac = K.lu_factor_f32(self.A.to(torch.float32))
         else:
-            self.lu, self.piv, _ = K.lu_factor(self.A)
+            self.fac = K.lu_factor(self.A)
         self.ws = torch.empty(L.lib().fj_refine_workspace_bytes(m * n) // 8 + 1, dtype=F64, device=dev)
         self.iters = torch.zeros(1, dtype=torch.int32, device=dev)
 
@@ -283,8 +282,9 @@ class ServingEngine:
             off += nb
             x = sl.gb.reshape(-1)
             fn = L.lib().fj_solve_refined if self.precision == "mixed" else L.lib().fj_solve_refined_f64
-            rc = fn(L.ptr(sl.A), nb, L.ptr(sl.lu), nb, L.ptr(sl.piv), nb, L.ptr(rhs), L.ptr(x), 10,
-                    2.0 * np.finfo(float).eps, L.ptr(sl.iters), L.ptr(sl.ws), L.stream())
+            fac = sl.fac
+            rc = fn(L.ptr(sl.A), nb, L.ptr(fac.lu), nb, L.ptr(fac.piv), L.ptr(fac.aux), nb, L.ptr(rhs), L.ptr(x), 10,
+                    4.0 * np.finfo(float).eps, L.ptr(sl.iters), L.ptr(sl.ws), L.stream())
             L.check(rc, "fj_solve_refined")
         # (3) forces: velocities at the filament midpoints (fused)
         seg_parts, tgt_parts, tasks = [], [], []

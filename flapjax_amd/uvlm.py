@@ -17,11 +17,13 @@ Per dynamic step (aero/uvlm.py:912-1103):
      alpha, circulation rate and unsteady force (K7).
 
 Arithmetic forms (``precision``):
-  * "fp64"  - the plain drop-in: FP64 everywhere, AIC re-assembled and
-              re-factored every step as the reference does (with
-              ``aic_reuse=True`` the FP64 factor is kept across steps and
-              every step's system is solved by FP64 refinement against the
-              freshly assembled AIC instead);
+  * "fp64"  - the plain drop-in: FP64 everywhere.  The AIC is re-assembled
+              every step as the reference does; by default (``aic_reuse``)
+              the FP64 LU factor is kept across steps and each step's system
+              is solved by FP64 iterative refinement against the freshly
+              assembled AIC (refactoring when refinement stops converging),
+              which gives the solution of that step's system to FP64
+              accuracy; ``aic_reuse=False`` re-factors every step literally;
   * "mixed" - FP32 far field / FP64 near field for the rollup and force pair
               sums, FP32 LU with FP64 iterative refinement against the FP64
               AIC (residual b - A x in FP64).  The AIC and the wake upwash in
@@ -105,7 +107,7 @@ class UVLM:
         self.batch_size = batch_size
         self.precision = precision
         self._prec = PRECISIONS[precision]
-        self.aic_reuse = (precision == "mixed") if aic_reuse is None else bool(aic_reuse)
+        self.aic_reuse = True if aic_reuse is None else bool(aic_reuse)
         if mirror_point is None:
             self.mirror = None
             self.mirror_point = self.mirror_normal = None
@@ -269,8 +271,7 @@ class UVLM:
 
     def _solve(self, A, rhs):
         if self._solver is None:
-            LU, piv, info = K.lu_factor(A, overwrite=True)
-            return K.lu_solve(LU, piv, rhs)
+            return K.lu_solve(K.lu_factor(A, overwrite=True), rhs)
         # convergence of the reused FP32 factor is checked on the host (one
         # sync) after every refactorisation and every 16th solve after that
         self._n_solves = getattr(self, "_n_solves", 0) + 1

@@ -155,41 +155,46 @@ int fj_aic_assemble(const double* colloc, const double* normals, int n_tgt, cons
 /* ---- K3: circulation solve -------------------------------------------------
  * Replaces jnp.linalg.solve(aic, -v_bc_n) at aero/uvlm.py:985 (and :1634). */
 
-/* Bytes of workspace fj_lu_factor needs for an n x n matrix. */
-size_t fj_lu_workspace_bytes(int n);
+/* Bytes of the auxiliary array that travels with a factor of order n: the
+ * inverses of its 64x64 diagonal blocks (L unit-lower and U), used by every
+ * solve with that factor. */
+size_t fj_lu_aux_bytes(int n);
 
 /* In-place LU with partial pivoting of a row-major n x n FP64 matrix
- * (P A = L U, unit-lower L).  ipiv (device, n int32) receives 0-based pivot
- * rows in LAPACK order.  info (device, 1 int32) receives 0 or 1 + the first
- * zero-pivot column. */
-int fj_lu_factor(double* A, int n, int lda, int32_t* ipiv, int32_t* info, void* workspace,
+ * (P A = L U, unit-lower L), n <= 8192.  ipiv (device, 2n int32) receives the
+ * 0-based pivot rows in LAPACK order, then the composed row permutation
+ * (row i of P A is row ipiv[n + i] of A).  info (device, 1 int32) receives 0
+ * or 1 + the first column without a non-zero pivot.  aux (device,
+ * fj_lu_aux_bytes(n)) receives the diagonal-block inverses. */
+int fj_lu_factor(double* A, int n, int lda, int32_t* ipiv, int32_t* info, void* aux,
                  void* stream);
 
 /* Solve (trans = 0) A x = b or (trans = 1) A^T x = b in place in b (n, nrhs
  * column-major: right-hand side r at b + r*n) from fj_lu_factor's output. */
-int fj_lu_solve(const double* LU, int n, int lda, const int32_t* ipiv, double* b, int nrhs,
-                int trans, void* stream);
+int fj_lu_solve(const double* LU, int n, int lda, const int32_t* ipiv, const void* aux,
+                double* b, int nrhs, int trans, void* stream);
 
-/* FP32 LU factor (row-major, in place) for the refined form. */
-int fj_lu_factor_f32(float* A, int n, int lda, int32_t* ipiv, int32_t* info, void* workspace,
+/* FP32 LU factor (row-major, in place; aux holds float block inverses). */
+int fj_lu_factor_f32(float* A, int n, int lda, int32_t* ipiv, int32_t* info, void* aux,
                      void* stream);
 
-/* x <- solve(A, b) by FP32 LU (LU32, ipiv from fj_lu_factor_f32) refined in
- * FP64: r = b - A x (FP64 A), x += LU32^{-1} r, for at most max_iter sweeps or
- * until |r|_inf <= tol * (|A|_inf |x|_inf + |b|_inf).  iters (device int32)
- * receives the sweeps done (negative if not converged).  workspace:
+/* x <- solve(A, b) from the FP32 factor LU32 (ipiv, aux from fj_lu_factor_f32)
+ * of A or of a nearby matrix, refined in FP64: r = b - A x with the FP64 A,
+ * x += LU32^{-1} r, until the correction is <= tol |x|_inf or max_iter
+ * corrections were made.  iters (device int32) receives the corrections made,
+ * negated if the tolerance was not reached.  workspace:
  * fj_refine_workspace_bytes(n). */
 size_t fj_refine_workspace_bytes(int n);
 int fj_solve_refined(const double* A, int lda, const float* LU32, int ldlu, const int32_t* ipiv,
-                     int n, const double* b, double* x, int max_iter, double tol,
-                     int32_t* iters, void* workspace, void* stream);
+                     const void* aux, int n, const double* b, double* x, int max_iter,
+                     double tol, int32_t* iters, void* workspace, void* stream);
 
 /* Same with an FP64 factor of a nearby matrix (a factor kept across time
  * steps while the AIC changes only by rounding, e.g. rigid motion): the
  * result solves the CURRENT A to FP64 accuracy. */
 int fj_solve_refined_f64(const double* A, int lda, const double* LU, int ldlu,
-                         const int32_t* ipiv, int n, const double* b, double* x, int max_iter,
-                         double tol, int32_t* iters, void* workspace, void* stream);
+                         const int32_t* ipiv, const void* aux, int n, const double* b, double* x,
+                         int max_iter, double tol, int32_t* iters, void* workspace, void* stream);
 
 /* ---- K4-K8: stencils, convection, forces ------------------------------- */
 

@@ -64,16 +64,17 @@ def test_lu_solve_random(dev, n):
     rng = np.random.default_rng(n)
     A = rng.normal(size=(n, n)) + np.eye(n) * 0.5
     b = rng.normal(size=n)
-    LU, piv, info = K.lu_factor(_t(A, dev))
-    assert int(info.item()) == 0
-    x = K.lu_solve(LU, piv, _t(b, dev)).cpu().numpy()
+    fac = K.lu_factor(_t(A, dev))
+    LU, piv = fac.lu, fac.piv
+    assert int(fac.info.item()) == 0
+    x = K.lu_solve(fac, _t(b, dev)).cpu().numpy()
     ref = np.linalg.solve(A, b)
     # backward error of a stable LU, and agreement with LAPACK
     res = np.linalg.norm(A @ x - b, np.inf) / (np.linalg.norm(A, np.inf) * np.linalg.norm(x, np.inf))
     assert res < 50 * n * np.finfo(float).eps
     assert rel_err(x, ref) < 1e-9
     # the transposed solve (adjoint)
-    xt = K.lu_solve(LU, piv, _t(b, dev), trans=True).cpu().numpy()
+    xt = K.lu_solve(fac, _t(b, dev), trans=True).cpu().numpy()
     assert rel_err(xt, np.linalg.solve(A.T, b)) < 1e-9
     # the permutation and factors reproduce P A = L U
     if n <= 257:
@@ -90,7 +91,8 @@ def test_lu_pivots_match_lapack_order(dev):
 
     rng = np.random.default_rng(7)
     A = rng.normal(size=(96, 96))
-    LU, piv, _ = K.lu_factor(_t(A, dev))
+    fac = K.lu_factor(_t(A, dev))
+    LU, piv = fac.lu, fac.piv
     lu_ref, piv_ref = scipy.linalg.lu_factor(A)
     assert np.array_equal(piv.cpu().numpy()[:96], piv_ref)
     assert rel_err(LU.cpu().numpy(), lu_ref) < 1e-12
@@ -100,8 +102,7 @@ def test_singular_reports_info(dev):
     from flapjax_amd import _kernels as K
 
     A = np.ones((40, 40))
-    _, _, info = K.lu_factor(_t(A, dev))
-    assert int(info.item()) == 2
+    assert int(K.lu_factor(_t(A, dev)).info.item()) == 2
 
 
 @pytest.mark.parametrize("n", [256, 1024])
@@ -113,7 +114,7 @@ def test_refined_solver_on_aic(dev, n):
     m = 8 if n == 256 else 16
     zb, zw = _wing(m, n // m, 20)
     c, nc = O.compute_surf_c(zb), O.compute_surf_nc(zb)
-    A = compute_aic_solve([_t(c, dev)], [_t(nc, dev)], [_t(zb, dev)], [_t(zw, dev)], None, None)
+    A = compute_aic_solve([_t(c, dev)], [_t(nc, dev)], [_t(zb, dev)], [_t(zw, dev)], None, [None])
     b = _t(-(np.array([1.0, 0.0, 0.0]) * nc).sum(-1).ravel(), dev)
     x64 = solve(A, b).cpu().numpy()
     rs = RefinedSolver()
@@ -121,3 +122,22 @@ def test_refined_solver_on_aic(dev, n):
     print(f"refined solve n={n}: sweeps={rs.last_iters}, rel err vs FP64 LU={rel_err(xr, x64):.2e}")
     assert rel_err(xr, x64) < 1e-12
     assert rs.last_iters >= 1
+
+
+@pytest.mark.parametrize("n", [40, 300, 1024])
+def test_fp32_factor(dev, n):
+    """FP32 LU on the matrix cores: P A = L U to FP32 accuracy, and the FP64
+    refinement from it reaches the FP64 solution."""
+    from flapjax_amd import _kernels as K
+
+    rng = np.random.default_rng(n + 1)
+    A = rng.normal(size=(n, n)) + np.eye(n) * 2
+    b = rng.normal(size=n)
+    fac = K.lu_factor_f32(torch.as_tensor(A, dtype=torch.float32, device=dev))
+    LUh = fac.lu.cpu().numpy().astype(float)
+    perm = fac.piv.cpu().numpy()[n:]
+    Lm = np.tril(LUh, -1) + np.eye(n)
+    assert rel_err(Lm @ np.triu(LUh), A[perm]) < 1e-5
+    x, it = K.solve_refined(_t(A, dev), fac, _t(b, dev), max_iter=20)
+    assert int(it.item()) > 0
+    assert rel_err(x.cpu().numpy(), np.linalg.solve(A, b)) < 1e-12
