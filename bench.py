@@ -171,6 +171,36 @@ def rollup_roofline(reps=20):
     return res, (tgt.cpu().numpy(), seg.cpu().numpy())
 
 
+def profile_rollup(reps=10, copy_mib=256):
+    """Counter-collection workload: the rollup pair-sum launch alone, ``reps``
+    times per form, then ``reps`` plain device copies of ``copy_mib`` MiB (the
+    scale for the HBM byte counters)."""
+    import torch
+
+    roof, _ = rollup_roofline(reps)
+    src = torch.ones(copy_mib << 17, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"rollup_ms": roof["fp64"]["ms"], "rollup_mixed_ms": roof["mixed"]["ms"],
+                      "pairs": roof["fp64"]["pairs"], "copy_bytes": src.numel() * 8, "reps": reps}))
+
+
+def serve(precision, n_cases, rate, m_star_cap, steps_range):
+    """Served mixed-size stream (continuous batching), one GPU."""
+    import torch
+
+    from flapjax_amd.serving import ServingEngine, poisson_stream
+
+    reqs = poisson_stream(n_cases, rate, seed=2, m_star=m_star_cap, steps_range=steps_range)
+    eng = ServingEngine(precision=precision, max_slots=64)
+    eng.run(poisson_stream(4, 1e6, seed=5, m_star=8, steps_range=(5, 8)))  # warm-up
+    torch.cuda.synchronize()
+    _, stats = eng.run(reqs)
+    return stats
+
+
 def cpu_baseline(sample, min_seconds=10.0):
     """The oracle's C restatement of the FP64 pair sum, all host threads."""
     from oracle import c_oracle
@@ -198,6 +228,10 @@ def main():
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve-cases", type=int, default=48)
+    ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--profile-rollup", action="store_true",
+                    help="only the rollup launch and a copy of known size (for counter runs)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the UVLM path here runs on a single MI355X "
@@ -208,11 +242,21 @@ def main():
     if not torch.cuda.is_available():
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if args.profile_rollup:
+        profile_rollup()
+        return
     pairs, rollup_pairs = pairs_per_step()
     f64 = time_form("fp64", args.steps, args.warmup)
     mix = time_form("mixed", args.steps, args.warmup)
     lit = time_form("fp64", max(3, args.steps // 4), 1, aic_reuse=False, whole_case=False)
     roof, sample = rollup_roofline()
+    srv = {}
+    if args.serve_cases > 0:
+        for prec in ("fp64", "mixed"):
+            st = serve(prec, args.serve_cases, args.serve_rate, 32, (50, 100))
+            srv[prec] = {"cases_per_s": round(st["cases_per_s"], 3), "p50_s": round(st["p50_s"], 4),
+                         "p99_s": round(st["p99_s"], 4), "gpairs_per_s": round(st["gpairs_per_s"], 3),
+                         "makespan_s": round(st["makespan_s"], 3)}
     cpu = None if args.no_cpu else cpu_baseline(sample)
     value = pairs * args.steps / f64["wall_s"] / 1e9
     value_mixed = pairs * args.steps / mix["wall_s"] / 1e9
@@ -245,6 +289,9 @@ def main():
                      "kernel": "k_pair_partial_f64 + k_pair_reduce (rollup pair sum)",
                      "launch_ms": round(roof["fp64"]["ms"], 4), "pairs_per_launch": roof["fp64"]["pairs"],
                      "mixed_launch_ms": round(roof["mixed"]["ms"], 4)},
+        "serving": {"stream": f"{args.serve_cases} Poisson arrivals at {args.serve_rate}/s, M in {{4,8,16,32}}, "
+                              "N in {16,32,64,128}, 50-100 steps, wake rows = min(steps, 32), pitch 0-5 deg, "
+                              "k 0.05-0.5", **srv},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

@@ -200,23 +200,55 @@ def lu_solve(fac, b, trans=False):
     return x.t().contiguous() if b.dim() == 2 else x.reshape(n)
 
 
-def solve_refined(A, fac, b, max_iter=8, tol=None, out=None):
-    """FP64-refined solution of A x = b from the factor (FP32 or FP64) of A or
-    of a nearby matrix.  Returns (x, iters tensor: corrections made, negative
-    if the tolerance was not reached)."""
+def _refine_ws(n, device):
+    return torch.empty(L.lib().fj_refine_workspace_bytes(n) // 8 + 1, dtype=F64, device=device)
+
+
+def solve_refined(A, fac, b, max_iter=8, tol=None, out=None, workspace=None, iters=None):
+    """FP64-refined solution of A x = b from the LU factor (FP32 or FP64) of A
+    or of a nearby matrix.  Returns (x, iters tensor: corrections made,
+    negative if the stop test ||r|| <= tol ||A|| ||x|| never passed).  tol None
+    is dsgesv's sqrt(n) eps."""
     A = _c64(A, None, "A")
     LU, piv, aux = fac[0], fac[1], fac[2]
     n = A.shape[0]
     b = _c64(b, None, "b").reshape(n)
-    if tol is None:
-        tol = 4.0 * np.finfo(np.float64).eps
     x = torch.empty_like(b) if out is None else out
-    iters = torch.zeros(1, dtype=torch.int32, device=A.device)
-    ws = torch.empty(L.lib().fj_refine_workspace_bytes(n) // 8 + 1, dtype=F64, device=A.device)
+    iters = torch.zeros(1, dtype=torch.int32, device=A.device) if iters is None else iters
+    ws = _refine_ws(n, A.device) if workspace is None else workspace
     fn = L.lib().fj_solve_refined if LU.dtype == torch.float32 else L.lib().fj_solve_refined_f64
-    rc = fn(L.ptr(A), n, L.ptr(LU), n, L.ptr(piv), L.ptr(aux), n, L.ptr(b), L.ptr(x), int(max_iter), float(tol),
-            L.ptr(iters), L.ptr(ws), L.stream())
+    rc = fn(L.ptr(A), n, L.ptr(LU), n, L.ptr(piv), L.ptr(aux), n, L.ptr(b), L.ptr(x), int(max_iter),
+            float(tol or 0.0), L.ptr(iters), L.ptr(ws), L.stream())
     L.check(rc, "fj_solve_refined")
+    return x, iters
+
+
+def lu_inverse(fac):
+    """Explicit inverse (same dtype as the factor) from a Factor."""
+    LU, piv, aux = fac[0], fac[1], fac[2]
+    n = LU.shape[0]
+    X = torch.empty_like(LU)
+    fn = L.lib().fj_lu_inverse_f32 if LU.dtype == torch.float32 else L.lib().fj_lu_inverse
+    L.check(fn(L.ptr(LU), n, n, L.ptr(piv), L.ptr(aux), L.ptr(X), n, L.stream()), "fj_lu_inverse")
+    return X
+
+
+def solve_refined_inv(A, Ainv, b, max_iter=4, tol=None, out=None, workspace=None, iters=None):
+    """FP64-refined solution of A x = b with M = Ainv (FP64 or FP32), an
+    inverse of A or of a nearby matrix; same return and stop test as
+    solve_refined."""
+    A = _c64(A, None, "A")
+    n = A.shape[0]
+    if Ainv.shape != (n, n) or not Ainv.is_contiguous():
+        raise ValueError("Ainv must be a contiguous (n, n) tensor")
+    b = _c64(b, None, "b").reshape(n)
+    x = torch.empty_like(b) if out is None else out
+    iters = torch.zeros(1, dtype=torch.int32, device=A.device) if iters is None else iters
+    ws = _refine_ws(n, A.device) if workspace is None else workspace
+    fn = L.lib().fj_solve_refined_inv_f32 if Ainv.dtype == torch.float32 else L.lib().fj_solve_refined_inv
+    rc = fn(L.ptr(A), n, L.ptr(Ainv), n, n, L.ptr(b), L.ptr(x), int(max_iter), float(tol or 0.0), L.ptr(iters),
+            L.ptr(ws), L.stream())
+    L.check(rc, "fj_solve_refined_inv")
     return x, iters
 
 

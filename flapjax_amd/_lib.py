@@ -72,6 +72,12 @@ _SIGS = {
                           c_double, c_void_p, c_void_p, c_void_p], c_int),
     "fj_solve_refined_f64": ([c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p,
                               c_int, c_double, c_void_p, c_void_p, c_void_p], c_int),
+    "fj_lu_inverse": ([c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p], c_int),
+    "fj_lu_inverse_f32": ([c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p], c_int),
+    "fj_solve_refined_inv": ([c_void_p, c_int, c_void_p, c_int, c_int, c_void_p, c_void_p, c_int, c_double,
+                              c_void_p, c_void_p, c_void_p], c_int),
+    "fj_solve_refined_inv_f32": ([c_void_p, c_int, c_void_p, c_int, c_int, c_void_p, c_void_p, c_int, c_double,
+                                  c_void_p, c_void_p, c_void_p], c_int),
     "fj_grid_from_hg": ([c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p], c_int),
     "fj_colloc_normals": ([c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p], c_int),
     "fj_filament_midpoints": ([c_void_p, c_int, c_int, c_void_p, c_void_p], c_int),

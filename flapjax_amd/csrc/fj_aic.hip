@@ -20,6 +20,7 @@ __device__ __forceinline__ double fj_proj(double x, double y, double z, const do
   const double r0sq = s[6] * s[6] + s[7] * s[7] + s[8] * s[8];
   s[9] = (KIND == FJ_KERNEL_CUTOFF) ? 1.0 / r0sq : kp.eps * r0sq * r0sq;
   s[10] = FJ_INV_4PI;
+  s[11] = r0sq;
   double vx, vy, vz;
   fj_bs_eval<KIND>(x, y, z, s, kp, vx, vy, vz);
   return vx * nx + vy * ny + vz * nz;

@@ -89,40 +89,63 @@ __global__ __launch_bounds__(FJ_BS_THREADS) void k_batched_partial(
   }
 }
 
-__global__ void k_batched_reduce(const FjTask* __restrict__ tasks, const int* __restrict__ counts,
-                                 const double* __restrict__ part, const double* __restrict__ tgt,
-                                 double* __restrict__ out) {
+// one workgroup per 64 packed targets (a block may straddle tasks: each lane
+// follows its own task); wave w sums chunks w, w + FJ_BRED_WAVES, ... and
+// wave 0 adds the wave sums in order (deterministic for given sizes)
+#define FJ_BRED_WAVES 8
+__global__ __launch_bounds__(64 * FJ_BRED_WAVES) void k_batched_reduce(
+    const FjTask* __restrict__ tasks, const int* __restrict__ counts,
+    const double* __restrict__ part, const double* __restrict__ tgt, double* __restrict__ out) {
+  __shared__ double red[FJ_BRED_WAVES - 1][3][64];
   const int n_tasks = counts[0];
   const int n_tgt_total = counts[2];
-  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < n_tgt_total; g += gridDim.x * blockDim.x) {
-    // task owning packed target g (tasks sorted by tgt_off)
-    int lo = 0, hi = n_tasks - 1;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = blockIdx.x * 64 + lane;
+  // task owning packed target g (tasks sorted by tgt_off)
+  int lo = 0;
+  if (g < n_tgt_total) {
+    int hi = n_tasks - 1;
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
       if (tasks[mid].tgt_off <= g) lo = mid; else hi = mid - 1;
     }
-    const FjTask& T = tasks[lo];
-    const int t = g - T.tgt_off;
-    if (t >= T.n_tgt) continue;
-    double vx = 0.0, vy = 0.0, vz = 0.0;
-    for (int c = 0; c < T.chunks; ++c) {
+  }
+  const FjTask& T = tasks[lo];
+  const int t = g - T.tgt_off;
+  const bool live = g < n_tgt_total && t < T.n_tgt;
+  double vx = 0.0, vy = 0.0, vz = 0.0;
+  if (live) {
+    for (int c = wave; c < T.chunks; c += FJ_BRED_WAVES) {
       const double* o = part + T.part_off + (size_t)c * 3 * T.ld;
       vx += o[t]; vy += o[T.ld + t]; vz += o[2 * T.ld + t];
     }
-    vx *= FJ_INV_4PI; vy *= FJ_INV_4PI; vz *= FJ_INV_4PI;
-    double* op = out + 3 * (size_t)(T.out_off + t);
-    if (T.mode == FJ_OUT_STORE) {
-      op[0] = vx; op[1] = vy; op[2] = vz;
-    } else if (T.mode == FJ_OUT_ADD) {
-      op[0] += vx; op[1] += vy; op[2] += vz;
-    } else {
-      const double* xp = tgt + 3 * (size_t)g;
-      double u[3];
-      fj_flow_eval(T.flow, xp[0], xp[1], xp[2], T.time, u);
-      op[0] = xp[0] + T.dt * (u[0] + vx);
-      op[1] = xp[1] + T.dt * (u[1] + vy);
-      op[2] = xp[2] + T.dt * (u[2] + vz);
-    }
+  }
+  if (wave > 0) {
+    red[wave - 1][0][lane] = vx;
+    red[wave - 1][1][lane] = vy;
+    red[wave - 1][2][lane] = vz;
+  }
+  __syncthreads();
+  if (wave != 0 || !live) return;
+#pragma unroll
+  for (int w = 0; w < FJ_BRED_WAVES - 1; ++w) {
+    vx += red[w][0][lane];
+    vy += red[w][1][lane];
+    vz += red[w][2][lane];
+  }
+  vx *= FJ_INV_4PI; vy *= FJ_INV_4PI; vz *= FJ_INV_4PI;
+  double* op = out + 3 * (size_t)(T.out_off + t);
+  if (T.mode == FJ_OUT_STORE) {
+    op[0] = vx; op[1] = vy; op[2] = vz;
+  } else if (T.mode == FJ_OUT_ADD) {
+    op[0] += vx; op[1] += vy; op[2] += vz;
+  } else {
+    const double* xp = tgt + 3 * (size_t)g;
+    double u[3];
+    fj_flow_eval(T.flow, xp[0], xp[1], xp[2], T.time, u);
+    op[0] = xp[0] + T.dt * (u[0] + vx);
+    op[1] = xp[1] + T.dt * (u[1] + vy);
+    op[2] = xp[2] + T.dt * (u[2] + vz);
   }
 }
 
@@ -208,7 +231,8 @@ extern "C" int fj_pair_sum_batched(int n_tasks, const int* tgt_off, const int* n
                          dtasks, dcounts, seg, targets, (double*)workspace, kp);
     FJ_CHECK_LAUNCH();
   }
-  hipLaunchKernelGGL(k_batched_reduce, dim3(fj_cdiv(n_tgt_total > 0 ? n_tgt_total : 1, 256)), dim3(256),
+  hipLaunchKernelGGL(k_batched_reduce, dim3(fj_cdiv(n_tgt_total > 0 ? n_tgt_total : 1, 64)),
+                     dim3(64 * FJ_BRED_WAVES),
                      0, st, dtasks, dcounts, (const double*)workspace, targets, out);
   FJ_CHECK_LAUNCH();
   return FJ_OK;

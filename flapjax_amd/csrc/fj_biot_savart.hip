@@ -219,18 +219,39 @@ __global__ __launch_bounds__(FJ_BS_THREADS) void k_pair_partial_f64(
   }
 }
 
+// Chunk partials -> result, one workgroup per 64 targets: wave w sums chunks
+// w, w + FJ_RED_WAVES, ... (independent loads in flight across the waves),
+// then wave 0 adds the wave sums in wave order -- a fixed order for given
+// sizes, so results are deterministic.
+#define FJ_RED_WAVES 8
 template <int MODE>
-__global__ void k_pair_reduce(const double* __restrict__ part, int chunks, int ld, int n_tgt,
-                              const double* __restrict__ tgt, double* __restrict__ out,
-                              FjFlow flow, double time, double dt) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n_tgt) return;
+__global__ __launch_bounds__(64 * FJ_RED_WAVES) void k_pair_reduce(
+    const double* __restrict__ part, int chunks, int ld, int n_tgt,
+    const double* __restrict__ tgt, double* __restrict__ out, FjFlow flow, double time,
+    double dt) {
+  __shared__ double red[FJ_RED_WAVES - 1][3][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int t = blockIdx.x * 64 + lane;  // < ld: every partial row holds ld >= 64 * grid entries
   double vx = 0.0, vy = 0.0, vz = 0.0;
-  for (int c = 0; c < chunks; ++c) {
+#pragma unroll 4
+  for (int c = wave; c < chunks; c += FJ_RED_WAVES) {
     const double* o = part + (size_t)c * 3 * ld;
     vx += o[t];
     vy += o[ld + t];
     vz += o[2 * ld + t];
+  }
+  if (wave > 0) {
+    red[wave - 1][0][lane] = vx;
+    red[wave - 1][1][lane] = vy;
+    red[wave - 1][2][lane] = vz;
+  }
+  __syncthreads();
+  if (wave != 0 || t >= n_tgt) return;
+#pragma unroll
+  for (int w = 0; w < FJ_RED_WAVES - 1; ++w) {
+    vx += red[w][0][lane];
+    vy += red[w][1][lane];
+    vz += red[w][2][lane];
   }
   vx *= FJ_INV_4PI; vy *= FJ_INV_4PI; vz *= FJ_INV_4PI;
   if (MODE == FJ_OUT_STORE) {
@@ -312,13 +333,13 @@ extern "C" int fj_pair_sum(const double* seg, int n_seg, const double* targets, 
 #undef FJ_DISPATCH_TPL
   }
   FJ_CHECK_LAUNCH();
-  const dim3 g(fj_cdiv(n_tgt, 256));
+  const dim3 g(fj_cdiv(n_tgt, 64)), blk(64 * FJ_RED_WAVES);
   if (mode == FJ_OUT_STORE)
-    hipLaunchKernelGGL(k_pair_reduce<FJ_OUT_STORE>, g, dim3(256), 0, st, part, p.chunks, p.ld, n_tgt, targets, out, fl, time, dt);
+    hipLaunchKernelGGL(k_pair_reduce<FJ_OUT_STORE>, g, blk, 0, st, part, p.chunks, p.ld, n_tgt, targets, out, fl, time, dt);
   else if (mode == FJ_OUT_ADD)
-    hipLaunchKernelGGL(k_pair_reduce<FJ_OUT_ADD>, g, dim3(256), 0, st, part, p.chunks, p.ld, n_tgt, targets, out, fl, time, dt);
+    hipLaunchKernelGGL(k_pair_reduce<FJ_OUT_ADD>, g, blk, 0, st, part, p.chunks, p.ld, n_tgt, targets, out, fl, time, dt);
   else
-    hipLaunchKernelGGL(k_pair_reduce<FJ_OUT_CONVECT>, g, dim3(256), 0, st, part, p.chunks, p.ld, n_tgt, targets, out, fl, time, dt);
+    hipLaunchKernelGGL(k_pair_reduce<FJ_OUT_CONVECT>, g, blk, 0, st, part, p.chunks, p.ld, n_tgt, targets, out, fl, time, dt);
   FJ_CHECK_LAUNCH();
   return FJ_OK;
 }

@@ -12,35 +12,33 @@
 
 #define FJ_BS_WAVES 4
 #define FJ_BS_THREADS (64 * FJ_BS_WAVES)
-#define FJ_NEAR_FACTOR2 16.0f  // (4 filament lengths)^2
+#define FJ_NEAR_FACTOR2 25.0f  // (5 filament lengths)^2 from the start point
 #define FJ_MIXED_FLUSH 16
 
+// FP32 far-field evaluation from r1 (rounded from the FP64 difference) and
+// the filament's FP32 r0 / aux / g: r2 = r1 - r0 (relative error ~eps32 since
+// far-field pairs have |r2| >= 4 |r0|), c = r0 x r1, r0.d = a1 w1 - a2 w2.
 template <int KIND>
-__device__ __forceinline__ void fj_bs_eval_f32(float r1x, float r1y, float r1z, float r2x,
-                                               float r2y, float r2z, const float* sf,
-                                               float eps2f, float rcut2f, float& vx, float& vy,
-                                               float& vz) {
-  const float cx = r1y * r2z - r1z * r2y;
-  const float cy = r1z * r2x - r1x * r2z;
-  const float cz = r1x * r2y - r1y * r2x;
-  const float cc = cx * cx + cy * cy + cz * cz;
+__device__ __forceinline__ void fj_bs_eval_f32(float r1x, float r1y, float r1z, float s1,
+                                               const float* sf, float r0sq_f, float eps2f,
+                                               float rcut2f, float& vx, float& vy, float& vz) {
+  const float r0x = sf[0], r0y = sf[1], r0z = sf[2];
+  const float r2x = r1x - r0x, r2y = r1y - r0y, r2z = r1z - r0z;
+  const float cx = r0y * r1z - r0z * r1y;
+  const float cy = r0z * r1x - r0x * r1z;
+  const float cz = r0x * r1y - r0y * r1x;
   const float e2 = (KIND == FJ_KERNEL_PLAIN) ? 0.0f : eps2f;
-  const float inv1 = __builtin_amdgcn_rsqf(r1x * r1x + r1y * r1y + r1z * r1z + e2);
-  const float inv2 = __builtin_amdgcn_rsqf(r2x * r2x + r2y * r2y + r2z * r2z + e2);
-  const float dx = r1x * inv1 - r2x * inv2;
-  const float dy = r1y * inv1 - r2y * inv2;
-  const float dz = r1z * inv1 - r2z * inv2;
-  const float num = sf[0] * dx + sf[1] * dy + sf[2] * dz;
-  float q;
-  if (KIND == FJ_KERNEL_EPSILON) {
-    q = sf[4] * num * __builtin_amdgcn_rcpf(cc + sf[3]);
-  } else if (KIND == FJ_KERNEL_PLAIN) {
-    q = sf[4] * num * __builtin_amdgcn_rcpf(cc);
-  } else {
-    const float sm = (sf[0] * r1x + sf[1] * r1y + sf[2] * r1z) * sf[3];
-    const float px = r1x - sm * sf[0], py = r1y - sm * sf[1], pz = r1z - sm * sf[2];
-    const float rad2 = px * px + py * py + pz * pz;
-    q = (rad2 > rcut2f) ? sf[4] * num * __builtin_amdgcn_rcpf(cc) : 0.0f;
+  const float s2 = r2x * r2x + r2y * r2y + r2z * r2z + e2;
+  const float a1 = r0x * r1x + r0y * r1y + r0z * r1z;
+  const float a2 = a1 - r0sq_f;
+  float D = cx * cx + cy * cy + cz * cz;
+  if (KIND == FJ_KERNEL_EPSILON) D += sf[3];
+  const float w1 = __builtin_amdgcn_rsqf(s1 + e2);
+  const float w2 = __builtin_amdgcn_rsqf(s2);
+  float q = sf[4] * (a1 * w1 - a2 * w2) * __builtin_amdgcn_rcpf(D);
+  if (KIND == FJ_KERNEL_CUTOFF) {
+    const float rad2 = s1 - a1 * (a1 * sf[3]);
+    q = (rad2 > rcut2f) ? q : 0.0f;
   }
   vx = cx * q; vy = cy * q; vz = cz * q;
 }
@@ -81,22 +79,22 @@ __global__ __launch_bounds__(FJ_BS_THREADS) void k_pair_partial_mixed(
     float sf[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) sf[k] = spf[k];
-    const float near2 = FJ_NEAR_FACTOR2 * (float)rec[11];
+    const float r0sq_f = (float)rec[11];
+    // near: within FJ_NEAR_FACTOR filament lengths of the start point, which
+    // (|r0| apart) also covers the end point's neighbourhood
+    const float near2 = FJ_NEAR_FACTOR2 * r0sq_f;
 #pragma unroll
     for (int p = 0; p < TPL; ++p) {
       const float r1x = (float)(x[p] - rec[0]), r1y = (float)(y[p] - rec[1]),
                   r1z = (float)(z[p] - rec[2]);
-      const float r2x = (float)(x[p] - rec[3]), r2y = (float)(y[p] - rec[4]),
-                  r2z = (float)(z[p] - rec[5]);
       const float s1 = r1x * r1x + r1y * r1y + r1z * r1z;
-      const float s2 = r2x * r2x + r2y * r2y + r2z * r2z;
-      if (fminf(s1, s2) < near2) {
+      if (s1 < near2) {
         double vx, vy, vz;
         fj_bs_eval<KIND>(x[p], y[p], z[p], rec, kp, vx, vy, vz);
         ax[p] += vx; ay[p] += vy; az[p] += vz;
       } else {
         float vx, vy, vz;
-        fj_bs_eval_f32<KIND>(r1x, r1y, r1z, r2x, r2y, r2z, sf, eps2f, rcut2f, vx, vy, vz);
+        fj_bs_eval_f32<KIND>(r1x, r1y, r1z, s1, sf, r0sq_f, eps2f, rcut2f, vx, vy, vz);
         bx[p] += vx; by[p] += vy; bz[p] += vz;
       }
     }

@@ -94,33 +94,36 @@ __device__ __forceinline__ double fj_rsqrt(double x) {
 // The division by D = |c|^2 (+ eps |r0|^4) is folded into the two reciprocal
 // square roots: (r0 . d) / D = r0 . (r1 / sqrt(s1 D^2) - r2 / sqrt(s2 D^2)),
 // which is the same quantity up to rounding and saves the FP64 divide.
+// r1 = x - y0 and r2 = x - y1 are formed as the reference forms them, and
+// c = r1 x r2 as well: at a target ON an end point (every wake vertex in the
+// rollup is one) one of r1, r2 is exactly zero, so c is exactly zero -- the
+// identity c = r0 x r1 would instead leave the rounding error of an FMA-
+// contracted cross product of a vector with itself.  r0 . d uses
+// r0 . r2 = r0 . r1 - |r0|^2 (one subtraction instead of a dot product).
 template <int KIND>
 __device__ __forceinline__ void fj_bs_eval(double x, double y, double z, const double* s,
                                            const FjKernelParams& kp, double& vx, double& vy,
                                            double& vz) {
+  const double r0x = s[6], r0y = s[7], r0z = s[8];
   const double r1x = x - s[0], r1y = y - s[1], r1z = z - s[2];
   const double r2x = x - s[3], r2y = y - s[4], r2z = z - s[5];
-  const double r0x = s[6], r0y = s[7], r0z = s[8];
   const double cx = r1y * r2z - r1z * r2y;
   const double cy = r1z * r2x - r1x * r2z;
   const double cz = r1x * r2y - r1y * r2x;
   const double e2 = (KIND == FJ_KERNEL_PLAIN) ? 0.0 : kp.eps2;
   const double s1 = r1x * r1x + r1y * r1y + r1z * r1z + e2;
   const double s2 = r2x * r2x + r2y * r2y + r2z * r2z + e2;
+  const double a1 = r0x * r1x + r0y * r1y + r0z * r1z;
+  const double a2 = a1 - s[11];
   double D = cx * cx + cy * cy + cz * cz;
   if (KIND == FJ_KERNEL_EPSILON) D += s[9];
   const double D2 = D * D;
   const double w1 = fj_rsqrt(s1 * D2);
   const double w2 = fj_rsqrt(s2 * D2);
-  const double dx = r1x * w1 - r2x * w2;
-  const double dy = r1y * w1 - r2y * w2;
-  const double dz = r1z * w1 - r2z * w2;
-  double q = s[10] * (r0x * dx + r0y * dy + r0z * dz);
+  double q = s[10] * (a1 * w1 - a2 * w2);
   if (KIND == FJ_KERNEL_CUTOFF) {
-    // distance from x to the filament line: |r1 - (r0.r1 / |r0|^2) r0|
-    const double sm = (r0x * r1x + r0y * r1y + r0z * r1z) * s[9];
-    const double px = r1x - sm * r0x, py = r1y - sm * r0y, pz = r1z - sm * r0z;
-    const double rad2 = px * px + py * py + pz * pz;
+    // squared distance from x to the filament line: |r1|^2 - (r0.r1)^2 / |r0|^2
+    const double rad2 = (s1 - e2) - a1 * (a1 * s[9]);
     q = (rad2 > kp.rcut * kp.rcut) ? q : 0.0;
   }
   vx = cx * q; vy = cy * q; vz = cz * q;

@@ -494,80 +494,197 @@ extern "C" int fj_lu_solve(const double* LU, int n, int lda, const int32_t* ipiv
   return FJ_OK;
 }
 
-// ---- FP64 iterative refinement around a kept factor -------------------------
-// r = b - A x (FP64), one wave per row, lanes over columns (coalesced rows).
-// flags[0]: 0 running, 1 converged, -1 gave up; flags[1]: corrections done.
-__global__ void k_residual(const double* __restrict__ A, int lda, int n,
-                           const double* __restrict__ b, const double* __restrict__ x,
-                           double* __restrict__ r, const int32_t* __restrict__ flags) {
-  if (flags[0]) return;
-  const int lane = threadIdx.x & 63;
-  const int row = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
-  if (row >= n) return;
-  const double* a = A + (size_t)row * lda;
-  double s = 0.0;
-  for (int c = lane; c < n; c += 64) s += a[c] * x[c];
+// ---- FP64 iterative refinement ------------------------------------------------
+// Mixed-precision refinement in the form of LAPACK dsgesv (Buttari, Dongarra
+// et al., "Mixed precision iterative refinement techniques for the solution
+// of dense linear systems", 2007): x0 = M b, then r = b - A x (FP64, the
+// CURRENT A), x += M r, where M is an approximate inverse of A -- the LU
+// factor of A (FP32 or FP64), or of a nearby matrix, or its explicit inverse.
+// Stop test (dsgesv's backward-error bound): converged when
+//   ||r||_inf <= tol ||A||_inf ||x||_inf       (tol default sqrt(n) eps64).
+//
+// Device state in the workspace: r (n doubles), then per sweep k = 0..max_iter
+// a slot of three non-negative doubles kept as bit patterns for unsigned
+// max-atomics {||A||_inf, ||r||_inf, ||x_k||_inf}, then flags {status
+// (0 running, 1 converged), corrections made}.  Every launch reads the flags
+// first, so once converged the remaining sweeps exit at entry.
+#define FJ_REFINE_MAX_ITER 64
+#define FJ_RV_WAVES 4
+
+typedef unsigned long long fj_u64;
+
+__device__ __forceinline__ fj_u64 fj_bits(double v) { return (fj_u64)__double_as_longlong(v); }
+__device__ __forceinline__ double fj_from_bits(fj_u64 b) { return __longlong_as_double((long long)b); }
+
+// max of a non-negative double over the workgroup, then one atomic per group
+__device__ __forceinline__ void fj_group_max(double v, fj_u64* dst, double* sred) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-  if (lane == 0) r[row] = b[row] - s;
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+  if (lane == 0) sred[wave] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = sred[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = fmax(m, sred[w]);
+    atomicMax(dst, fj_bits(m));
+  }
 }
 
-// one workgroup: correction d = LU^{-1} r, x += d; converged when
-// |d|_inf <= tol |x|_inf (the correction no longer changes x at FP64 level)
-template <typename T>
-__global__ __launch_bounds__(FJ_LU_THREADS) void k_refine_step(
-    const T* __restrict__ LU, int ldlu, const int32_t* __restrict__ ipiv,
-    const T* __restrict__ aux, int n, const double* __restrict__ r, double* __restrict__ x,
-    double tol, int32_t* __restrict__ flags, int last) {
-  __shared__ double xs[FJ_SOLVE_MAXN];
-  __shared__ double red[2][FJ_LU_WAVES];
+__device__ __forceinline__ bool fj_refine_converged(const fj_u64* slot, double tol) {
+  const double an = fj_from_bits(slot[0]), rn = fj_from_bits(slot[1]), xn = fj_from_bits(slot[2]);
+  return rn <= tol * an * xn;
+}
+
+__global__ void k_refine_init(fj_u64* __restrict__ slots, int nslots, int32_t* __restrict__ flags) {
+  for (int i = threadIdx.x; i < 3 * nslots; i += blockDim.x) slots[i] = 0ull;
+  if (threadIdx.x == 0) { flags[0] = 0; flags[1] = 0; }
+}
+
+// r = b - A x, one wave per row (lanes over the columns, coalesced);
+// slot k gathers ||A||_inf and ||r||_inf.
+__global__ __launch_bounds__(64 * FJ_RV_WAVES) void k_refine_residual(
+    const double* __restrict__ A, int lda, int n, const double* __restrict__ b,
+    const double* __restrict__ x, double* __restrict__ r, fj_u64* __restrict__ slot,
+    const int32_t* __restrict__ flags) {
+  __shared__ double sred[2][FJ_RV_WAVES];
   if (flags[0]) return;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  fj_lu_solve_block<T>(LU, n, ldlu, ipiv, aux, r, xs, 0, xs);
-  double dm = 0.0, xm = 0.0;
-  for (int i = tid; i < n; i += blockDim.x) {
-    const double xn = x[i] + xs[i];
-    x[i] = xn;
-    dm = fmax(dm, fabs(xs[i]));
-    xm = fmax(xm, fabs(xn));
+  const int lane = threadIdx.x & 63;
+  const int row = blockIdx.x * FJ_RV_WAVES + (threadIdx.x >> 6);
+  double s = 0.0, a = 0.0;
+  if (row < n) {
+    const double* ar = A + (size_t)row * lda;
+#pragma unroll 4
+    for (int c = lane; c < n; c += 64) {
+      const double v = ar[c];
+      s += v * x[c];
+      a += fabs(v);
+    }
   }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
-    dm = fmax(dm, __shfl_xor(dm, off));
-    xm = fmax(xm, __shfl_xor(xm, off));
+    s += __shfl_xor(s, off);
+    a += __shfl_xor(a, off);
   }
-  if (lane == 0) { red[0][wave] = dm; red[1][wave] = xm; }
-  __syncthreads();
-  if (tid == 0) {
-    dm = red[0][0]; xm = red[1][0];
-    for (int w = 1; w < FJ_LU_WAVES; ++w) { dm = fmax(dm, red[0][w]); xm = fmax(xm, red[1][w]); }
-    flags[1] += 1;
-    if (dm <= tol * xm) flags[0] = 1;
-    else if (last) flags[0] = -1;
+  double rr = 0.0;
+  if (row < n) {
+    rr = b[row] - s;
+    if (lane == 0) r[row] = rr;
   }
+  fj_group_max(a, &slot[0], sred[0]);
+  fj_group_max(fabs(rr), &slot[1], sred[1]);
 }
 
-__global__ void k_refine_init(int32_t* flags) {
-  flags[0] = 0;
-  flags[1] = 0;
+// x = M v (first solve, v = b) or x += M v (correction, v = r) with the
+// explicit inverse M (FP64 or FP32), one wave per row; ||x||_inf into xslot.
+// A correction first applies the stop test of sweep k.
+template <typename T, bool CORRECT>
+__global__ __launch_bounds__(64 * FJ_RV_WAVES) void k_refine_apply_inv(
+    const T* __restrict__ M, int ldm, int n, const double* __restrict__ v,
+    double* __restrict__ x, fj_u64* __restrict__ slots, int k, double tol,
+    int32_t* __restrict__ flags) {
+  __shared__ double sred[FJ_RV_WAVES];
+  if (flags[0]) return;
+  if (CORRECT && fj_refine_converged(slots + 3 * k, tol)) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) { flags[0] = 1; flags[1] = k; }
+    return;
+  }
+  const int lane = threadIdx.x & 63;
+  const int row = blockIdx.x * FJ_RV_WAVES + (threadIdx.x >> 6);
+  double s = 0.0;
+  if (row < n) {
+    const T* mr = M + (size_t)row * ldm;
+#pragma unroll 4
+    for (int c = lane; c < n; c += 64) s += (double)mr[c] * v[c];
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+  double xn = 0.0;
+  if (row < n) {
+    xn = CORRECT ? x[row] + s : s;
+    if (lane == 0) x[row] = xn;
+  }
+  fj_group_max(fabs(xn), &slots[3 * (CORRECT ? k + 1 : 0) + 2], sred);
+  if (CORRECT && blockIdx.x == 0 && threadIdx.x == 0) flags[1] = k + 1;
 }
 
-__global__ void k_refine_finish(const int32_t* flags, int32_t* iters) {
-  iters[0] = (flags[0] == 1) ? flags[1] : -flags[1];
-}
-
-template <typename T>
-__global__ __launch_bounds__(FJ_LU_THREADS) void k_first_solve(const T* __restrict__ LU, int n,
-                                                                int lda,
-                                                                const int32_t* __restrict__ ipiv,
-                                                                const T* __restrict__ aux,
-                                                                const double* b, double* x) {
+// x = LU^{-1} b (first solve) or x += LU^{-1} r (correction) in one
+// workgroup with the triangular sweeps above; same slots and stop test.
+template <typename T, bool CORRECT>
+__global__ __launch_bounds__(FJ_LU_THREADS) void k_refine_apply_lu(
+    const T* __restrict__ LU, int ldlu, const int32_t* __restrict__ ipiv,
+    const T* __restrict__ aux, int n, const double* __restrict__ v, double* __restrict__ x,
+    fj_u64* __restrict__ slots, int k, double tol, int32_t* __restrict__ flags) {
   __shared__ double xs[FJ_SOLVE_MAXN];
-  fj_lu_solve_block<T>(LU, n, lda, ipiv, aux, b, x, 0, xs);
+  __shared__ double sred[FJ_LU_WAVES];
+  if (flags[0]) return;
+  if (CORRECT && fj_refine_converged(slots + 3 * k, tol)) {
+    if (threadIdx.x == 0) { flags[0] = 1; flags[1] = k; }
+    return;
+  }
+  fj_lu_solve_block<T>(LU, n, ldlu, ipiv, aux, v, xs, 0, xs);
+  double xm = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const double xn = CORRECT ? x[i] + xs[i] : xs[i];
+    x[i] = xn;
+    xm = fmax(xm, fabs(xn));
+  }
+  fj_group_max(xm, &slots[3 * (CORRECT ? k + 1 : 0) + 2], sred);
+  if (CORRECT && threadIdx.x == 0) flags[1] = k + 1;
+}
+
+// iters = corrections made, negated when the stop test never passed (the
+// last slot is tested here, after the final residual)
+__global__ void k_refine_finish(const fj_u64* __restrict__ slots, int max_iter, double tol,
+                                const int32_t* __restrict__ flags, int32_t* __restrict__ iters) {
+  if (flags[0] == 1) iters[0] = flags[1];
+  else iters[0] = fj_refine_converged(slots + 3 * max_iter, tol) ? max_iter : -max_iter;
 }
 
 extern "C" size_t fj_refine_workspace_bytes(int n) {
-  return ((size_t)(n + 8) * sizeof(double) + 64 + 255) / 256 * 256;
+  return ((size_t)(n + 4) * sizeof(double) + 3 * (FJ_REFINE_MAX_ITER + 1) * sizeof(fj_u64) + 64 +
+          255) / 256 * 256;
+}
+
+struct FjRefineWs {
+  double* r;
+  fj_u64* slots;
+  int32_t* flags;
+};
+
+static FjRefineWs fj_refine_ws(void* workspace, int n) {
+  FjRefineWs w;
+  w.r = (double*)workspace;
+  w.slots = (fj_u64*)(w.r + n + 4);
+  w.flags = (int32_t*)(w.slots + 3 * (FJ_REFINE_MAX_ITER + 1));
+  return w;
+}
+
+static double fj_refine_tol(double tol, int n) {
+  return tol > 0.0 ? tol : sqrt((double)n) * 1.1102230246251565e-16;
+}
+
+// apply(first, k) launches x = M b (first) or the correction of sweep k
+template <typename Apply>
+static int fj_refine_run(const double* A, int lda, int n, const double* b, double* x,
+                         int max_iter, double tol, int32_t* iters, void* workspace,
+                         hipStream_t st, Apply apply) {
+  const FjRefineWs w = fj_refine_ws(workspace, n);
+  const int rgrid = fj_cdiv(n, FJ_RV_WAVES);
+  hipLaunchKernelGGL(k_refine_init, dim3(1), dim3(256), 0, st, w.slots, max_iter + 1, w.flags);
+  FJ_CHECK_LAUNCH();
+  apply(true, 0, w);
+  for (int k = 0; k < max_iter; ++k) {
+    hipLaunchKernelGGL(k_refine_residual, dim3(rgrid), dim3(64 * FJ_RV_WAVES), 0, st, A, lda, n, b,
+                       x, w.r, w.slots + 3 * k, w.flags);
+    FJ_CHECK_LAUNCH();
+    apply(false, k, w);
+  }
+  hipLaunchKernelGGL(k_refine_residual, dim3(rgrid), dim3(64 * FJ_RV_WAVES), 0, st, A, lda, n, b, x,
+                     w.r, w.slots + 3 * max_iter, w.flags);
+  hipLaunchKernelGGL(k_refine_finish, dim3(1), dim3(1), 0, st, w.slots, max_iter, tol, w.flags,
+                     iters);
+  FJ_CHECK_LAUNCH();
+  return FJ_OK;
 }
 
 template <typename T>
@@ -576,24 +693,20 @@ static int fj_solve_refined_t(const double* A, int lda, const T* LU, int ldlu,
                               double* x, int max_iter, double tol, int32_t* iters,
                               void* workspace, hipStream_t st) {
   if (n <= 0 || n > FJ_SOLVE_MAXN || lda < n || ldlu < n || !A || !LU || !ipiv || !aux || !b ||
-      !x || !iters || !workspace || max_iter < 1)
+      !x || !iters || !workspace || max_iter < 1 || max_iter > FJ_REFINE_MAX_ITER)
     return FJ_ERR_ARG;
-  double* r = (double*)workspace;
-  int32_t* flags = (int32_t*)(r + n + 4);
-  hipLaunchKernelGGL(k_refine_init, dim3(1), dim3(1), 0, st, flags);
-  hipLaunchKernelGGL(k_first_solve<T>, dim3(1), dim3(FJ_LU_THREADS), 0, st, LU, n, ldlu, ipiv, aux,
-                     b, x);
-  FJ_CHECK_LAUNCH();
-  for (int it = 0; it < max_iter; ++it) {
-    hipLaunchKernelGGL(k_residual, dim3(fj_cdiv(n, 4)), dim3(256), 0, st, A, lda, n, b, x, r,
-                       flags);
-    hipLaunchKernelGGL(k_refine_step<T>, dim3(1), dim3(FJ_LU_THREADS), 0, st, LU, ldlu, ipiv, aux,
-                       n, r, x, tol, flags, it == max_iter - 1 ? 1 : 0);
-    FJ_CHECK_LAUNCH();
-  }
-  hipLaunchKernelGGL(k_refine_finish, dim3(1), dim3(1), 0, st, flags, iters);
-  FJ_CHECK_LAUNCH();
-  return FJ_OK;
+  tol = fj_refine_tol(tol, n);
+  return fj_refine_run(A, lda, n, b, x, max_iter, tol, iters, workspace, st,
+                       [&](bool first, int k, const FjRefineWs& w) {
+                         if (first)
+                           hipLaunchKernelGGL((k_refine_apply_lu<T, false>), dim3(1),
+                                              dim3(FJ_LU_THREADS), 0, st, LU, ldlu, ipiv, aux, n,
+                                              b, x, w.slots, 0, tol, w.flags);
+                         else
+                           hipLaunchKernelGGL((k_refine_apply_lu<T, true>), dim3(1),
+                                              dim3(FJ_LU_THREADS), 0, st, LU, ldlu, ipiv, aux, n,
+                                              w.r, x, w.slots, k, tol, w.flags);
+                       });
 }
 
 extern "C" int fj_solve_refined(const double* A, int lda, const float* LU32, int ldlu,
@@ -610,4 +723,175 @@ extern "C" int fj_solve_refined_f64(const double* A, int lda, const double* LU, 
                                     void* workspace, void* stream) {
   return fj_solve_refined_t<double>(A, lda, LU, ldlu, ipiv, (const double*)aux, n, b, x,
                                     max_iter, tol, iters, workspace, (hipStream_t)stream);
+}
+
+template <typename T>
+static int fj_solve_refined_inv_t(const double* A, int lda, const T* M, int ldm, int n,
+                                  const double* b, double* x, int max_iter, double tol,
+                                  int32_t* iters, void* workspace, hipStream_t st) {
+  if (n <= 0 || lda < n || ldm < n || !A || !M || !b || !x || !iters || !workspace ||
+      max_iter < 1 || max_iter > FJ_REFINE_MAX_ITER)
+    return FJ_ERR_ARG;
+  tol = fj_refine_tol(tol, n);
+  const int rgrid = fj_cdiv(n, FJ_RV_WAVES);
+  return fj_refine_run(A, lda, n, b, x, max_iter, tol, iters, workspace, st,
+                       [&](bool first, int k, const FjRefineWs& w) {
+                         if (first)
+                           hipLaunchKernelGGL((k_refine_apply_inv<T, false>), dim3(rgrid),
+                                              dim3(64 * FJ_RV_WAVES), 0, st, M, ldm, n, b, x,
+                                              w.slots, 0, tol, w.flags);
+                         else
+                           hipLaunchKernelGGL((k_refine_apply_inv<T, true>), dim3(rgrid),
+                                              dim3(64 * FJ_RV_WAVES), 0, st, M, ldm, n, w.r, x,
+                                              w.slots, k, tol, w.flags);
+                       });
+}
+
+extern "C" int fj_solve_refined_inv(const double* A, int lda, const double* Ainv, int ldi, int n,
+                                    const double* b, double* x, int max_iter, double tol,
+                                    int32_t* iters, void* workspace, void* stream) {
+  return fj_solve_refined_inv_t<double>(A, lda, Ainv, ldi, n, b, x, max_iter, tol, iters,
+                                        workspace, (hipStream_t)stream);
+}
+
+extern "C" int fj_solve_refined_inv_f32(const double* A, int lda, const float* Ainv, int ldi,
+                                        int n, const double* b, double* x, int max_iter,
+                                        double tol, int32_t* iters, void* workspace,
+                                        void* stream) {
+  return fj_solve_refined_inv_t<float>(A, lda, Ainv, ldi, n, b, x, max_iter, tol, iters,
+                                       workspace, (hipStream_t)stream);
+}
+
+// ---- explicit inverse from the factor ------------------------------------------
+// A^{-1} = U^{-1} L^{-1} P by two sweeps over 64-row block rows, each a
+// launch with one workgroup per 64-column tile, in place in X:
+//   forward  (k = 0 .. nb-1): Y_k = Linv_kk (P_k - sum_{j<k} L_kj Y_j)
+//   backward (k = nb-1 .. 0): X_k = Uinv_kk (Y_k - sum_{j>k} U_kj X_j)
+// P_k holds the rows of the permutation matrix (row i: unit at column
+// ipiv[n + i]); Linv_kk / Uinv_kk are the factor's diagonal-block inverses.
+// Both products run on the matrix cores with the trailing update's tiling.
+template <typename T>
+__global__ __launch_bounds__(256) void k_lu_inverse_row(const T* __restrict__ LU, int lda,
+                                                        const T* __restrict__ aux,
+                                                        const int32_t* __restrict__ ipiv,
+                                                        T* __restrict__ X, int ldx, int n, int kb,
+                                                        int backward) {
+  __shared__ T sA[FJ_GT][FJ_GK + 1];
+  __shared__ T sB[FJ_GK][FJ_GT + 1];
+  __shared__ T sR[FJ_GT][FJ_GT + 1];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+  const int nblk = fj_nblk(n);
+  const int r0 = kb * FJ_TS_B, rs = min(FJ_TS_B, n - r0);
+  const int c0 = blockIdx.x * FJ_GT;
+  const int q_begin = backward ? r0 + rs : 0;
+  const int q_end = backward ? n : r0;
+  typedef typename FjMfma<T>::acc_t acc_t;
+  acc_t acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) acc[a][c] = acc_t{0, 0, 0, 0};
+
+  // acc = sum over the solved block rows of (L or U) block * X block
+  for (int q0 = q_begin; q0 < q_end; q0 += FJ_GK) {
+    const int kq = min(FJ_GK, q_end - q0);
+    for (int e = tid; e < FJ_GT * FJ_GK; e += 256) {
+      const int i = e / FJ_GK, q = e % FJ_GK;
+      sA[i][q] = (i < rs && q < kq) ? LU[(size_t)(r0 + i) * lda + q0 + q] : T(0);
+    }
+    for (int e = tid; e < FJ_GK * FJ_GT; e += 256) {
+      const int q = e / FJ_GT, j = e % FJ_GT;
+      sB[q][j] = (q < kq && c0 + j < n) ? X[(size_t)(q0 + q) * ldx + c0 + j] : T(0);
+    }
+    __syncthreads();
+    for (int q = 0; q < kq; q += 4) {
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        const T av = sA[wr + a * 16 + li][q + lk];
+#pragma unroll
+        for (int c = 0; c < 2; ++c) acc[a][c] = FjMfma<T>::mma(av, sB[q + lk][wc + c * 16 + li], acc[a][c]);
+      }
+    }
+    __syncthreads();
+  }
+  // right-hand side tile: P_k - acc (forward) or Y_k - acc (backward)
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = wr + a * 16 + FjMfma<T>::row(lk, v);
+        const int j = wc + c * 16 + li;
+        T base = T(0);
+        if (i < rs && c0 + j < n)
+          base = backward ? X[(size_t)(r0 + i) * ldx + c0 + j]
+                          : ((ipiv[n + r0 + i] == c0 + j) ? T(1) : T(0));
+        sR[i][j] = base - acc[a][c][v];
+        acc[a][c][v] = T(0);
+      }
+  __syncthreads();
+  // X_k = D_kk^{-1} * rhs, D = L (forward) or U (backward)
+  const T* D = aux + ((size_t)(backward ? nblk : 0) + kb) * FJ_TS_B * FJ_TS_B;
+  for (int q0 = 0; q0 < FJ_TS_B; q0 += FJ_GK) {
+    for (int e = tid; e < FJ_GT * FJ_GK; e += 256) {
+      const int i = e / FJ_GK, q = e % FJ_GK;
+      sA[i][q] = D[i * FJ_TS_B + q0 + q];
+    }
+    __syncthreads();
+    for (int q = 0; q < FJ_GK; q += 4) {
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        const T av = sA[wr + a * 16 + li][q + lk];
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+          acc[a][c] = FjMfma<T>::mma(av, sR[q0 + q + lk][wc + c * 16 + li], acc[a][c]);
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = wr + a * 16 + FjMfma<T>::row(lk, v);
+        const int j = c0 + wc + c * 16 + li;
+        if (i < rs && j < n) X[(size_t)(r0 + i) * ldx + j] = acc[a][c][v];
+      }
+}
+
+template <typename T>
+static int fj_lu_inverse_t(const T* LU, int n, int lda, const int32_t* ipiv, const T* aux, T* X,
+                           int ldx, hipStream_t st) {
+  if (n <= 0 || n > FJ_SOLVE_MAXN || lda < n || ldx < n || !LU || !ipiv || !aux || !X)
+    return FJ_ERR_ARG;
+  const int nblk = fj_nblk(n);
+  const dim3 grid(fj_cdiv(n, FJ_GT));
+  for (int kb = 0; kb < nblk; ++kb) {
+    hipLaunchKernelGGL(k_lu_inverse_row<T>, grid, dim3(256), 0, st, LU, lda, aux, ipiv, X, ldx, n,
+                       kb, 0);
+    FJ_CHECK_LAUNCH();
+  }
+  for (int kb = nblk - 1; kb >= 0; --kb) {
+    hipLaunchKernelGGL(k_lu_inverse_row<T>, grid, dim3(256), 0, st, LU, lda, aux, ipiv, X, ldx, n,
+                       kb, 1);
+    FJ_CHECK_LAUNCH();
+  }
+  return FJ_OK;
+}
+
+extern "C" int fj_lu_inverse(const double* LU, int n, int lda, const int32_t* ipiv,
+                             const void* aux, double* Ainv, int ldi, void* stream) {
+  return fj_lu_inverse_t<double>(LU, n, lda, ipiv, (const double*)aux, Ainv, ldi,
+                                 (hipStream_t)stream);
+}
+
+extern "C" int fj_lu_inverse_f32(const float* LU, int n, int lda, const int32_t* ipiv,
+                                 const void* aux, float* Ainv, int ldi, void* stream) {
+  return fj_lu_inverse_t<float>(LU, n, lda, ipiv, (const float*)aux, Ainv, ldi,
+                                (hipStream_t)stream);
 }

@@ -2,12 +2,17 @@
 
 Two forms:
   * ``solve``           FP64 LU with partial pivoting + triangular solves;
-  * ``RefinedSolver``   LU (FP32 on the matrix cores by default, or FP64)
-                        with FP64 iterative refinement against the FP64
-                        matrix (residual b - A x in FP64); the factor can be
-                        reused across steps while the matrix stays close
-                        (rigid motion), refactoring when refinement stops
-                        converging.
+  * ``RefinedSolver``   a kept factor: the LU (FP64, or FP32 on the matrix
+                        cores) of the matrix at the last (re)factorisation and
+                        its explicit inverse M; every solve is x0 = M b then
+                        FP64 refinement against the CURRENT matrix
+                        (r = b - A x, x += M r) until LAPACK dsgesv's stop test
+                        ||r|| <= sqrt(n) eps ||A|| ||x|| passes.  Each solve is
+                        a few mat-vecs over all CUs.  The stop-test outcome is
+                        copied to pinned host memory behind the solve, so the
+                        host learns it without draining the stream; when a
+                        solve fails (the matrix drifted too far from the
+                        factored one) the caller refactors and solves again.
 """
 
 from __future__ import annotations
@@ -34,40 +39,75 @@ def solve(A, b):
 
 
 class RefinedSolver:
-    """FP32-factor + FP64-refinement solver with optional factor reuse."""
+    """Kept-factor solver with FP64 refinement (see the module docstring).
 
-    def __init__(self, max_iter: int = 10, tol: float | None = None, reuse: bool = True,
-                 factor_dtype=torch.float32):
-        self.max_iter = max_iter
+    ``submit(A, b)`` enqueues a solve and the asynchronous copy of its status;
+    ``confirm()`` waits for that copy only and returns the corrections made
+    (negative: the stop test never passed).  ``solve(A, b)`` does both and, on
+    failure with ``reuse``, refactors on A and solves once more."""
+
+    def __init__(self, max_iter: int | None = None, tol: float | None = None, reuse: bool = True,
+                 factor_dtype=torch.float64):
+        self.factor_dtype = factor_dtype
+        self.max_iter = max_iter if max_iter is not None else (3 if factor_dtype == torch.float64 else 8)
         self.tol = tol
         self.reuse = reuse
-        self.factor_dtype = factor_dtype
-        self._fac = None
+        self._inv = None
+        self._ws = None
+        self._iters = None
+        self._iters_host = None
+        self._event = None
         self.factorisations = 0
         self.last_iters = None
 
+    @property
+    def inverse(self):
+        return self._inv
+
     def factor(self, A):
+        A = to_device(A)
         if self.factor_dtype == torch.float32:
-            self._fac = K.lu_factor_f32(A.to(torch.float32))
+            fac = K.lu_factor_f32(A.to(torch.float32))
         else:
-            self._fac = K.lu_factor(A)
+            fac = K.lu_factor(A)
+        self._inv = K.lu_inverse(fac)
+        self._info = fac.info
         self.factorisations += 1
 
-    def solve(self, A, b, check: bool = True):
+    def _buffers(self, n, dev):
+        if self._ws is None or self._ws.device != dev or self._n != n:
+            self._n = n
+            self._ws = torch.empty(L.lib().fj_refine_workspace_bytes(n) // 8 + 1, dtype=torch.float64, device=dev)
+            self._iters = torch.zeros(1, dtype=torch.int32, device=dev)
+            self._iters_host = torch.zeros(1, dtype=torch.int32, pin_memory=True)
+            self._event = torch.cuda.Event()
+
+    def submit(self, A, b, out=None):
         A = to_device(A)
         b = to_device(b)
-        if self._fac is None or not self.reuse or self._fac.lu.shape != A.shape:
+        n = A.shape[0]
+        if self._inv is None or not self.reuse or self._inv.shape != A.shape:
             self.factor(A)
-        x, iters = K.solve_refined(A, self._fac, b, self.max_iter, self.tol)
-        if check:
-            it = int(iters.item())
-            if it < 0 and self.reuse:
-                # the stored factor is too far from A: refactor and retry once
-                self.factor(A)
-                x, iters = K.solve_refined(A, self._fac, b, self.max_iter, self.tol)
-                it = int(iters.item())
-            if it < 0:
-                raise L.FlapjaxLibError(f"refinement did not converge in {self.max_iter} sweeps")
-            self.last_iters = it
-        self.iters_tensor = iters
+        self._buffers(n, A.device)
+        x, _ = K.solve_refined_inv(A, self._inv, b, self.max_iter, self.tol, out=out, workspace=self._ws,
+                                   iters=self._iters)
+        self._iters_host.copy_(self._iters, non_blocking=True)
+        self._event.record()
+        return x
+
+    def confirm(self) -> int:
+        self._event.synchronize()
+        self.last_iters = int(self._iters_host[0])
+        return self.last_iters
+
+    def solve(self, A, b, out=None):
+        x = self.submit(A, b, out=out)
+        if self.confirm() < 0 and self.reuse:
+            self.factor(A)
+            x = self.submit(A, b, out=x)
+            self.confirm()
+        if self.last_iters < 0:
+            if int(self._info.item()) != 0:
+                raise L.FlapjaxLibError("singular matrix in the circulation solve")
+            raise L.FlapjaxLibError(f"refinement did not pass its stop test in {self.max_iter} corrections")
         return x

@@ -19,11 +19,12 @@ Cases that arrive while others run join at the next tick; finished cases
 leave and free their slot and memory.  Latency = completion - arrival.
 
 Per-step semantics are those of ``uvlm.UVLM`` (aero/uvlm.py:824-1117) with
-the circulation solve done by a factor kept for the case (rigid motion keeps
-the AIC fixed up to rounding) and refined in FP64 against the AIC
-re-assembled every step, so each step solves the current system to FP64
-accuracy.  Precision "fp64": FP64 factor and FP64 pair sums; "mixed": FP32
-factor on the matrix cores, FP32 far-field pair sums for rollup and forces.
+the circulation solve done with the explicit inverse of the dynamic AIC kept
+for the case (rigid motion keeps the AIC fixed up to rounding) and refined in
+FP64 against the AIC re-assembled every step, so each step solves the current
+system to FP64 accuracy; the stop-test outcome of every case's solve is read
+back at the end of the tick.  Precision "fp64": FP64 pair sums; "mixed":
+FP32 far-field pair sums for rollup and forces.
 """
 
 from __future__ import annotations
@@ -107,7 +108,7 @@ def footprint_bytes(req: CaseRequest, precision: str = "fp64") -> int:
     words += 2 * (ms + 1) * (n + 1) * 3 + 2 * ms * n  # wake (double buffered)
     words += 6 * nb * 3 + 6 * nb                     # stencils, circulations
     words += nb * nb                                 # FP64 AIC
-    words += (nb * nb) // (2 if precision == "mixed" else 1)  # factor
+    words += nb * nb                                 # kept inverse
     words += 16 * (s_b + s_w)                        # filament table
     words += 3 * (ms * (n + 1) + s_b + nb)           # pair-sum targets/outputs
     words += req.steps                               # lift history
@@ -157,10 +158,7 @@ class _Slot:
         # the factor kept for the dynamic (bound-only) system
         c, nc, _ = K.colloc_normals(self.zb)
         K.aic_assemble(c, nc, self.zb, kernel_kind, EPSILON, R_CUTOFF, self.A, 0)
-        if precision == "mixed":
-            self.fac = K.lu_factor_f32(self.A.to(torch.float32))
-        else:
-            self.fac = K.lu_factor(self.A)
+        self.Ainv = K.lu_inverse(K.lu_factor(self.A))
         self.ws = torch.empty(L.lib().fj_refine_workspace_bytes(m * n) // 8 + 1, dtype=F64, device=dev)
         self.iters = torch.zeros(1, dtype=torch.int32, device=dev)
 
@@ -280,12 +278,13 @@ class ServingEngine:
             K.aic_assemble(c, nc, sl.zb, self.kind, EPSILON, R_CUTOFF, sl.A, 0)
             rhs = K.rhs(c, nc, cd, vw_all[off:off + nb], fl, i * sl.dt)
             off += nb
-            x = sl.gb.reshape(-1)
-            fn = L.lib().fj_solve_refined if self.precision == "mixed" else L.lib().fj_solve_refined_f64
-            fac = sl.fac
-            rc = fn(L.ptr(sl.A), nb, L.ptr(fac.lu), nb, L.ptr(fac.piv), L.ptr(fac.aux), nb, L.ptr(rhs), L.ptr(x), 10,
-                    4.0 * np.finfo(float).eps, L.ptr(sl.iters), L.ptr(sl.ws), L.stream())
-            L.check(rc, "fj_solve_refined")
+            K.solve_refined_inv(sl.A, sl.Ainv, rhs, 3, out=sl.gb.reshape(-1), workspace=sl.ws, iters=sl.iters)
+        # stop-test outcomes of this tick's solves, read back behind the forces
+        status = torch.cat([sl.iters for sl in slots])
+        status_host = torch.empty(status.shape, dtype=status.dtype, pin_memory=True)
+        status_host.copy_(status, non_blocking=True)
+        status_ev = torch.cuda.Event()
+        status_ev.record()
         # (3) forces: velocities at the filament midpoints (fused)
         seg_parts, tgt_parts, tasks = [], [], []
         s_off = t_off = 0
@@ -311,6 +310,10 @@ class ServingEngine:
             sl.lift[i] = f[..., 2].sum()
             K.gamma_dot_unsteady(sl.gb, sl.gb_prev, sl.gbd, None, sl.dt, 0.7, fl.rho, gamma_dot=sl.gbd)
             sl.i = i
+        status_ev.synchronize()
+        bad = [sl.req.case_id for sl, it in zip(slots, status_host.tolist()) if it < 0]
+        if bad:
+            raise L.FlapjaxLibError(f"circulation solve failed its stop test for cases {bad}")
 
     def run(self, requests, time_source=time.perf_counter, log=None):
         """Serve requests (sorted by arrival, seconds from start).  Returns

@@ -18,17 +18,19 @@ Per dynamic step (aero/uvlm.py:912-1103):
 
 Arithmetic forms (``precision``):
   * "fp64"  - the plain drop-in: FP64 everywhere.  The AIC is re-assembled
-              every step as the reference does; by default (``aic_reuse``)
-              the FP64 LU factor is kept across steps and each step's system
-              is solved by FP64 iterative refinement against the freshly
-              assembled AIC (refactoring when refinement stops converging),
-              which gives the solution of that step's system to FP64
-              accuracy; ``aic_reuse=False`` re-factors every step literally;
+              every step as the reference does.  By default (``aic_reuse``)
+              the static and the dynamic system each keep an FP64 factor and
+              its explicit inverse; each step's system is solved by FP64
+              iterative refinement against the freshly assembled AIC (LAPACK
+              dsgesv's stop test), which gives the solution of that step's
+              system to FP64 accuracy.  The stop-test outcome of every step is
+              read back behind the step's remaining work; a step whose
+              refinement failed is refactored and computed again.
+              ``aic_reuse=False`` factors and solves every step literally;
   * "mixed" - FP32 far field / FP64 near field for the rollup and force pair
-              sums, FP32 LU with FP64 iterative refinement against the FP64
-              AIC (residual b - A x in FP64).  The AIC and the wake upwash in
-              the right-hand side stay FP64, so the circulation solve is
-              FP64-accurate for the given wake.
+              sums; the AIC, the wake upwash in the right-hand side and the
+              circulation solve stay FP64, so the solve is FP64-accurate for
+              the given wake.
 """
 
 from __future__ import annotations
@@ -129,12 +131,10 @@ class UVLM:
         self._dt = None
         self.zeta_b0 = None
         self.hg_ref = None
-        if precision == "mixed":
-            self._solver = RefinedSolver(reuse=self.aic_reuse)
-        elif self.aic_reuse:
-            self._solver = RefinedSolver(reuse=True, factor_dtype=torch.float64)
-        else:
-            self._solver = None
+        # one kept factor per system kind: the static AIC carries the lumped
+        # wake, the dynamic one does not
+        self._solvers = {True: RefinedSolver(), False: RefinedSolver()} if self.aic_reuse else None
+        self._refactor = False
         self._aic = None
 
     # ------------------------------------------------------------------ setup
@@ -269,14 +269,20 @@ class UVLM:
                                mirror=self.mirror)
         return A
 
-    def _solve(self, A, rhs):
-        if self._solver is None:
+    def _solve(self, A, rhs, static):
+        if self._solvers is None:
             return K.lu_solve(K.lu_factor(A, overwrite=True), rhs)
-        # convergence of the reused FP32 factor is checked on the host (one
-        # sync) after every refactorisation and every 16th solve after that
-        self._n_solves = getattr(self, "_n_solves", 0) + 1
-        check = self._solver.factorisations == 0 or self._n_solves % 16 == 1
-        return self._solver.solve(A, rhs, check=check)
+        solver = self._solvers[bool(static)]
+        if self._refactor:
+            solver.factor(A)
+        return solver.submit(A, rhs)
+
+    def _confirm(self, static) -> bool:
+        """Stop-test outcome of the step's circulation solve (waits for the
+        solve only, not for the rest of the step)."""
+        if self._solvers is None:
+            return True
+        return self._solvers[bool(static)].confirm() >= 0
 
     def _tables(self, zetas, gammas):
         return segment_tables(zetas, gammas, [self.kernel] * len(zetas), self.mirror)
@@ -355,7 +361,7 @@ class UVLM:
                     v_w = K.pair_sum(seg, c_all, kind, eps, rcut, L.PREC_FP64,
                                      L.OUT_STORE if v_w is None else L.OUT_ADD, out=v_w)
         rhs = K.rhs(c_all, nc_all, cd_all if not static else None, v_w, ff, t_n)
-        gvec = self._solve(A, rhs)
+        gvec = self._solve(A, rhs, static)
         gamma_b_n = []
         for i, gd in enumerate(self.grid_disc):
             g = gvec[self._b_off[i]:self._b_off[i + 1]].reshape(gd.m, gd.n)
@@ -433,6 +439,18 @@ class UVLM:
         out = None if static else snap
         res = self.base_solve_from_grid(q_nm1, case.t[i_ts], zeta_b_n, zeta_b_nm1, zeta_b_dot_n, static, horseshoe,
                                         out=out)
+        if not self._confirm(static):
+            # the kept factor no longer serves this step's AIC: refactor on it
+            # and compute the step again (a pure function of its inputs)
+            self._refactor = True
+            try:
+                res = self.base_solve_from_grid(q_nm1, case.t[i_ts], zeta_b_n, zeta_b_nm1, zeta_b_dot_n, static,
+                                                horseshoe, out=out)
+            finally:
+                self._refactor = False
+            if not self._confirm(static):
+                raise L.FlapjaxLibError("circulation solve failed its stop test after refactorisation "
+                                        "(singular or ill-conditioned AIC, or non-finite input)")
         self._store(case, i_ts, res, static, horseshoe)
         if horseshoe:
             self.initialise_wake(zeta_b_n, out=snap["zeta_w"])

@@ -178,23 +178,44 @@ int fj_lu_solve(const double* LU, int n, int lda, const int32_t* ipiv, const voi
 int fj_lu_factor_f32(float* A, int n, int lda, int32_t* ipiv, int32_t* info, void* aux,
                      void* stream);
 
-/* x <- solve(A, b) from the FP32 factor LU32 (ipiv, aux from fj_lu_factor_f32)
- * of A or of a nearby matrix, refined in FP64: r = b - A x with the FP64 A,
- * x += LU32^{-1} r, until the correction is <= tol |x|_inf or max_iter
- * corrections were made.  iters (device int32) receives the corrections made,
- * negated if the tolerance was not reached.  workspace:
- * fj_refine_workspace_bytes(n). */
+/* Mixed-precision iterative refinement (LAPACK dsgesv's scheme): x0 = M b,
+ * then r = b - A x in FP64 with the CURRENT FP64 A and x += M r, where M is an
+ * approximation of A^{-1}: an LU factor of A or of a nearby matrix (e.g. the
+ * AIC of an earlier step of a rigid-body motion, which changes by rounding
+ * only), FP32 or FP64, or its explicit inverse.  Stop test (dsgesv's):
+ * ||r||_inf <= tol ||A||_inf ||x||_inf, tol <= 0 meaning sqrt(n) * 2^-53.
+ * At most max_iter (1..64) corrections; iters (device int32) receives the
+ * corrections made, negated if the stop test never passed (also for NaN).
+ * workspace: fj_refine_workspace_bytes(n).  All work stays on the stream; no
+ * host synchronisation. */
 size_t fj_refine_workspace_bytes(int n);
+
+/* ... with the FP32 factor LU32 (ipiv, aux from fj_lu_factor_f32). */
 int fj_solve_refined(const double* A, int lda, const float* LU32, int ldlu, const int32_t* ipiv,
                      const void* aux, int n, const double* b, double* x, int max_iter,
                      double tol, int32_t* iters, void* workspace, void* stream);
 
-/* Same with an FP64 factor of a nearby matrix (a factor kept across time
- * steps while the AIC changes only by rounding, e.g. rigid motion): the
- * result solves the CURRENT A to FP64 accuracy. */
+/* ... with an FP64 factor (fj_lu_factor). */
 int fj_solve_refined_f64(const double* A, int lda, const double* LU, int ldlu,
                          const int32_t* ipiv, const void* aux, int n, const double* b, double* x,
                          int max_iter, double tol, int32_t* iters, void* workspace, void* stream);
+
+/* Explicit inverse (row-major, leading dimension ldi) of the matrix whose
+ * factor fj_lu_factor / fj_lu_factor_f32 produced: A^{-1} = U^{-1} L^{-1} P by
+ * blocked triangular sweeps on the matrix cores.  Used as M above when one
+ * factor serves many solves: each solve is then a few mat-vecs over all CUs. */
+int fj_lu_inverse(const double* LU, int n, int lda, const int32_t* ipiv, const void* aux,
+                  double* Ainv, int ldi, void* stream);
+int fj_lu_inverse_f32(const float* LU, int n, int lda, const int32_t* ipiv, const void* aux,
+                      float* Ainv, int ldi, void* stream);
+
+/* Refinement (as above) with M = an explicit inverse, FP64 or FP32. */
+int fj_solve_refined_inv(const double* A, int lda, const double* Ainv, int ldi, int n,
+                         const double* b, double* x, int max_iter, double tol, int32_t* iters,
+                         void* workspace, void* stream);
+int fj_solve_refined_inv_f32(const double* A, int lda, const float* Ainv, int ldi, int n,
+                             const double* b, double* x, int max_iter, double tol,
+                             int32_t* iters, void* workspace, void* stream);
 
 /* ---- K4-K8: stencils, convection, forces ------------------------------- */
 

@@ -121,7 +121,12 @@ def test_refined_solver_on_aic(dev, n):
     xr = rs.solve(A, b).cpu().numpy()
     print(f"refined solve n={n}: sweeps={rs.last_iters}, rel err vs FP64 LU={rel_err(xr, x64):.2e}")
     assert rel_err(xr, x64) < 1e-12
-    assert rs.last_iters >= 1
+    assert rs.last_iters >= 0 and rs.factorisations == 1
+    # a nearby matrix (same kept inverse) is solved to FP64 accuracy too
+    A2 = A * (1.0 + 1e-7 * torch.cos(torch.arange(n * n, device=dev, dtype=torch.float64)).reshape(n, n))
+    x2 = rs.solve(A2, b).cpu().numpy()
+    assert rs.factorisations == 1 and rs.last_iters >= 1
+    assert rel_err(x2, np.linalg.solve(A2.cpu().numpy(), b.cpu().numpy())) < 1e-12
 
 
 @pytest.mark.parametrize("n", [40, 300, 1024])
@@ -140,4 +145,67 @@ def test_fp32_factor(dev, n):
     assert rel_err(Lm @ np.triu(LUh), A[perm]) < 1e-5
     x, it = K.solve_refined(_t(A, dev), fac, _t(b, dev), max_iter=20)
     assert int(it.item()) > 0
-    assert rel_err(x.cpu().numpy(), np.linalg.solve(A, b)) < 1e-12
+    # dsgesv's stop test holds (backward error within sqrt(n) eps), so the
+    # forward error is within cond(A) times that
+    xh = x.cpu().numpy()
+    bw = np.abs(A @ xh - b).max() / (np.abs(A).sum(1).max() * np.abs(xh).max())
+    assert bw <= np.sqrt(n) * 2.0 ** -53
+    assert rel_err(xh, np.linalg.solve(A, b)) < 4 * np.linalg.cond(A, np.inf) * np.sqrt(n) * 2.0 ** -53
+
+
+@pytest.mark.parametrize("n", [1, 37, 64, 65, 300, 1024])
+def test_lu_inverse(dev, n):
+    """Explicit inverse from the factor (FP64 and FP32) against LAPACK's."""
+    from flapjax_amd import _kernels as K
+
+    rng = np.random.default_rng(n + 3)
+    A = rng.normal(size=(n, n)) + np.eye(n) * 2
+    ref = np.linalg.inv(A)
+    X = K.lu_inverse(K.lu_factor(_t(A, dev))).cpu().numpy()
+    assert rel_err(X, ref) < 1e-11
+    assert np.abs(X @ A - np.eye(n)).max() < 1e-11
+    X32 = K.lu_inverse(K.lu_factor_f32(torch.as_tensor(A, dtype=torch.float32, device=dev))).cpu().numpy()
+    assert X32.dtype == np.float32
+    assert rel_err(X32.astype(float), ref) < 1e-3
+
+
+@pytest.mark.parametrize("n", [64, 500, 1024])
+def test_solve_refined_inv(dev, n):
+    """Refinement around an explicit inverse: FP64 and FP32 inverses reach the
+    FP64 solution of the CURRENT matrix; failures report negative counts."""
+    from flapjax_amd import _kernels as K
+
+    rng = np.random.default_rng(n + 5)
+    A = rng.normal(size=(n, n)) + np.eye(n) * 4
+    b = rng.normal(size=n)
+    ref = np.linalg.solve(A, b)
+    X = K.lu_inverse(K.lu_factor(_t(A, dev)))
+    x, it = K.solve_refined_inv(_t(A, dev), X, _t(b, dev))
+    assert int(it.item()) >= 0
+    assert rel_err(x.cpu().numpy(), ref) < 1e-12
+    # the stop test: backward error within sqrt(n) eps
+    xh = x.cpu().numpy()
+    bw = np.abs(A @ xh - b).max() / (np.abs(A).sum(1).max() * np.abs(xh).max())
+    assert bw <= np.sqrt(n) * 2.0 ** -53 * 4
+    # FP32 inverse: more corrections, same answer
+    X32 = K.lu_inverse(K.lu_factor_f32(torch.as_tensor(A, dtype=torch.float32, device=dev)))
+    x32, it32 = K.solve_refined_inv(_t(A, dev), X32, _t(b, dev), max_iter=10)
+    assert int(it32.item()) >= 1
+    assert rel_err(x32.cpu().numpy(), ref) < 1e-12
+    # a nearby matrix with the kept inverse
+    A2 = A + 1e-6 * rng.normal(size=(n, n))
+    x2, it2 = K.solve_refined_inv(_t(A2, dev), X, _t(b, dev))
+    assert int(it2.item()) >= 1
+    assert rel_err(x2.cpu().numpy(), np.linalg.solve(A2, b)) < 1e-12
+    # an unrelated inverse does not pass the stop test: negative count
+    Xbad = K.lu_inverse(K.lu_factor(_t(rng.normal(size=(n, n)) + np.eye(n) * 4, dev)))
+    _, itb = K.solve_refined_inv(_t(A, dev), Xbad, _t(b, dev), max_iter=2)
+    assert int(itb.item()) == -2
+    # non-finite right-hand side: negative count
+    bn = b.copy()
+    bn[n // 2] = np.nan
+    _, itn = K.solve_refined_inv(_t(A, dev), X, _t(bn, dev), max_iter=2)
+    assert int(itn.item()) < 0
+    # zero right-hand side: x = 0, converged at once
+    x0, it0 = K.solve_refined_inv(_t(A, dev), X, _t(np.zeros(n), dev))
+    assert int(it0.item()) == 0 and not x0.any()
