@@ -171,6 +171,72 @@ def rollup_roofline(reps=20):
     return res, (tgt.cpu().numpy(), seg.cpu().numpy())
 
 
+def pair_sweep(exps=range(12, 18), reps=3):
+    """Square pair-sum sweep T = S = 2^k (seeded random filaments of length
+    ~0.05 in a unit box), FP64 and mixed: Gpairs/s per size."""
+    import torch
+
+    from flapjax_amd import _kernels as K
+    from flapjax_amd import _lib as L
+
+    rng = np.random.default_rng(0)
+    res = {}
+    for k in exps:
+        n = 1 << k
+        y0 = rng.uniform(0.0, 1.0, (n, 3))
+        ends = torch.as_tensor(np.stack((y0, y0 + 0.05 * rng.normal(size=(n, 3))), 1), device="cuda")
+        seg = K.segments_from_endpoints(ends, torch.as_tensor(rng.uniform(-1, 1, n), device="cuda"),
+                                        L.KERNEL_EPSILON, 1e-7)
+        tgt = torch.as_tensor(rng.uniform(0.0, 1.0, (n, 3)), device="cuda")
+        out = torch.empty_like(tgt)
+        row = {}
+        for name, prec in (("fp64", L.PREC_FP64), ("mixed", L.PREC_MIXED)):
+            K.pair_sum(seg, tgt, L.KERNEL_EPSILON, 1e-7, 1e-4, prec, out=out)
+            s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            s.record()
+            for _ in range(reps):
+                K.pair_sum(seg, tgt, L.KERNEL_EPSILON, 1e-7, 1e-4, prec, out=out)
+            e.record()
+            torch.cuda.synchronize()
+            row[name] = round(n * n * reps / (s.elapsed_time(e) * 1e-3) / 1e9, 2)
+        res[f"2^{k}"] = row
+    return res
+
+
+def convection_bench(m_star=4000, n=1000, reps=20):
+    """Prescribed-wake convection step (circulation shift + TE row +
+    x + dt U for every wake vertex) on a large synthetic wake: algorithmic
+    bytes 48 m*(n+1) + 24(n+1) + 16 m* n per step against HBM."""
+    import torch
+
+    import flapjax_amd as fj
+    from flapjax_amd import _kernels as K
+
+    dev = "cuda"
+    rng = np.random.default_rng(3)
+    zb = torch.as_tensor(rng.normal(size=(2, n + 1, 3)), device=dev)
+    gb = torch.as_tensor(rng.normal(size=(1, n)), device=dev)
+    zw = [torch.as_tensor(rng.normal(size=(m_star + 1, n + 1, 3)), device=dev) for _ in range(2)]
+    gw = [torch.as_tensor(rng.normal(size=(m_star, n)), device=dev) for _ in range(2)]
+    flow = fj.Constant(np.array([1.0, 0.0, 0.0]), 1.225, True)
+
+    def step(a, b):
+        K.wake_shift(gb, gw[a], zb, m_star, gw[b], zw[b])
+        K.wake_convect(zw[a], None, flow, 0.0, 0.01, zw[b])
+
+    step(0, 1)
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.record()
+    for r in range(reps):
+        step(r % 2, 1 - r % 2)
+    e.record()
+    torch.cuda.synchronize()
+    ms = s.elapsed_time(e) / reps
+    nbytes = 48 * m_star * (n + 1) + 24 * (n + 1) + 16 * m_star * n
+    return {"wake": f"{m_star} rows x {n} panels", "us_per_step": round(1e3 * ms, 2),
+            "bytes_per_step": nbytes, "gb_s": round(nbytes / (ms * 1e-3) / 1e9, 1), "peak_gb_s": 8000}
+
+
 def profile_rollup(reps=10, copy_mib=256):
     """Counter-collection workload: the rollup pair-sum launch alone, ``reps``
     times per form, then ``reps`` plain device copies of ``copy_mib`` MiB (the
@@ -201,25 +267,46 @@ def serve(precision, n_cases, rate, m_star_cap, steps_range):
     return stats
 
 
+def _timed(fn, min_seconds, max_reps=50):
+    t0 = time.perf_counter()
+    reps = 0
+    while True:
+        fn()
+        reps += 1
+        el = time.perf_counter() - t0
+        if el >= min_seconds or reps >= max_reps:
+            return reps, el
+
+
 def cpu_baseline(sample, min_seconds=10.0):
-    """The oracle's C restatement of the FP64 pair sum, all host threads."""
+    """The oracle's C restatement of the FP64 pair sum on all host threads
+    (``value``), plus the same at one thread and the oracle's chunked NumPy
+    pair sum on a smaller sample of the same targets."""
     from oracle import c_oracle
+    from oracle import uvlm_oracle as O
 
     tgt, seg = sample
     y0, y1, g = seg[:, 0:3], seg[:, 3:6], seg[:, 10]
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    t0 = time.perf_counter()
-    reps = 0
-    while True:
-        c_oracle.pair_sum_eps(tgt, y0, y1, g)
-        reps += 1
-        el = time.perf_counter() - t0
-        if el >= min_seconds or reps >= 50:
-            break
+    c_oracle.set_threads(threads)
+    reps, el = _timed(lambda: c_oracle.pair_sum_eps(tgt, y0, y1, g), min_seconds)
     pairs = reps * tgt.shape[0] * y0.shape[0]
-    return {"value": pairs / el / 1e9, "unit": "Gpairs/s", "cores": threads, "kind": "port",
-            "sample": f"{reps} x rollup pair sum of one step ({tgt.shape[0]} targets x {y0.shape[0]} filaments), "
-                      f"FP64 C (gcc -O3 -fopenmp), {el:.1f} s"}
+    out = {"value": pairs / el / 1e9, "unit": "Gpairs/s", "cores": threads, "kind": "port",
+           "sample": f"{reps} x rollup pair sum of one step ({tgt.shape[0]} targets x {y0.shape[0]} filaments), "
+                     f"FP64 C (gcc -O3 -fopenmp), {el:.1f} s"}
+    # one thread, 1/13 of the targets
+    t1 = tgt[::13]
+    c_oracle.set_threads(1)
+    reps, el = _timed(lambda: c_oracle.pair_sum_eps(t1, y0, y1, g), min_seconds / 4)
+    c_oracle.set_threads(threads)
+    out["single_core"] = {"value": reps * t1.shape[0] * y0.shape[0] / el / 1e9, "cores": 1,
+                          "sample": f"{reps} x {t1.shape[0]} targets x {y0.shape[0]} filaments, {el:.1f} s"}
+    # the oracle's NumPy form (chunked targets x filaments, as the reference's lax.map rows)
+    tn = tgt[::130]
+    reps, el = _timed(lambda: O.pair_sum(tn, y0, y1, g, "epsilon"), min_seconds / 4)
+    out["numpy"] = {"value": reps * tn.shape[0] * y0.shape[0] / el / 1e9,
+                    "sample": f"{reps} x {tn.shape[0]} targets x {y0.shape[0]} filaments, {el:.1f} s"}
+    return out
 
 
 def main():
@@ -257,6 +344,8 @@ def main():
             srv[prec] = {"cases_per_s": round(st["cases_per_s"], 3), "p50_s": round(st["p50_s"], 4),
                          "p99_s": round(st["p99_s"], 4), "gpairs_per_s": round(st["gpairs_per_s"], 3),
                          "makespan_s": round(st["makespan_s"], 3)}
+    sweep = pair_sweep()
+    conv = convection_bench()
     cpu = None if args.no_cpu else cpu_baseline(sample)
     value = pairs * args.steps / f64["wall_s"] / 1e9
     value_mixed = pairs * args.steps / mix["wall_s"] / 1e9
@@ -292,6 +381,8 @@ def main():
         "serving": {"stream": f"{args.serve_cases} Poisson arrivals at {args.serve_rate}/s, M in {{4,8,16,32}}, "
                               "N in {16,32,64,128}, 50-100 steps, wake rows = min(steps, 32), pitch 0-5 deg, "
                               "k 0.05-0.5", **srv},
+        "pair_sweep_gpairs_s": sweep,
+        "convection": conv,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

@@ -12,8 +12,6 @@
 // reduction pass applies each task's epilogue (store / add / convect).
 #include "fj_common.hpp"
 
-#define FJ_BS_WAVES 4
-#define FJ_BS_THREADS (64 * FJ_BS_WAVES)
 
 struct FjTask {
   int tgt_off, n_tgt;      // packed target rows

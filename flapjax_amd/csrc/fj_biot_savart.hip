@@ -16,10 +16,6 @@
 // free-wake convection).
 #include "fj_common.hpp"
 
-#define FJ_BS_WAVES 4
-#define FJ_BS_THREADS (64 * FJ_BS_WAVES)
-#define FJ_BS_TARGET_BLOCKS 2048
-#define FJ_BS_MIN_SEG_PER_WAVE 8
 
 // ---------------------------------------------------------------------------
 // filament table
@@ -132,7 +128,7 @@ extern "C" int fj_segments_from_endpoints(const double* ends, const double* gamm
 // ---------------------------------------------------------------------------
 static FjPairPlan fj_pair_plan(int n_tgt, int n_seg) {
   FjPairPlan p;
-  p.tpl = (n_tgt >= 16384) ? 2 : 1;
+  p.tpl = (n_tgt >= FJ_BS_TPL2_MIN_TARGETS) ? 2 : 1;
   p.tiles = fj_cdiv(n_tgt, 64 * p.tpl);
   if (p.tiles < 1) p.tiles = 1;
   int chunks = fj_cdiv(FJ_BS_TARGET_BLOCKS, p.tiles);
@@ -162,7 +158,9 @@ __global__ __launch_bounds__(FJ_BS_THREADS) void k_pair_partial_f64(
   __shared__ double red[FJ_BS_WAVES - 1][3][64 * TPL];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int t0 = blockIdx.x * 64 * TPL;
+  int tile, chunk;
+  fj_tile_chunk(tile, chunk);
+  const int t0 = tile * 64 * TPL;
 
   double x[TPL], y[TPL], z[TPL], ax[TPL], ay[TPL], az[TPL];
 #pragma unroll
@@ -175,7 +173,7 @@ __global__ __launch_bounds__(FJ_BS_THREADS) void k_pair_partial_f64(
     ax[p] = 0.0; ay[p] = 0.0; az[p] = 0.0;
   }
 
-  const int s_begin = blockIdx.y * chunk_len;
+  const int s_begin = chunk * chunk_len;
   const int s_end = min(n_seg, s_begin + chunk_len);
   for (int s = s_begin + wave; s < s_end; s += FJ_BS_WAVES) {
     const double* sp = seg + (size_t)s * FJ_SEG_STRIDE;
@@ -211,7 +209,7 @@ __global__ __launch_bounds__(FJ_BS_THREADS) void k_pair_partial_f64(
         sz += red[w][2][p * 64 + lane];
       }
       const int t = t0 + p * 64 + lane;
-      double* o = part + (size_t)blockIdx.y * 3 * ld;
+      double* o = part + (size_t)chunk * 3 * ld;
       o[t] = sx;
       o[ld + t] = sy;
       o[2 * ld + t] = sz;

@@ -10,8 +10,6 @@
 // FP64 accumulators.  Launch shape and reduction are those of the FP64 path.
 #include "fj_common.hpp"
 
-#define FJ_BS_WAVES 4
-#define FJ_BS_THREADS (64 * FJ_BS_WAVES)
 #define FJ_NEAR_FACTOR2 25.0f  // (5 filament lengths)^2 from the start point
 #define FJ_MIXED_FLUSH 16
 
@@ -50,7 +48,9 @@ __global__ __launch_bounds__(FJ_BS_THREADS) void k_pair_partial_mixed(
   __shared__ double red[FJ_BS_WAVES - 1][3][64 * TPL];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int t0 = blockIdx.x * 64 * TPL;
+  int tile, chunk;
+  fj_tile_chunk(tile, chunk);
+  const int t0 = tile * 64 * TPL;
   const float eps2f = (float)kp.eps2;
   const float rcut2f = (float)(kp.rcut * kp.rcut);
 
@@ -67,7 +67,7 @@ __global__ __launch_bounds__(FJ_BS_THREADS) void k_pair_partial_mixed(
     bx[p] = 0.0f; by[p] = 0.0f; bz[p] = 0.0f;
   }
 
-  const int s_begin = blockIdx.y * chunk_len;
+  const int s_begin = chunk * chunk_len;
   const int s_end = min(n_seg, s_begin + chunk_len);
   int since_flush = 0;
   for (int s = s_begin + wave; s < s_end; s += FJ_BS_WAVES) {
@@ -132,7 +132,7 @@ __global__ __launch_bounds__(FJ_BS_THREADS) void k_pair_partial_mixed(
         sz += red[w][2][p * 64 + lane];
       }
       const int t = t0 + p * 64 + lane;
-      double* o = part + (size_t)blockIdx.y * 3 * ld;
+      double* o = part + (size_t)chunk * 3 * ld;
       o[t] = sx;
       o[ld + t] = sy;
       o[2 * ld + t] = sz;

@@ -111,12 +111,12 @@ __device__ __forceinline__ void fj_bs_eval(double x, double y, double z, const d
   const double cy = r1z * r2x - r1x * r2z;
   const double cz = r1x * r2y - r1y * r2x;
   const double e2 = (KIND == FJ_KERNEL_PLAIN) ? 0.0 : kp.eps2;
-  const double s1 = r1x * r1x + r1y * r1y + r1z * r1z + e2;
-  const double s2 = r2x * r2x + r2y * r2y + r2z * r2z + e2;
+  // sums of squares as FMA chains seeded with the additive term
+  const double s1 = fma(r1z, r1z, fma(r1y, r1y, fma(r1x, r1x, e2)));
+  const double s2 = fma(r2z, r2z, fma(r2y, r2y, fma(r2x, r2x, e2)));
   const double a1 = r0x * r1x + r0y * r1y + r0z * r1z;
   const double a2 = a1 - s[11];
-  double D = cx * cx + cy * cy + cz * cz;
-  if (KIND == FJ_KERNEL_EPSILON) D += s[9];
+  const double D = fma(cz, cz, fma(cy, cy, fma(cx, cx, (KIND == FJ_KERNEL_EPSILON) ? s[9] : 0.0)));
   const double D2 = D * D;
   const double w1 = fj_rsqrt(s1 * D2);
   const double w2 = fj_rsqrt(s2 * D2);
@@ -127,6 +127,44 @@ __device__ __forceinline__ void fj_bs_eval(double x, double y, double z, const d
     q = (rad2 > kp.rcut * kp.rcut) ? q : 0.0;
   }
   vx = cx * q; vy = cy * q; vz = cz * q;
+}
+
+// Launch shape of the pair-sum kernels (FP64 and mixed share the plan):
+// FJ_BS_WAVES waves per workgroup take interleaved filaments of a chunk;
+// grid.x = target tiles of 64 * tpl, grid.y = filament chunks sized so that
+// about FJ_BS_TARGET_BLOCKS workgroups run, each wave with at least
+// FJ_BS_MIN_SEG_PER_WAVE filaments; two targets per lane from
+// FJ_BS_TPL2_MIN_TARGETS targets on.
+#ifndef FJ_BS_WAVES
+#define FJ_BS_WAVES 4
+#endif
+#define FJ_BS_THREADS (64 * FJ_BS_WAVES)
+#ifndef FJ_BS_TARGET_BLOCKS
+#define FJ_BS_TARGET_BLOCKS 2048
+#endif
+#ifndef FJ_BS_MIN_SEG_PER_WAVE
+#define FJ_BS_MIN_SEG_PER_WAVE 8
+#endif
+#ifndef FJ_BS_TPL2_MIN_TARGETS
+#define FJ_BS_TPL2_MIN_TARGETS 16384
+#endif
+
+// (tile, chunk) of this workgroup in a (tiles, chunks) pair-sum grid, mapped
+// so that each XCD works on a contiguous band of chunks: workgroups are
+// handed to the 8 XCDs round-robin by linear id, so XCD k runs ids
+// k, k + 8, ...; giving those ids consecutive chunk-major slots means each
+// XCD's L2 holds only its own chunks' filament records.
+#define FJ_NUM_XCD 8
+__device__ __forceinline__ void fj_tile_chunk(int& tile, int& chunk) {
+  const int tiles = gridDim.x;
+  const int total = tiles * gridDim.y;
+  const int lin = blockIdx.x + blockIdx.y * tiles;
+  const int k = lin % FJ_NUM_XCD, idx = lin / FJ_NUM_XCD;
+  int base = 0;  // workgroups on XCDs 0 .. k-1
+  for (int j = 0; j < k; ++j) base += (total - j + FJ_NUM_XCD - 1) / FJ_NUM_XCD;
+  const int slot = base + idx;
+  chunk = slot / tiles;
+  tile = slot - chunk * tiles;
 }
 
 // Launch plan of a pair sum (fj_biot_savart.hip)

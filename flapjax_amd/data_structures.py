@@ -62,6 +62,15 @@ class AeroCase:
     alpha: list = field(default_factory=list)
     c: list = field(default_factory=list)
     nc: list = field(default_factory=list)
+    surf_b_names: list = field(default_factory=list)
+    surf_w_names: list = field(default_factory=list)
+
+    def __post_init__(self):
+        # aero/uvlm.py:330-331 naming
+        if not self.surf_b_names:
+            self.surf_b_names = [f"surf_{i}_bound" for i in range(len(self.grid_disc))]
+        if not self.surf_w_names:
+            self.surf_w_names = [f"surf_{i}_wake" for i in range(len(self.grid_disc))]
 
     @classmethod
     def zeros(cls, grid_disc, n_tstep: int, device) -> "AeroCase":
@@ -83,6 +92,14 @@ class AeroCase:
                 dst[i_ts].copy_(src[i_ts if self.n_tstep > 1 else 0])
         out.t[i_ts] = self.t[i_ts if self.n_tstep > 1 else 0]
         return out
+
+    def plot(self, directory, plot_bound: bool = True, plot_wake: bool = True, index=None):
+        """VTK output (aero/data_structures.py:401-461): per-surface ``.vts``
+        files for a single-step case, per-step files and one ``.pvd`` per
+        surface and kind for a history."""
+        from .vtk_output import plot_case
+
+        return plot_case(self, directory, plot_bound, plot_wake, index)
 
     def to_numpy(self) -> dict:
         res = {"t": self.t.copy()}

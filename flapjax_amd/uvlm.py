@@ -275,6 +275,7 @@ class UVLM:
         solver = self._solvers[bool(static)]
         if self._refactor:
             solver.factor(A)
+        self._last_system = (A, rhs)
         return solver.submit(A, rhs)
 
     def _confirm(self, static) -> bool:
@@ -449,8 +450,13 @@ class UVLM:
             finally:
                 self._refactor = False
             if not self._confirm(static):
-                raise L.FlapjaxLibError("circulation solve failed its stop test after refactorisation "
-                                        "(singular or ill-conditioned AIC, or non-finite input)")
+                A, rhs = self._last_system
+                if bool(torch.isfinite(A).all()) and bool(torch.isfinite(rhs).all()):
+                    raise L.FlapjaxLibError("circulation solve failed its stop test after refactorisation "
+                                            "(singular or ill-conditioned AIC)")
+                # non-finite geometry or circulation: like the reference, the
+                # NaN/inf propagates into the results instead of raising
+                warn(f"Non-finite values in the circulation system at time step {i_ts}; results are not finite.")
         self._store(case, i_ts, res, static, horseshoe)
         if horseshoe:
             self.initialise_wake(zeta_b_n, out=snap["zeta_w"])

@@ -6,7 +6,12 @@
  *   d = r1 / sqrt(|r1|^2 + eps^2) - r2 / sqrt(|r2|^2 + eps^2).
  * out[t] = sum_s g[s] K(x_t; y0[s], y1[s]); OpenMP over targets. */
 #include <math.h>
+#include <omp.h>
 #include <stddef.h>
+
+/* OpenMP team size of the following calls (CPU-baseline runs at 1 core and
+ * at all host cores in one process). */
+void fj_oracle_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
 
 void fj_oracle_pair_sum_eps(const double* x, long nt, const double* y0, const double* y1,
                             const double* g, long ns, double eps, double* out) {

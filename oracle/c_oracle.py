@@ -22,7 +22,13 @@ def _lib():
     lib = ctypes.CDLL(str(LIB))
     p = ctypes.POINTER(ctypes.c_double)
     lib.fj_oracle_pair_sum_eps.argtypes = [p, ctypes.c_long, p, p, p, ctypes.c_long, ctypes.c_double, p]
+    lib.fj_oracle_set_threads.argtypes = [ctypes.c_int]
     return lib
+
+
+def set_threads(n: int) -> None:
+    """OpenMP threads of the following pair sums."""
+    _lib().fj_oracle_set_threads(int(n))
 
 
 def pair_sum_eps(targets, y0, y1, g, eps=1e-7):

@@ -140,3 +140,18 @@ def test_variable_wake_discretisation(dev, free_wake):
     dg = gpu.solve_prescribed_dynamic(gpu.solve_static(), hg_t, hgd_t)
     _cmp(dg, do, ("gamma_b", "gamma_w", "f_steady"), TOL)
     _cmp(dg, do, ("zeta_w",), TOL_WAKE)
+
+
+def test_non_finite_motion_propagates_like_the_reference():
+    """A NaN in the prescribed motion at step k gives NaN results from step k
+    on (with a warning), not an exception -- the reference's behaviour."""
+    m, n, m_star, steps = 4, 8, 6, 6
+    _, gpu, dt = _setup(m, n, m_star, 4.0, 5.0, free_wake=True)
+    hg_t, hg_dot_t = O.pitch_plunge_motion(n, 4.0, steps, dt, np.deg2rad(5.0), np.deg2rad(3.0), 0.05, 1.6)
+    hg_t = hg_t.copy()
+    hg_t[4, 2, 0, 3] = np.nan
+    with pytest.warns(UserWarning, match="Non-finite"):
+        d = gpu.solve_prescribed_dynamic(gpu.solve_static(hg=hg_t[0]), hg_t, hg_dot_t)
+    g = d.gamma_b[0].cpu().numpy()
+    assert np.isfinite(g[:4]).all()
+    assert not np.isfinite(g[4]).all()
