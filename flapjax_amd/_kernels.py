@@ -72,6 +72,63 @@ def segments_from_grid(zeta, gamma, kernel_kind, eps, out=None, mirror=None):
     return out
 
 
+def segments_from_grids(grids, kernel_kind, eps, mirror=None, out=None):
+    """One table for several grids: ``grids`` is a list of (zeta, gamma,
+    mirrored) with zeta (m+1, n+1, 3) and gamma (m, n) or None; records are
+    concatenated in list order (fj_segments_from_grids)."""
+    refs = (L.FjGridRef * max(1, len(grids)))()
+    total = 0
+    for k, (zeta, gamma, mirrored) in enumerate(grids):
+        if zeta.dim() != 3 or zeta.shape[-1] != 3 or zeta.dtype != F64 or not zeta.is_contiguous():
+            raise ValueError("each grid must be a contiguous float64 (m+1, n+1, 3) tensor")
+        m, n = zeta.shape[0] - 1, zeta.shape[1] - 1
+        if gamma is not None and (tuple(gamma.shape) != (m, n) or gamma.dtype != F64 or not gamma.is_contiguous()):
+            raise ValueError(f"gamma must be a contiguous float64 ({m}, {n}) tensor")
+        refs[k].zeta = zeta.data_ptr()
+        refs[k].gamma = gamma.data_ptr() if gamma is not None else None
+        refs[k].m, refs[k].n, refs[k].mirror = m, n, int(bool(mirrored))
+        total += n_segments(m, n)
+    if out is None:
+        dev = grids[0][0].device if grids else _dev()
+        out = torch.empty((total, L.SEG_STRIDE), dtype=F64, device=dev)
+    elif out.shape[0] != total or out.dtype != F64 or not out.is_contiguous():
+        raise ValueError(f"table output must be a contiguous float64 ({total}, {L.SEG_STRIDE}) tensor")
+    mp = mn = None
+    keep = []
+    if mirror is not None:
+        mp, a = L.host3(mirror[0])
+        mn, b = L.host3(mirror[1])
+        keep += [a, b]
+    if total:
+        rc = L.lib().fj_segments_from_grids(len(grids), refs, mp, mn, int(kernel_kind), float(eps), L.ptr(out),
+                                            L.stream())
+        L.check(rc, "fj_segments_from_grids")
+    return out
+
+
+def step_geometry(hg_n, hg_nm1, hg_dot_n, dof, zeta0, zeta_n=None, zeta_nm1=None, zeta_dot_n=None, c=None,
+                  nc=None, c_dot=None, mid=None):
+    """fj_step_geometry: every output is optional (pass preallocated tensors)."""
+    m, n = zeta0.shape[0] - 1, zeta0.shape[1] - 1
+    p = lambda t: L.ptr(t) if t is not None else None  # noqa: E731
+    rc = L.lib().fj_step_geometry(L.ptr(hg_n), p(hg_nm1), p(hg_dot_n), L.ptr(dof), L.ptr(zeta0), m, n, p(zeta_n),
+                                  p(zeta_nm1), p(zeta_dot_n), p(c), p(nc), p(c_dot), p(mid), L.stream())
+    L.check(rc, "fj_step_geometry")
+
+
+def step_forces(zeta, zeta_dot, gamma_b, gamma_b_nm1, gamma_dot_nm1, gamma_w0, v_mid, nc, flow, time, rho, dt,
+                relax, f_steady, gamma_dot_n=None, f_unsteady=None):
+    """fj_step_forces into the given output tensors."""
+    m, n = gamma_b.shape
+    p = lambda t: L.ptr(t) if t is not None else None  # noqa: E731
+    fd = flow_desc(flow) if flow is not None else None
+    rc = L.lib().fj_step_forces(L.ptr(zeta), p(zeta_dot), L.ptr(gamma_b), p(gamma_b_nm1), p(gamma_dot_nm1),
+                                p(gamma_w0), L.ptr(v_mid), p(nc), m, n, ctypes.byref(fd) if fd is not None else None,
+                                float(time), float(rho), float(dt), float(relax), L.ptr(f_steady), p(gamma_dot_n),
+                                p(f_unsteady), L.stream())
+    L.check(rc, "fj_step_forces")
+
+
 def segments_from_endpoints(ends, gamma, kernel_kind, eps):
     """Filament table from end points (S, 2, 3) and circulations (S,)."""
     ends = _c64(ends, (2, 3), "filaments").reshape(-1, 2, 3)
@@ -89,8 +146,10 @@ def segments_from_endpoints(ends, gamma, kernel_kind, eps):
 
 
 def pair_sum(seg, targets, kernel_kind, eps, rcut, precision=L.PREC_FP64, mode=L.OUT_STORE, out=None, flow=None,
-             time=0.0, dt=0.0):
-    """Induced velocity of the filament table `seg` at `targets` (T, 3)."""
+             time=0.0, dt=0.0, workspace=None):
+    """Induced velocity of the filament table `seg` at `targets` (T, 3).
+    ``workspace`` (float64 device tensor of at least
+    fj_pair_sum_workspace_bytes) avoids an allocation per call."""
     targets = _c64(targets, (3,), "targets")
     flat = targets.reshape(-1, 3)
     nt = flat.shape[0]
@@ -102,7 +161,10 @@ def pair_sum(seg, targets, kernel_kind, eps, rcut, precision=L.PREC_FP64, mode=L
     if nt == 0:
         return out
     ws_bytes = L.lib().fj_pair_sum_workspace_bytes(nt, ns, int(precision))
-    ws = torch.empty(ws_bytes // 8 + 1, dtype=F64, device=flat.device)
+    if workspace is not None and workspace.numel() * 8 >= ws_bytes:
+        ws = workspace
+    else:
+        ws = torch.empty(ws_bytes // 8 + 1, dtype=F64, device=flat.device)
     fd = flow_desc(flow) if flow is not None else None
     rc = L.lib().fj_pair_sum(L.ptr(seg) if ns else None, ns, L.ptr(flat), nt, L.ptr(out), int(kernel_kind),
                              float(eps), float(rcut), int(precision), int(mode),

@@ -78,6 +78,10 @@ _SIGS = {
                               c_void_p, c_void_p, c_void_p], c_int),
     "fj_solve_refined_inv_f32": ([c_void_p, c_int, c_void_p, c_int, c_int, c_void_p, c_void_p, c_int, c_double,
                                   c_void_p, c_void_p, c_void_p], c_int),
+    "fj_segments_from_grids": ([c_int, c_void_p, c_void_p, c_void_p, c_int, c_double, c_void_p, c_void_p], c_int),
+    "fj_step_geometry": ([c_void_p] * 5 + [c_int, c_int] + [c_void_p] * 8, c_int),
+    "fj_step_forces": ([c_void_p] * 8 + [c_int, c_int, POINTER(FjFlowfield), c_double, c_double, c_double, c_double,
+                        c_void_p, c_void_p, c_void_p, c_void_p], c_int),
     "fj_grid_from_hg": ([c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p], c_int),
     "fj_colloc_normals": ([c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p], c_int),
     "fj_filament_midpoints": ([c_void_p, c_int, c_int, c_void_p, c_void_p], c_int),
@@ -107,6 +111,14 @@ _SIGS = {
     "fj_flow_eval": ([POINTER(FjFlowfield), c_void_p, c_int, c_double, c_void_p, c_void_p], c_int),
 }
 
+
+
+class FjGridRef(ctypes.Structure):
+    """fj_grid_ref: one source grid of fj_segments_from_grids."""
+
+    _fields_ = [("zeta", c_void_p), ("gamma", c_void_p), ("m", c_int), ("n", c_int), ("mirror", c_int)]
+
+
 _lib = None
 
 
@@ -134,8 +146,18 @@ def load(require_gpu: bool = False):
     return _lib
 
 
+_gpu_checked = False
+
+
 def lib():
-    return load(require_gpu=True)
+    """The loaded library; raises if it is not built or no GPU is visible
+    (checked once per process)."""
+    global _gpu_checked
+    if _lib is not None and _gpu_checked:
+        return _lib
+    out = load(require_gpu=True)
+    _gpu_checked = True
+    return out
 
 
 def check(rc: int, what: str) -> None:

@@ -20,15 +20,14 @@
 // ---------------------------------------------------------------------------
 // filament table
 // ---------------------------------------------------------------------------
-__global__ void k_segments_from_grid(const double* __restrict__ zeta,
-                                     const double* __restrict__ gamma, int m, int n, int mirror,
-                                     double mpx, double mpy, double mpz, double mnx, double mny,
-                                     double mnz, int kernel, double eps,
-                                     double* __restrict__ seg) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+// record of filament s of grid (m+1, n+1) with ring strengths gamma (or unit
+// strength when gamma is NULL), optionally reflected in the mirror plane
+__device__ __forceinline__ void fj_segment_record(const double* __restrict__ zeta,
+                                                  const double* __restrict__ gamma, int m,
+                                                  int n, int s, int mirror, const double* mp,
+                                                  const double* mn, int kernel, double eps,
+                                                  double* __restrict__ o) {
   const int n_ch = m * (n + 1);
-  const int n_all = n_ch + (m + 1) * n;
-  if (s >= n_all) return;
   int ia, ja, ib, jb;
   double g = 0.0;
   if (s < n_ch) {
@@ -49,15 +48,14 @@ __global__ void k_segments_from_grid(const double* __restrict__ zeta,
   double y1[3] = {b[0], b[1], b[2]};
   if (mirror) {
     // aero/utils.py:651-663, then the image is subtracted (aero/aic.py:306-308)
-    const double d0 = (y0[0] - mpx) * mnx + (y0[1] - mpy) * mny + (y0[2] - mpz) * mnz;
-    const double d1 = (y1[0] - mpx) * mnx + (y1[1] - mpy) * mny + (y1[2] - mpz) * mnz;
-    y0[0] -= 2.0 * d0 * mnx; y0[1] -= 2.0 * d0 * mny; y0[2] -= 2.0 * d0 * mnz;
-    y1[0] -= 2.0 * d1 * mnx; y1[1] -= 2.0 * d1 * mny; y1[2] -= 2.0 * d1 * mnz;
+    const double d0 = (y0[0] - mp[0]) * mn[0] + (y0[1] - mp[1]) * mn[1] + (y0[2] - mp[2]) * mn[2];
+    const double d1 = (y1[0] - mp[0]) * mn[0] + (y1[1] - mp[1]) * mn[1] + (y1[2] - mp[2]) * mn[2];
+    y0[0] -= 2.0 * d0 * mn[0]; y0[1] -= 2.0 * d0 * mn[1]; y0[2] -= 2.0 * d0 * mn[2];
+    y1[0] -= 2.0 * d1 * mn[0]; y1[1] -= 2.0 * d1 * mn[1]; y1[2] -= 2.0 * d1 * mn[2];
     g = -g;
   }
   const double r0x = y1[0] - y0[0], r0y = y1[1] - y0[1], r0z = y1[2] - y0[2];
   const double r0sq = r0x * r0x + r0y * r0y + r0z * r0z;
-  double* o = seg + (size_t)s * FJ_SEG_STRIDE;
   o[0] = y0[0]; o[1] = y0[1]; o[2] = y0[2];
   o[3] = y1[0]; o[4] = y1[1]; o[5] = y1[2];
   o[6] = r0x; o[7] = r0y; o[8] = r0z;
@@ -69,6 +67,74 @@ __global__ void k_segments_from_grid(const double* __restrict__ zeta,
   f[0] = (float)r0x; f[1] = (float)r0y; f[2] = (float)r0z;
   f[3] = (float)aux; f[4] = (float)g;
   f[5] = 0.0f; f[6] = 0.0f; f[7] = 0.0f;
+}
+
+__global__ void k_segments_from_grid(const double* __restrict__ zeta,
+                                     const double* __restrict__ gamma, int m, int n, int mirror,
+                                     double mpx, double mpy, double mpz, double mnx, double mny,
+                                     double mnz, int kernel, double eps,
+                                     double* __restrict__ seg) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= m * (n + 1) + (m + 1) * n) return;
+  const double mp[3] = {mpx, mpy, mpz}, mn[3] = {mnx, mny, mnz};
+  fj_segment_record(zeta, gamma, m, n, s, mirror, mp, mn, kernel, eps,
+                    seg + (size_t)s * FJ_SEG_STRIDE);
+}
+
+// several grids into one table (their records concatenated in list order)
+#define FJ_MAX_GRIDS 16
+struct FjGridList {
+  const double* zeta[FJ_MAX_GRIDS];
+  const double* gamma[FJ_MAX_GRIDS];
+  int m[FJ_MAX_GRIDS], n[FJ_MAX_GRIDS], mirror[FJ_MAX_GRIDS], off[FJ_MAX_GRIDS + 1];
+  int count;
+  double mp[3], mn[3];
+};
+
+__global__ void k_segments_from_grids(FjGridList gl, int kernel, double eps,
+                                      double* __restrict__ seg) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= gl.off[gl.count]) return;
+  int k = 0;
+  while (s >= gl.off[k + 1]) ++k;
+  fj_segment_record(gl.zeta[k], gl.gamma[k], gl.m[k], gl.n[k], s - gl.off[k], gl.mirror[k], gl.mp,
+                    gl.mn, kernel, eps, seg + (size_t)s * FJ_SEG_STRIDE);
+}
+
+extern "C" int fj_segments_from_grids(int n_grids, const fj_grid_ref* grids,
+                                      const double* mirror_point, const double* mirror_normal,
+                                      int kernel, double eps, double* seg_out, void* stream) {
+  if (n_grids < 0 || (n_grids > 0 && (!grids || !seg_out))) return FJ_ERR_ARG;
+  if (kernel < FJ_KERNEL_EPSILON || kernel > FJ_KERNEL_PLAIN) return FJ_ERR_ARG;
+  size_t base = 0;
+  for (int g0 = 0; g0 < n_grids; g0 += FJ_MAX_GRIDS) {
+    FjGridList gl;
+    gl.count = (n_grids - g0 < FJ_MAX_GRIDS) ? n_grids - g0 : FJ_MAX_GRIDS;
+    gl.off[0] = 0;
+    for (int k = 0; k < gl.count; ++k) {
+      const fj_grid_ref& r = grids[g0 + k];
+      if (r.m < 0 || r.n < 0 || !r.zeta) return FJ_ERR_ARG;
+      if (r.mirror && (!mirror_point || !mirror_normal)) return FJ_ERR_ARG;
+      gl.zeta[k] = r.zeta;
+      gl.gamma[k] = r.gamma;
+      gl.m[k] = r.m;
+      gl.n[k] = r.n;
+      gl.mirror[k] = r.mirror ? 1 : 0;
+      gl.off[k + 1] = gl.off[k] + r.m * (r.n + 1) + (r.m + 1) * r.n;
+    }
+    for (int a = 0; a < 3; ++a) {
+      gl.mp[a] = mirror_point ? mirror_point[a] : 0.0;
+      gl.mn[a] = mirror_normal ? mirror_normal[a] : 0.0;
+    }
+    const int total = gl.off[gl.count];
+    if (total > 0) {
+      hipLaunchKernelGGL(k_segments_from_grids, dim3(fj_cdiv(total, 256)), dim3(256), 0,
+                         (hipStream_t)stream, gl, kernel, eps, seg_out + base * FJ_SEG_STRIDE);
+      FJ_CHECK_LAUNCH();
+    }
+    base += (size_t)total;
+  }
+  return FJ_OK;
 }
 
 extern "C" int fj_segments_from_grid(const double* zeta, const double* gamma, int m, int n,

@@ -140,7 +140,7 @@ __device__ __forceinline__ void fj_bs_eval(double x, double y, double z, const d
 #endif
 #define FJ_BS_THREADS (64 * FJ_BS_WAVES)
 #ifndef FJ_BS_TARGET_BLOCKS
-#define FJ_BS_TARGET_BLOCKS 2048
+#define FJ_BS_TARGET_BLOCKS 8192
 #endif
 #ifndef FJ_BS_MIN_SEG_PER_WAVE
 #define FJ_BS_MIN_SEG_PER_WAVE 8

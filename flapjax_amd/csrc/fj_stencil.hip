@@ -8,23 +8,25 @@
 #define FJ_EW 256
 
 // ---- K8: zeta = R zeta0 + x, zeta_dot = W zeta0 + v (aero/uvlm.py:537-621) --
+// vertex v of the grid mapped by the (4, 4) pose of its beam node
+__device__ __forceinline__ void fj_map_vertex(const double* __restrict__ hg,
+                                              const int32_t* __restrict__ dof,
+                                              const double* __restrict__ z0, int n, int v,
+                                              double* o) {
+  const double* p = z0 + 3 * v;
+  const double* h = hg + 16 * (size_t)dof[v % (n + 1)];
+  for (int a = 0; a < 3; ++a)
+    o[a] = (h[4 * a] * p[0] + h[4 * a + 1] * p[1] + h[4 * a + 2] * p[2]) + h[4 * a + 3];
+}
+
 __global__ void k_grid_from_hg(const double* __restrict__ hg, const double* __restrict__ hg_dot,
                                const int32_t* __restrict__ dof, const double* __restrict__ z0,
                                int m, int n, double* __restrict__ zeta,
                                double* __restrict__ zeta_dot) {
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= (m + 1) * (n + 1)) return;
-  const int j = v % (n + 1);
-  const double* p = z0 + 3 * v;
-  const double* h = hg + 16 * (size_t)dof[j];
-  for (int a = 0; a < 3; ++a)
-    zeta[3 * v + a] = (h[4 * a] * p[0] + h[4 * a + 1] * p[1] + h[4 * a + 2] * p[2]) + h[4 * a + 3];
-  if (hg_dot && zeta_dot) {
-    const double* w = hg_dot + 16 * (size_t)dof[j];
-    for (int a = 0; a < 3; ++a)
-      zeta_dot[3 * v + a] =
-          (w[4 * a] * p[0] + w[4 * a + 1] * p[1] + w[4 * a + 2] * p[2]) + w[4 * a + 3];
-  }
+  fj_map_vertex(hg, dof, z0, n, v, zeta + 3 * v);
+  if (hg_dot && zeta_dot) fj_map_vertex(hg_dot, dof, z0, n, v, zeta_dot + 3 * v);
 }
 
 extern "C" int fj_grid_from_hg(const double* hg, const double* hg_dot, const int32_t* dof_map,
@@ -82,12 +84,16 @@ extern "C" int fj_colloc_normals(const double* zeta, const double* zeta_dot, int
   return FJ_OK;
 }
 
-// ---- filament midpoints (aero/utils.py:162-164) ------------------------------
-__global__ void k_midpoints(const double* __restrict__ z, int m, int n, double* __restrict__ mid) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+// ---- fused step geometry ------------------------------------------------------
+// One launch for everything a time step derives from the beam poses
+// (aero/uvlm.py:537-621, aero/utils.py:87-105 and 162-164, uvlm.py:888-893):
+// threads [0, V) map the vertices at n (and n-1, and the vertex velocities);
+// threads [V, V+P) recompute their panel's four corners from the poses and
+// write collocation point, normal and collocation velocity; threads
+// [V+P, V+P+S) write filament midpoints.  Every value is computed with the
+// same expressions as the separate kernels above, so results are identical.
+__device__ __forceinline__ void fj_filament_ends(int m, int n, int s, int& a, int& b) {
   const int n_ch = m * (n + 1);
-  if (s >= n_ch + (m + 1) * n) return;
-  int a, b;
   if (s < n_ch) {
     const int i = s / (n + 1), j = s % (n + 1);
     a = i * (n + 1) + j; b = a + (n + 1);
@@ -95,6 +101,79 @@ __global__ void k_midpoints(const double* __restrict__ z, int m, int n, double* 
     const int r = s - n_ch, i = r / n, j = r % n;
     a = i * (n + 1) + j; b = a + 1;
   }
+}
+
+__global__ void k_step_geometry(const double* __restrict__ hg, const double* __restrict__ hg_nm1,
+                                const double* __restrict__ hg_dot,
+                                const int32_t* __restrict__ dof, const double* __restrict__ z0,
+                                int m, int n, double* __restrict__ zeta,
+                                double* __restrict__ zeta_nm1, double* __restrict__ zeta_dot,
+                                double* __restrict__ c, double* __restrict__ nc,
+                                double* __restrict__ cd, double* __restrict__ mid) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int V = (m + 1) * (n + 1), P = m * n, S = m * (n + 1) + (m + 1) * n;
+  if (e < V) {
+    if (zeta) fj_map_vertex(hg, dof, z0, n, e, zeta + 3 * e);
+    if (hg_nm1 && zeta_nm1) fj_map_vertex(hg_nm1, dof, z0, n, e, zeta_nm1 + 3 * e);
+    if (hg_dot && zeta_dot) fj_map_vertex(hg_dot, dof, z0, n, e, zeta_dot + 3 * e);
+  } else if (e < V + P) {
+    const int p = e - V;
+    const int i = p / n, j = p % n;
+    const int v00 = i * (n + 1) + j, v01 = v00 + 1, v10 = v00 + (n + 1), v11 = v10 + 1;
+    double z00[3], z01[3], z10[3], z11[3];
+    fj_map_vertex(hg, dof, z0, n, v00, z00);
+    fj_map_vertex(hg, dof, z0, n, v01, z01);
+    fj_map_vertex(hg, dof, z0, n, v10, z10);
+    fj_map_vertex(hg, dof, z0, n, v11, z11);
+    if (c)
+      for (int k = 0; k < 3; ++k) c[3 * p + k] = fj_quad_avg(z00[k], z10[k], z01[k], z11[k]);
+    if (nc) {
+      double d1[3], d2[3];
+      for (int k = 0; k < 3; ++k) {
+        d1[k] = z11[k] - z00[k];
+        d2[k] = z10[k] - z01[k];
+      }
+      nc[3 * p + 0] = d1[1] * d2[2] - d1[2] * d2[1];
+      nc[3 * p + 1] = d1[2] * d2[0] - d1[0] * d2[2];
+      nc[3 * p + 2] = d1[0] * d2[1] - d1[1] * d2[0];
+    }
+    if (cd && hg_dot) {
+      fj_map_vertex(hg_dot, dof, z0, n, v00, z00);
+      fj_map_vertex(hg_dot, dof, z0, n, v01, z01);
+      fj_map_vertex(hg_dot, dof, z0, n, v10, z10);
+      fj_map_vertex(hg_dot, dof, z0, n, v11, z11);
+      for (int k = 0; k < 3; ++k) cd[3 * p + k] = fj_quad_avg(z00[k], z10[k], z01[k], z11[k]);
+    }
+  } else if (e < V + P + S && mid) {
+    const int s = e - V - P;
+    int a, b;
+    fj_filament_ends(m, n, s, a, b);
+    double za[3], zb[3];
+    fj_map_vertex(hg, dof, z0, n, a, za);
+    fj_map_vertex(hg, dof, z0, n, b, zb);
+    for (int k = 0; k < 3; ++k) mid[3 * s + k] = 0.5 * (za[k] + zb[k]);
+  }
+}
+
+extern "C" int fj_step_geometry(const double* hg_n, const double* hg_nm1, const double* hg_dot_n,
+                                const int32_t* dof_map, const double* zeta0, int m, int n,
+                                double* zeta_n, double* zeta_nm1, double* zeta_dot_n, double* c,
+                                double* nc, double* c_dot, double* mid, void* stream) {
+  if (m < 0 || n < 0 || !hg_n || !dof_map || !zeta0) return FJ_ERR_ARG;
+  const int total = (m + 1) * (n + 1) + m * n + m * (n + 1) + (m + 1) * n;
+  hipLaunchKernelGGL(k_step_geometry, dim3(fj_cdiv(total, FJ_EW)), dim3(FJ_EW), 0,
+                     (hipStream_t)stream, hg_n, hg_nm1, hg_dot_n, dof_map, zeta0, m, n, zeta_n,
+                     zeta_nm1, zeta_dot_n, c, nc, c_dot, mid);
+  FJ_CHECK_LAUNCH();
+  return FJ_OK;
+}
+
+// ---- filament midpoints (aero/utils.py:162-164) ------------------------------
+__global__ void k_midpoints(const double* __restrict__ z, int m, int n, double* __restrict__ mid) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= m * (n + 1) + (m + 1) * n) return;
+  int a, b;
+  fj_filament_ends(m, n, s, a, b);
   for (int k = 0; k < 3; ++k) mid[3 * s + k] = 0.5 * (z[3 * a + k] + z[3 * b + k]);
 }
 
@@ -275,12 +354,10 @@ __device__ __forceinline__ void fj_fil_force(const double* __restrict__ z,
   f[2] = rho * (g * (vr[0] * r[1] - vr[1] * r[0]));
 }
 
-__global__ void k_steady_forces(const double* __restrict__ z, const double* __restrict__ zd,
-                                const double* __restrict__ G, const double* __restrict__ gw0,
-                                const double* __restrict__ vmid, int m, int n, FjFlow flow,
-                                double time, double rho, double* __restrict__ fo) {
-  const int v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= (m + 1) * (n + 1)) return;
+__device__ __forceinline__ void fj_vertex_steady_force(
+    const double* __restrict__ z, const double* __restrict__ zd, const double* __restrict__ G,
+    const double* __restrict__ gw0, const double* __restrict__ vmid, int m, int n,
+    const FjFlow& flow, double time, double rho, int v, double* __restrict__ fo) {
   const int i = v / (n + 1), j = v % (n + 1);
   const int n_ch = m * (n + 1);
   double fc_lo[3] = {0, 0, 0}, fc_hi[3] = {0, 0, 0}, fs_lo[3] = {0, 0, 0}, fs_hi[3] = {0, 0, 0};
@@ -318,6 +395,15 @@ __global__ void k_steady_forces(const double* __restrict__ z, const double* __re
   }
 }
 
+__global__ void k_steady_forces(const double* __restrict__ z, const double* __restrict__ zd,
+                                const double* __restrict__ G, const double* __restrict__ gw0,
+                                const double* __restrict__ vmid, int m, int n, FjFlow flow,
+                                double time, double rho, double* __restrict__ fo) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= (m + 1) * (n + 1)) return;
+  fj_vertex_steady_force(z, zd, G, gw0, vmid, m, n, flow, time, rho, v, fo);
+}
+
 extern "C" int fj_steady_forces(const double* zeta, const double* zeta_dot, const double* gamma_b,
                                 const double* gamma_w0, const double* v_mid, int m, int n,
                                 const fj_flowfield* flow, double time, double rho, double* f_out,
@@ -334,19 +420,34 @@ extern "C" int fj_steady_forces(const double* zeta, const double* zeta_dot, cons
 }
 
 // ---- K7: circulation rate and unsteady force (aero/uvlm.py:691-715, 1086-1103)
+__device__ __forceinline__ double fj_gamma_rate(const double* __restrict__ gn,
+                                                const double* __restrict__ gp,
+                                                const double* __restrict__ gdp, int p, double dt,
+                                                double relax) {
+  return (gn[p] - gp[p]) / dt * relax + gdp[p] * (1.0 - relax);
+}
+
 __global__ void k_gamma_dot(const double* __restrict__ gn, const double* __restrict__ gp,
                             const double* __restrict__ gdp, int np, double dt, double relax,
                             double* __restrict__ gd) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= np) return;
-  gd[p] = (gn[p] - gp[p]) / dt * relax + gdp[p] * (1.0 - relax);
+  gd[p] = fj_gamma_rate(gn, gp, gdp, p, dt, relax);
 }
 
-__global__ void k_unsteady_force(const double* __restrict__ gd, const double* __restrict__ nc,
-                                 int m, int n, double rho, double* __restrict__ fo) {
-  const int v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= (m + 1) * (n + 1)) return;
+// f_unsteady at vertex v = split_to_vertex(rho Gamma_dot n) (aero/uvlm.py:1093-1103);
+// gd(p) gives the circulation rate of panel p
+template <typename GD>
+__device__ __forceinline__ void fj_vertex_unsteady_force(GD gd, const double* __restrict__ nc,
+                                                         int m, int n, double rho, int v,
+                                                         double* __restrict__ fo) {
   const int i = v / (n + 1), j = v % (n + 1);
+  double r[2][2];  // rho * Gamma_dot of panels (i-1 or i, j-1 or j)
+  for (int a = 0; a < 2; ++a)
+    for (int q = 0; q < 2; ++q) {
+      const int ii = i - 1 + a, jj = j - 1 + q;
+      r[a][q] = (ii >= 0 && ii < m && jj >= 0 && jj < n) ? rho * gd(ii * n + jj) : 0.0;
+    }
   // split along the chordwise axis, then along the spanwise axis
   for (int k = 0; k < 3; ++k) {
     double s0[2];  // chordwise-split value at panel columns j-1 (0) and j (1)
@@ -354,13 +455,20 @@ __global__ void k_unsteady_force(const double* __restrict__ gd, const double* __
       const int jj = j - 1 + q;
       double hi = 0.0, lo = 0.0;
       if (jj >= 0 && jj < n) {
-        if (i < m) hi = 0.5 * ((rho * gd[i * n + jj]) * nc[3 * (i * n + jj) + k]);
-        if (i > 0) lo = 0.5 * ((rho * gd[(i - 1) * n + jj]) * nc[3 * ((i - 1) * n + jj) + k]);
+        if (i < m) hi = 0.5 * (r[1][q] * nc[3 * (i * n + jj) + k]);
+        if (i > 0) lo = 0.5 * (r[0][q] * nc[3 * ((i - 1) * n + jj) + k]);
       }
       s0[q] = hi + lo;
     }
     fo[3 * v + k] = 0.5 * s0[1] + 0.5 * s0[0];
   }
+}
+
+__global__ void k_unsteady_force(const double* __restrict__ gd, const double* __restrict__ nc,
+                                 int m, int n, double rho, double* __restrict__ fo) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= (m + 1) * (n + 1)) return;
+  fj_vertex_unsteady_force([=](int p) { return gd[p]; }, nc, m, n, rho, v, fo);
 }
 
 extern "C" int fj_gamma_dot_unsteady(const double* gamma_n, const double* gamma_nm1,
@@ -379,6 +487,50 @@ extern "C" int fj_gamma_dot_unsteady(const double* gamma_n, const double* gamma_
                        st, gamma_dot_n, nc, m, n, rho, f_unsteady);
     FJ_CHECK_LAUNCH();
   }
+  return FJ_OK;
+}
+
+// ---- fused step forces -------------------------------------------------------
+// Steady vertex forces, circulation rate and unsteady vertex forces of a
+// dynamic step in one launch: threads [0, V) write both forces of vertex v
+// (the rates of the up to four adjacent panels are recomputed with the rate
+// expression), threads [V, V+P) write the panel rates.
+__global__ void k_step_forces(const double* __restrict__ z, const double* __restrict__ zd,
+                              const double* __restrict__ G, const double* __restrict__ Gp,
+                              const double* __restrict__ Gdp, const double* __restrict__ gw0,
+                              const double* __restrict__ vmid, const double* __restrict__ nc,
+                              int m, int n, FjFlow flow, double time, double rho, double dt,
+                              double relax, double* __restrict__ f_steady,
+                              double* __restrict__ gdot, double* __restrict__ f_uns) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int V = (m + 1) * (n + 1), P = m * n;
+  if (e < V) {
+    fj_vertex_steady_force(z, zd, G, gw0, vmid, m, n, flow, time, rho, e, f_steady);
+    if (f_uns && Gp)
+      fj_vertex_unsteady_force([=](int p) { return fj_gamma_rate(G, Gp, Gdp, p, dt, relax); }, nc,
+                               m, n, rho, e, f_uns);
+  } else if (e < V + P && Gp && gdot) {
+    gdot[e - V] = fj_gamma_rate(G, Gp, Gdp, e - V, dt, relax);
+  }
+}
+
+extern "C" int fj_step_forces(const double* zeta, const double* zeta_dot, const double* gamma_b,
+                              const double* gamma_b_nm1, const double* gamma_dot_nm1,
+                              const double* gamma_w0, const double* v_mid, const double* nc,
+                              int m, int n, const fj_flowfield* flow, double time, double rho,
+                              double dt, double relax, double* f_steady, double* gamma_dot_n,
+                              double* f_unsteady, void* stream) {
+  if (m < 0 || n < 0 || !zeta || !gamma_b || !v_mid || !f_steady) return FJ_ERR_ARG;
+  if (gamma_b_nm1 && (!gamma_dot_nm1 || !gamma_dot_n || !(dt > 0.0))) return FJ_ERR_ARG;
+  if (f_unsteady && (!gamma_b_nm1 || !nc)) return FJ_ERR_ARG;
+  const int total = (m + 1) * (n + 1) + m * n;
+  FjFlow fl = fj_flow_from_desc(flow);
+  if (!flow) fl.kind = -1;
+  hipLaunchKernelGGL(k_step_forces, dim3(fj_cdiv(total, FJ_EW)), dim3(FJ_EW), 0,
+                     (hipStream_t)stream, zeta, zeta_dot, gamma_b, gamma_b_nm1, gamma_dot_nm1,
+                     gamma_w0, v_mid, nc, m, n, fl, time, rho, dt, relax, f_steady, gamma_dot_n,
+                     f_unsteady);
+  FJ_CHECK_LAUNCH();
   return FJ_OK;
 }
 

@@ -136,6 +136,7 @@ class UVLM:
         self._solvers = {True: RefinedSolver(), False: RefinedSolver()} if self.aic_reuse else None
         self._refactor = False
         self._aic = None
+        self._bufs = {}
 
     # ------------------------------------------------------------------ setup
     @property
@@ -278,6 +279,29 @@ class UVLM:
         self._last_system = (A, rhs)
         return solver.submit(A, rhs)
 
+    def _run_checked(self, step, static, i_ts):
+        """Run ``step`` (a pure function of its inputs that ends with the
+        circulation solve enqueued) and check the solve's stop test; if the
+        kept factor no longer serves this step's AIC, refactor on it and run
+        the step again."""
+        res = step()
+        if self._confirm(static):
+            return res
+        self._refactor = True
+        try:
+            res = step()
+        finally:
+            self._refactor = False
+        if not self._confirm(static):
+            A, rhs = self._last_system
+            if bool(torch.isfinite(A).all()) and bool(torch.isfinite(rhs).all()):
+                raise L.FlapjaxLibError("circulation solve failed its stop test after refactorisation "
+                                        "(singular or ill-conditioned AIC)")
+            # non-finite geometry or circulation: like the reference, the
+            # NaN/inf propagates into the results instead of raising
+            warn(f"Non-finite values in the circulation system at time step {i_ts}; results are not finite.")
+        return res
+
     def _confirm(self, static) -> bool:
         """Stop-test outcome of the step's circulation solve (waits for the
         solve only, not for the rest of the step)."""
@@ -408,6 +432,130 @@ class UVLM:
         return (c_n, nc_n, gamma_b_n, gamma_w_n, gamma_b_dot_n, zeta_b_n, zeta_w_n, zeta_b_dot_n, f_steady,
                 f_unsteady, alpha)
 
+    # ------------------------------------------------------ fused dynamic step
+    def _buf(self, name, shape, dtype=F64):
+        """Work buffer kept across steps (re-allocated when the shape changes)."""
+        b = self._bufs.get(name)
+        if b is None or tuple(b.shape) != tuple(shape) or b.dtype != dtype:
+            b = torch.empty(shape, dtype=dtype, device=self.hg_ref.device)
+            self._bufs[name] = b
+        return b
+
+    def _grid_list(self, zetas, gammas):
+        """(zeta, gamma, mirrored) in filament-table order: every non-empty
+        source grid, followed by its mirror image when mirrored
+        (aero/aic.py:276-308)."""
+        out = []
+        for z, g in zip(zetas, gammas):
+            if z.numel() == 0 or g.numel() == 0:
+                continue
+            out.append((z, g, False))
+            if self.mirror is not None:
+                out.append((z, g, True))
+        return out
+
+    @staticmethod
+    def _table_len(grids):
+        return sum(K.n_segments(z.shape[0] - 1, z.shape[1] - 1) for z, _, _ in grids)
+
+    def _pair_ws(self, name, n_tgt, n_seg):
+        nbytes = L.lib().fj_pair_sum_workspace_bytes(n_tgt, n_seg, int(self._prec))
+        return self._buf(name, (nbytes // 8 + 1,))
+
+    def _fast_step_ok(self, static, hg_nm1, hg_dot_n):
+        return (not static and hg_nm1 is not None and hg_dot_n is not None
+                and all(d is None for d in self.delta_w))
+
+    def _dynamic_step(self, case, i_ts, hg_n, hg_nm1, hg_dot_n):
+        """The dynamic branch of base_solve_from_grid (aero/uvlm.py:824-1117)
+        for a uniform wake, written straight into ``case`` at ``i_ts`` with
+        the fused entry points: one geometry launch per surface (grids at n
+        and n-1, grid velocities, stencils, force midpoints), one filament
+        table per stage (no concatenation; the wake table at n serves the
+        upwash and, with the bound table after the solve, the forces), and
+        one force launch per surface.  Same arithmetic and the same table
+        order as the general path, so the same results."""
+        ff = self.flowfield
+        t_n = float(case.t[i_ts])
+        dt = self.dt
+        kern = self.kernel
+        snap = case.snapshot(i_ts)
+        prev = {k: [a[i_ts - 1] for a in getattr(case, k)] for k in ("gamma_b", "gamma_w", "gamma_b_dot", "zeta_w")}
+        hg_n, hg_nm1, hg_dot_n = to_device(hg_n), to_device(hg_nm1), to_device(hg_dot_n)
+        single = self.n_surf == 1
+        n_b = self.n_b
+        n_mid = int(self._s_off[-1])
+        c_all = snap["c"][0].view(-1, 3) if single else self._buf("c_all", (n_b, 3))
+        nc_all = snap["nc"][0].view(-1, 3) if single else self._buf("nc_all", (n_b, 3))
+        cd_all = self._buf("cd_all", (n_b, 3))
+        mids = self._buf("mids", (n_mid, 3))
+        zb_nm1 = []
+        for s, gd in enumerate(self.grid_disc):
+            a, b = self._b_off[s], self._b_off[s + 1]
+            sa, sb = self._s_off[s], self._s_off[s + 1]
+            zb_nm1.append(self._buf(f"zb_nm1_{s}", (gd.m + 1, gd.n + 1, 3)))
+            K.step_geometry(hg_n, hg_nm1, hg_dot_n, self._dof[s], self.zeta_b0[s], zeta_n=snap["zeta_b"][s],
+                            zeta_nm1=zb_nm1[s], zeta_dot_n=snap["zeta_b_dot"][s], c=c_all[a:b], nc=nc_all[a:b],
+                            c_dot=cd_all[a:b], mid=mids[sa:sb])
+        # ---- wake: circulation shift, TE row, convection (aero/utils.py:400-577)
+        zw_n, gw_n = snap["zeta_w"], snap["gamma_w"]
+        seg_nm1 = None
+        if self.free_wake:
+            grids = self._grid_list(zb_nm1 + prev["zeta_w"], prev["gamma_b"] + prev["gamma_w"])
+            seg_nm1 = K.segments_from_grids(grids, kern.kind, kern.eps, self.mirror,
+                                            out=self._buf("seg_nm1", (self._table_len(grids), L.SEG_STRIDE)))
+        for s, gd in enumerate(self.grid_disc):
+            K.wake_shift(prev["gamma_b"][s], prev["gamma_w"][s], snap["zeta_b"][s], gd.m_star, gw_n[s], zw_n[s])
+            if gd.m_star == 0:
+                continue
+            if self.free_wake:
+                base = prev["zeta_w"][s][:-1].reshape(-1, 3)
+                K.pair_sum(seg_nm1, base, kern.kind, kern.eps, kern.rcut, self._prec, L.OUT_CONVECT,
+                           out=zw_n[s][1:].reshape(-1, 3), flow=ff, time=t_n, dt=dt,
+                           workspace=self._pair_ws(f"ws_roll_{s}", base.shape[0], seg_nm1.shape[0]))
+            else:
+                K.wake_convect(prev["zeta_w"][s], None, ff, t_n, dt, zw_n[s])
+        # ---- system (aero/uvlm.py:944-993)
+        A = self._assemble_aic(c_all, nc_all, snap["zeta_b"], None)
+        grids_b = self._grid_list(snap["zeta_b"], snap["gamma_b"])
+        grids_w = self._grid_list(zw_n, gw_n)
+        s_b, s_w = self._table_len(grids_b), self._table_len(grids_w)
+        seg_n = self._buf("seg_n", (s_b + s_w, L.SEG_STRIDE))
+        v_w = None
+        if s_w:
+            K.segments_from_grids(grids_w, kern.kind, kern.eps, self.mirror, out=seg_n[s_b:])
+            v_w = K.pair_sum(seg_n[s_b:], c_all, kern.kind, kern.eps, kern.rcut, L.PREC_FP64, L.OUT_STORE,
+                             out=self._buf("v_w", (n_b, 3)), workspace=self._pair_ws("ws_up", n_b, s_w))
+        rhs = K.rhs(c_all, nc_all, cd_all, v_w, ff, t_n, out=self._buf("rhs", (n_b,)))
+        gout = snap["gamma_b"][0].view(-1) if single else self._buf("gvec", (n_b,))
+        if self._solvers is None:
+            gout.copy_(K.lu_solve(K.lu_factor(A, overwrite=True), rhs))
+        else:
+            solver = self._solvers[False]
+            if self._refactor:
+                solver.factor(A)
+            self._last_system = (A, rhs)
+            solver.submit(A, rhs, out=gout)
+        if not single:
+            for s, gd in enumerate(self.grid_disc):
+                snap["gamma_b"][s].copy_(gout[self._b_off[s]:self._b_off[s + 1]].view(gd.m, gd.n))
+                snap["c"][s].copy_(c_all[self._b_off[s]:self._b_off[s + 1]].view(gd.m, gd.n, 3))
+                snap["nc"][s].copy_(nc_all[self._b_off[s]:self._b_off[s + 1]].view(gd.m, gd.n, 3))
+        # ---- forces (aero/uvlm.py:1018-1103)
+        if s_b:
+            K.segments_from_grids(grids_b, kern.kind, kern.eps, self.mirror, out=seg_n[:s_b])
+        v_mid = K.pair_sum(seg_n if s_b + s_w else None, mids, kern.kind, kern.eps, kern.rcut, self._prec,
+                           L.OUT_STORE, out=self._buf("v_mid", (n_mid, 3)),
+                           workspace=self._pair_ws("ws_mid", n_mid, s_b + s_w))
+        for s, gd in enumerate(self.grid_disc):
+            sa, sb = self._s_off[s], self._s_off[s + 1]
+            gw0 = gw_n[s][0] if gd.m_star > 0 else None
+            K.step_forces(snap["zeta_b"][s], snap["zeta_b_dot"][s], snap["gamma_b"][s], prev["gamma_b"][s],
+                          prev["gamma_b_dot"][s], gw0, v_mid[sa:sb], nc_all[self._b_off[s]:self._b_off[s + 1]], ff,
+                          t_n, ff.rho, dt, self.gamma_dot_relaxation, snap["f_steady"][s], snap["gamma_b_dot"][s],
+                          snap["f_unsteady"][s])
+            K.strip_alpha(snap["zeta_b"][s], snap["f_steady"][s], ff, t_n, ff.rho, out=snap["alpha"][s])
+
     def _store(self, case, i_ts, res, static, horseshoe=False):
         names = ("c", "nc", "gamma_b", "gamma_w", "gamma_b_dot", "zeta_b", "zeta_w", "zeta_b_dot", "f_steady",
                  "f_unsteady", "alpha")
@@ -431,6 +579,9 @@ class UVLM:
         if not static:
             q_nm1 = {k: [a[i_ts - 1] for a in getattr(case, k)] for k in ("gamma_b", "gamma_w", "gamma_b_dot", "zeta_w")}
             case.t[i_ts] = case.t[i_ts - 1] + self.dt if i_ts else 0.0
+        if i_ts > 0 and self._fast_step_ok(static, hg_nm1, hg_dot_n):
+            self._run_checked(lambda: self._dynamic_step(case, i_ts, hg_n, hg_nm1, hg_dot_n), False, i_ts)
+            return case
         snap = case.snapshot(i_ts)
         zeta_b_n = self.hg_to_zeta_b(hg_n if hg_n is not None else self.hg_ref, out=snap["zeta_b"])
         zeta_b_dot_n = zeta_b_nm1 = None
@@ -438,25 +589,9 @@ class UVLM:
             zeta_b_dot_n = self.hg_dot_to_zeta_b_dot(hg_n, hg_dot_n, out=snap["zeta_b_dot"])
             zeta_b_nm1 = self.hg_to_zeta_b(hg_nm1)
         out = None if static else snap
-        res = self.base_solve_from_grid(q_nm1, case.t[i_ts], zeta_b_n, zeta_b_nm1, zeta_b_dot_n, static, horseshoe,
-                                        out=out)
-        if not self._confirm(static):
-            # the kept factor no longer serves this step's AIC: refactor on it
-            # and compute the step again (a pure function of its inputs)
-            self._refactor = True
-            try:
-                res = self.base_solve_from_grid(q_nm1, case.t[i_ts], zeta_b_n, zeta_b_nm1, zeta_b_dot_n, static,
-                                                horseshoe, out=out)
-            finally:
-                self._refactor = False
-            if not self._confirm(static):
-                A, rhs = self._last_system
-                if bool(torch.isfinite(A).all()) and bool(torch.isfinite(rhs).all()):
-                    raise L.FlapjaxLibError("circulation solve failed its stop test after refactorisation "
-                                            "(singular or ill-conditioned AIC)")
-                # non-finite geometry or circulation: like the reference, the
-                # NaN/inf propagates into the results instead of raising
-                warn(f"Non-finite values in the circulation system at time step {i_ts}; results are not finite.")
+        res = self._run_checked(
+            lambda: self.base_solve_from_grid(q_nm1, case.t[i_ts], zeta_b_n, zeta_b_nm1, zeta_b_dot_n, static,
+                                              horseshoe, out=out), static, i_ts)
         self._store(case, i_ts, res, static, horseshoe)
         if horseshoe:
             self.initialise_wake(zeta_b_n, out=snap["zeta_w"])

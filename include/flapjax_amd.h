@@ -89,6 +89,26 @@ int fj_segments_from_grid(const double* zeta, const double* gamma, int m, int n,
                           const double* mirror_point, const double* mirror_normal,
                           int kernel, double eps, double* seg_out, void* stream);
 
+/* One source grid of fj_segments_from_grids: device vertex grid (m+1, n+1, 3),
+ * ring strengths (m, n) or NULL, and whether to take its mirror image. */
+typedef struct {
+  const double* zeta;
+  const double* gamma;
+  int m;
+  int n;
+  int mirror;
+} fj_grid_ref;
+
+/* The tables of several grids (host array `grids`) concatenated in list order
+ * into seg_out, in as few launches as possible (one per 16 grids): the table
+ * of all source surfaces that compute_v_ind sums over (aero/aic.py:276-308,
+ * bound and wake surfaces, each followed by its mirror image when mirrored).
+ * mirror_point/mirror_normal (HOST, 3 doubles) are required if any grid has
+ * mirror set. */
+int fj_segments_from_grids(int n_grids, const fj_grid_ref* grids, const double* mirror_point,
+                           const double* mirror_normal, int kernel, double eps, double* seg_out,
+                           void* stream);
+
 /* Filament table from end points ends (n_seg, 2, 3) and circulations gamma
  * (n_seg; NULL = unit), the filament-grid input of v_ind_vmap
  * (aero/aic.py:208-245). */
@@ -238,6 +258,18 @@ int fj_colloc_normals(const double* zeta, const double* zeta_dot, int m, int n, 
  * (m+1, n), the targets of calculate_steady_forcing (aero/utils.py:162-164). */
 int fj_filament_midpoints(const double* zeta, int m, int n, double* mid, void* stream);
 
+/* Everything a time step derives from the beam poses, in one launch (the
+ * case_solve preamble, aero/uvlm.py:1162-1185, with the stencils of
+ * base_solve_from_grid, :888-893, and the force midpoints, utils.py:162-164):
+ * zeta_n from hg_n, zeta_nm1 from hg_nm1, zeta_dot_n from hg_n/hg_dot_n as in
+ * fj_grid_from_hg; c, nc, c_dot as in fj_colloc_normals; mid as in
+ * fj_filament_midpoints.  hg_nm1 / hg_dot_n and every output may be NULL.
+ * Results are identical to those separate entry points. */
+int fj_step_geometry(const double* hg_n, const double* hg_nm1, const double* hg_dot_n,
+                     const int32_t* dof_map, const double* zeta0, int m, int n, double* zeta_n,
+                     double* zeta_nm1, double* zeta_dot_n, double* c, double* nc, double* c_dot,
+                     double* mid, void* stream);
+
 /* rhs[t] = -(U(c_t, time) - c_dot_t + v_w_t) . n_t  (aero/uvlm.py:961-985);
  * c_dot and v_w may be NULL (static solve). */
 int fj_rhs(const double* c, const double* nc, const double* c_dot, const double* v_w, int n_tgt,
@@ -301,6 +333,16 @@ int fj_gamma_dot_unsteady(const double* gamma_n, const double* gamma_nm1,
                           const double* gamma_dot_nm1, const double* nc, int m, int n, double dt,
                           double relax, double rho, double* gamma_dot_n, double* f_unsteady,
                           void* stream);
+
+/* The force stage of a dynamic step in one launch (aero/uvlm.py:1023-1103):
+ * steady vertex forces as fj_steady_forces and, if gamma_b_nm1 is non-NULL,
+ * the circulation rate and unsteady force as fj_gamma_dot_unsteady
+ * (f_unsteady may be NULL).  Results are identical to those entry points. */
+int fj_step_forces(const double* zeta, const double* zeta_dot, const double* gamma_b,
+                   const double* gamma_b_nm1, const double* gamma_dot_nm1, const double* gamma_w0,
+                   const double* v_mid, const double* nc, int m, int n, const fj_flowfield* flow,
+                   double time, double rho, double dt, double relax, double* f_steady,
+                   double* gamma_dot_n, double* f_unsteady, void* stream);
 
 /* Per-strip effective angle of attack (strip_alpha, aero/utils.py:226-273). */
 int fj_strip_alpha(const double* zeta, const double* f_steady, int m, int n,

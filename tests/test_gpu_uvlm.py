@@ -155,3 +155,36 @@ def test_non_finite_motion_propagates_like_the_reference():
     g = d.gamma_b[0].cpu().numpy()
     assert np.isfinite(g[:4]).all()
     assert not np.isfinite(g[4]).all()
+
+
+@pytest.mark.parametrize("mirror,two_surf,free", [(False, False, True), (True, False, True), (False, True, True),
+                                                   (False, False, False)])
+def test_fused_step_equals_general_path(mirror, two_surf, free):
+    """The fused dynamic step (one geometry launch per surface, shared
+    filament tables, fused forces) gives the results of the general
+    base_solve_from_grid path, step by step."""
+    import flapjax_amd as fj
+    from flapjax_amd.uvlm import UVLM
+
+    m, n, m_star, steps = 4, 8, 6, 7
+    span = 4.0
+    zeta0 = O.make_rectangular_grid(m, n, 1.0, 0.25)
+    hg_t, hg_dot_t = O.pitch_plunge_motion(n, span, steps, 0.25, np.deg2rad(5.0), np.deg2rad(3.0), 0.05, 1.6)
+    shapes = [(m, n, m_star)] * (2 if two_surf else 1)
+    dofs = [np.arange(n + 1)] * len(shapes)
+    z0s = [zeta0, zeta0 + np.array([0.0, 0.0, 0.5])] if two_surf else zeta0
+    mp = np.zeros(3) if mirror else None
+    mn = np.array([0.0, 1.0, 0.0]) if mirror else None
+    runs = []
+    for fused in (True, False):
+        uv = UVLM(shapes, dofs, free_wake=free, mirror_point=mp, mirror_normal=mn)
+        uv.set_design_variables(0.25, fj.Constant(np.array([1.0, 0.0, 0.0]), 1.225, True), z0s, hg_t[0])
+        if not fused:
+            uv._fast_step_ok = lambda *a: False
+        runs.append(uv.solve_prescribed_dynamic(uv.solve_static(hg=hg_t[0]), hg_t, hg_dot_t))
+    a, b = runs
+    for key in ("gamma_b", "gamma_w", "zeta_w", "f_steady", "f_unsteady", "gamma_b_dot", "alpha", "c", "nc",
+                "zeta_b_dot"):
+        for s in range(len(shapes)):
+            x, y = getattr(a, key)[s].cpu().numpy(), getattr(b, key)[s].cpu().numpy()
+            assert rel_err(x, y) <= 1e-15, (key, s, rel_err(x, y))
