@@ -33,6 +33,12 @@ N_TSTEP = 200
 ALPHA0, PITCH, PLUNGE, K_RED = 5.0, 3.0, 0.05, 0.2
 F_PAIR = 57  # algorithmic FLOP per pair interaction
 PEAK_FP64 = 78.6  # TFLOP/s, MI355X FP64 vector (datasheet)
+# HBM bytes per rollup launch (partial + reduce kernels), from rocprofv3
+# --pmc FETCH_SIZE and WRITE_SIZE runs of `bench.py --profile-rollup`, each
+# scaled by the same counter on a 256 MiB device copy (FETCH_SIZE reads 0.5x
+# the copied bytes on MI355X, WRITE_SIZE 1.0x); re-measured whenever the
+# pair-sum launch shape changes.
+ROLLUP_TRAFFIC_BYTES = 36_979_000
 
 
 def n_seg(m, n):
@@ -374,7 +380,7 @@ def main():
                   "latency_ms": {"p50": round(mix["p50_ms"], 4), "p99": round(mix["p99_ms"], 4)},
                   "cases_per_s": round(1.0 / mix["case_s"], 3), "speedup_vs_fp64": round(value_mixed / value, 3)},
         "roofline": {"bound": "valu", "achieved": round(roof["fp64"]["tflops"], 3), "peak": PEAK_FP64,
-                     "unit": "TFLOP/s", "frac": round(roof["fp64"]["tflops"] / PEAK_FP64, 4), "traffic": None,
+                     "unit": "TFLOP/s", "frac": round(roof["fp64"]["tflops"] / PEAK_FP64, 4), "traffic": ROLLUP_TRAFFIC_BYTES,
                      "kernel": "k_pair_partial_f64 + k_pair_reduce (rollup pair sum)",
                      "launch_ms": round(roof["fp64"]["ms"], 4), "pairs_per_launch": roof["fp64"]["pairs"],
                      "mixed_launch_ms": round(roof["mixed"]["ms"], 4)},

@@ -136,7 +136,7 @@ __device__ __forceinline__ void fj_bs_eval(double x, double y, double z, const d
 // FJ_BS_MIN_SEG_PER_WAVE filaments; two targets per lane from
 // FJ_BS_TPL2_MIN_TARGETS targets on.
 #ifndef FJ_BS_WAVES
-#define FJ_BS_WAVES 4
+#define FJ_BS_WAVES 8
 #endif
 #define FJ_BS_THREADS (64 * FJ_BS_WAVES)
 #ifndef FJ_BS_TARGET_BLOCKS
