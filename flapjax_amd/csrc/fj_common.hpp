@@ -111,12 +111,15 @@ __device__ __forceinline__ void fj_bs_eval(double x, double y, double z, const d
   const double cy = r1z * r2x - r1x * r2z;
   const double cz = r1x * r2y - r1y * r2x;
   const double e2 = (KIND == FJ_KERNEL_PLAIN) ? 0.0 : kp.eps2;
-  // sums of squares as FMA chains seeded with the additive term
+  // |r|^2 + eps^2 as FMA chains seeded with eps^2
   const double s1 = fma(r1z, r1z, fma(r1y, r1y, fma(r1x, r1x, e2)));
   const double s2 = fma(r2z, r2z, fma(r2y, r2y, fma(r2x, r2x, e2)));
   const double a1 = r0x * r1x + r0y * r1y + r0z * r1z;
   const double a2 = a1 - s[11];
-  const double D = fma(cz, cz, fma(cy, cy, fma(cx, cx, (KIND == FJ_KERNEL_EPSILON) ? s[9] : 0.0)));
+  // |c|^2 then + eps |r0|^4, the reference's order (and no register copy of
+  // the scalar operand)
+  double D = fma(cz, cz, fma(cy, cy, cx * cx));
+  if (KIND == FJ_KERNEL_EPSILON) D += s[9];
   const double D2 = D * D;
   const double w1 = fj_rsqrt(s1 * D2);
   const double w2 = fj_rsqrt(s2 * D2);

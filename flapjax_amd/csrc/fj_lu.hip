@@ -35,108 +35,139 @@ __device__ __forceinline__ unsigned long long fj_pivot_key(double a, int row) {
   return (b & ~0x1FFFull) | (unsigned long long)(8191 - row);
 }
 
+// 64-bit lane value moved by a DPP pattern (both halves), rows outside
+// ROW_MASK keep their own value
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ unsigned long long fj_dpp64(unsigned long long v) {
+  const int lo = (int)(unsigned)(v & 0xffffffffull), hi = (int)(unsigned)(v >> 32);
+  const int nlo = __builtin_amdgcn_update_dpp(lo, lo, CTRL, ROW_MASK, 0xf, false);
+  const int nhi = __builtin_amdgcn_update_dpp(hi, hi, CTRL, ROW_MASK, 0xf, false);
+  return ((unsigned long long)(unsigned)nhi << 32) | (unsigned long long)(unsigned)nlo;
+}
+
+__device__ __forceinline__ unsigned long long fj_umax64(unsigned long long a,
+                                                        unsigned long long b) {
+  return a > b ? a : b;
+}
+
+// wave64 maximum through DPP (no LDS round trips): quad swaps, half-row and
+// row mirrors give every lane its 16-lane row maximum, then the row_bcast15 /
+// row_bcast31 steps carry it into row 3; lane 63 holds the wave maximum.
+__device__ __forceinline__ unsigned long long fj_wave_max_u64(unsigned long long v) {
+  v = fj_umax64(v, fj_dpp64<0xB1, 0xF>(v));   // quad_perm [1,0,3,2]
+  v = fj_umax64(v, fj_dpp64<0x4E, 0xF>(v));   // quad_perm [2,3,0,1]
+  v = fj_umax64(v, fj_dpp64<0x141, 0xF>(v));  // row_half_mirror
+  v = fj_umax64(v, fj_dpp64<0x140, 0xF>(v));  // row_mirror
+  v = fj_umax64(v, fj_dpp64<0x142, 0xA>(v));  // row_bcast15 -> rows 1, 3
+  v = fj_umax64(v, fj_dpp64<0x143, 0xC>(v));  // row_bcast31 -> rows 2, 3
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(v & 0xffffffffull), 63);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(v >> 32), 63);
+  return ((unsigned long long)hi << 32) | lo;
+}
+
+// Panel factorisation (LAPACK getf2 on the (n-k) x jb panel), one workgroup.
+// The panel sits in registers: thread tid owns local rows tid + r * THREADS
+// and JB values of each.  Pivoting is implicit: rows never move between
+// threads; each row carries its logical position (LAPACK's row order after
+// the exchanges so far), used for the tie break of the pivot search and
+// updated by the exchange of each column.  Register slot 0 always holds the
+// current column: the rank-1 update writes a[c] = a[c+1] - l u[c+1] (update
+// and shift in one FMA) and a[JB-1] = l, so the finished multipliers rotate
+// in behind the remaining columns; the published pivot row carries zeros in
+// its own multiplier slots, so they shift through unchanged.  The column
+// loop is short, rolled and free of global memory traffic; at the end each
+// row is written once, to its logical position, undoing its rotation (the
+// step it was chosen as pivot, or jb).
 template <typename T, int R, int JB>
 __global__ __launch_bounds__(FJ_PANEL_THREADS) void k_lu_panel(T* __restrict__ A, int n, int lda,
                                                                int k, int32_t* __restrict__ ipiv,
                                                                int32_t* __restrict__ info) {
   __shared__ unsigned long long s_key[FJ_PANEL_WAVES];
-  __shared__ T s_piv[2][JB];
-  __shared__ T s_swp[2][JB];
+  __shared__ T s_piv[JB];
   __shared__ int s_ipiv[JB];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int jb = min(JB, n - k);
   const int rows = n - k;
+  T* P = A + (size_t)k * lda + k;  // panel origin
 
-  // load the panel: thread tid owns local rows tid + r * FJ_PANEL_THREADS
   T a[R][JB];
+  int pos[R], rot[R];
+  bool live[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int lr = tid + r * FJ_PANEL_THREADS;
+    live[r] = lr < rows;
+    pos[r] = lr;
+    rot[r] = jb;
 #pragma unroll
-    for (int c = 0; c < JB; ++c)
-      a[r][c] = (lr < rows && c < jb) ? A[(size_t)(k + lr) * lda + k + c] : T(0);
+    for (int c = 0; c < JB; ++c) a[r][c] = (live[r] && c < jb) ? P[(size_t)lr * lda + c] : T(0);
   }
 
-  // fully unrolled so that every register index below is a compile-time constant
+#pragma unroll 1
+  for (int jj = 0; jj < jb; ++jj) {
+    unsigned long long key = 0ull;
 #pragma unroll
-  for (int jj = 0; jj < JB; ++jj) {
-    if (jj < jb) {  // uniform: jb is the same for the whole workgroup
-      const int buf = jj & 1;
-      unsigned long long key = 0ull;
+    for (int r = 0; r < R; ++r)
+      if (live[r]) key = fj_umax64(key, fj_pivot_key((double)a[r][0], pos[r]));
+    key = fj_wave_max_u64(key);
+    if (lane == 0) s_key[wave] = key;
+    __syncthreads();
+    key = s_key[0];
 #pragma unroll
-      for (int r = 0; r < R; ++r) {
-        const int lr = tid + r * FJ_PANEL_THREADS;
-        if (lr >= jj && lr < rows) {
-          const unsigned long long kk = fj_pivot_key((double)a[r][jj], lr);
-          key = kk > key ? kk : key;
-        }
+    for (int w = 1; w < FJ_PANEL_WAVES; ++w) key = fj_umax64(key, s_key[w]);
+    const int p = 8191 - (int)(key & 0x1FFFull);  // logical position of the pivot row
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      if (live[r] && pos[r] == p) {
+        // the pivot row: publish it (zeros over its multiplier slots), retire it
+#pragma unroll
+        for (int c = 0; c < JB; ++c) s_piv[c] = (c < JB - jj) ? a[r][c] : T(0);
+        live[r] = false;
+        pos[r] = jj;
+        rot[r] = jj;
+      } else if (live[r] && pos[r] == jj) {
+        pos[r] = p;  // the row at position jj takes the pivot's place
       }
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) {
-        const unsigned long long o = __shfl_xor(key, off);
-        key = o > key ? o : key;
-      }
-      if (lane == 0) s_key[wave] = key;
-      __syncthreads();
-      key = s_key[0];
-#pragma unroll
-      for (int w = 1; w < FJ_PANEL_WAVES; ++w) key = s_key[w] > key ? s_key[w] : key;
-      const int p = 8191 - (int)(key & 0x1FFFull);  // local pivot row
-      // owners publish the pivot row and the current row jj
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        const int lr = tid + r * FJ_PANEL_THREADS;
-        if (lr == p) {
-#pragma unroll
-          for (int c = 0; c < JB; ++c) s_piv[buf][c] = a[r][c];
-        }
-        if (lr == jj) {
-#pragma unroll
-          for (int c = 0; c < JB; ++c) s_swp[buf][c] = a[r][c];
-        }
-      }
-      if (tid == 0) {
-        s_ipiv[jj] = k + p;
-        if ((key >> 13) == 0ull && *info == 0) *info = k + jj + 1;  // zero column
-      }
-      __syncthreads();
-      const T piv = s_piv[buf][jj];
-      const T inv = (piv != T(0)) ? T(1) / piv : T(0);
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        const int lr = tid + r * FJ_PANEL_THREADS;
-        if (lr == jj) {
-#pragma unroll
-          for (int c = 0; c < JB; ++c) a[r][c] = s_piv[buf][c];
-        } else if (lr == p) {
-#pragma unroll
-          for (int c = 0; c < JB; ++c) a[r][c] = s_swp[buf][c];
-        }
-        if (lr > jj && lr < rows) {
-          const T l = a[r][jj] * inv;
-          a[r][jj] = l;
-#pragma unroll
-          for (int c = 0; c < JB; ++c)
-            if (c > jj) a[r][c] -= l * s_piv[buf][c];
-        }
-      }
-      // the next column writes the other LDS buffers (s_key is protected by
-      // the second barrier of this column)
     }
+    if (tid == 0) {
+      s_ipiv[jj] = k + p;
+      if ((key >> 13) == 0ull && *info == 0) *info = k + jj + 1;  // zero column
+    }
+    __syncthreads();
+    const T piv = s_piv[0];
+    const T inv = (piv != T(0)) ? T(1) / piv : T(0);
+    T u[JB];
+#pragma unroll
+    for (int c = 1; c < JB; ++c) u[c] = s_piv[c];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      if (live[r]) {
+        const T l = a[r][0] * inv;
+#pragma unroll
+        for (int c = 0; c < JB - 1; ++c) a[r][c] = a[r][c + 1] - l * u[c + 1];
+        a[r][JB - 1] = l;
+      }
+    }
+    // the next column's LDS writes follow its first barrier, after every
+    // thread has read this column's s_key / s_piv
   }
-  __syncthreads();
 
-  // write the panel back
+  // each row to its logical position; column c sits in slot (c - rot) mod JB
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int lr = tid + r * FJ_PANEL_THREADS;
     if (lr < rows) {
+      T* dst = P + (size_t)pos[r] * lda;
 #pragma unroll
-      for (int c = 0; c < JB; ++c)
-        if (c < jb) A[(size_t)(k + lr) * lda + k + c] = a[r][c];
+      for (int sl = 0; sl < JB; ++sl) {
+        int c = sl + rot[r];
+        c = (c >= JB) ? c - JB : c;
+        if (c < jb) dst[c] = a[r][sl];
+      }
     }
   }
+  __syncthreads();
   if (tid < jb) ipiv[k + tid] = s_ipiv[tid];
   // composed permutation ipiv[n + q]: only positions k..k+jb-1 and the pivot
   // rows change; each is traced back through this panel's exchanges (last
@@ -161,40 +192,91 @@ __global__ __launch_bounds__(FJ_PANEL_THREADS) void k_lu_panel(T* __restrict__ A
 }
 
 // Columns outside the panel, one thread each: the panel's row exchanges
-// (laswp), then U12 = L11^{-1} A12 for the columns right of the panel (trsm).
+// (laswp) and, right of the panel, U12 = L11^{-1} A12 (trsm).  The jb
+// sequential exchanges only touch positions k..k+jb-1 and the pivot rows
+// beyond them; their composition (which old position lands where) is built
+// once per workgroup, so each column is one round of independent loads,
+// the exchanged values written back, and the unit-lower solve in registers
+// with L11 from LDS.
+#define FJ_SWAP_THREADS 64
 template <typename T, int JB>
-__global__ __launch_bounds__(256) void k_lu_swap_trsm(T* __restrict__ A, int n, int lda, int k,
-                                                      int jb, const int32_t* __restrict__ ipiv) {
+__global__ __launch_bounds__(FJ_SWAP_THREADS) void k_lu_swap_trsm(T* __restrict__ A, int n,
+                                                                 int lda, int k, int jb,
+                                                                 const int32_t* __restrict__ ipiv) {
   __shared__ T s_L11[JB][JB + 1];
-  __shared__ T s_u[JB][256];
+  __shared__ T s_v[2 * JB][FJ_SWAP_THREADS];
+  __shared__ T s_x[JB][FJ_SWAP_THREADS];
   __shared__ int s_ipiv[JB];
+  __shared__ int s_pos[2 * JB];  // affected positions: k..k+jb-1, then the pivot rows beyond
+  __shared__ int s_src[2 * JB];  // list index of the position whose old value lands here
+  __shared__ int s_cnt;
   const int tid = threadIdx.x;
-  for (int e = tid; e < JB * JB; e += blockDim.x) {
+  for (int e = tid; e < JB * JB; e += FJ_SWAP_THREADS) {
     const int r = e / JB, q = e % JB;
     s_L11[r][q] = (r < jb && q < jb) ? A[(size_t)(k + r) * lda + k + q] : T(0);
   }
   if (tid < JB) s_ipiv[tid] = (tid < jb) ? ipiv[k + tid] : k + tid;
   __syncthreads();
-  int c = blockIdx.x * blockDim.x + tid;
+  if (tid == 0) {
+    int cnt = jb;
+    for (int jj = 0; jj < jb; ++jj) s_pos[jj] = k + jj;
+    for (int jj = 0; jj < jb; ++jj) {
+      const int p = s_ipiv[jj];
+      if (p < k + jb) continue;
+      bool seen = false;
+      for (int e = jb; e < cnt; ++e) seen = seen || (s_pos[e] == p);
+      if (!seen) s_pos[cnt++] = p;
+    }
+    s_cnt = cnt;
+  }
+  __syncthreads();
+  const int cnt = s_cnt;
+  if (tid < cnt) {
+    // trace the content of position s_pos[tid] back through the exchanges
+    int src = s_pos[tid];
+    for (int jj = jb - 1; jj >= 0; --jj) {
+      const int a0 = k + jj, b0 = s_ipiv[jj];
+      if (src == a0) src = b0;
+      else if (src == b0) src = a0;
+    }
+    int idx = 0;
+    for (int e = 0; e < cnt; ++e) idx = (s_pos[e] == src) ? e : idx;
+    s_src[tid] = idx;
+  }
+  __syncthreads();
+  int c = blockIdx.x * FJ_SWAP_THREADS + tid;
   if (c >= n - jb) return;
   if (c >= k) c += jb;  // skip the panel columns
-  for (int jj = 0; jj < jb; ++jj) {
-    const int p = s_ipiv[jj];
-    if (p != k + jj) {
-      const T t0 = A[(size_t)(k + jj) * lda + c];
-      A[(size_t)(k + jj) * lda + c] = A[(size_t)p * lda + c];
-      A[(size_t)p * lda + c] = t0;
-    }
+#pragma unroll
+  for (int e = 0; e < 2 * JB; ++e)
+    if (e < cnt) s_v[e][tid] = A[(size_t)s_pos[e] * lda + c];
+  // positions beyond the panel block: exchanged values only
+  for (int e = jb; e < cnt; ++e) {
+    const int se = s_src[e];
+    if (se != e) A[(size_t)s_pos[e] * lda + c] = s_v[se][tid];
   }
   if (c >= k + jb) {
-    // forward substitution with the column staged in this thread's LDS slot
-    for (int r = 0; r < jb; ++r) s_u[r][tid] = A[(size_t)(k + r) * lda + c];
+    // forward substitution with the unit-lower L11, the column in this
+    // thread's LDS slot; four partial sums break the FMA chain
+    for (int r = 0; r < jb; ++r) s_x[r][tid] = s_v[s_src[r]][tid];
     for (int r = 1; r < jb; ++r) {
-      T acc = s_u[r][tid];
-      for (int q = 0; q < r; ++q) acc -= s_L11[r][q] * s_u[q][tid];
-      s_u[r][tid] = acc;
-      A[(size_t)(k + r) * lda + c] = acc;
+      T a0 = s_x[r][tid], a1 = T(0), a2 = T(0), a3 = T(0);
+      int q = 0;
+      for (; q + 3 < r; q += 4) {
+        a0 -= s_L11[r][q] * s_x[q][tid];
+        a1 -= s_L11[r][q + 1] * s_x[q + 1][tid];
+        a2 -= s_L11[r][q + 2] * s_x[q + 2][tid];
+        a3 -= s_L11[r][q + 3] * s_x[q + 3][tid];
+      }
+      for (; q < r; ++q) a0 -= s_L11[r][q] * s_x[q][tid];
+      const T v = (a0 + a1) + (a2 + a3);
+      s_x[r][tid] = v;
+      A[(size_t)(k + r) * lda + c] = v;
     }
+    A[(size_t)k * lda + c] = s_x[0][tid];
+  } else {
+    for (int r = 0; r < jb; ++r)
+      if (s_src[r] != r) A[(size_t)(k + r) * lda + c] = s_v[s_src[r]][tid];
   }
 }
 
@@ -299,8 +381,8 @@ static int fj_lu_factor_t(T* A, int n, int lda, int32_t* ipiv, int32_t* info, T*
                        ipiv, info);
     FJ_CHECK_LAUNCH();
     if (n - jb > 0) {
-      hipLaunchKernelGGL((k_lu_swap_trsm<T, JB>), dim3(fj_cdiv(n - jb, 256)), dim3(256), 0, st, A, n,
-                         lda, k, jb, ipiv);
+      hipLaunchKernelGGL((k_lu_swap_trsm<T, JB>), dim3(fj_cdiv(n - jb, FJ_SWAP_THREADS)),
+                         dim3(FJ_SWAP_THREADS), 0, st, A, n, lda, k, jb, ipiv);
       FJ_CHECK_LAUNCH();
     }
     const int rest = n - k - jb;
@@ -763,47 +845,50 @@ extern "C" int fj_solve_refined_inv_f32(const double* A, int lda, const float* A
 }
 
 // ---- explicit inverse from the factor ------------------------------------------
-// A^{-1} = U^{-1} L^{-1} P by two sweeps over 64-row block rows, each a
-// launch with one workgroup per 64-column tile, in place in X:
-//   forward  (k = 0 .. nb-1): Y_k = Linv_kk (P_k - sum_{j<k} L_kj Y_j)
-//   backward (k = nb-1 .. 0): X_k = Uinv_kk (Y_k - sum_{j>k} U_kj X_j)
-// P_k holds the rows of the permutation matrix (row i: unit at column
-// ipiv[n + i]); Linv_kk / Uinv_kk are the factor's diagonal-block inverses.
-// Both products run on the matrix cores with the trailing update's tiling.
+// A^{-1} = U^{-1} L^{-1} P, right-looking over 64-row block rows, in place in
+// X (initialised to P: row i has its unit at column ipiv[n + i]):
+//   forward,  k = 0 .. nb-1:  X_k <- Linv_kk X_k;  X_j -= L_jk X_k  (j > k)
+//   backward, k = nb-1 .. 0:  X_k <- Uinv_kk X_k;  X_j -= U_jk X_k  (j < k)
+// Linv_kk / Uinv_kk are the factor's diagonal-block inverses.  Each step is
+// two launches: the 64 x n diagonal product (one workgroup per 64-column
+// tile) and the update of every other block row in one launch (one
+// workgroup per 64 x 64 tile), so each update is a wide matrix-core launch
+// with K = 64 instead of a long K loop in a few workgroups.
+__global__ void k_inv_perm_init_f64(double* __restrict__ X, int ldx, int n,
+                                    const int32_t* __restrict__ ipiv) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)n * n) return;
+  const int i = (int)(e / n), j = (int)(e % n);
+  X[(size_t)i * ldx + j] = (ipiv[n + i] == j) ? 1.0 : 0.0;
+}
+__global__ void k_inv_perm_init_f32(float* __restrict__ X, int ldx, int n,
+                                    const int32_t* __restrict__ ipiv) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)n * n) return;
+  const int i = (int)(e / n), j = (int)(e % n);
+  X[(size_t)i * ldx + j] = (ipiv[n + i] == j) ? 1.0f : 0.0f;
+}
+
+// C (64 x 64 tile in LDS-free registers) = Ablk (64 x K, row stride lda) *
+// Bblk (K x 64, row stride ldb), K <= 64, staged through LDS in halves of 32;
+// rows/columns beyond (ra, cb) and K beyond kk read as zero.
 template <typename T>
-__global__ __launch_bounds__(256) void k_lu_inverse_row(const T* __restrict__ LU, int lda,
-                                                        const T* __restrict__ aux,
-                                                        const int32_t* __restrict__ ipiv,
-                                                        T* __restrict__ X, int ldx, int n, int kb,
-                                                        int backward) {
-  __shared__ T sA[FJ_GT][FJ_GK + 1];
-  __shared__ T sB[FJ_GK][FJ_GT + 1];
-  __shared__ T sR[FJ_GT][FJ_GT + 1];
+__device__ __forceinline__ void fj_tile_mma(const T* __restrict__ Ab, int lda, int ra,
+                                            const T* __restrict__ Bb, int ldb, int cb, int kk,
+                                            T (*sA)[FJ_GK + 1], T (*sB)[FJ_GT + 1],
+                                            typename FjMfma<T>::acc_t (&acc)[2][2]) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int li = lane & 15, lk = lane >> 4;
   const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
-  const int nblk = fj_nblk(n);
-  const int r0 = kb * FJ_TS_B, rs = min(FJ_TS_B, n - r0);
-  const int c0 = blockIdx.x * FJ_GT;
-  const int q_begin = backward ? r0 + rs : 0;
-  const int q_end = backward ? n : r0;
-  typedef typename FjMfma<T>::acc_t acc_t;
-  acc_t acc[2][2];
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int c = 0; c < 2; ++c) acc[a][c] = acc_t{0, 0, 0, 0};
-
-  // acc = sum over the solved block rows of (L or U) block * X block
-  for (int q0 = q_begin; q0 < q_end; q0 += FJ_GK) {
-    const int kq = min(FJ_GK, q_end - q0);
+  for (int q0 = 0; q0 < kk; q0 += FJ_GK) {
+    const int kq = min(FJ_GK, kk - q0);
     for (int e = tid; e < FJ_GT * FJ_GK; e += 256) {
       const int i = e / FJ_GK, q = e % FJ_GK;
-      sA[i][q] = (i < rs && q < kq) ? LU[(size_t)(r0 + i) * lda + q0 + q] : T(0);
+      sA[i][q] = (i < ra && q < kq) ? Ab[(size_t)i * lda + q0 + q] : T(0);
     }
     for (int e = tid; e < FJ_GK * FJ_GT; e += 256) {
       const int q = e / FJ_GT, j = e % FJ_GT;
-      sB[q][j] = (q < kq && c0 + j < n) ? X[(size_t)(q0 + q) * ldx + c0 + j] : T(0);
+      sB[q][j] = (q < kq && j < cb) ? Bb[(size_t)(q0 + q) * ldb + j] : T(0);
     }
     __syncthreads();
     for (int q = 0; q < kq; q += 4) {
@@ -816,52 +901,82 @@ __global__ __launch_bounds__(256) void k_lu_inverse_row(const T* __restrict__ LU
     }
     __syncthreads();
   }
-  // right-hand side tile: P_k - acc (forward) or Y_k - acc (backward)
+}
+
+// X_k (block row kb, this workgroup's 64-column tile) <- D X_k
+template <typename T>
+__global__ __launch_bounds__(256) void k_inv_diag(T* __restrict__ X, int ldx, int n,
+                                                  const T* __restrict__ D, int kb) {
+  __shared__ T sA[FJ_GT][FJ_GK + 1];
+  __shared__ T sB[FJ_GK][FJ_GT + 1];
+  typedef typename FjMfma<T>::acc_t acc_t;
+  acc_t acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) acc[a][c] = acc_t{0, 0, 0, 0};
+  const int r0 = kb * FJ_TS_B, rs = min(FJ_TS_B, n - r0);
+  const int c0 = blockIdx.x * FJ_GT, cb = min(FJ_GT, n - c0);
+  fj_tile_mma<T>(D, FJ_TS_B, rs, X + (size_t)r0 * ldx + c0, ldx, cb, rs, sA, sB, acc);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int c = 0; c < 2; ++c)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int i = wr + a * 16 + FjMfma<T>::row(lk, v);
-        const int j = wc + c * 16 + li;
-        T base = T(0);
-        if (i < rs && c0 + j < n)
-          base = backward ? X[(size_t)(r0 + i) * ldx + c0 + j]
-                          : ((ipiv[n + r0 + i] == c0 + j) ? T(1) : T(0));
-        sR[i][j] = base - acc[a][c][v];
-        acc[a][c][v] = T(0);
+        const int i = wr + a * 16 + FjMfma<T>::row(lk, v), j = wc + c * 16 + li;
+        if (i < rs && j < cb) X[(size_t)(r0 + i) * ldx + c0 + j] = acc[a][c][v];
       }
-  __syncthreads();
-  // X_k = D_kk^{-1} * rhs, D = L (forward) or U (backward)
-  const T* D = aux + ((size_t)(backward ? nblk : 0) + kb) * FJ_TS_B * FJ_TS_B;
-  for (int q0 = 0; q0 < FJ_TS_B; q0 += FJ_GK) {
-    for (int e = tid; e < FJ_GT * FJ_GK; e += 256) {
-      const int i = e / FJ_GK, q = e % FJ_GK;
-      sA[i][q] = D[i * FJ_TS_B + q0 + q];
-    }
-    __syncthreads();
-    for (int q = 0; q < FJ_GK; q += 4) {
+}
+
+// X_j -= F_{j,kb} X_kb for the block rows j after kb (lower) or before it
+// (upper); grid (column tiles, block rows updated)
+template <typename T>
+__global__ __launch_bounds__(256) void k_inv_update(const T* __restrict__ LU, int lda,
+                                                    T* __restrict__ X, int ldx, int n, int kb,
+                                                    int lower) {
+  __shared__ T sA[FJ_GT][FJ_GK + 1];
+  __shared__ T sB[FJ_GK][FJ_GT + 1];
+  typedef typename FjMfma<T>::acc_t acc_t;
+  acc_t acc[2][2];
 #pragma unroll
-      for (int a = 0; a < 2; ++a) {
-        const T av = sA[wr + a * 16 + li][q + lk];
+  for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int c = 0; c < 2; ++c)
-          acc[a][c] = FjMfma<T>::mma(av, sR[q0 + q + lk][wc + c * 16 + li], acc[a][c]);
-      }
-    }
-    __syncthreads();
-  }
+    for (int c = 0; c < 2; ++c) acc[a][c] = acc_t{0, 0, 0, 0};
+  const int jblk = lower ? kb + 1 + (int)blockIdx.y : (int)blockIdx.y;
+  const int r0 = jblk * FJ_TS_B, rs = min(FJ_TS_B, n - r0);
+  const int k0 = kb * FJ_TS_B, ks = min(FJ_TS_B, n - k0);
+  const int c0 = blockIdx.x * FJ_GT, cb = min(FJ_GT, n - c0);
+  fj_tile_mma<T>(LU + (size_t)r0 * lda + k0, lda, rs, X + (size_t)k0 * ldx + c0, ldx, cb, ks, sA,
+                 sB, acc);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int c = 0; c < 2; ++c)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int i = wr + a * 16 + FjMfma<T>::row(lk, v);
-        const int j = c0 + wc + c * 16 + li;
-        if (i < rs && j < n) X[(size_t)(r0 + i) * ldx + j] = acc[a][c][v];
+        const int i = wr + a * 16 + FjMfma<T>::row(lk, v), j = wc + c * 16 + li;
+        if (i < rs && j < cb) X[(size_t)(r0 + i) * ldx + c0 + j] -= acc[a][c][v];
       }
+}
+
+template <typename T>
+static void fj_inv_perm_init(T* X, int ldx, int n, const int32_t* ipiv, hipStream_t st);
+template <>
+void fj_inv_perm_init<double>(double* X, int ldx, int n, const int32_t* ipiv, hipStream_t st) {
+  hipLaunchKernelGGL(k_inv_perm_init_f64, dim3((unsigned)(((size_t)n * n + 255) / 256)), dim3(256),
+                     0, st, X, ldx, n, ipiv);
+}
+template <>
+void fj_inv_perm_init<float>(float* X, int ldx, int n, const int32_t* ipiv, hipStream_t st) {
+  hipLaunchKernelGGL(k_inv_perm_init_f32, dim3((unsigned)(((size_t)n * n + 255) / 256)), dim3(256),
+                     0, st, X, ldx, n, ipiv);
 }
 
 template <typename T>
@@ -870,16 +985,30 @@ static int fj_lu_inverse_t(const T* LU, int n, int lda, const int32_t* ipiv, con
   if (n <= 0 || n > FJ_SOLVE_MAXN || lda < n || ldx < n || !LU || !ipiv || !aux || !X)
     return FJ_ERR_ARG;
   const int nblk = fj_nblk(n);
-  const dim3 grid(fj_cdiv(n, FJ_GT));
+  const int tiles = fj_cdiv(n, FJ_GT);
+  const T* Linv = aux;
+  const T* Uinv = aux + (size_t)nblk * FJ_TS_B * FJ_TS_B;
+  fj_inv_perm_init<T>(X, ldx, n, ipiv, st);
+  FJ_CHECK_LAUNCH();
   for (int kb = 0; kb < nblk; ++kb) {
-    hipLaunchKernelGGL(k_lu_inverse_row<T>, grid, dim3(256), 0, st, LU, lda, aux, ipiv, X, ldx, n,
-                       kb, 0);
+    hipLaunchKernelGGL(k_inv_diag<T>, dim3(tiles), dim3(256), 0, st, X, ldx, n,
+                       Linv + (size_t)kb * FJ_TS_B * FJ_TS_B, kb);
     FJ_CHECK_LAUNCH();
+    if (nblk - kb - 1 > 0) {
+      hipLaunchKernelGGL(k_inv_update<T>, dim3(tiles, nblk - kb - 1), dim3(256), 0, st, LU, lda, X,
+                         ldx, n, kb, 1);
+      FJ_CHECK_LAUNCH();
+    }
   }
   for (int kb = nblk - 1; kb >= 0; --kb) {
-    hipLaunchKernelGGL(k_lu_inverse_row<T>, grid, dim3(256), 0, st, LU, lda, aux, ipiv, X, ldx, n,
-                       kb, 1);
+    hipLaunchKernelGGL(k_inv_diag<T>, dim3(tiles), dim3(256), 0, st, X, ldx, n,
+                       Uinv + (size_t)kb * FJ_TS_B * FJ_TS_B, kb);
     FJ_CHECK_LAUNCH();
+    if (kb > 0) {
+      hipLaunchKernelGGL(k_inv_update<T>, dim3(tiles, kb), dim3(256), 0, st, LU, lda, X, ldx, n, kb,
+                         0);
+      FJ_CHECK_LAUNCH();
+    }
   }
   return FJ_OK;
 }
