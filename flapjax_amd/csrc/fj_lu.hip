@@ -35,139 +35,108 @@ __device__ __forceinline__ unsigned long long fj_pivot_key(double a, int row) {
   return (b & ~0x1FFFull) | (unsigned long long)(8191 - row);
 }
 
-// 64-bit lane value moved by a DPP pattern (both halves), rows outside
-// ROW_MASK keep their own value
-template <int CTRL, int ROW_MASK>
-__device__ __forceinline__ unsigned long long fj_dpp64(unsigned long long v) {
-  const int lo = (int)(unsigned)(v & 0xffffffffull), hi = (int)(unsigned)(v >> 32);
-  const int nlo = __builtin_amdgcn_update_dpp(lo, lo, CTRL, ROW_MASK, 0xf, false);
-  const int nhi = __builtin_amdgcn_update_dpp(hi, hi, CTRL, ROW_MASK, 0xf, false);
-  return ((unsigned long long)(unsigned)nhi << 32) | (unsigned long long)(unsigned)nlo;
-}
-
-__device__ __forceinline__ unsigned long long fj_umax64(unsigned long long a,
-                                                        unsigned long long b) {
-  return a > b ? a : b;
-}
-
-// wave64 maximum through DPP (no LDS round trips): quad swaps, half-row and
-// row mirrors give every lane its 16-lane row maximum, then the row_bcast15 /
-// row_bcast31 steps carry it into row 3; lane 63 holds the wave maximum.
-__device__ __forceinline__ unsigned long long fj_wave_max_u64(unsigned long long v) {
-  v = fj_umax64(v, fj_dpp64<0xB1, 0xF>(v));   // quad_perm [1,0,3,2]
-  v = fj_umax64(v, fj_dpp64<0x4E, 0xF>(v));   // quad_perm [2,3,0,1]
-  v = fj_umax64(v, fj_dpp64<0x141, 0xF>(v));  // row_half_mirror
-  v = fj_umax64(v, fj_dpp64<0x140, 0xF>(v));  // row_mirror
-  v = fj_umax64(v, fj_dpp64<0x142, 0xA>(v));  // row_bcast15 -> rows 1, 3
-  v = fj_umax64(v, fj_dpp64<0x143, 0xC>(v));  // row_bcast31 -> rows 2, 3
-  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(v & 0xffffffffull), 63);
-  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(v >> 32), 63);
-  return ((unsigned long long)hi << 32) | lo;
-}
-
-// Panel factorisation (LAPACK getf2 on the (n-k) x jb panel), one workgroup.
-// The panel sits in registers: thread tid owns local rows tid + r * THREADS
-// and JB values of each.  Pivoting is implicit: rows never move between
-// threads; each row carries its logical position (LAPACK's row order after
-// the exchanges so far), used for the tie break of the pivot search and
-// updated by the exchange of each column.  Register slot 0 always holds the
-// current column: the rank-1 update writes a[c] = a[c+1] - l u[c+1] (update
-// and shift in one FMA) and a[JB-1] = l, so the finished multipliers rotate
-// in behind the remaining columns; the published pivot row carries zeros in
-// its own multiplier slots, so they shift through unchanged.  The column
-// loop is short, rolled and free of global memory traffic; at the end each
-// row is written once, to its logical position, undoing its rotation (the
-// step it was chosen as pivot, or jb).
 template <typename T, int R, int JB>
 __global__ __launch_bounds__(FJ_PANEL_THREADS) void k_lu_panel(T* __restrict__ A, int n, int lda,
                                                                int k, int32_t* __restrict__ ipiv,
                                                                int32_t* __restrict__ info) {
   __shared__ unsigned long long s_key[FJ_PANEL_WAVES];
-  __shared__ T s_piv[JB];
+  __shared__ T s_piv[2][JB];
+  __shared__ T s_swp[2][JB];
   __shared__ int s_ipiv[JB];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int jb = min(JB, n - k);
   const int rows = n - k;
-  T* P = A + (size_t)k * lda + k;  // panel origin
 
+  // load the panel: thread tid owns local rows tid + r * FJ_PANEL_THREADS
   T a[R][JB];
-  int pos[R], rot[R];
-  bool live[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int lr = tid + r * FJ_PANEL_THREADS;
-    live[r] = lr < rows;
-    pos[r] = lr;
-    rot[r] = jb;
 #pragma unroll
-    for (int c = 0; c < JB; ++c) a[r][c] = (live[r] && c < jb) ? P[(size_t)lr * lda + c] : T(0);
+    for (int c = 0; c < JB; ++c)
+      a[r][c] = (lr < rows && c < jb) ? A[(size_t)(k + lr) * lda + k + c] : T(0);
   }
 
-#pragma unroll 1
-  for (int jj = 0; jj < jb; ++jj) {
-    unsigned long long key = 0ull;
+  // fully unrolled so that every register index below is a compile-time constant
 #pragma unroll
-    for (int r = 0; r < R; ++r)
-      if (live[r]) key = fj_umax64(key, fj_pivot_key((double)a[r][0], pos[r]));
-    key = fj_wave_max_u64(key);
-    if (lane == 0) s_key[wave] = key;
-    __syncthreads();
-    key = s_key[0];
+  for (int jj = 0; jj < JB; ++jj) {
+    if (jj < jb) {  // uniform: jb is the same for the whole workgroup
+      const int buf = jj & 1;
+      unsigned long long key = 0ull;
 #pragma unroll
-    for (int w = 1; w < FJ_PANEL_WAVES; ++w) key = fj_umax64(key, s_key[w]);
-    const int p = 8191 - (int)(key & 0x1FFFull);  // logical position of the pivot row
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      if (live[r] && pos[r] == p) {
-        // the pivot row: publish it (zeros over its multiplier slots), retire it
-#pragma unroll
-        for (int c = 0; c < JB; ++c) s_piv[c] = (c < JB - jj) ? a[r][c] : T(0);
-        live[r] = false;
-        pos[r] = jj;
-        rot[r] = jj;
-      } else if (live[r] && pos[r] == jj) {
-        pos[r] = p;  // the row at position jj takes the pivot's place
+      for (int r = 0; r < R; ++r) {
+        const int lr = tid + r * FJ_PANEL_THREADS;
+        if (lr >= jj && lr < rows) {
+          const unsigned long long kk = fj_pivot_key((double)a[r][jj], lr);
+          key = kk > key ? kk : key;
+        }
       }
-    }
-    if (tid == 0) {
-      s_ipiv[jj] = k + p;
-      if ((key >> 13) == 0ull && *info == 0) *info = k + jj + 1;  // zero column
-    }
-    __syncthreads();
-    const T piv = s_piv[0];
-    const T inv = (piv != T(0)) ? T(1) / piv : T(0);
-    T u[JB];
 #pragma unroll
-    for (int c = 1; c < JB; ++c) u[c] = s_piv[c];
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      if (live[r]) {
-        const T l = a[r][0] * inv;
-#pragma unroll
-        for (int c = 0; c < JB - 1; ++c) a[r][c] = a[r][c + 1] - l * u[c + 1];
-        a[r][JB - 1] = l;
+      for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(key, off);
+        key = o > key ? o : key;
       }
+      if (lane == 0) s_key[wave] = key;
+      __syncthreads();
+      key = s_key[0];
+#pragma unroll
+      for (int w = 1; w < FJ_PANEL_WAVES; ++w) key = s_key[w] > key ? s_key[w] : key;
+      const int p = 8191 - (int)(key & 0x1FFFull);  // local pivot row
+      // owners publish the pivot row and the current row jj
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int lr = tid + r * FJ_PANEL_THREADS;
+        if (lr == p) {
+#pragma unroll
+          for (int c = 0; c < JB; ++c) s_piv[buf][c] = a[r][c];
+        }
+        if (lr == jj) {
+#pragma unroll
+          for (int c = 0; c < JB; ++c) s_swp[buf][c] = a[r][c];
+        }
+      }
+      if (tid == 0) {
+        s_ipiv[jj] = k + p;
+        if ((key >> 13) == 0ull && *info == 0) *info = k + jj + 1;  // zero column
+      }
+      __syncthreads();
+      const T piv = s_piv[buf][jj];
+      const T inv = (piv != T(0)) ? T(1) / piv : T(0);
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int lr = tid + r * FJ_PANEL_THREADS;
+        if (lr == jj) {
+#pragma unroll
+          for (int c = 0; c < JB; ++c) a[r][c] = s_piv[buf][c];
+        } else if (lr == p) {
+#pragma unroll
+          for (int c = 0; c < JB; ++c) a[r][c] = s_swp[buf][c];
+        }
+        if (lr > jj && lr < rows) {
+          const T l = a[r][jj] * inv;
+          a[r][jj] = l;
+#pragma unroll
+          for (int c = 0; c < JB; ++c)
+            if (c > jj) a[r][c] -= l * s_piv[buf][c];
+        }
+      }
+      // the next column writes the other LDS buffers (s_key is protected by
+      // the second barrier of this column)
     }
-    // the next column's LDS writes follow its first barrier, after every
-    // thread has read this column's s_key / s_piv
   }
+  __syncthreads();
 
-  // each row to its logical position; column c sits in slot (c - rot) mod JB
+  // write the panel back
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int lr = tid + r * FJ_PANEL_THREADS;
     if (lr < rows) {
-      T* dst = P + (size_t)pos[r] * lda;
 #pragma unroll
-      for (int sl = 0; sl < JB; ++sl) {
-        int c = sl + rot[r];
-        c = (c >= JB) ? c - JB : c;
-        if (c < jb) dst[c] = a[r][sl];
-      }
+      for (int c = 0; c < JB; ++c)
+        if (c < jb) A[(size_t)(k + lr) * lda + k + c] = a[r][c];
     }
   }
-  __syncthreads();
   if (tid < jb) ipiv[k + tid] = s_ipiv[tid];
   // composed permutation ipiv[n + q]: only positions k..k+jb-1 and the pivot
   // rows change; each is traced back through this panel's exchanges (last
@@ -192,91 +161,40 @@ __global__ __launch_bounds__(FJ_PANEL_THREADS) void k_lu_panel(T* __restrict__ A
 }
 
 // Columns outside the panel, one thread each: the panel's row exchanges
-// (laswp) and, right of the panel, U12 = L11^{-1} A12 (trsm).  The jb
-// sequential exchanges only touch positions k..k+jb-1 and the pivot rows
-// beyond them; their composition (which old position lands where) is built
-// once per workgroup, so each column is one round of independent loads,
-// the exchanged values written back, and the unit-lower solve in registers
-// with L11 from LDS.
-#define FJ_SWAP_THREADS 64
+// (laswp), then U12 = L11^{-1} A12 for the columns right of the panel (trsm).
 template <typename T, int JB>
-__global__ __launch_bounds__(FJ_SWAP_THREADS) void k_lu_swap_trsm(T* __restrict__ A, int n,
-                                                                 int lda, int k, int jb,
-                                                                 const int32_t* __restrict__ ipiv) {
+__global__ __launch_bounds__(256) void k_lu_swap_trsm(T* __restrict__ A, int n, int lda, int k,
+                                                      int jb, const int32_t* __restrict__ ipiv) {
   __shared__ T s_L11[JB][JB + 1];
-  __shared__ T s_v[2 * JB][FJ_SWAP_THREADS];
-  __shared__ T s_x[JB][FJ_SWAP_THREADS];
+  __shared__ T s_u[JB][256];
   __shared__ int s_ipiv[JB];
-  __shared__ int s_pos[2 * JB];  // affected positions: k..k+jb-1, then the pivot rows beyond
-  __shared__ int s_src[2 * JB];  // list index of the position whose old value lands here
-  __shared__ int s_cnt;
   const int tid = threadIdx.x;
-  for (int e = tid; e < JB * JB; e += FJ_SWAP_THREADS) {
+  for (int e = tid; e < JB * JB; e += blockDim.x) {
     const int r = e / JB, q = e % JB;
     s_L11[r][q] = (r < jb && q < jb) ? A[(size_t)(k + r) * lda + k + q] : T(0);
   }
   if (tid < JB) s_ipiv[tid] = (tid < jb) ? ipiv[k + tid] : k + tid;
   __syncthreads();
-  if (tid == 0) {
-    int cnt = jb;
-    for (int jj = 0; jj < jb; ++jj) s_pos[jj] = k + jj;
-    for (int jj = 0; jj < jb; ++jj) {
-      const int p = s_ipiv[jj];
-      if (p < k + jb) continue;
-      bool seen = false;
-      for (int e = jb; e < cnt; ++e) seen = seen || (s_pos[e] == p);
-      if (!seen) s_pos[cnt++] = p;
-    }
-    s_cnt = cnt;
-  }
-  __syncthreads();
-  const int cnt = s_cnt;
-  if (tid < cnt) {
-    // trace the content of position s_pos[tid] back through the exchanges
-    int src = s_pos[tid];
-    for (int jj = jb - 1; jj >= 0; --jj) {
-      const int a0 = k + jj, b0 = s_ipiv[jj];
-      if (src == a0) src = b0;
-      else if (src == b0) src = a0;
-    }
-    int idx = 0;
-    for (int e = 0; e < cnt; ++e) idx = (s_pos[e] == src) ? e : idx;
-    s_src[tid] = idx;
-  }
-  __syncthreads();
-  int c = blockIdx.x * FJ_SWAP_THREADS + tid;
+  int c = blockIdx.x * blockDim.x + tid;
   if (c >= n - jb) return;
   if (c >= k) c += jb;  // skip the panel columns
-#pragma unroll
-  for (int e = 0; e < 2 * JB; ++e)
-    if (e < cnt) s_v[e][tid] = A[(size_t)s_pos[e] * lda + c];
-  // positions beyond the panel block: exchanged values only
-  for (int e = jb; e < cnt; ++e) {
-    const int se = s_src[e];
-    if (se != e) A[(size_t)s_pos[e] * lda + c] = s_v[se][tid];
+  for (int jj = 0; jj < jb; ++jj) {
+    const int p = s_ipiv[jj];
+    if (p != k + jj) {
+      const T t0 = A[(size_t)(k + jj) * lda + c];
+      A[(size_t)(k + jj) * lda + c] = A[(size_t)p * lda + c];
+      A[(size_t)p * lda + c] = t0;
+    }
   }
   if (c >= k + jb) {
-    // forward substitution with the unit-lower L11, the column in this
-    // thread's LDS slot; four partial sums break the FMA chain
-    for (int r = 0; r < jb; ++r) s_x[r][tid] = s_v[s_src[r]][tid];
+    // forward substitution with the column staged in this thread's LDS slot
+    for (int r = 0; r < jb; ++r) s_u[r][tid] = A[(size_t)(k + r) * lda + c];
     for (int r = 1; r < jb; ++r) {
-      T a0 = s_x[r][tid], a1 = T(0), a2 = T(0), a3 = T(0);
-      int q = 0;
-      for (; q + 3 < r; q += 4) {
-        a0 -= s_L11[r][q] * s_x[q][tid];
-        a1 -= s_L11[r][q + 1] * s_x[q + 1][tid];
-        a2 -= s_L11[r][q + 2] * s_x[q + 2][tid];
-        a3 -= s_L11[r][q + 3] * s_x[q + 3][tid];
-      }
-      for (; q < r; ++q) a0 -= s_L11[r][q] * s_x[q][tid];
-      const T v = (a0 + a1) + (a2 + a3);
-      s_x[r][tid] = v;
-      A[(size_t)(k + r) * lda + c] = v;
+      T acc = s_u[r][tid];
+      for (int q = 0; q < r; ++q) acc -= s_L11[r][q] * s_u[q][tid];
+      s_u[r][tid] = acc;
+      A[(size_t)(k + r) * lda + c] = acc;
     }
-    A[(size_t)k * lda + c] = s_x[0][tid];
-  } else {
-    for (int r = 0; r < jb; ++r)
-      if (s_src[r] != r) A[(size_t)(k + r) * lda + c] = s_v[s_src[r]][tid];
   }
 }
 
@@ -381,8 +299,8 @@ static int fj_lu_factor_t(T* A, int n, int lda, int32_t* ipiv, int32_t* info, T*
                        ipiv, info);
     FJ_CHECK_LAUNCH();
     if (n - jb > 0) {
-      hipLaunchKernelGGL((k_lu_swap_trsm<T, JB>), dim3(fj_cdiv(n - jb, FJ_SWAP_THREADS)),
-                         dim3(FJ_SWAP_THREADS), 0, st, A, n, lda, k, jb, ipiv);
+      hipLaunchKernelGGL((k_lu_swap_trsm<T, JB>), dim3(fj_cdiv(n - jb, 256)), dim3(256), 0, st, A, n,
+                         lda, k, jb, ipiv);
       FJ_CHECK_LAUNCH();
     }
     const int rest = n - k - jb;
