@@ -345,7 +345,7 @@ def main():
     roof, sample = rollup_roofline()
     srv = {}
     if args.serve_cases > 0:
-        for prec in ("fp64", "mixed"):
+        for prec in ("fp64",):
             st = serve(prec, args.serve_cases, args.serve_rate, 32, (50, 100))
             srv[prec] = {"cases_per_s": round(st["cases_per_s"], 3), "p50_s": round(st["p50_s"], 4),
                          "p99_s": round(st["p99_s"], 4), "gpairs_per_s": round(st["gpairs_per_s"], 3),

@@ -108,9 +108,23 @@ _SIGS = {
     "fj_pair_sum_batched": ([c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                              c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_double, c_double, c_void_p, c_void_p,
                              c_void_p, c_void_p], c_int),
+    "fj_step_batched_workspace_bytes": ([c_int, c_void_p], c_size_t),
+    "fj_step_batched_table_bytes": ([c_int], c_size_t),
+    "fj_step_batched": ([c_int, c_void_p, POINTER(FjFlowfield), c_double, c_double, c_int, c_double, c_double,
+                         c_int, c_int, c_double, c_void_p, c_void_p, c_void_p, c_void_p], c_int),
     "fj_flow_eval": ([POINTER(FjFlowfield), c_void_p, c_int, c_double, c_void_p, c_void_p], c_int),
 }
 
+
+
+class FjStepCase(ctypes.Structure):
+    """fj_step_case: one active case of a batched serving tick."""
+
+    _fields_ = [("m", c_int), ("n", c_int), ("m_star", c_int), ("dt", c_double), ("time", c_double)] + [
+        (name, c_void_p) for name in (
+            "hg", "hg_dot", "dof", "zeta0", "zeta_b_nm1", "zeta_b", "zeta_b_dot", "gamma_b_nm1", "gamma_w_nm1",
+            "zeta_w_nm1", "gamma_w", "zeta_w", "aic", "aic_inv", "gamma_b", "gamma_b_dot", "f_steady", "lift",
+            "refine_ws", "iters")]
 
 
 class FjGridRef(ctypes.Structure):
@@ -176,7 +190,16 @@ def ptr(t):
     return ctypes.c_void_p(t.data_ptr())
 
 
+_raw_stream = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+_cur_device = getattr(torch._C, "_cuda_getDevice", None)
+
+
 def stream():
+    """torch's current HIP stream on the current device, as a raw handle
+    (through torch's raw-stream accessor where present: the public
+    current_stream() re-validates the device on every call)."""
+    if _raw_stream is not None and _cur_device is not None:
+        return ctypes.c_void_p(_raw_stream(_cur_device()))
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 

@@ -14,9 +14,10 @@
 
 
 struct FjTask {
-  int tgt_off, n_tgt;      // packed target rows
-  int seg_off, n_seg;      // packed filament rows
-  int out_off;             // packed output rows
+  const double* tgt;       // this task's targets (n_tgt, 3)
+  double* out;             // this task's outputs (n_tgt, 3)
+  int tgt_off, n_tgt;      // first row in the tasks' concatenated target list
+  int seg_off, n_seg;      // filament rows in the packed table
   int tiles, chunks, chunk_len;
   int work_off;            // first work item of this task
   int part_off;            // first double of this task's partials
@@ -39,7 +40,7 @@ __device__ __forceinline__ int fj_find_task(const FjTask* __restrict__ tasks, in
 template <int KIND>
 __global__ __launch_bounds__(FJ_BS_THREADS) void k_batched_partial(
     const FjTask* __restrict__ tasks, const int* __restrict__ counts, const double* __restrict__ seg,
-    const double* __restrict__ tgt, double* __restrict__ part, FjKernelParams kp) {
+    double* __restrict__ part, FjKernelParams kp) {
   __shared__ double red[FJ_BS_WAVES - 1][3][64];
   const int n_tasks = counts[0];
   const int total = counts[1];
@@ -53,7 +54,7 @@ __global__ __launch_bounds__(FJ_BS_THREADS) void k_batched_partial(
     const int chunk = local / T.tiles;
     int t = tile * 64 + lane;
     const int tc = (t < T.n_tgt) ? t : (T.n_tgt - 1);
-    const double* xp = tgt + 3 * (size_t)(T.tgt_off + tc);
+    const double* xp = T.tgt + 3 * (size_t)tc;
     const double x = xp[0], y = xp[1], z = xp[2];
     double ax = 0.0, ay = 0.0, az = 0.0;
     const int s_begin = chunk * T.chunk_len;
@@ -93,7 +94,7 @@ __global__ __launch_bounds__(FJ_BS_THREADS) void k_batched_partial(
 #define FJ_BRED_WAVES 8
 __global__ __launch_bounds__(64 * FJ_BRED_WAVES) void k_batched_reduce(
     const FjTask* __restrict__ tasks, const int* __restrict__ counts,
-    const double* __restrict__ part, const double* __restrict__ tgt, double* __restrict__ out) {
+    const double* __restrict__ part) {
   __shared__ double red[FJ_BRED_WAVES - 1][3][64];
   const int n_tasks = counts[0];
   const int n_tgt_total = counts[2];
@@ -132,13 +133,13 @@ __global__ __launch_bounds__(64 * FJ_BRED_WAVES) void k_batched_reduce(
     vz += red[w][2][lane];
   }
   vx *= FJ_INV_4PI; vy *= FJ_INV_4PI; vz *= FJ_INV_4PI;
-  double* op = out + 3 * (size_t)(T.out_off + t);
+  double* op = T.out + 3 * (size_t)t;
   if (T.mode == FJ_OUT_STORE) {
     op[0] = vx; op[1] = vy; op[2] = vz;
   } else if (T.mode == FJ_OUT_ADD) {
     op[0] += vx; op[1] += vy; op[2] += vz;
   } else {
-    const double* xp = tgt + 3 * (size_t)g;
+    const double* xp = T.tgt + 3 * (size_t)t;
     double u[3];
     fj_flow_eval(T.flow, xp[0], xp[1], xp[2], T.time, u);
     op[0] = xp[0] + T.dt * (u[0] + vx);
@@ -150,16 +151,99 @@ __global__ __launch_bounds__(64 * FJ_BRED_WAVES) void k_batched_reduce(
 // ---------------------------------------------------------------------------
 // host side: the caller passes per-case descriptors in HOST memory; the
 // library fills the device task table (one async copy) and launches.
+size_t fj_batched_desc_bytes(int n_tasks) { return (size_t)n_tasks * sizeof(FjTaskDesc); }
+
+// device task table: the task records and three counters; the host staging
+// area is the same size plus room for the task descriptors
 extern "C" size_t fj_batched_task_bytes(int n_tasks) {
-  return (size_t)n_tasks * sizeof(FjTask) + 64;
+  return (size_t)n_tasks * sizeof(FjTask) + 64 + fj_batched_desc_bytes(n_tasks);
+}
+
+static void fj_task_tiling(int n_tgt, int n_seg, int& tiles, int& chunks, int& chunk_len) {
+  tiles = n_tgt > 0 ? fj_cdiv(n_tgt, 64) : 0;  // no work for an empty target set
+  chunks = n_seg > 0 ? fj_cdiv(n_seg, 256) : 1;
+  chunk_len = n_seg > 0 ? fj_cdiv(n_seg, chunks) : 0;
+}
+
+size_t fj_batched_ws_bytes(const FjTaskDesc* d, int n) {
+  size_t total = 0;
+  for (int i = 0; i < n; ++i) {
+    int tiles, chunks, chunk_len;
+    fj_task_tiling(d[i].n_tgt, d[i].n_seg, tiles, chunks, chunk_len);
+    total += (size_t)chunks * 3 * (tiles > 0 ? tiles : 1) * 64;
+  }
+  return total * sizeof(double) + 256;
+}
+
+int fj_batched_launch(const FjTaskDesc* d, int n_tasks, const double* seg, int kernel,
+                      const FjKernelParams& kp, void* task_table, void* host_staging,
+                      void* workspace, hipStream_t st) {
+  if (n_tasks <= 0) return FJ_OK;
+  if (!d || !task_table || !host_staging || !workspace) return FJ_ERR_ARG;
+  if (kernel < FJ_KERNEL_EPSILON || kernel > FJ_KERNEL_PLAIN) return FJ_ERR_ARG;
+  // host_staging: fj_batched_task_bytes(n_tasks) bytes of host memory that
+  // stays valid until the copy has run (the caller keeps it)
+  FjTask* h = (FjTask*)host_staging;
+  int work = 0, part = 0, n_tgt_total = 0;
+  for (int i = 0; i < n_tasks; ++i) {
+    FjTask& T = h[i];
+    if (d[i].n_tgt < 0 || d[i].n_seg < 0 || d[i].mode < FJ_OUT_STORE || d[i].mode > FJ_OUT_CONVECT)
+      return FJ_ERR_ARG;
+    if (d[i].n_tgt > 0 && (!d[i].tgt || !d[i].out || (d[i].n_seg > 0 && !seg))) return FJ_ERR_ARG;
+    T.tgt = d[i].tgt;
+    T.out = d[i].out;
+    T.tgt_off = n_tgt_total;
+    T.n_tgt = d[i].n_tgt;
+    T.seg_off = d[i].seg_off;
+    T.n_seg = d[i].n_seg;
+    fj_task_tiling(T.n_tgt, T.n_seg, T.tiles, T.chunks, T.chunk_len);
+    T.work_off = work;
+    T.part_off = part;
+    T.ld = T.tiles * 64;
+    T.mode = d[i].mode;
+    T.dt = d[i].dt;
+    T.time = d[i].time;
+    T.flow = d[i].flow;
+    work += T.tiles * T.chunks;
+    part += T.chunks * 3 * T.ld;
+    n_tgt_total += T.n_tgt;
+  }
+  int* counts = (int*)(h + n_tasks);
+  counts[0] = n_tasks;
+  counts[1] = work;
+  counts[2] = n_tgt_total;
+  if (hipMemcpyAsync(task_table, h, fj_batched_task_bytes(n_tasks), hipMemcpyHostToDevice, st) !=
+      hipSuccess)
+    return FJ_ERR_LAUNCH;
+  const FjTask* dtasks = (const FjTask*)task_table;
+  const int* dcounts = (const int*)(dtasks + n_tasks);
+  const int grid = work < 4096 ? work : 4096;
+  if (work > 0) {
+    if (kernel == FJ_KERNEL_EPSILON)
+      hipLaunchKernelGGL(k_batched_partial<FJ_KERNEL_EPSILON>, dim3(grid), dim3(FJ_BS_THREADS), 0, st,
+                         dtasks, dcounts, seg, (double*)workspace, kp);
+    else if (kernel == FJ_KERNEL_CUTOFF)
+      hipLaunchKernelGGL(k_batched_partial<FJ_KERNEL_CUTOFF>, dim3(grid), dim3(FJ_BS_THREADS), 0, st,
+                         dtasks, dcounts, seg, (double*)workspace, kp);
+    else
+      hipLaunchKernelGGL(k_batched_partial<FJ_KERNEL_PLAIN>, dim3(grid), dim3(FJ_BS_THREADS), 0, st,
+                         dtasks, dcounts, seg, (double*)workspace, kp);
+    FJ_CHECK_LAUNCH();
+  }
+  if (n_tgt_total > 0) {
+    hipLaunchKernelGGL(k_batched_reduce, dim3(fj_cdiv(n_tgt_total, 64)), dim3(64 * FJ_BRED_WAVES),
+                       0, st, dtasks, dcounts, (const double*)workspace);
+    FJ_CHECK_LAUNCH();
+  }
+  return FJ_OK;
 }
 
 extern "C" size_t fj_batched_workspace_bytes(int n_tasks, const int* n_tgt, const int* n_seg) {
   size_t total = 0;
   for (int i = 0; i < n_tasks; ++i) {
-    const int tiles = fj_cdiv(n_tgt[i] > 0 ? n_tgt[i] : 1, 64);
-    int chunks = n_seg[i] > 0 ? fj_cdiv(n_seg[i], 256) : 1;
-    total += (size_t)chunks * 3 * tiles * 64;
+    int tiles, chunks, chunk_len;
+    fj_task_tiling(n_tgt[i], n_seg[i], tiles, chunks, chunk_len);
+    total += (size_t)chunks * 3 * (tiles > 0 ? tiles : 1) * 64;
   }
   return total * sizeof(double) + 256;
 }
@@ -175,63 +259,24 @@ extern "C" int fj_pair_sum_batched(int n_tasks, const int* tgt_off, const int* n
   if (!tgt_off || !n_tgt || !seg_off || !n_seg || !out_off || !mode || !task_table ||
       !host_staging || !workspace || !targets || !out)
     return FJ_ERR_ARG;
-  if (kernel < FJ_KERNEL_EPSILON || kernel > FJ_KERNEL_PLAIN) return FJ_ERR_ARG;
-  hipStream_t st = (hipStream_t)stream;
-  // host_staging: fj_batched_task_bytes(n_tasks) bytes of pinned or pageable
-  // host memory that stays valid until the copy has run (the caller keeps it)
-  FjTask* h = (FjTask*)host_staging;
-  int work = 0, part = 0, n_tgt_total = 0;
+  // the descriptors are built in the tail of host_staging (after the task
+  // records), which fj_batched_task_bytes sizes for both
+  FjTaskDesc* d = (FjTaskDesc*)((char*)host_staging + fj_batched_task_bytes(n_tasks) -
+                                fj_batched_desc_bytes(n_tasks));
   for (int i = 0; i < n_tasks; ++i) {
-    FjTask& T = h[i];
-    if (n_tgt[i] < 0 || n_seg[i] < 0 || mode[i] < FJ_OUT_STORE || mode[i] > FJ_OUT_CONVECT)
-      return FJ_ERR_ARG;
-    T.tgt_off = tgt_off[i];
-    T.n_tgt = n_tgt[i];
-    T.seg_off = seg_off[i];
-    T.n_seg = n_seg[i];
-    T.out_off = out_off[i];
-    T.tiles = n_tgt[i] > 0 ? fj_cdiv(n_tgt[i], 64) : 0;  // no work for an empty target set
-    T.chunks = n_seg[i] > 0 ? fj_cdiv(n_seg[i], 256) : 1;
-    T.chunk_len = n_seg[i] > 0 ? fj_cdiv(n_seg[i], T.chunks) : 0;
-    T.work_off = work;
-    T.part_off = part;
-    T.ld = T.tiles * 64;
-    T.mode = mode[i];
-    T.dt = dt ? dt[i] : 0.0;
-    T.time = time ? time[i] : 0.0;
-    T.flow = fj_flow_from_desc(flows ? &flows[i] : nullptr);
-    work += T.tiles * T.chunks;
-    part += T.chunks * 3 * T.ld;
-    if (i > 0 && T.tgt_off < h[i - 1].tgt_off + h[i - 1].n_tgt) return FJ_ERR_ARG;  // sorted, disjoint
-    n_tgt_total = T.tgt_off + T.n_tgt;
+    if (i > 0 && tgt_off[i] < tgt_off[i - 1] + n_tgt[i - 1]) return FJ_ERR_ARG;  // sorted, disjoint
+    d[i].tgt = targets + 3 * (size_t)tgt_off[i];
+    d[i].out = out + 3 * (size_t)out_off[i];
+    d[i].n_tgt = n_tgt[i];
+    d[i].seg_off = seg_off[i];
+    d[i].n_seg = n_seg[i];
+    d[i].mode = mode[i];
+    d[i].dt = dt ? dt[i] : 0.0;
+    d[i].time = time ? time[i] : 0.0;
+    d[i].flow = fj_flow_from_desc(flows ? &flows[i] : nullptr);
   }
-  int* counts = (int*)(h + n_tasks);
-  counts[0] = n_tasks;
-  counts[1] = work;
-  counts[2] = n_tgt_total;
-  if (hipMemcpyAsync(task_table, h, fj_batched_task_bytes(n_tasks), hipMemcpyHostToDevice, st) !=
-      hipSuccess)
-    return FJ_ERR_LAUNCH;
-  const FjTask* dtasks = (const FjTask*)task_table;
-  const int* dcounts = (const int*)(dtasks + n_tasks);
   FjKernelParams kp;
   kp.eps = eps; kp.eps2 = eps * eps; kp.rcut = rcut;
-  const int grid = work < 4096 ? work : 4096;
-  if (work > 0) {
-    if (kernel == FJ_KERNEL_EPSILON)
-      hipLaunchKernelGGL(k_batched_partial<FJ_KERNEL_EPSILON>, dim3(grid), dim3(FJ_BS_THREADS), 0, st,
-                         dtasks, dcounts, seg, targets, (double*)workspace, kp);
-    else if (kernel == FJ_KERNEL_CUTOFF)
-      hipLaunchKernelGGL(k_batched_partial<FJ_KERNEL_CUTOFF>, dim3(grid), dim3(FJ_BS_THREADS), 0, st,
-                         dtasks, dcounts, seg, targets, (double*)workspace, kp);
-    else
-      hipLaunchKernelGGL(k_batched_partial<FJ_KERNEL_PLAIN>, dim3(grid), dim3(FJ_BS_THREADS), 0, st,
-                         dtasks, dcounts, seg, targets, (double*)workspace, kp);
-    FJ_CHECK_LAUNCH();
-  }
-  hipLaunchKernelGGL(k_batched_reduce, dim3(fj_cdiv(n_tgt_total > 0 ? n_tgt_total : 1, 64)),
-                     dim3(64 * FJ_BRED_WAVES),
-                     0, st, dtasks, dcounts, (const double*)workspace, targets, out);
-  FJ_CHECK_LAUNCH();
-  return FJ_OK;
+  return fj_batched_launch(d, n_tasks, seg, kernel, kp, task_table, host_staging, workspace,
+                           (hipStream_t)stream);
 }

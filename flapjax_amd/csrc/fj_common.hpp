@@ -179,13 +179,28 @@ struct FjPairPlan {
   int ld;         // padded target count of the partial buffer
 };
 
+// one task of a fused multi-case pair sum (fj_batched.hip): targets and
+// outputs by pointer, filaments by row range of a shared table
+struct FjTaskDesc {
+  const double* tgt;
+  double* out;
+  int n_tgt, seg_off, n_seg, mode;
+  double dt, time;
+  FjFlow flow;
+};
+size_t fj_batched_ws_bytes(const FjTaskDesc* d, int n);
+size_t fj_batched_desc_bytes(int n_tasks);
+int fj_batched_launch(const FjTaskDesc* d, int n_tasks, const double* seg, int kernel,
+                      const FjKernelParams& kp, void* task_table, void* host_staging,
+                      void* workspace, hipStream_t st);
+
 // mixed-precision partial sums (fj_biot_savart_mixed.hip)
 int fj_pair_partial_mixed(const FjPairPlan* p, const double* seg, int n_seg, const double* tgt,
                           int n_tgt, double* part, int kernel, const FjKernelParams* kp,
                           hipStream_t st);
 
 // Launch-shape helpers
-static inline int fj_cdiv(long long a, long long b) { return (int)((a + b - 1) / b); }
+__host__ __device__ static inline int fj_cdiv(long long a, long long b) { return (int)((a + b - 1) / b); }
 
 #define FJ_CHECK_LAUNCH()                         \
   do {                                            \

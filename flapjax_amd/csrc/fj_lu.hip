@@ -17,6 +17,7 @@
 // one wave in LDS, the off-diagonal GEMV by all waves (one row per wave, lanes
 // over the block columns, coalesced), x kept in LDS.
 #include "fj_common.hpp"
+#include "fj_refine_dev.hpp"
 
 #define FJ_LU_THREADS 1024
 #define FJ_LU_WAVES (FJ_LU_THREADS / 64)
@@ -508,33 +509,6 @@ extern "C" int fj_lu_solve(const double* LU, int n, int lda, const int32_t* ipiv
 // max-atomics {||A||_inf, ||r||_inf, ||x_k||_inf}, then flags {status
 // (0 running, 1 converged), corrections made}.  Every launch reads the flags
 // first, so once converged the remaining sweeps exit at entry.
-#define FJ_REFINE_MAX_ITER 64
-#define FJ_RV_WAVES 4
-
-typedef unsigned long long fj_u64;
-
-__device__ __forceinline__ fj_u64 fj_bits(double v) { return (fj_u64)__double_as_longlong(v); }
-__device__ __forceinline__ double fj_from_bits(fj_u64 b) { return __longlong_as_double((long long)b); }
-
-// max of a non-negative double over the workgroup, then one atomic per group
-__device__ __forceinline__ void fj_group_max(double v, fj_u64* dst, double* sred) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
-  if (lane == 0) sred[wave] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double m = sred[0];
-    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = fmax(m, sred[w]);
-    atomicMax(dst, fj_bits(m));
-  }
-}
-
-__device__ __forceinline__ bool fj_refine_converged(const fj_u64* slot, double tol) {
-  const double an = fj_from_bits(slot[0]), rn = fj_from_bits(slot[1]), xn = fj_from_bits(slot[2]);
-  return rn <= tol * an * xn;
-}
-
 __global__ void k_refine_init(fj_u64* __restrict__ slots, int nslots, int32_t* __restrict__ flags) {
   for (int i = threadIdx.x; i < 3 * nslots; i += blockDim.x) slots[i] = 0ull;
   if (threadIdx.x == 0) { flags[0] = 0; flags[1] = 0; }
@@ -643,24 +617,6 @@ __global__ void k_refine_finish(const fj_u64* __restrict__ slots, int max_iter, 
 extern "C" size_t fj_refine_workspace_bytes(int n) {
   return ((size_t)(n + 4) * sizeof(double) + 3 * (FJ_REFINE_MAX_ITER + 1) * sizeof(fj_u64) + 64 +
           255) / 256 * 256;
-}
-
-struct FjRefineWs {
-  double* r;
-  fj_u64* slots;
-  int32_t* flags;
-};
-
-static FjRefineWs fj_refine_ws(void* workspace, int n) {
-  FjRefineWs w;
-  w.r = (double*)workspace;
-  w.slots = (fj_u64*)(w.r + n + 4);
-  w.flags = (int32_t*)(w.slots + 3 * (FJ_REFINE_MAX_ITER + 1));
-  return w;
-}
-
-static double fj_refine_tol(double tol, int n) {
-  return tol > 0.0 ? tol : sqrt((double)n) * 1.1102230246251565e-16;
 }
 
 // apply(first, k) launches x = M b (first) or the correction of sweep k

@@ -23,8 +23,10 @@ the circulation solve done with the explicit inverse of the dynamic AIC kept
 for the case (rigid motion keeps the AIC fixed up to rounding) and refined in
 FP64 against the AIC re-assembled every step, so each step solves the current
 system to FP64 accuracy; the stop-test outcome of every case's solve is read
-back at the end of the tick.  Precision "fp64": FP64 pair sums; "mixed":
-FP32 far-field pair sums for rollup and forces.
+back at the end of the tick.  A tick is one native call (fj_step_batched):
+each stage -- geometry, tables, the three fused pair sums, AIC, right-hand
+side, refinement, forces, lift -- is one launch over all active cases.  The
+fused launches evaluate in FP64 for either precision setting.
 """
 
 from __future__ import annotations
@@ -112,6 +114,8 @@ def footprint_bytes(req: CaseRequest, precision: str = "fp64") -> int:
     words += 16 * (s_b + s_w)                        # filament table
     words += 3 * (ms * (n + 1) + s_b + nb)           # pair-sum targets/outputs
     words += req.steps                               # lift history
+    words += 16 * (s_b + s_w) + 10 * nb + 6 * s_b     # tick workspace share: second table, stencils, midpoints
+    words += 3 * (m + 1) * (n + 1)                   # steady forces
     return 8 * words + 4096
 
 
@@ -124,8 +128,10 @@ class _Slot:
         self.dt = 1.0 / m
         self.flow = flow
         hg, hgd = motion(req)
-        self.hg = torch.as_tensor(hg, device=dev)
-        self.hgd = torch.as_tensor(hgd, device=dev)
+        # pinned staging, asynchronous upload (the staging block is held by
+        # torch's host allocator until the copy has run)
+        self.hg = torch.from_numpy(hg).pin_memory().to(dev, non_blocking=True)
+        self.hgd = torch.from_numpy(hgd).pin_memory().to(dev, non_blocking=True)
         self.dof = torch.arange(n + 1, dtype=torch.int32, device=dev)
         z0 = np.zeros((m + 1, n + 1, 3))
         z0[..., 0] = (np.linspace(0.0, 1.0, m + 1) - 0.25)[:, None]
@@ -161,6 +167,7 @@ class _Slot:
         self.Ainv = K.lu_inverse(K.lu_factor(self.A))
         self.ws = torch.empty(L.lib().fj_refine_workspace_bytes(m * n) // 8 + 1, dtype=F64, device=dev)
         self.iters = torch.zeros(1, dtype=torch.int32, device=dev)
+        self.fs = torch.empty((m + 1, n + 1, 3), dtype=F64, device=dev)
 
     @property
     def done(self):
@@ -180,6 +187,7 @@ class ServingEngine:
         self.flow = Constant(np.array([1.0, 0.0, 0.0]), 1.225, True)
         self.kind = L.KERNEL_EPSILON
         self._staging = {}
+        self._bufs = {}
 
     # ---- fused pair sums ---------------------------------------------------
     def _batched(self, tasks, seg, targets, out, prec):
@@ -215,105 +223,64 @@ class ServingEngine:
 
     # ---- one engine tick: every active case advances one step ----------------
     def _tick(self, slots):
-        dev = self.dev
-        fl = self.flow
-        # (1) geometry at n, wake circulation shift; rollup filament tables at n-1
-        seg_parts, tgt_parts, tasks = [], [], []
-        s_off = t_off = 0
-        for sl in slots:
+        """One time step of every active case in one native call
+        (fj_step_batched): geometry, wake shift and rollup, upwash, AIC,
+        right-hand side and kept-inverse refinement, forces and lift of all
+        cases, each stage one launch over all cases."""
+        nc = len(slots)
+        cases = (L.FjStepCase * nc)()
+        for k, sl in enumerate(slots):
             i = sl.i + 1
-            m, n, ms = sl.req.m, sl.req.n, sl.req.m_star
+            # (previous step, this step) buffer pairs
             sl.zb_prev, sl.zb = sl.zb, sl.zb_prev
-            K.grid_from_hg(sl.hg[i], sl.hgd[i], sl.dof, sl.zeta0, zeta=sl.zb, zeta_dot=sl.zbd)
-            K.wake_shift(sl.gb, sl.gw, sl.zb, ms, sl.gw_next, sl.zw_next)
-            if ms > 0 and self.free_wake:
-                seg = torch.cat((K.segments_from_grid(sl.zb_prev, sl.gb, self.kind, EPSILON),
-                                 K.segments_from_grid(sl.zw, sl.gw, self.kind, EPSILON)))
-                seg_parts.append(seg)
-                tgt_parts.append(sl.zw[:-1].reshape(-1, 3))
-                tasks.append((t_off, ms * (n + 1), s_off, seg.shape[0], t_off, L.OUT_CONVECT, sl.dt,
-                              i * sl.dt))
-                s_off += seg.shape[0]
-                t_off += ms * (n + 1)
-            elif ms > 0:
-                K.wake_convect(sl.zw, None, fl, i * sl.dt, sl.dt, sl.zw_next)
-        if tasks:
-            seg_all = torch.cat(seg_parts)
-            tgt_all = torch.cat(tgt_parts)
-            out_all = torch.empty_like(tgt_all)
-            self._batched(tasks, seg_all, tgt_all, out_all, L.PREC_FP64)
-            off = 0
-            for sl in slots:
-                ms, n = sl.req.m_star, sl.req.n
-                if ms > 0 and self.free_wake:
-                    sl.zw_next[1:].reshape(-1, 3).copy_(out_all[off:off + ms * (n + 1)])
-                    off += ms * (n + 1)
+            sl.gb_prev, sl.gb = sl.gb, sl.gb_prev
+            c = cases[k]
+            c.m, c.n, c.m_star = sl.req.m, sl.req.n, sl.req.m_star
+            c.dt, c.time = sl.dt, i * sl.dt
+            c.hg, c.hg_dot = sl.hg[i].data_ptr(), sl.hgd[i].data_ptr()
+            c.dof, c.zeta0 = sl.dof.data_ptr(), sl.zeta0.data_ptr()
+            c.zeta_b_nm1, c.zeta_b, c.zeta_b_dot = sl.zb_prev.data_ptr(), sl.zb.data_ptr(), sl.zbd.data_ptr()
+            c.gamma_b_nm1, c.gamma_b = sl.gb_prev.data_ptr(), sl.gb.data_ptr()
+            c.gamma_w_nm1, c.zeta_w_nm1 = sl.gw.data_ptr(), sl.zw.data_ptr()
+            c.gamma_w, c.zeta_w = sl.gw_next.data_ptr(), sl.zw_next.data_ptr()
+            c.aic, c.aic_inv = sl.A.data_ptr(), sl.Ainv.data_ptr()
+            c.gamma_b_dot, c.f_steady = sl.gbd.data_ptr(), sl.fs.data_ptr()
+            c.lift = sl.lift.data_ptr() + 8 * i
+            c.refine_ws, c.iters = sl.ws.data_ptr(), sl.iters.data_ptr()
+        lib = L.lib()
+        wsb = lib.fj_step_batched_workspace_bytes(nc, cases)
+        tb = lib.fj_step_batched_table_bytes(nc)
+        ws = self._grow("ws", wsb)
+        table = self._grow("table", tb)
+        staging = self._grow("staging", tb, pinned=True)
+        fd = self.flow.descriptor()
+        rc = lib.fj_step_batched(nc, cases, ctypes.byref(fd), self.flow.rho, 0.7, self.kind, EPSILON, R_CUTOFF,
+                                 int(self.free_wake), 3, 0.0, L.ptr(table), ctypes.c_void_p(staging.data_ptr()),
+                                 L.ptr(ws), L.stream())
+        L.check(rc, "fj_step_batched")
         for sl in slots:
             sl.zw, sl.zw_next = sl.zw_next, sl.zw
             sl.gw, sl.gw_next = sl.gw_next, sl.gw
-            sl.gb_prev, sl.gb = sl.gb, sl.gb_prev
-        # (2) wake upwash at the collocation points (fused) and the solve
-        seg_parts, tgt_parts, tasks, cs = [], [], [], []
-        s_off = t_off = 0
-        for sl in slots:
-            c, nc, cd = K.colloc_normals(sl.zb, sl.zbd)
-            cs.append((c, nc, cd))
-            nb = sl.req.m * sl.req.n
-            if sl.req.m_star > 0:
-                seg = K.segments_from_grid(sl.zw, sl.gw, self.kind, EPSILON)
-                seg_parts.append(seg)
-                tasks.append((t_off, nb, s_off, seg.shape[0], t_off, L.OUT_STORE, 0.0, 0.0))
-                s_off += seg.shape[0]
-            tgt_parts.append(c.reshape(-1, 3))
-            t_off += nb
-        tgt_all = torch.cat(tgt_parts)
-        vw_all = torch.zeros_like(tgt_all)
-        if tasks:
-            self._batched(tasks, torch.cat(seg_parts), tgt_all, vw_all, L.PREC_FP64)
-        off = 0
-        for sl, (c, nc, cd) in zip(slots, cs):
-            i = sl.i + 1
-            m, n = sl.req.m, sl.req.n
-            nb = m * n
-            K.aic_assemble(c, nc, sl.zb, self.kind, EPSILON, R_CUTOFF, sl.A, 0)
-            rhs = K.rhs(c, nc, cd, vw_all[off:off + nb], fl, i * sl.dt)
-            off += nb
-            K.solve_refined_inv(sl.A, sl.Ainv, rhs, 3, out=sl.gb.reshape(-1), workspace=sl.ws, iters=sl.iters)
-        # stop-test outcomes of this tick's solves, read back behind the forces
+            sl.i += 1
         status = torch.cat([sl.iters for sl in slots])
-        status_host = torch.empty(status.shape, dtype=status.dtype, pin_memory=True)
+        status_host = self._grow("status", 4 * nc, pinned=True)[:4 * nc].view(torch.int32)
         status_host.copy_(status, non_blocking=True)
-        status_ev = torch.cuda.Event()
-        status_ev.record()
-        # (3) forces: velocities at the filament midpoints (fused)
-        seg_parts, tgt_parts, tasks = [], [], []
-        s_off = t_off = 0
-        for sl in slots:
-            mid = K.filament_midpoints(sl.zb)
-            seg = torch.cat((K.segments_from_grid(sl.zb, sl.gb, self.kind, EPSILON),
-                             K.segments_from_grid(sl.zw, sl.gw, self.kind, EPSILON)))
-            seg_parts.append(seg)
-            tgt_parts.append(mid)
-            tasks.append((t_off, mid.shape[0], s_off, seg.shape[0], t_off, L.OUT_STORE, 0.0, 0.0))
-            s_off += seg.shape[0]
-            t_off += mid.shape[0]
-        tgt_all = torch.cat(tgt_parts)
-        v_all = torch.empty_like(tgt_all)
-        self._batched(tasks, torch.cat(seg_parts), tgt_all, v_all, L.PREC_FP64)
-        off = 0
-        for sl in slots:
-            i = sl.i + 1
-            n_mid = sl.s_b
-            gw0 = sl.gw[0].contiguous() if sl.req.m_star > 0 else None
-            f = K.steady_forces(sl.zb, sl.zbd, sl.gb, gw0, v_all[off:off + n_mid], fl, i * sl.dt, fl.rho)
-            off += n_mid
-            sl.lift[i] = f[..., 2].sum()
-            K.gamma_dot_unsteady(sl.gb, sl.gb_prev, sl.gbd, None, sl.dt, 0.7, fl.rho, gamma_dot=sl.gbd)
-            sl.i = i
-        status_ev.synchronize()
+        torch.cuda.current_stream().synchronize()
         bad = [sl.req.case_id for sl, it in zip(slots, status_host.tolist()) if it < 0]
         if bad:
             raise L.FlapjaxLibError(f"circulation solve failed its stop test for cases {bad}")
+
+    def _grow(self, name, nbytes, pinned=False):
+        """Byte buffer kept across ticks, grown when a tick needs more."""
+        buf = self._bufs.get(name)
+        if buf is None or buf.numel() < nbytes:
+            size = max(int(nbytes), 256)
+            if pinned:
+                buf = torch.empty(size, dtype=torch.uint8, pin_memory=True)
+            else:
+                buf = torch.empty(size, dtype=torch.uint8, device=self.dev)
+            self._bufs[name] = buf
+        return buf
 
     def run(self, requests, time_source=time.perf_counter, log=None):
         """Serve requests (sorted by arrival, seconds from start).  Returns

@@ -349,6 +349,54 @@ int fj_strip_alpha(const double* zeta, const double* f_steady, int m, int n,
                    const fj_flowfield* flow, double time, double rho, double* alpha,
                    void* stream);
 
+/* ---- Serving: one time step of many independent cases ----------------- */
+
+/* One active case of a batched serving tick.  All pointers are device
+ * pointers to this case's own buffers; the grid / wake / circulation pairs
+ * are (previous step, this step) and must not alias. */
+typedef struct {
+  int m, n, m_star;          /* bound panels (m chordwise, n spanwise), wake rows */
+  double dt, time;           /* step size, time of this step */
+  const double* hg;          /* (n+1, 4, 4) beam-node poses of this step */
+  const double* hg_dot;      /* (n+1, 4, 4) pose rates */
+  const int32_t* dof;        /* (n+1) beam node of each spanwise station */
+  const double* zeta0;       /* (m+1, n+1, 3) reference grid */
+  const double* zeta_b_nm1;  /* (m+1, n+1, 3) bound grid of the previous step */
+  double* zeta_b;            /* out: bound grid of this step */
+  double* zeta_b_dot;        /* out: its velocity */
+  const double* gamma_b_nm1; /* (m, n) */
+  const double* gamma_w_nm1; /* (m_star, n) */
+  const double* zeta_w_nm1;  /* (m_star+1, n+1, 3) */
+  double* gamma_w;           /* out (m_star, n) */
+  double* zeta_w;            /* out (m_star+1, n+1, 3) */
+  double* aic;               /* (m n, m n): assembled here (bound-only dynamic AIC) */
+  const double* aic_inv;     /* (m n, m n): kept inverse of that AIC (fj_lu_inverse) */
+  double* gamma_b;           /* out (m, n) */
+  double* gamma_b_dot;       /* (m, n): previous rate in, this step's rate out */
+  double* f_steady;          /* out (m+1, n+1, 3) steady vertex forces */
+  double* lift;              /* out: 1 double, sum of the z force over the vertices */
+  double* refine_ws;         /* fj_refine_workspace_bytes(m n) bytes */
+  int32_t* iters;            /* out: refinement corrections, negative if the stop test failed */
+} fj_step_case;
+
+/* Sizes for fj_step_batched: device workspace, device table and host
+ * staging (which must stay valid until the tick's work has run). */
+size_t fj_step_batched_workspace_bytes(int n_cases, const fj_step_case* cases);
+size_t fj_step_batched_table_bytes(int n_cases);
+
+/* One dynamic UVLM step (the per-step work of base_solve_from_grid,
+ * aero/uvlm.py:824-1117, uniform wake, no mirror) for n_cases independent
+ * cases in one set of launches: geometry and wake shift, the three filament
+ * tables, the rollup / upwash / force pair sums as fused multi-case launches,
+ * the AIC, right-hand side and kept-inverse refinement of every case, and
+ * the steady forces, circulation rate and lift.  free_wake: 1 free-wake
+ * rollup, 0 prescribed convection.  No host synchronisation; the stop-test
+ * outcome of each case is in its `iters`. */
+int fj_step_batched(int n_cases, const fj_step_case* cases, const fj_flowfield* flow,
+                    double rho, double relax, int kernel, double eps, double rcut,
+                    int free_wake, int max_iter, double tol, void* device_table,
+                    void* host_staging, void* workspace, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -2,7 +2,7 @@
 
 * each case served in a mix equals the same case served alone (1e-12);
 * a served case equals the plain UVLM drop-in run of that case (1e-10 on the
-  lift history, circulation and wake);
+  lift history, circulation and wake), free and prescribed wake;
 * the fused multi-case pair sum equals per-case pair sums.
 """
 
@@ -52,23 +52,24 @@ def test_batched_pair_sum_equals_single(dev):
         assert torch.allclose(got, ref, rtol=1e-13, atol=1e-15)
 
 
-def test_mix_equals_alone_and_equals_uvlm(dev):
+@pytest.mark.parametrize("free", [True, False])
+def test_mix_equals_alone_and_equals_uvlm(dev, free):
     from flapjax_amd import Constant
     from flapjax_amd.serving import ServingEngine, motion
     from flapjax_amd.uvlm import UVLM
 
     reqs = _stream()
-    mix, stats = ServingEngine().run(reqs)
+    mix, stats = ServingEngine(free_wake=free).run(reqs)
     assert stats["cases"] == len(reqs)
     by_id = {r.case_id: r for r in mix}
     for req in reqs:
-        alone, _ = ServingEngine().run([req])
+        alone, _ = ServingEngine(free_wake=free).run([req])
         a, b = alone[0], by_id[req.case_id]
         assert rel_err(b.lift, a.lift) < 1e-12
         assert rel_err(b.gamma_b_final, a.gamma_b_final) < 1e-12
         # the plain drop-in UVLM on the same case
         hg, hgd = motion(req)
-        uv = UVLM([(req.m, req.n, req.m_star)], np.arange(req.n + 1), free_wake=True)
+        uv = UVLM([(req.m, req.n, req.m_star)], np.arange(req.n + 1), free_wake=free)
         z0 = np.zeros((req.m + 1, req.n + 1, 3))
         z0[..., 0] = (np.linspace(0.0, 1.0, req.m + 1) - 0.25)[:, None]
         uv.set_design_variables(1.0 / req.m, Constant(np.array([1.0, 0, 0]), 1.225, True), z0, hg[0])
