@@ -259,6 +259,53 @@ def profile_rollup(reps=10, copy_mib=256):
                       "pairs": roof["fp64"]["pairs"], "copy_bytes": src.numel() * 8, "reps": reps}))
 
 
+def profile_kernels(reps=5, copy_mib=256):
+    """Counter-collection workload for the kernels the roofline figures are
+    quoted on, each a fixed number of times: the rollup pair-sum launch (FP64
+    and mixed), the FP64 LU factorisation at n = 1024 and 4096, the FP32 LU
+    and the explicit inverse at n = 1024, the prescribed-wake convection step
+    on a 4000 x 1000 wake, and plain device copies of ``copy_mib`` MiB (the
+    scale for the HBM byte counters).  Prints the event-timed mean of each."""
+    import torch
+
+    from flapjax_amd import _kernels as K
+
+    out = {}
+    roof, _ = rollup_roofline(reps)
+    out["rollup_ms"] = roof["fp64"]["ms"]
+    out["rollup_mixed_ms"] = roof["mixed"]["ms"]
+    rng = np.random.default_rng(4)
+
+    def timed(fn, n_rep):
+        fn()
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record()
+        for _ in range(n_rep):
+            fn()
+        e.record()
+        torch.cuda.synchronize()
+        return s.elapsed_time(e) / n_rep
+
+    for n, n_rep in ((1024, reps), (4096, 2)):
+        A = torch.as_tensor(rng.normal(size=(n, n)) + n ** 0.5 * np.eye(n), device="cuda")
+        ms = timed(lambda: K.lu_factor(A), n_rep)
+        out[f"lu_f64_{n}_ms"] = ms
+        out[f"lu_f64_{n}_tflops"] = 2.0 / 3.0 * n ** 3 / (ms * 1e-3) / 1e12
+        if n == 1024:
+            A32 = A.float()
+            out["lu_f32_1024_ms"] = timed(lambda: K.lu_factor_f32(A32), n_rep)
+            fac = K.lu_factor(A)
+            out["inverse_f64_1024_ms"] = timed(lambda: K.lu_inverse(fac), n_rep)
+    out["convection"] = convection_bench(reps=4 * reps)
+    src = torch.ones(copy_mib << 17, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(2 * reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    out["copy_bytes"] = src.numel() * 8
+    print(json.dumps(out))
+
+
 def serve(precision, n_cases, rate, m_star_cap, steps_range):
     """Served mixed-size stream (continuous batching), one GPU."""
     import torch
@@ -325,6 +372,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--profile-rollup", action="store_true",
                     help="only the rollup launch and a copy of known size (for counter runs)")
+    ap.add_argument("--profile-kernels", action="store_true",
+                    help="rollup, LU, inverse, convection and copy kernels a fixed number of times (for counter runs)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the UVLM path here runs on a single MI355X "
@@ -338,6 +387,9 @@ def main():
     if args.profile_rollup:
         profile_rollup()
         return
+    if args.profile_kernels:
+        profile_kernels()
+        return
     pairs, rollup_pairs = pairs_per_step()
     f64 = time_form("fp64", args.steps, args.warmup)
     mix = time_form("mixed", args.steps, args.warmup)
@@ -345,11 +397,12 @@ def main():
     roof, sample = rollup_roofline()
     srv = {}
     if args.serve_cases > 0:
-        for prec in ("fp64",):
+        for prec in ("fp64", "mixed"):
             st = serve(prec, args.serve_cases, args.serve_rate, 32, (50, 100))
             srv[prec] = {"cases_per_s": round(st["cases_per_s"], 3), "p50_s": round(st["p50_s"], 4),
                          "p99_s": round(st["p99_s"], 4), "gpairs_per_s": round(st["gpairs_per_s"], 3),
                          "makespan_s": round(st["makespan_s"], 3)}
+        srv["mixed_speedup_vs_fp64"] = round(srv["mixed"]["cases_per_s"] / srv["fp64"]["cases_per_s"], 3)
     sweep = pair_sweep()
     conv = convection_bench()
     cpu = None if args.no_cpu else cpu_baseline(sample)

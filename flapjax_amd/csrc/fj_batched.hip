@@ -37,7 +37,10 @@ __device__ __forceinline__ int fj_find_task(const FjTask* __restrict__ tasks, in
   return lo;
 }
 
-template <int KIND>
+// MIXED: the mixed form of the single-case kernels (FP32 far field, FP64
+// near field, FP32 partial sums folded into FP64 every FJ_MIXED_FLUSH
+// filaments of a wave)
+template <int KIND, bool MIXED>
 __global__ __launch_bounds__(FJ_BS_THREADS) void k_batched_partial(
     const FjTask* __restrict__ tasks, const int* __restrict__ counts, const double* __restrict__ seg,
     double* __restrict__ part, FjKernelParams kp) {
@@ -46,6 +49,8 @@ __global__ __launch_bounds__(FJ_BS_THREADS) void k_batched_partial(
   const int total = counts[1];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const float eps2f = (float)kp.eps2;
+  const float rcut2f = (float)(kp.rcut * kp.rcut);
   for (int w = blockIdx.x; w < total; w += gridDim.x) {
     const int ti = __builtin_amdgcn_readfirstlane(fj_find_task(tasks, n_tasks, w));
     const FjTask& T = tasks[ti];
@@ -60,14 +65,43 @@ __global__ __launch_bounds__(FJ_BS_THREADS) void k_batched_partial(
     const int s_begin = chunk * T.chunk_len;
     const int s_end = min(T.n_seg, s_begin + T.chunk_len);
     const double* sb = seg + (size_t)T.seg_off * FJ_SEG_STRIDE;
+    float bx = 0.0f, by = 0.0f, bz = 0.0f;
+    int since_flush = 0;
     for (int s = s_begin + wave; s < s_end; s += FJ_BS_WAVES) {
       const double* sp = sb + (size_t)s * FJ_SEG_STRIDE;
       double rec[12];
 #pragma unroll
       for (int k = 0; k < 12; ++k) rec[k] = sp[k];
-      double vx, vy, vz;
-      fj_bs_eval<KIND>(x, y, z, rec, kp, vx, vy, vz);
-      ax += vx; ay += vy; az += vz;
+      if (MIXED) {
+        const float* spf = (const float*)(sp + 12);
+        float sf[5];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) sf[k] = spf[k];
+        const float r0sq_f = (float)rec[11];
+        const float r1x = (float)(x - rec[0]), r1y = (float)(y - rec[1]), r1z = (float)(z - rec[2]);
+        const float s1 = r1x * r1x + r1y * r1y + r1z * r1z;
+        if (s1 < FJ_NEAR_FACTOR2 * r0sq_f) {
+          double vx, vy, vz;
+          fj_bs_eval<KIND>(x, y, z, rec, kp, vx, vy, vz);
+          ax += vx; ay += vy; az += vz;
+        } else {
+          float vx, vy, vz;
+          fj_bs_eval_f32<KIND>(r1x, r1y, r1z, s1, sf, r0sq_f, eps2f, rcut2f, vx, vy, vz);
+          bx += vx; by += vy; bz += vz;
+        }
+        if (++since_flush == FJ_MIXED_FLUSH) {
+          since_flush = 0;
+          ax += (double)bx; ay += (double)by; az += (double)bz;
+          bx = 0.0f; by = 0.0f; bz = 0.0f;
+        }
+      } else {
+        double vx, vy, vz;
+        fj_bs_eval<KIND>(x, y, z, rec, kp, vx, vy, vz);
+        ax += vx; ay += vy; az += vz;
+      }
+    }
+    if (MIXED) {
+      ax += (double)bx; ay += (double)by; az += (double)bz;
     }
     if (wave > 0) {
       red[wave - 1][0][lane] = ax;
@@ -175,12 +209,25 @@ size_t fj_batched_ws_bytes(const FjTaskDesc* d, int n) {
   return total * sizeof(double) + 256;
 }
 
+template <int KIND>
+static void fj_batched_partial_launch(int precision, int grid, const FjTask* dtasks,
+                                      const int* dcounts, const double* seg, double* part,
+                                      const FjKernelParams& kp, hipStream_t st) {
+  if (precision == FJ_PREC_MIXED)
+    hipLaunchKernelGGL((k_batched_partial<KIND, true>), dim3(grid), dim3(FJ_BS_THREADS), 0, st,
+                       dtasks, dcounts, seg, part, kp);
+  else
+    hipLaunchKernelGGL((k_batched_partial<KIND, false>), dim3(grid), dim3(FJ_BS_THREADS), 0, st,
+                       dtasks, dcounts, seg, part, kp);
+}
+
 int fj_batched_launch(const FjTaskDesc* d, int n_tasks, const double* seg, int kernel,
-                      const FjKernelParams& kp, void* task_table, void* host_staging,
-                      void* workspace, hipStream_t st) {
+                      int precision, const FjKernelParams& kp, void* task_table,
+                      void* host_staging, void* workspace, hipStream_t st) {
   if (n_tasks <= 0) return FJ_OK;
   if (!d || !task_table || !host_staging || !workspace) return FJ_ERR_ARG;
   if (kernel < FJ_KERNEL_EPSILON || kernel > FJ_KERNEL_PLAIN) return FJ_ERR_ARG;
+  if (precision != FJ_PREC_FP64 && precision != FJ_PREC_MIXED) return FJ_ERR_ARG;
   // host_staging: fj_batched_task_bytes(n_tasks) bytes of host memory that
   // stays valid until the copy has run (the caller keeps it)
   FjTask* h = (FjTask*)host_staging;
@@ -219,15 +266,13 @@ int fj_batched_launch(const FjTaskDesc* d, int n_tasks, const double* seg, int k
   const int* dcounts = (const int*)(dtasks + n_tasks);
   const int grid = work < 4096 ? work : 4096;
   if (work > 0) {
+    double* part = (double*)workspace;
     if (kernel == FJ_KERNEL_EPSILON)
-      hipLaunchKernelGGL(k_batched_partial<FJ_KERNEL_EPSILON>, dim3(grid), dim3(FJ_BS_THREADS), 0, st,
-                         dtasks, dcounts, seg, (double*)workspace, kp);
+      fj_batched_partial_launch<FJ_KERNEL_EPSILON>(precision, grid, dtasks, dcounts, seg, part, kp, st);
     else if (kernel == FJ_KERNEL_CUTOFF)
-      hipLaunchKernelGGL(k_batched_partial<FJ_KERNEL_CUTOFF>, dim3(grid), dim3(FJ_BS_THREADS), 0, st,
-                         dtasks, dcounts, seg, (double*)workspace, kp);
+      fj_batched_partial_launch<FJ_KERNEL_CUTOFF>(precision, grid, dtasks, dcounts, seg, part, kp, st);
     else
-      hipLaunchKernelGGL(k_batched_partial<FJ_KERNEL_PLAIN>, dim3(grid), dim3(FJ_BS_THREADS), 0, st,
-                         dtasks, dcounts, seg, (double*)workspace, kp);
+      fj_batched_partial_launch<FJ_KERNEL_PLAIN>(precision, grid, dtasks, dcounts, seg, part, kp, st);
     FJ_CHECK_LAUNCH();
   }
   if (n_tgt_total > 0) {
@@ -277,6 +322,6 @@ extern "C" int fj_pair_sum_batched(int n_tasks, const int* tgt_off, const int* n
   }
   FjKernelParams kp;
   kp.eps = eps; kp.eps2 = eps * eps; kp.rcut = rcut;
-  return fj_batched_launch(d, n_tasks, seg, kernel, kp, task_table, host_staging, workspace,
-                           (hipStream_t)stream);
+  return fj_batched_launch(d, n_tasks, seg, kernel, FJ_PREC_FP64, kp, task_table, host_staging,
+                           workspace, (hipStream_t)stream);
 }

@@ -10,37 +10,6 @@
 // FP64 accumulators.  Launch shape and reduction are those of the FP64 path.
 #include "fj_common.hpp"
 
-#define FJ_NEAR_FACTOR2 25.0f  // (5 filament lengths)^2 from the start point
-#define FJ_MIXED_FLUSH 16
-
-// FP32 far-field evaluation from r1 (rounded from the FP64 difference) and
-// the filament's FP32 r0 / aux / g: r2 = r1 - r0 (relative error ~eps32 since
-// far-field pairs have |r2| >= 4 |r0|), c = r0 x r1, r0.d = a1 w1 - a2 w2.
-template <int KIND>
-__device__ __forceinline__ void fj_bs_eval_f32(float r1x, float r1y, float r1z, float s1,
-                                               const float* sf, float r0sq_f, float eps2f,
-                                               float rcut2f, float& vx, float& vy, float& vz) {
-  const float r0x = sf[0], r0y = sf[1], r0z = sf[2];
-  const float r2x = r1x - r0x, r2y = r1y - r0y, r2z = r1z - r0z;
-  const float cx = r0y * r1z - r0z * r1y;
-  const float cy = r0z * r1x - r0x * r1z;
-  const float cz = r0x * r1y - r0y * r1x;
-  const float e2 = (KIND == FJ_KERNEL_PLAIN) ? 0.0f : eps2f;
-  const float s2 = r2x * r2x + r2y * r2y + r2z * r2z + e2;
-  const float a1 = r0x * r1x + r0y * r1y + r0z * r1z;
-  const float a2 = a1 - r0sq_f;
-  float D = cx * cx + cy * cy + cz * cz;
-  if (KIND == FJ_KERNEL_EPSILON) D += sf[3];
-  const float w1 = __builtin_amdgcn_rsqf(s1 + e2);
-  const float w2 = __builtin_amdgcn_rsqf(s2);
-  float q = sf[4] * (a1 * w1 - a2 * w2) * __builtin_amdgcn_rcpf(D);
-  if (KIND == FJ_KERNEL_CUTOFF) {
-    const float rad2 = s1 - a1 * (a1 * sf[3]);
-    q = (rad2 > rcut2f) ? q : 0.0f;
-  }
-  vx = cx * q; vy = cy * q; vz = cz * q;
-}
-
 template <int KIND, int TPL>
 __global__ __launch_bounds__(FJ_BS_THREADS) void k_pair_partial_mixed(
     const double* __restrict__ seg, int n_seg, int chunk_len, const double* __restrict__ tgt,

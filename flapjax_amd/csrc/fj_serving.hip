@@ -459,11 +459,13 @@ extern "C" size_t fj_step_batched_table_bytes(int n_cases) {
 
 extern "C" int fj_step_batched(int n_cases, const fj_step_case* cases, const fj_flowfield* flow,
                                double rho, double relax, int kernel, double eps, double rcut,
-                               int free_wake, int max_iter, double tol, void* device_table,
-                               void* host_staging, void* workspace, void* stream) {
+                               int precision, int free_wake, int max_iter, double tol,
+                               void* device_table, void* host_staging, void* workspace,
+                               void* stream) {
   if (n_cases <= 0) return FJ_OK;
   if (!cases || !flow || !device_table || !host_staging || !workspace) return FJ_ERR_ARG;
   if (kernel < FJ_KERNEL_EPSILON || kernel > FJ_KERNEL_PLAIN) return FJ_ERR_ARG;
+  if (precision != FJ_PREC_FP64 && precision != FJ_PREC_MIXED) return FJ_ERR_ARG;
   if (max_iter < 1 || max_iter > FJ_REFINE_MAX_ITER) return FJ_ERR_ARG;
   for (int i = 0; i < n_cases; ++i) {
     const fj_step_case& K = cases[i];
@@ -522,8 +524,8 @@ extern "C" int fj_step_batched(int n_cases, const fj_step_case* cases, const fj_
     hipLaunchKernelGGL(k_bt_segments, grid_of(L.roll_tot), dim3(FJ_SV_EW), 0, st, dcs, n_cases,
                        L.roll_tot, 0, kernel, eps, seg_roll);
     FJ_CHECK_LAUNCH();
-    rc = fj_batched_launch(roll.data(), n_cases, seg_roll, kernel, kp, dt_base + rec_bytes,
-                           hs_base + rec_bytes, ws + L.off_ws_roll, st);
+    rc = fj_batched_launch(roll.data(), n_cases, seg_roll, kernel, precision, kp,
+                           dt_base + rec_bytes, hs_base + rec_bytes, ws + L.off_ws_roll, st);
     if (rc != FJ_OK) return rc;
   }
   // 3. wake tables at n and the fused upwash at the collocation points
@@ -532,8 +534,10 @@ extern "C" int fj_step_batched(int n_cases, const fj_step_case* cases, const fj_
                        L.wk_tot, 1, kernel, eps, seg_frc);
     FJ_CHECK_LAUNCH();
   }
-  rc = fj_batched_launch(up.data(), n_cases, seg_frc, kernel, kp, dt_base + rec_bytes + task_bytes,
-                         hs_base + rec_bytes + task_bytes, ws + L.off_ws_up, st);
+  // the upwash stays FP64 in both forms (it enters the right-hand side)
+  rc = fj_batched_launch(up.data(), n_cases, seg_frc, kernel, FJ_PREC_FP64, kp,
+                         dt_base + rec_bytes + task_bytes, hs_base + rec_bytes + task_bytes,
+                         ws + L.off_ws_up, st);
   if (rc != FJ_OK) return rc;
   // 4. AIC, right-hand sides, refinement
   if (kernel == FJ_KERNEL_EPSILON)
@@ -567,7 +571,7 @@ extern "C" int fj_step_batched(int n_cases, const fj_step_case* cases, const fj_
   hipLaunchKernelGGL(k_bt_segments, grid_of(L.bnd_tot), dim3(FJ_SV_EW), 0, st, dcs, n_cases,
                      L.bnd_tot, 2, kernel, eps, seg_frc);
   FJ_CHECK_LAUNCH();
-  rc = fj_batched_launch(frc.data(), n_cases, seg_frc, kernel, kp,
+  rc = fj_batched_launch(frc.data(), n_cases, seg_frc, kernel, precision, kp,
                          dt_base + rec_bytes + 2 * task_bytes, hs_base + rec_bytes + 2 * task_bytes,
                          ws + L.off_ws_frc, st);
   if (rc != FJ_OK) return rc;

@@ -12,8 +12,8 @@ step per engine tick:
   * across cases: the three Biot-Savart pair sums of the step (free-wake
     rollup with the convection epilogue, wake upwash at the collocation
     points, velocities at the filament midpoints) are each ONE fused launch
-    over all active cases (fj_pair_sum_batched, a persistent grid striding
-    over a device task table).
+    over all active cases (a persistent grid striding over a device task
+    table).
 
 Cases that arrive while others run join at the next tick; finished cases
 leave and free their slot and memory.  Latency = completion - arrival.
@@ -25,8 +25,10 @@ FP64 against the AIC re-assembled every step, so each step solves the current
 system to FP64 accuracy; the stop-test outcome of every case's solve is read
 back at the end of the tick.  A tick is one native call (fj_step_batched):
 each stage -- geometry, tables, the three fused pair sums, AIC, right-hand
-side, refinement, forces, lift -- is one launch over all active cases.  The
-fused launches evaluate in FP64 for either precision setting.
+side, refinement, forces, lift -- is one launch over all active cases.  With
+precision="mixed" the rollup and force pair sums take the mixed form (FP32
+far field, FP64 near field) as uvlm.UVLM does; the upwash, AIC and solve
+stay FP64.
 """
 
 from __future__ import annotations
@@ -116,7 +118,32 @@ def footprint_bytes(req: CaseRequest, precision: str = "fp64") -> int:
     words += req.steps                               # lift history
     words += 16 * (s_b + s_w) + 10 * nb + 6 * s_b     # tick workspace share: second table, stencils, midpoints
     words += 3 * (m + 1) * (n + 1)                   # steady forces
+    # pair-sum partials (filament chunks of 256, target tiles of 64)
+    for n_tgt, n_seg in (((ms + 1) * (n + 1), s_b + s_w), (nb, s_w), (s_b, s_b + s_w)):
+        words += 3 * 64 * (-(-n_tgt // 64)) * max(1, -(-n_seg // 256))
     return 8 * words + 4096
+
+
+def _static_solve(A, Ainv, U, te, b):
+    """x = A_s^{-1} b for A_s = A + U E^T (E selects the columns `te`) from
+    the kept inverse of A, by the Sherman-Morrison-Woodbury identity
+    x = y - Z (I + E^T Z)^{-1} E^T y with y = A^{-1} b, Z = A^{-1} U, followed
+    by one correction sweep with the FP64 residual b - A_s x.  The capacitance
+    matrix I + E^T Z = E^T A^{-1} A_s E is as well conditioned as A_s relative
+    to A (its size is the number of trailing-edge rings), and solving it with
+    the small LU replaces a second full factorisation."""
+    Z = Ainv @ U
+    C = Z[te].clone()
+    C.diagonal().add_(1.0)
+    fac = K.lu_factor(C, overwrite=True)
+
+    def apply(r):
+        y = Ainv @ r
+        return y - Z @ K.lu_solve(fac, y[te])
+
+    x = apply(b)
+    res = b - A @ x - U @ x[te]
+    return x + apply(res)
 
 
 class _Slot:
@@ -154,17 +181,21 @@ class _Slot:
         self.gw_next = torch.empty_like(self.gw)
         self.gbd = torch.zeros((m, n), dtype=F64, device=dev)
         self.A = torch.empty((m * n, m * n), dtype=F64, device=dev)
-        # static initial solve (aero/uvlm.py:895-1012): AIC with lumped wake
         c, nc, _ = K.colloc_normals(self.zb)
-        K.aic_assemble(c, nc, self.zb, kernel_kind, EPSILON, R_CUTOFF, self.A, 0, zeta_w=self.zw)
         rhs = K.rhs(c, nc, None, None, flow, 0.0)
-        self.gb = K.lu_solve(K.lu_factor(self.A), rhs).reshape(m, n).contiguous()
+        # the static AIC (aero/uvlm.py:895-1012) is the dynamic, bound-only
+        # AIC plus the lumped wake in the n trailing-edge ring columns
+        # (add_wake_influence, aero/aic.py:190-205): A_s = A_d + U E^T
+        K.aic_assemble(c, nc, self.zb, kernel_kind, EPSILON, R_CUTOFF, self.A, 0, zeta_w=self.zw)
+        te = slice((m - 1) * n, m * n)
+        U = self.A[:, te].clone()
+        K.aic_assemble(c, nc, self.zb, kernel_kind, EPSILON, R_CUTOFF, self.A, 0)
+        U -= self.A[:, te]
+        # the inverse kept for the dynamic (bound-only) system
+        self.Ainv = K.lu_inverse(K.lu_factor(self.A))
+        self.gb = _static_solve(self.A, self.Ainv, U, te, rhs).reshape(m, n).contiguous()
         K.static_wake_gamma(self.gb, ms, out=self.gw)
         self.gb_prev = self.gb.clone()
-        # the factor kept for the dynamic (bound-only) system
-        c, nc, _ = K.colloc_normals(self.zb)
-        K.aic_assemble(c, nc, self.zb, kernel_kind, EPSILON, R_CUTOFF, self.A, 0)
-        self.Ainv = K.lu_inverse(K.lu_factor(self.A))
         self.ws = torch.empty(L.lib().fj_refine_workspace_bytes(m * n) // 8 + 1, dtype=F64, device=dev)
         self.iters = torch.zeros(1, dtype=torch.int32, device=dev)
         self.fs = torch.empty((m + 1, n + 1, 3), dtype=F64, device=dev)
@@ -188,6 +219,7 @@ class ServingEngine:
         self.kind = L.KERNEL_EPSILON
         self._staging = {}
         self._bufs = {}
+        self._adm = None
 
     # ---- fused pair sums ---------------------------------------------------
     def _batched(self, tasks, seg, targets, out, prec):
@@ -254,8 +286,9 @@ class ServingEngine:
         table = self._grow("table", tb)
         staging = self._grow("staging", tb, pinned=True)
         fd = self.flow.descriptor()
+        prec = L.PREC_MIXED if self.precision == "mixed" else L.PREC_FP64
         rc = lib.fj_step_batched(nc, cases, ctypes.byref(fd), self.flow.rho, 0.7, self.kind, EPSILON, R_CUTOFF,
-                                 int(self.free_wake), 3, 0.0, L.ptr(table), ctypes.c_void_p(staging.data_ptr()),
+                                 prec, int(self.free_wake), 3, 0.0, L.ptr(table), ctypes.c_void_p(staging.data_ptr()),
                                  L.ptr(ws), L.stream())
         L.check(rc, "fj_step_batched")
         for sl in slots:
@@ -287,34 +320,57 @@ class ServingEngine:
         (results, stats)."""
         pending = sorted(requests, key=lambda r: r.arrival)
         active: list[_Slot] = []
+        warming: list[_Slot] = []
         used = 0
         results = []
         t0 = time_source()
         ticks = 0
         pairs = 0
         peak = 0
-        while pending or active:
+        main = torch.cuda.current_stream(self.dev)
+        if self._adm is None:
+            self._adm = torch.cuda.Stream(device=self.dev)
+        adm = self._adm
+        while pending or active or warming:
             now = time_source() - t0
-            # admission: arrived, slot free, footprint fits
-            while pending and pending[0].arrival <= now and len(active) < self.max_slots:
+            # admission: arrived, slot free, footprint fits.  A new case's
+            # set-up (grids, AIC, factorisation and inverse, static solve) is
+            # enqueued on the admission stream, so it runs on the device
+            # beside the ticks of the cases already active; the case joins
+            # the ticks once its set-up has completed.  The admission stream
+            # first waits for the work already on the main stream (memory
+            # that finished cases released may still be read there); a
+            # case's buffers are only released after the tick that last used
+            # them has been synchronised.
+            while pending and pending[0].arrival <= now and len(active) + len(warming) < self.max_slots:
                 fp = footprint_bytes(pending[0], self.precision)
-                if used + fp > self.budget and active:
+                if used + fp > self.budget and (active or warming):
                     break
                 req = pending.pop(0)
-                sl = _Slot(req, self.dev, self.precision, self.flow, self.kind)
+                adm.wait_stream(main)
+                with torch.cuda.stream(adm):
+                    sl = _Slot(req, self.dev, self.precision, self.flow, self.kind)
+                    sl.ready = torch.cuda.Event()
+                    sl.ready.record(adm)
                 sl.start = time_source() - t0
                 sl.fp = fp
                 used += fp
                 peak = max(peak, used)
+                warming.append(sl)
+            if warming and not active:
+                warming[0].ready.synchronize()
+            for sl in [w for w in warming if w.ready.query()]:
+                main.wait_event(sl.ready)
+                warming.remove(sl)
                 active.append(sl)
             if not active:
-                wait = pending[0].arrival - (time_source() - t0)
-                if wait > 0:
-                    time.sleep(min(wait, 0.01))
+                if not warming:
+                    wait = pending[0].arrival - (time_source() - t0)
+                    if wait > 0:
+                        time.sleep(min(wait, 0.01))
                 continue
             self._tick(active)
             ticks += 1
-            torch.cuda.synchronize()
             now = time_source() - t0
             still = []
             for sl in active:

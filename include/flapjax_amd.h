@@ -390,10 +390,12 @@ size_t fj_step_batched_table_bytes(int n_cases);
  * tables, the rollup / upwash / force pair sums as fused multi-case launches,
  * the AIC, right-hand side and kept-inverse refinement of every case, and
  * the steady forces, circulation rate and lift.  free_wake: 1 free-wake
- * rollup, 0 prescribed convection.  No host synchronisation; the stop-test
+ * rollup, 0 prescribed convection.  precision: FJ_PREC_MIXED evaluates the
+ * rollup and force pair sums in the mixed form (the upwash, AIC and solve
+ * stay FP64), as fj_pair_sum does.  No host synchronisation; the stop-test
  * outcome of each case is in its `iters`. */
 int fj_step_batched(int n_cases, const fj_step_case* cases, const fj_flowfield* flow,
-                    double rho, double relax, int kernel, double eps, double rcut,
+                    double rho, double relax, int kernel, double eps, double rcut, int precision,
                     int free_wake, int max_iter, double tol, void* device_table,
                     void* host_staging, void* workspace, void* stream);
 

@@ -3,6 +3,8 @@
 * each case served in a mix equals the same case served alone (1e-12);
 * a served case equals the plain UVLM drop-in run of that case (1e-10 on the
   lift history, circulation and wake), free and prescribed wake;
+* the mixed form served in a mix equals the same case served alone and stays
+  within 1e-6 (circulation, wake) / 1e-5 (lift) of the FP64 drop-in;
 * the fused multi-case pair sum equals per-case pair sums.
 """
 
@@ -79,6 +81,38 @@ def test_mix_equals_alone_and_equals_uvlm(dev, free):
         assert rel_err(b.gamma_b_final, d.gamma_b[0][-1].cpu().numpy()) < 1e-10
         assert rel_err(b.zeta_w_final, d.zeta_w[0][-1].cpu().numpy()) < 1e-10
     print("serving stats:", stats)
+
+
+def test_mixed_form_served(dev):
+    """Mixed form (FP32 far field in the rollup and force pair sums): a case
+    gives the same result alone and in a mix (the per-case arithmetic does not
+    depend on the other cases), and stays within the mixed form's error of
+    the plain FP64 drop-in run."""
+    from flapjax_amd import Constant
+    from flapjax_amd.serving import ServingEngine, motion
+    from flapjax_amd.uvlm import UVLM
+
+    reqs = _stream()
+    mix, stats = ServingEngine(precision="mixed").run(reqs)
+    assert stats["cases"] == len(reqs)
+    by_id = {r.case_id: r for r in mix}
+    for req in reqs:
+        alone, _ = ServingEngine(precision="mixed").run([req])
+        a, b = alone[0], by_id[req.case_id]
+        assert rel_err(b.lift, a.lift) < 1e-12
+        assert rel_err(b.gamma_b_final, a.gamma_b_final) < 1e-12
+        hg, hgd = motion(req)
+        uv = UVLM([(req.m, req.n, req.m_star)], np.arange(req.n + 1), free_wake=True)
+        z0 = np.zeros((req.m + 1, req.n + 1, 3))
+        z0[..., 0] = (np.linspace(0.0, 1.0, req.m + 1) - 0.25)[:, None]
+        uv.set_design_variables(1.0 / req.m, Constant(np.array([1.0, 0, 0]), 1.225, True), z0, hg[0])
+        d = uv.solve_prescribed_dynamic(uv.solve_static(hg=hg[0]), hg, hgd)
+        lift = d.f_steady[0][..., 2].sum(dim=(1, 2)).cpu().numpy()
+        e_lift = rel_err(b.lift[1:], lift[1:])
+        e_gam = rel_err(b.gamma_b_final, d.gamma_b[0][-1].cpu().numpy())
+        e_wake = rel_err(b.zeta_w_final, d.zeta_w[0][-1].cpu().numpy())
+        print(f"case {req.case_id}: mixed vs FP64 lift {e_lift:.2e} gamma {e_gam:.2e} wake {e_wake:.2e}")
+        assert e_lift < 1e-5 and e_gam < 1e-6 and e_wake < 1e-6
 
 
 def test_admission_by_footprint(dev):
