@@ -160,14 +160,18 @@ def rollup_roofline(reps=20):
                      K.segments_from_grid(zw, gw, L.KERNEL_EPSILON, 1e-7)])
     tgt = zw[:-1].reshape(-1, 3).contiguous()
     out = torch.empty_like(tgt)
+    # the table holds the bound and the wake lattice, as in the step
+    lats = [(0, M, N), (M * (N + 1) + (M + 1) * N, M_STAR, N)]
     res = {}
     for name, prec in (("fp64", L.PREC_FP64), ("mixed", L.PREC_MIXED)):
+        ws = torch.empty(K.pair_workspace_bytes(tgt.shape[0], seg.shape[0], prec, lats) // 8 + 1,
+                         dtype=torch.float64, device=tgt.device)
         for _ in range(3):
-            K.pair_sum(seg, tgt, L.KERNEL_EPSILON, 1e-7, 1e-4, prec, out=out)
+            K.pair_sum(seg, tgt, L.KERNEL_EPSILON, 1e-7, 1e-4, prec, out=out, workspace=ws, lattices=lats)
         s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         s.record()
         for _ in range(reps):
-            K.pair_sum(seg, tgt, L.KERNEL_EPSILON, 1e-7, 1e-4, prec, out=out)
+            K.pair_sum(seg, tgt, L.KERNEL_EPSILON, 1e-7, 1e-4, prec, out=out, workspace=ws, lattices=lats)
         e.record()
         torch.cuda.synchronize()
         ms = s.elapsed_time(e) / reps
@@ -313,9 +317,12 @@ def serve(precision, n_cases, rate, m_star_cap, steps_range):
     from flapjax_amd.serving import ServingEngine, poisson_stream
 
     reqs = poisson_stream(n_cases, rate, seed=2, m_star=m_star_cap, steps_range=steps_range)
-    eng = ServingEngine(precision=precision, max_slots=64)
-    eng.run(poisson_stream(4, 1e6, seed=5, m_star=8, steps_range=(5, 8)))  # warm-up
+    # warm-up on an engine of its own, so that the measured engine starts
+    # with an empty inverse cache
+    ServingEngine(precision=precision, max_slots=64).run(
+        poisson_stream(4, 1e6, seed=5, m_star=8, steps_range=(5, 8)))
     torch.cuda.synchronize()
+    eng = ServingEngine(precision=precision, max_slots=64)
     _, stats = eng.run(reqs)
     return stats
 
@@ -368,8 +375,11 @@ def main():
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--serve-cases", type=int, default=48)
+    ap.add_argument("--serve-cases", type=int, default=256)
     ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--serve-steps", type=int, nargs=2, default=(50, 400))
+    ap.add_argument("--serve-wake-cap", type=int, default=32,
+                    help="wake rows per served case = min(steps, cap); 0: wake rows = steps")
     ap.add_argument("--profile-rollup", action="store_true",
                     help="only the rollup launch and a copy of known size (for counter runs)")
     ap.add_argument("--profile-kernels", action="store_true",
@@ -398,7 +408,8 @@ def main():
     srv = {}
     if args.serve_cases > 0:
         for prec in ("fp64", "mixed"):
-            st = serve(prec, args.serve_cases, args.serve_rate, 32, (50, 100))
+            st = serve(prec, args.serve_cases, args.serve_rate, args.serve_wake_cap or None,
+                       tuple(args.serve_steps))
             srv[prec] = {"cases_per_s": round(st["cases_per_s"], 3), "p50_s": round(st["p50_s"], 4),
                          "p99_s": round(st["p99_s"], 4), "gpairs_per_s": round(st["gpairs_per_s"], 3),
                          "makespan_s": round(st["makespan_s"], 3)}
@@ -434,12 +445,14 @@ def main():
                   "cases_per_s": round(1.0 / mix["case_s"], 3), "speedup_vs_fp64": round(value_mixed / value, 3)},
         "roofline": {"bound": "valu", "achieved": round(roof["fp64"]["tflops"], 3), "peak": PEAK_FP64,
                      "unit": "TFLOP/s", "frac": round(roof["fp64"]["tflops"] / PEAK_FP64, 4), "traffic": ROLLUP_TRAFFIC_BYTES,
-                     "kernel": "k_pair_partial_f64 + k_pair_reduce (rollup pair sum)",
+                     "kernel": "k_lattice_records + k_lattice_partial + k_pair_reduce (rollup pair sum)",
                      "launch_ms": round(roof["fp64"]["ms"], 4), "pairs_per_launch": roof["fp64"]["pairs"],
                      "mixed_launch_ms": round(roof["mixed"]["ms"], 4)},
-        "serving": {"stream": f"{args.serve_cases} Poisson arrivals at {args.serve_rate}/s, M in {{4,8,16,32}}, "
-                              "N in {16,32,64,128}, 50-100 steps, wake rows = min(steps, 32), pitch 0-5 deg, "
-                              "k 0.05-0.5", **srv},
+        "serving": {"stream": f"{args.serve_cases} Poisson arrivals at {args.serve_rate}/s (seed 2), "
+                              "M in {4,8,16,32}, N in {16,32,64,128}, "
+                              f"{args.serve_steps[0]}-{args.serve_steps[1]} steps, wake rows = "
+                              + (f"min(steps, {args.serve_wake_cap})" if args.serve_wake_cap else "steps")
+                              + ", pitch 0-5 deg, k 0.05-0.5", **srv},
         "pair_sweep_gpairs_s": sweep,
         "convection": conv,
         "cpu_baseline": cpu,

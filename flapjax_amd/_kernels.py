@@ -145,11 +145,38 @@ def segments_from_endpoints(ends, gamma, kernel_kind, eps):
     return out
 
 
+def _lattice_path(seg, precision, lattices):
+    """The lattices of ``seg`` when the FP64 lattice kernel applies to it:
+    the explicit list, else the list a table built from whole grids carries
+    (``fj_lattices``, set by aic.segment_tables)."""
+    if seg is None or int(precision) != L.PREC_FP64:
+        return None
+    lats = lattices if lattices is not None else getattr(seg, "fj_lattices", None)
+    if not lats or len(lats) > MAX_LATTICES:
+        return None
+    return lats
+
+
+def pair_workspace_bytes(n_tgt, n_seg, precision=L.PREC_FP64, lattices=None):
+    """Workspace bytes of the pair sum that ``pair_sum`` runs for this size,
+    precision and lattice list."""
+    if int(precision) == L.PREC_FP64 and lattices and len(lattices) <= MAX_LATTICES and n_tgt > 0:
+        return L.lib().fj_pair_sum_lattices_workspace_bytes(n_tgt, len(lattices), lattice_refs(lattices))
+    return L.lib().fj_pair_sum_workspace_bytes(n_tgt, n_seg, int(precision))
+
+
 def pair_sum(seg, targets, kernel_kind, eps, rcut, precision=L.PREC_FP64, mode=L.OUT_STORE, out=None, flow=None,
-             time=0.0, dt=0.0, workspace=None):
+             time=0.0, dt=0.0, workspace=None, lattices=None):
     """Induced velocity of the filament table `seg` at `targets` (T, 3).
-    ``workspace`` (float64 device tensor of at least
-    fj_pair_sum_workspace_bytes) avoids an allocation per call."""
+    ``workspace`` (float64 device tensor of at least pair_workspace_bytes)
+    avoids an allocation per call.  When the table holds whole lattices
+    (``lattices``: (seg_off, m, n) per grid, or the ``fj_lattices`` a table
+    from aic.segment_tables carries) the FP64 form runs the lattice kernel
+    (fj_pair_sum_lattices), which shares the per-vertex terms between
+    filaments."""
+    lats = _lattice_path(seg, precision, lattices)
+    if lats is not None:
+        return pair_sum_lattices(seg, lats, targets, kernel_kind, eps, rcut, mode, out, flow, time, dt, workspace)
     targets = _c64(targets, (3,), "targets")
     flat = targets.reshape(-1, 3)
     nt = flat.shape[0]
@@ -171,6 +198,60 @@ def pair_sum(seg, targets, kernel_kind, eps, rcut, precision=L.PREC_FP64, mode=L
                              ctypes.byref(fd) if fd is not None else None, float(time), float(dt), L.ptr(ws),
                              L.stream())
     L.check(rc, "fj_pair_sum")
+    return out
+
+
+MAX_LATTICES = 16
+
+
+def lattices_of(grids, base=0):
+    """(seg_off, m, n) of each grid of a table built by segments_from_grids
+    from ``grids`` [(zeta, gamma, mirrored), ...], starting at row ``base``."""
+    out, off = [], base
+    for zeta, _, _ in grids:
+        m, n = zeta.shape[0] - 1, zeta.shape[1] - 1
+        out.append((off, m, n))
+        off += n_segments(m, n)
+    return out
+
+
+def lattice_refs(lattices):
+    """ctypes array of fj_lattice_ref from (seg_off, m, n) tuples."""
+    arr = (L.FjLatticeRef * max(1, len(lattices)))()
+    for k, (off, m, n) in enumerate(lattices):
+        arr[k].seg_off, arr[k].m, arr[k].n = int(off), int(m), int(n)
+    return arr
+
+
+def pair_sum_lattices(seg, lattices, targets, kernel_kind, eps, rcut, mode=L.OUT_STORE, out=None, flow=None,
+                      time=0.0, dt=0.0, workspace=None):
+    """FP64 induced velocity at `targets` of the lattices (seg_off, m, n) of
+    the filament table `seg` (at most MAX_LATTICES), with the per-vertex part
+    of the kernel shared between filaments."""
+    if len(lattices) > MAX_LATTICES:
+        raise ValueError(f"at most {MAX_LATTICES} lattices per call")
+    targets = _c64(targets, (3,), "targets")
+    flat = targets.reshape(-1, 3)
+    nt = flat.shape[0]
+    if out is None:
+        if mode == L.OUT_ADD:
+            raise ValueError("OUT_ADD needs an output tensor")
+        out = torch.empty_like(flat)
+    if nt == 0:
+        return out
+    refs = lattice_refs(lattices)
+    nl = len(lattices)
+    ws_bytes = L.lib().fj_pair_sum_lattices_workspace_bytes(nt, nl, refs)
+    if workspace is not None and workspace.numel() * 8 >= ws_bytes:
+        ws = workspace
+    else:
+        ws = torch.empty(ws_bytes // 8 + 1, dtype=F64, device=flat.device)
+    fd = flow_desc(flow) if flow is not None else None
+    rc = L.lib().fj_pair_sum_lattices(L.ptr(seg) if seg is not None else None, nl, refs, L.ptr(flat), nt,
+                                      L.ptr(out), int(kernel_kind), float(eps), float(rcut), int(mode),
+                                      ctypes.byref(fd) if fd is not None else None, float(time), float(dt),
+                                      L.ptr(ws), L.stream())
+    L.check(rc, "fj_pair_sum_lattices")
     return out
 
 

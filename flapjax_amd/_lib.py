@@ -61,6 +61,9 @@ _SIGS = {
     "fj_segments_from_grid": ([c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_int, c_double, c_void_p, c_void_p], c_int),
     "fj_pair_sum": ([c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, c_double, c_double, c_int, c_int,
                      POINTER(FjFlowfield), c_double, c_double, c_void_p, c_void_p], c_int),
+    "fj_pair_sum_lattices_workspace_bytes": ([c_int, c_int, c_void_p], c_size_t),
+    "fj_pair_sum_lattices": ([c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p, c_int, c_double, c_double,
+                              c_int, POINTER(FjFlowfield), c_double, c_double, c_void_p, c_void_p], c_int),
     "fj_aic_assemble": ([c_void_p, c_void_p, c_int, c_void_p, c_int, c_int, c_void_p, c_int, c_void_p, c_void_p,
                          c_int, c_double, c_double, c_void_p, c_int, c_int, c_int, c_void_p], c_int),
     "fj_lu_aux_bytes": ([c_int], c_size_t),
@@ -125,6 +128,13 @@ class FjStepCase(ctypes.Structure):
             "hg", "hg_dot", "dof", "zeta0", "zeta_b_nm1", "zeta_b", "zeta_b_dot", "gamma_b_nm1", "gamma_w_nm1",
             "zeta_w_nm1", "gamma_w", "zeta_w", "aic", "aic_inv", "gamma_b", "gamma_b_dot", "f_steady", "lift",
             "refine_ws", "iters")]
+
+
+class FjLatticeRef(ctypes.Structure):
+    """fj_lattice_ref: one lattice (its filaments' first row and size) in a
+    filament table."""
+
+    _fields_ = [("seg_off", c_int), ("m", c_int), ("n", c_int)]
 
 
 class FjGridRef(ctypes.Structure):

@@ -36,6 +36,7 @@ def segment_tables(zetas, gammas, kernels, mirror=None) -> dict:
     if isinstance(kernels, SegmentKernel) or kernels is None:
         kernels = [kernels] * len(zetas)
     groups: dict = {}
+    lats: dict = {}
     for zeta, gamma, kern in zip(zetas, gammas, kernels):
         kern = as_segment_kernel(kern)
         zeta = to_device(zeta)
@@ -46,8 +47,19 @@ def segment_tables(zetas, gammas, kernels, mirror=None) -> dict:
         if mirror is not None:
             tabs.append(K.segments_from_grid(zeta, gamma, kern.kind, kern.eps, mirror=mirror))
         key = (kern.kind, kern.eps, kern.rcut)
-        groups.setdefault(key, []).extend(tabs)
-    return {k: torch.cat(v, dim=0) for k, v in groups.items()}
+        m, n = zeta.shape[0] - 1, zeta.shape[1] - 1
+        for t in tabs:
+            off = sum(x.shape[0] for x in groups.get(key, []))
+            lats.setdefault(key, []).append((off, m, n))
+            groups.setdefault(key, []).append(t)
+    out = {}
+    for k, v in groups.items():
+        t = torch.cat(v, dim=0)
+        # the table holds whole lattices: the FP64 pair sums run the lattice
+        # kernel on it (_kernels.pair_sum)
+        t.fj_lattices = lats[k]
+        out[k] = t
+    return out
 
 
 def _pair_sum_tables(tables, targets, precision=L.PREC_FP64):

@@ -193,9 +193,15 @@ extern "C" size_t fj_batched_task_bytes(int n_tasks) {
   return (size_t)n_tasks * sizeof(FjTask) + 64 + fj_batched_desc_bytes(n_tasks);
 }
 
+// a task's tiling depends on its own sizes only, so a case's arithmetic (and
+// result) is the same whatever else shares the launch: filament chunks of
+// 256, but at most about FJ_BS_TARGET_BLOCKS work items per task (large
+// tasks take longer chunks, which bounds their partial-sum buffer)
 static void fj_task_tiling(int n_tgt, int n_seg, int& tiles, int& chunks, int& chunk_len) {
   tiles = n_tgt > 0 ? fj_cdiv(n_tgt, 64) : 0;  // no work for an empty target set
   chunks = n_seg > 0 ? fj_cdiv(n_seg, 256) : 1;
+  const int cap = tiles > 0 ? fj_cdiv(FJ_BS_TARGET_BLOCKS, tiles) : 1;
+  if (chunks > cap) chunks = cap;
   chunk_len = n_seg > 0 ? fj_cdiv(n_seg, chunks) : 0;
 }
 

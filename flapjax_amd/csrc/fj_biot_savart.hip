@@ -310,6 +310,31 @@ static void launch_partial_f64(const FjPairPlan& p, const double* seg, int n_seg
                      kp);
 }
 
+// Epilogue of a pair sum: the chunk partials (or nothing, for an empty
+// filament set: part == NULL) to out by `mode`.
+int fj_pair_epilogue(const double* part, int chunks, int ld, int n_tgt, const double* targets,
+                     double* out, int mode, const FjFlow& fl, double time, double dt,
+                     hipStream_t st) {
+  if (!part) {
+    const dim3 g(fj_cdiv(n_tgt, 256));
+    if (mode == FJ_OUT_STORE)
+      hipLaunchKernelGGL(k_pair_empty<FJ_OUT_STORE>, g, dim3(256), 0, st, n_tgt, targets, out, fl, time, dt);
+    else if (mode == FJ_OUT_CONVECT)
+      hipLaunchKernelGGL(k_pair_empty<FJ_OUT_CONVECT>, g, dim3(256), 0, st, n_tgt, targets, out, fl, time, dt);
+    FJ_CHECK_LAUNCH();
+    return FJ_OK;
+  }
+  const dim3 g(fj_cdiv(n_tgt, 64)), blk(64 * FJ_RED_WAVES);
+  if (mode == FJ_OUT_STORE)
+    hipLaunchKernelGGL(k_pair_reduce<FJ_OUT_STORE>, g, blk, 0, st, part, chunks, ld, n_tgt, targets, out, fl, time, dt);
+  else if (mode == FJ_OUT_ADD)
+    hipLaunchKernelGGL(k_pair_reduce<FJ_OUT_ADD>, g, blk, 0, st, part, chunks, ld, n_tgt, targets, out, fl, time, dt);
+  else
+    hipLaunchKernelGGL(k_pair_reduce<FJ_OUT_CONVECT>, g, blk, 0, st, part, chunks, ld, n_tgt, targets, out, fl, time, dt);
+  FJ_CHECK_LAUNCH();
+  return FJ_OK;
+}
+
 extern "C" int fj_pair_sum(const double* seg, int n_seg, const double* targets, int n_tgt,
                            double* out, int kernel, double eps, double rcut, int precision,
                            int mode, const fj_flowfield* flow, double time, double dt,
@@ -322,15 +347,7 @@ extern "C" int fj_pair_sum(const double* seg, int n_seg, const double* targets, 
   if (!targets || !out) return FJ_ERR_ARG;
   hipStream_t st = (hipStream_t)stream;
   const FjFlow fl = fj_flow_from_desc(flow);
-  if (n_seg == 0) {
-    const dim3 g(fj_cdiv(n_tgt, 256));
-    if (mode == FJ_OUT_STORE)
-      hipLaunchKernelGGL(k_pair_empty<FJ_OUT_STORE>, g, dim3(256), 0, st, n_tgt, targets, out, fl, time, dt);
-    else if (mode == FJ_OUT_CONVECT)
-      hipLaunchKernelGGL(k_pair_empty<FJ_OUT_CONVECT>, g, dim3(256), 0, st, n_tgt, targets, out, fl, time, dt);
-    FJ_CHECK_LAUNCH();
-    return FJ_OK;
-  }
+  if (n_seg == 0) return fj_pair_epilogue(nullptr, 0, 0, n_tgt, targets, out, mode, fl, time, dt, st);
   if (!seg || !workspace) return FJ_ERR_ARG;
   FjKernelParams kp;
   kp.eps = eps; kp.eps2 = eps * eps; kp.rcut = rcut;
@@ -349,13 +366,5 @@ extern "C" int fj_pair_sum(const double* seg, int n_seg, const double* targets, 
 #undef FJ_DISPATCH_TPL
   }
   FJ_CHECK_LAUNCH();
-  const dim3 g(fj_cdiv(n_tgt, 64)), blk(64 * FJ_RED_WAVES);
-  if (mode == FJ_OUT_STORE)
-    hipLaunchKernelGGL(k_pair_reduce<FJ_OUT_STORE>, g, blk, 0, st, part, p.chunks, p.ld, n_tgt, targets, out, fl, time, dt);
-  else if (mode == FJ_OUT_ADD)
-    hipLaunchKernelGGL(k_pair_reduce<FJ_OUT_ADD>, g, blk, 0, st, part, p.chunks, p.ld, n_tgt, targets, out, fl, time, dt);
-  else
-    hipLaunchKernelGGL(k_pair_reduce<FJ_OUT_CONVECT>, g, blk, 0, st, part, p.chunks, p.ld, n_tgt, targets, out, fl, time, dt);
-  FJ_CHECK_LAUNCH();
-  return FJ_OK;
+  return fj_pair_epilogue(part, p.chunks, p.ld, n_tgt, targets, out, mode, fl, time, dt, st);
 }

@@ -229,6 +229,11 @@ int fj_batched_launch(const FjTaskDesc* d, int n_tasks, const double* seg, int k
                       int precision, const FjKernelParams& kp, void* task_table,
                       void* host_staging, void* workspace, hipStream_t st);
 
+// pair-sum epilogue (fj_biot_savart.hip): chunk partials -> out by mode
+int fj_pair_epilogue(const double* part, int chunks, int ld, int n_tgt, const double* targets,
+                     double* out, int mode, const FjFlow& fl, double time, double dt,
+                     hipStream_t st);
+
 // mixed-precision partial sums (fj_biot_savart_mixed.hip)
 int fj_pair_partial_mixed(const FjPairPlan* p, const double* seg, int n_seg, const double* tgt,
                           int n_tgt, double* part, int kernel, const FjKernelParams* kp,

@@ -118,10 +118,38 @@ def footprint_bytes(req: CaseRequest, precision: str = "fp64") -> int:
     words += req.steps                               # lift history
     words += 16 * (s_b + s_w) + 10 * nb + 6 * s_b     # tick workspace share: second table, stencils, midpoints
     words += 3 * (m + 1) * (n + 1)                   # steady forces
-    # pair-sum partials (filament chunks of 256, target tiles of 64)
+    # pair-sum partials (target tiles of 64, filament chunks of 256, at most
+    # about 8192 work items per pair sum)
     for n_tgt, n_seg in (((ms + 1) * (n + 1), s_b + s_w), (nb, s_w), (s_b, s_b + s_w)):
-        words += 3 * 64 * (-(-n_tgt // 64)) * max(1, -(-n_seg // 256))
+        tiles = -(-n_tgt // 64)
+        words += 3 * 64 * tiles * max(1, min(-(-n_seg // 256), -(-8192 // tiles)))
     return 8 * words + 4096
+
+
+class _InverseCache:
+    """Explicit inverses of initial dynamic AICs keyed by (lattice, kernel,
+    initial pose): cases that start from the same pose of the same lattice
+    have the same matrix, so they share one factorisation.  Holds at most
+    ``max_bytes`` of matrices (oldest entries dropped first; a dropped entry
+    stays alive while an active case still holds it)."""
+
+    def __init__(self, max_bytes: int):
+        self.max_bytes = max_bytes
+        self.entries: dict = {}
+        self.bytes = 0
+
+    def get(self, key):
+        return self.entries.get(key)
+
+    def put(self, key, ent):
+        nbytes = sum(t.numel() * t.element_size() for t in ent)
+        if nbytes > self.max_bytes:
+            return
+        while self.entries and self.bytes + nbytes > self.max_bytes:
+            old = self.entries.pop(next(iter(self.entries)))
+            self.bytes -= sum(t.numel() * t.element_size() for t in old)
+        self.entries[key] = ent
+        self.bytes += nbytes
 
 
 def _static_solve(A, Ainv, U, te, b):
@@ -149,7 +177,8 @@ def _static_solve(A, Ainv, U, te, b):
 class _Slot:
     """Resident state of one admitted case."""
 
-    def __init__(self, req: CaseRequest, dev, precision: str, flow: Constant, kernel_kind: int):
+    def __init__(self, req: CaseRequest, dev, precision: str, flow: Constant, kernel_kind: int,
+                 cache: "_InverseCache | None" = None):
         self.req = req
         m, n, ms = req.m, req.n, req.m_star
         self.dt = 1.0 / m
@@ -180,20 +209,31 @@ class _Slot:
         self.gw = torch.empty((ms, n), dtype=F64, device=dev)
         self.gw_next = torch.empty_like(self.gw)
         self.gbd = torch.zeros((m, n), dtype=F64, device=dev)
-        self.A = torch.empty((m * n, m * n), dtype=F64, device=dev)
+        nb = m * n
+        self.A = torch.empty((nb, nb), dtype=F64, device=dev)  # this step's AIC, written by the ticks
         c, nc, _ = K.colloc_normals(self.zb)
         rhs = K.rhs(c, nc, None, None, flow, 0.0)
-        # the static AIC (aero/uvlm.py:895-1012) is the dynamic, bound-only
-        # AIC plus the lumped wake in the n trailing-edge ring columns
-        # (add_wake_influence, aero/aic.py:190-205): A_s = A_d + U E^T
-        K.aic_assemble(c, nc, self.zb, kernel_kind, EPSILON, R_CUTOFF, self.A, 0, zeta_w=self.zw)
+        # the dynamic (bound-only) AIC at the initial pose and its inverse,
+        # kept for the refinement of every step; shared with every case whose
+        # initial pose, lattice and kernel are the same (the same matrix)
+        key = (m, n, kernel_kind, hg[0].tobytes())
+        ent = cache.get(key) if cache is not None else None
+        if ent is None:
+            A0 = torch.empty((nb, nb), dtype=F64, device=dev)
+            K.aic_assemble(c, nc, self.zb, kernel_kind, EPSILON, R_CUTOFF, A0, 0)
+            ent = (A0, K.lu_inverse(K.lu_factor(A0)))
+            if cache is not None:
+                cache.put(key, ent)
+        A0, self.Ainv = ent
+        # the static AIC (aero/uvlm.py:895-1012) is the dynamic AIC plus the
+        # lumped wake in the n trailing-edge ring columns (add_wake_influence,
+        # aero/aic.py:190-205): A_s = A_0 + U E^T, U = (TE ring + wake column
+        # sums) - TE ring, from the trailing-edge ring row alone
         te = slice((m - 1) * n, m * n)
-        U = self.A[:, te].clone()
-        K.aic_assemble(c, nc, self.zb, kernel_kind, EPSILON, R_CUTOFF, self.A, 0)
-        U -= self.A[:, te]
-        # the inverse kept for the dynamic (bound-only) system
-        self.Ainv = K.lu_inverse(K.lu_factor(self.A))
-        self.gb = _static_solve(self.A, self.Ainv, U, te, rhs).reshape(m, n).contiguous()
+        U = torch.empty((nb, n), dtype=F64, device=dev)
+        K.aic_assemble(c, nc, self.zb[m - 1:], kernel_kind, EPSILON, R_CUTOFF, U, 0, zeta_w=self.zw)
+        U -= A0[:, te]
+        self.gb = _static_solve(A0, self.Ainv, U, te, rhs).reshape(m, n).contiguous()
         K.static_wake_gamma(self.gb, ms, out=self.gw)
         self.gb_prev = self.gb.clone()
         self.ws = torch.empty(L.lib().fj_refine_workspace_bytes(m * n) // 8 + 1, dtype=F64, device=dev)
@@ -207,7 +247,7 @@ class _Slot:
 
 class ServingEngine:
     def __init__(self, precision: str = "fp64", max_slots: int = 32, hbm_budget_bytes: int = 64 << 30,
-                 free_wake: bool = True, device=None):
+                 free_wake: bool = True, device=None, inverse_cache_bytes: int = 8 << 30):
         if precision not in ("fp64", "mixed"):
             raise ValueError("precision must be 'fp64' or 'mixed'")
         self.precision = precision
@@ -220,6 +260,7 @@ class ServingEngine:
         self._staging = {}
         self._bufs = {}
         self._adm = None
+        self._cache = _InverseCache(inverse_cache_bytes) if inverse_cache_bytes > 0 else None
 
     # ---- fused pair sums ---------------------------------------------------
     def _batched(self, tasks, seg, targets, out, prec):
@@ -349,7 +390,7 @@ class ServingEngine:
                 req = pending.pop(0)
                 adm.wait_stream(main)
                 with torch.cuda.stream(adm):
-                    sl = _Slot(req, self.dev, self.precision, self.flow, self.kind)
+                    sl = _Slot(req, self.dev, self.precision, self.flow, self.kind, self._cache)
                     sl.ready = torch.cuda.Event()
                     sl.ready.record(adm)
                 sl.start = time_source() - t0

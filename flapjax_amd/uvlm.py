@@ -458,8 +458,8 @@ class UVLM:
     def _table_len(grids):
         return sum(K.n_segments(z.shape[0] - 1, z.shape[1] - 1) for z, _, _ in grids)
 
-    def _pair_ws(self, name, n_tgt, n_seg):
-        nbytes = L.lib().fj_pair_sum_workspace_bytes(n_tgt, n_seg, int(self._prec))
+    def _pair_ws(self, name, n_tgt, n_seg, lattices=None):
+        nbytes = K.pair_workspace_bytes(n_tgt, n_seg, self._prec, lattices)
         return self._buf(name, (nbytes // 8 + 1,))
 
     def _fast_step_ok(self, static, hg_nm1, hg_dot_n):
@@ -510,9 +510,11 @@ class UVLM:
                 continue
             if self.free_wake:
                 base = prev["zeta_w"][s][:-1].reshape(-1, 3)
+                lats = K.lattices_of(grids)
                 K.pair_sum(seg_nm1, base, kern.kind, kern.eps, kern.rcut, self._prec, L.OUT_CONVECT,
                            out=zw_n[s][1:].reshape(-1, 3), flow=ff, time=t_n, dt=dt,
-                           workspace=self._pair_ws(f"ws_roll_{s}", base.shape[0], seg_nm1.shape[0]))
+                           workspace=self._pair_ws(f"ws_roll_{s}", base.shape[0], seg_nm1.shape[0], lats),
+                           lattices=lats)
             else:
                 K.wake_convect(prev["zeta_w"][s], None, ff, t_n, dt, zw_n[s])
         # ---- system (aero/uvlm.py:944-993)
@@ -524,8 +526,11 @@ class UVLM:
         v_w = None
         if s_w:
             K.segments_from_grids(grids_w, kern.kind, kern.eps, self.mirror, out=seg_n[s_b:])
+            lats_w = K.lattices_of(grids_w)
+            ws_up = K.pair_workspace_bytes(n_b, s_w, L.PREC_FP64, lats_w)
             v_w = K.pair_sum(seg_n[s_b:], c_all, kern.kind, kern.eps, kern.rcut, L.PREC_FP64, L.OUT_STORE,
-                             out=self._buf("v_w", (n_b, 3)), workspace=self._pair_ws("ws_up", n_b, s_w))
+                             out=self._buf("v_w", (n_b, 3)), workspace=self._buf("ws_up", (ws_up // 8 + 1,)),
+                             lattices=lats_w)
         rhs = K.rhs(c_all, nc_all, cd_all, v_w, ff, t_n, out=self._buf("rhs", (n_b,)))
         gout = snap["gamma_b"][0].view(-1) if single else self._buf("gvec", (n_b,))
         if self._solvers is None:
@@ -544,9 +549,10 @@ class UVLM:
         # ---- forces (aero/uvlm.py:1018-1103)
         if s_b:
             K.segments_from_grids(grids_b, kern.kind, kern.eps, self.mirror, out=seg_n[:s_b])
+        lats_n = K.lattices_of(grids_b + grids_w)
         v_mid = K.pair_sum(seg_n if s_b + s_w else None, mids, kern.kind, kern.eps, kern.rcut, self._prec,
                            L.OUT_STORE, out=self._buf("v_mid", (n_mid, 3)),
-                           workspace=self._pair_ws("ws_mid", n_mid, s_b + s_w))
+                           workspace=self._pair_ws("ws_mid", n_mid, s_b + s_w, lats_n), lattices=lats_n)
         for s, gd in enumerate(self.grid_disc):
             sa, sb = self._s_off[s], self._s_off[s + 1]
             gw0 = gw_n[s][0] if gd.m_star > 0 else None

@@ -126,6 +126,29 @@ int fj_pair_sum(const double* seg, int n_seg, const double* targets, int n_tgt, 
                 const fj_flowfield* flow, double time, double dt, void* workspace,
                 void* stream);
 
+/* One vortex lattice inside a filament table: the S(m, n) = m(n+1) + (m+1)n
+ * records written for an (m+1, n+1) vertex grid by fj_segments_from_grid(s)
+ * (chordwise filaments first, then spanwise), starting at row seg_off. */
+typedef struct {
+  int seg_off;
+  int m;
+  int n;
+} fj_lattice_ref;
+
+/* fj_pair_sum (FP64) over the filaments of whole lattices -- the surfaces
+ * that compute_v_ind sums over (aero/aic.py:248-310) -- with the per-vertex
+ * part of biot_savart_epsilon / _cutoff / biot_savart (aero/utils.py:580-648:
+ * r = x - y, 1/sqrt(|r|^2 + eps^2)) evaluated once per vertex and shared by
+ * the filaments that meet there.  Same epilogue modes as fj_pair_sum.  At
+ * most 16 lattices per call; lattices with m = 0 or n = 0 (all filament
+ * strengths zero) contribute nothing.
+ * workspace: fj_pair_sum_lattices_workspace_bytes(n_tgt, n_lat, lats) bytes. */
+size_t fj_pair_sum_lattices_workspace_bytes(int n_tgt, int n_lat, const fj_lattice_ref* lats);
+int fj_pair_sum_lattices(const double* seg, int n_lat, const fj_lattice_ref* lats,
+                         const double* targets, int n_tgt, double* out, int kernel, double eps,
+                         double rcut, int mode, const fj_flowfield* flow, double time, double dt,
+                         void* workspace, void* stream);
+
 /* Reverse mode of fj_pair_sum (FJ_OUT_STORE) for hand-written adjoints:
  * with upstream weights grad_v (n_tgt, 3),
  *   grad_targets (n_tgt, 3)   = dL/dx          (written, or added if accumulate)

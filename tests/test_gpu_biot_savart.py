@@ -157,3 +157,80 @@ def test_convect_epilogue(dev):
                      time=0.3, dt=0.0625).cpu().numpy()
     ref = x + 0.0625 * (np.array([1.0, 0.0, 0.05]) + O.pair_sum(x, y0, y1, g))
     assert rel_err(out, ref) < 1e-14
+
+
+def _lattice_case(dev, kind, rng):
+    """Bound-like and wake-like grids (ring rows not a multiple of the strip
+    height), a mirrored copy, an empty wake, and targets on and off the
+    vertices."""
+    from flapjax_amd import _kernels as K
+
+    grids, lat, off = [], [], 0
+    shapes = [(7, 13, False), (9, 13, False), (0, 13, False), (7, 13, True), (2, 5, False), (1, 1, False)]
+    verts = []
+    for m, n, mir in shapes:
+        z = O.make_rectangular_grid(m, n, 1.0, 0.25)
+        z[..., 1] = np.linspace(0.0, 3.0, n + 1)[None, :]
+        z += rng.normal(size=z.shape) * 0.02
+        z[..., 0] += 1.5 * (len(grids) % 2)
+        verts.append(z.reshape(-1, 3))
+        zt = _t(z, dev)
+        gt = _t(rng.uniform(-1, 1, (m, n)), dev) if m * n else None
+        grids.append((zt, gt, mir))
+        lat.append((off, m, n))
+        off += m * (n + 1) + (m + 1) * n
+    mirror = (np.array([0.0, -0.5, 0.0]), np.array([0.0, 1.0, 0.0]))
+    seg = K.segments_from_grids(grids, kind, O.EPSILON, mirror=mirror)
+    pts = rng.normal(size=(700, 3)) * 1.5
+    if kind != 2:  # the un-regularised kernel is singular on its own vertices
+        pts = np.concatenate([pts] + verts)
+    return seg, lat, pts
+
+
+@pytest.mark.parametrize("kernel", ["epsilon", "cutoff", "plain"])
+def test_lattice_pair_sum_equals_table_pair_sum(dev, kernel):
+    """fj_pair_sum_lattices (per-vertex terms shared between filaments) gives
+    the table pair sum of the same filaments, and the oracle's, to 1e-12."""
+    from flapjax_amd import _kernels as K
+    from flapjax_amd import _lib as L
+
+    kind = {"epsilon": L.KERNEL_EPSILON, "cutoff": L.KERNEL_CUTOFF, "plain": L.KERNEL_PLAIN}[kernel]
+    rng = np.random.default_rng(21)
+    seg, lat, pts = _lattice_case(dev, kind, rng)
+    x = _t(pts, dev)
+    v_lat = K.pair_sum_lattices(seg, lat, x, kind, O.EPSILON, O.R_CUTOFF).cpu().numpy()
+    v_tab = K.pair_sum(seg, x, kind, O.EPSILON, O.R_CUTOFF).cpu().numpy()
+    s = seg.cpu().numpy()
+    ref = O.pair_sum(pts, s[:, 0:3], s[:, 3:6], s[:, 10], kernel)
+    assert np.isfinite(v_lat).all()
+    assert rel_err(v_lat, v_tab) < 1e-12
+    assert rel_err(v_lat, ref) < 1e-12
+
+
+def test_lattice_pair_sum_modes_and_sizes(dev):
+    """Epilogue modes (store, add, convect) and launch plans from one target
+    to many column blocks."""
+    import flapjax_amd as fj
+    from flapjax_amd import _kernels as K
+    from flapjax_amd import _lib as L
+
+    rng = np.random.default_rng(22)
+    seg, lat, pts = _lattice_case(dev, L.KERNEL_EPSILON, rng)
+    flow = fj.Constant(np.array([1.0, 0.0, 0.2]), 1.225, True)
+    for nt in (1, 63, 65, 4000):
+        x = _t(pts[:nt] if nt <= len(pts) else rng.normal(size=(nt, 3)), dev)
+        ref = K.pair_sum(seg, x, L.KERNEL_EPSILON, O.EPSILON, O.R_CUTOFF)
+        got = K.pair_sum_lattices(seg, lat, x, L.KERNEL_EPSILON, O.EPSILON, O.R_CUTOFF)
+        assert rel_err(got.cpu().numpy(), ref.cpu().numpy()) < 1e-12
+        acc = torch.ones_like(x)
+        K.pair_sum_lattices(seg, lat, x, L.KERNEL_EPSILON, O.EPSILON, O.R_CUTOFF, mode=L.OUT_ADD, out=acc)
+        assert rel_err((acc - 1.0).cpu().numpy(), ref.cpu().numpy()) < 1e-12
+        cv = K.pair_sum_lattices(seg, lat, x, L.KERNEL_EPSILON, O.EPSILON, O.R_CUTOFF, mode=L.OUT_CONVECT,
+                                 flow=flow, time=0.3, dt=0.05)
+        cref = K.pair_sum(seg, x, L.KERNEL_EPSILON, O.EPSILON, O.R_CUTOFF, mode=L.OUT_CONVECT, flow=flow,
+                          time=0.3, dt=0.05)
+        assert rel_err(cv.cpu().numpy(), cref.cpu().numpy()) < 1e-14
+    # only empty lattices: zero velocity / pure convection
+    x = _t(pts[:10], dev)
+    z = K.pair_sum_lattices(seg, [(lat[2][0], 0, 13)], x, L.KERNEL_EPSILON, O.EPSILON, O.R_CUTOFF)
+    assert (z == 0).all()
