@@ -10,7 +10,13 @@
 // replayed (e.g. from a captured graph) while the table contents change.
 // Partial sums go to a per-task region of a packed partial buffer; the
 // reduction pass applies each task's epilogue (store / add / convect).
+//
+// FP64 tasks whose filaments are whole lattices (the serving tick's) run the
+// lattice kernel of fj_lattice.hip instead: a work item is a 64-target tile
+// and FJ_BS_WAVES (strip, column block) lattice tasks, one per wave, over
+// strip-column records built for all tasks by one launch.
 #include "fj_common.hpp"
+#include "fj_lattice_dev.hpp"
 
 
 struct FjTask {
@@ -25,6 +31,13 @@ struct FjTask {
   int mode;                // FJ_OUT_*
   double dt, time;
   FjFlow flow;
+  // lattice form: lattices (table row, rings m x n, column blocks per strip,
+  // first lattice task, first strip-column record), column-block width,
+  // lattice tasks, records
+  int n_lat;
+  int lat_off[FJ_TASK_MAX_LAT], lat_m[FJ_TASK_MAX_LAT], lat_n[FJ_TASK_MAX_LAT];
+  int lat_cb[FJ_TASK_MAX_LAT], lat_task[FJ_TASK_MAX_LAT], lat_rec[FJ_TASK_MAX_LAT];
+  int cb_width, n_ltasks, rec_off, n_recs;
 };
 
 __device__ __forceinline__ int fj_find_task(const FjTask* __restrict__ tasks, int n_tasks, int w) {
@@ -102,6 +115,87 @@ __global__ __launch_bounds__(FJ_BS_THREADS) void k_batched_partial(
     }
     if (MIXED) {
       ax += (double)bx; ay += (double)by; az += (double)bz;
+    }
+    if (wave > 0) {
+      red[wave - 1][0][lane] = ax;
+      red[wave - 1][1][lane] = ay;
+      red[wave - 1][2][lane] = az;
+    }
+    __syncthreads();
+    if (wave == 0) {
+      for (int q = 0; q < FJ_BS_WAVES - 1; ++q) {
+        ax += red[q][0][lane]; ay += red[q][1][lane]; az += red[q][2][lane];
+      }
+      double* o = part + T.part_off + (size_t)chunk * 3 * T.ld;
+      o[t] = ax;
+      o[T.ld + t] = ay;
+      o[2 * T.ld + t] = az;
+    }
+    __syncthreads();
+  }
+}
+
+// strip-column records of every lattice task: one thread per record word
+__global__ void k_batched_lattice_records(const FjTask* __restrict__ tasks,
+                                          const int* __restrict__ counts,
+                                          const double* __restrict__ seg,
+                                          double* __restrict__ tab, int kernel) {
+  const int n_tasks = counts[0];
+  const long long total = (long long)counts[3] * FJ_LAT_REC;
+  for (long long w = (long long)blockIdx.x * blockDim.x + threadIdx.x; w < total;
+       w += (long long)gridDim.x * blockDim.x) {
+    const int q = (int)(w / FJ_LAT_REC), f = (int)(w - (long long)q * FJ_LAT_REC);
+    // last task whose rec_off <= q among tasks with records
+    int lo = 0, hi = n_tasks - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (tasks[mid].rec_off <= q) lo = mid; else hi = mid - 1;
+    }
+    const FjTask& T = tasks[lo];
+    const int li = (T.n_lat > 1 && q >= T.lat_rec[1]) ? 1 : 0;
+    const int local = q - T.lat_rec[li];
+    const int n = T.lat_n[li];
+    const int s = local / (n + 1), j = local - s * (n + 1);
+    tab[w] = fj_lat_record_word(seg, T.lat_off[li], T.lat_m[li], n, s, j, f, kernel);
+  }
+}
+
+template <int KIND>
+__global__ __launch_bounds__(FJ_BS_THREADS, KIND == FJ_KERNEL_CUTOFF ? 4 : 5) void
+k_batched_lattice_partial(const FjTask* __restrict__ tasks, const int* __restrict__ counts,
+                          const double* __restrict__ tab, double* __restrict__ part,
+                          FjKernelParams kp) {
+  __shared__ double red[FJ_BS_WAVES - 1][3][64];
+  const int n_tasks = counts[0];
+  const int total = counts[1];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const double e2 = (KIND == FJ_KERNEL_PLAIN) ? 0.0 : kp.eps2;
+  const double rcut2 = kp.rcut * kp.rcut;
+  for (int w = blockIdx.x; w < total; w += gridDim.x) {
+    const int ti = __builtin_amdgcn_readfirstlane(fj_find_task(tasks, n_tasks, w));
+    const FjTask& T = tasks[ti];
+    const int local = w - T.work_off;
+    const int tile = local % T.tiles;
+    const int chunk = local / T.tiles;
+    const int t = tile * 64 + lane;
+    const int tc = (t < T.n_tgt) ? t : (T.n_tgt - 1);
+    const double* xp = T.tgt + 3 * (size_t)tc;
+    const double x = xp[0], y = xp[1], z = xp[2];
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    const int lt = chunk * FJ_BS_WAVES + wave;
+    if (lt < T.n_ltasks) {
+      const int li = (T.n_lat > 1 && lt >= T.lat_task[1]) ? 1 : 0;
+      const int m = T.lat_m[li], n = T.lat_n[li];
+      const int l2 = lt - T.lat_task[li];
+      const int s = l2 / T.lat_cb[li], b = l2 - s * T.lat_cb[li];
+      const int i0 = s * FJ_LAT_ROWS;
+      const int rows = min(FJ_LAT_ROWS, m - i0);
+      const int last = (i0 + rows == m) ? 1 : 0;
+      const int jb = b * T.cb_width;
+      const int je = min(n + 1, jb + T.cb_width);
+      const double* col = tab + (size_t)(T.lat_rec[li] + s * (n + 1)) * FJ_LAT_REC;
+      fj_lat_strip<KIND>(col, rows, last, jb, je, x, y, z, e2, rcut2, ax, ay, az);
     }
     if (wave > 0) {
       red[wave - 1][0][lane] = ax;
@@ -205,14 +299,65 @@ static void fj_task_tiling(int n_tgt, int n_seg, int& tiles, int& chunks, int& c
   chunk_len = n_seg > 0 ? fj_cdiv(n_seg, chunks) : 0;
 }
 
-size_t fj_batched_ws_bytes(const FjTaskDesc* d, int n) {
-  size_t total = 0;
+// lattice form of a task: column blocks of 16 chordwise-filament columns
+// (about 64 filaments per wave task), widened until the task has at most
+// about FJ_BS_TARGET_BLOCKS work items; depends on the task's own sizes only
+struct FjLatTiling {
+  int tiles, chunks, width, n_ltasks, n_recs;
+  int cb[FJ_TASK_MAX_LAT], ltask[FJ_TASK_MAX_LAT], rec[FJ_TASK_MAX_LAT];
+};
+
+static FjLatTiling fj_task_lattice_tiling(const FjTaskDesc& d) {
+  FjLatTiling L;
+  L.tiles = d.n_tgt > 0 ? fj_cdiv(d.n_tgt, 64) : 0;
+  const int n_lat = d.n_tgt > 0 ? d.n_lat : 0;  // no targets: no records, no work
+  for (int k = 0; k < FJ_TASK_MAX_LAT; ++k) L.cb[k] = L.ltask[k] = L.rec[k] = 0;
+  int max_cols = 1;
+  for (int k = 0; k < n_lat; ++k) max_cols = d.lat_n[k] + 1 > max_cols ? d.lat_n[k] + 1 : max_cols;
+  L.width = max_cols < 16 ? max_cols : 16;
+  for (;;) {
+    L.n_ltasks = 0;
+    L.n_recs = 0;
+    for (int k = 0; k < n_lat; ++k) {
+      const bool work = d.lat_m[k] > 0 && d.lat_n[k] > 0;
+      L.cb[k] = work ? fj_cdiv(d.lat_n[k] + 1, L.width) : 0;
+      L.ltask[k] = L.n_ltasks;
+      L.rec[k] = L.n_recs;
+      if (work) {
+        L.n_ltasks += fj_cdiv(d.lat_m[k], FJ_LAT_ROWS) * L.cb[k];
+        L.n_recs += fj_cdiv(d.lat_m[k], FJ_LAT_ROWS) * (d.lat_n[k] + 1);
+      }
+    }
+    L.chunks = fj_cdiv(L.n_ltasks > 0 ? L.n_ltasks : 1, FJ_BS_WAVES);
+    if ((long long)L.tiles * L.chunks <= FJ_BS_TARGET_BLOCKS || L.width >= max_cols) break;
+    L.width *= 2;
+  }
+  return L;
+}
+
+static bool fj_batched_lattice_form(const FjTaskDesc* d, int n, int precision) {
+  if (precision != FJ_PREC_FP64) return false;
+  for (int i = 0; i < n; ++i)
+    if (d[i].n_tgt > 0 && (d[i].n_lat < 1 || d[i].n_lat > FJ_TASK_MAX_LAT)) return false;
+  return true;
+}
+
+size_t fj_batched_ws_bytes(const FjTaskDesc* d, int n, int precision) {
+  const bool lat = fj_batched_lattice_form(d, n, precision);
+  size_t total = 0, recs = 0;
   for (int i = 0; i < n; ++i) {
     int tiles, chunks, chunk_len;
-    fj_task_tiling(d[i].n_tgt, d[i].n_seg, tiles, chunks, chunk_len);
+    if (lat) {
+      const FjLatTiling L = fj_task_lattice_tiling(d[i]);
+      tiles = L.tiles;
+      chunks = L.chunks;
+      recs += (size_t)L.n_recs;
+    } else {
+      fj_task_tiling(d[i].n_tgt, d[i].n_seg, tiles, chunks, chunk_len);
+    }
     total += (size_t)chunks * 3 * (tiles > 0 ? tiles : 1) * 64;
   }
-  return total * sizeof(double) + 256;
+  return ((total * sizeof(double) + 255) / 256) * 256 + recs * FJ_LAT_REC * sizeof(double) + 256;
 }
 
 template <int KIND>
@@ -236,8 +381,9 @@ int fj_batched_launch(const FjTaskDesc* d, int n_tasks, const double* seg, int k
   if (precision != FJ_PREC_FP64 && precision != FJ_PREC_MIXED) return FJ_ERR_ARG;
   // host_staging: fj_batched_task_bytes(n_tasks) bytes of host memory that
   // stays valid until the copy has run (the caller keeps it)
+  const bool lat = fj_batched_lattice_form(d, n_tasks, precision);
   FjTask* h = (FjTask*)host_staging;
-  int work = 0, part = 0, n_tgt_total = 0;
+  int work = 0, part = 0, n_tgt_total = 0, recs = 0;
   for (int i = 0; i < n_tasks; ++i) {
     FjTask& T = h[i];
     if (d[i].n_tgt < 0 || d[i].n_seg < 0 || d[i].mode < FJ_OUT_STORE || d[i].mode > FJ_OUT_CONVECT)
@@ -249,7 +395,33 @@ int fj_batched_launch(const FjTaskDesc* d, int n_tasks, const double* seg, int k
     T.n_tgt = d[i].n_tgt;
     T.seg_off = d[i].seg_off;
     T.n_seg = d[i].n_seg;
-    fj_task_tiling(T.n_tgt, T.n_seg, T.tiles, T.chunks, T.chunk_len);
+    T.n_lat = 0;
+    T.n_ltasks = 0;
+    T.n_recs = 0;
+    T.rec_off = recs;
+    T.cb_width = 1;
+    if (lat) {
+      const FjLatTiling L = fj_task_lattice_tiling(d[i]);
+      T.tiles = L.tiles;
+      T.chunks = L.chunks;
+      T.chunk_len = 0;
+      T.n_lat = T.n_tgt > 0 ? d[i].n_lat : 0;
+      for (int k = 0; k < FJ_TASK_MAX_LAT; ++k) {
+        const bool on = k < T.n_lat;
+        T.lat_off[k] = on ? d[i].lat_off[k] : 0;
+        T.lat_m[k] = on ? d[i].lat_m[k] : 0;
+        T.lat_n[k] = on ? d[i].lat_n[k] : 0;
+        T.lat_cb[k] = on ? L.cb[k] : 1;
+        T.lat_task[k] = on ? L.ltask[k] : L.n_ltasks;
+        T.lat_rec[k] = recs + (on ? L.rec[k] : L.n_recs);
+      }
+      T.cb_width = L.width;
+      T.n_ltasks = L.n_ltasks;
+      T.n_recs = L.n_recs;
+      recs += L.n_recs;
+    } else {
+      fj_task_tiling(T.n_tgt, T.n_seg, T.tiles, T.chunks, T.chunk_len);
+    }
     T.work_off = work;
     T.part_off = part;
     T.ld = T.tiles * 64;
@@ -265,13 +437,30 @@ int fj_batched_launch(const FjTaskDesc* d, int n_tasks, const double* seg, int k
   counts[0] = n_tasks;
   counts[1] = work;
   counts[2] = n_tgt_total;
+  counts[3] = recs;
+  const int part_total = part;
   if (hipMemcpyAsync(task_table, h, fj_batched_task_bytes(n_tasks), hipMemcpyHostToDevice, st) !=
       hipSuccess)
     return FJ_ERR_LAUNCH;
   const FjTask* dtasks = (const FjTask*)task_table;
   const int* dcounts = (const int*)(dtasks + n_tasks);
   const int grid = work < 4096 ? work : 4096;
-  if (work > 0) {
+  if (work > 0 && lat) {
+    double* part = (double*)workspace;
+    double* tab = (double*)((char*)workspace + (((size_t)part_total * sizeof(double) + 255) / 256) * 256);
+    const long long words = (long long)recs * FJ_LAT_REC;
+    const int rgrid = (int)(fj_cdiv(words, 256) < 65536 ? fj_cdiv(words, 256) : 65536);
+    hipLaunchKernelGGL(k_batched_lattice_records, dim3(rgrid), dim3(256), 0, st, dtasks, dcounts,
+                       seg, tab, kernel);
+    FJ_CHECK_LAUNCH();
+    if (kernel == FJ_KERNEL_EPSILON)
+      hipLaunchKernelGGL(k_batched_lattice_partial<FJ_KERNEL_EPSILON>, dim3(grid), dim3(FJ_BS_THREADS), 0, st, dtasks, dcounts, tab, part, kp);
+    else if (kernel == FJ_KERNEL_CUTOFF)
+      hipLaunchKernelGGL(k_batched_lattice_partial<FJ_KERNEL_CUTOFF>, dim3(grid), dim3(FJ_BS_THREADS), 0, st, dtasks, dcounts, tab, part, kp);
+    else
+      hipLaunchKernelGGL(k_batched_lattice_partial<FJ_KERNEL_PLAIN>, dim3(grid), dim3(FJ_BS_THREADS), 0, st, dtasks, dcounts, tab, part, kp);
+    FJ_CHECK_LAUNCH();
+  } else if (work > 0) {
     double* part = (double*)workspace;
     if (kernel == FJ_KERNEL_EPSILON)
       fj_batched_partial_launch<FJ_KERNEL_EPSILON>(precision, grid, dtasks, dcounts, seg, part, kp, st);
@@ -325,6 +514,7 @@ extern "C" int fj_pair_sum_batched(int n_tasks, const int* tgt_off, const int* n
     d[i].dt = dt ? dt[i] : 0.0;
     d[i].time = time ? time[i] : 0.0;
     d[i].flow = fj_flow_from_desc(flows ? &flows[i] : nullptr);
+    d[i].n_lat = 0;
   }
   FjKernelParams kp;
   kp.eps = eps; kp.eps2 = eps * eps; kp.rcut = rcut;

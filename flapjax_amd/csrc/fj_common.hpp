@@ -216,14 +216,19 @@ struct FjPairPlan {
 
 // one task of a fused multi-case pair sum (fj_batched.hip): targets and
 // outputs by pointer, filaments by row range of a shared table
+// n_lat > 0: the task's filaments are whole lattices (rows lat_off of the
+// table, m x n rings each), which lets the FP64 form run the lattice kernel
+#define FJ_TASK_MAX_LAT 2
 struct FjTaskDesc {
   const double* tgt;
   double* out;
   int n_tgt, seg_off, n_seg, mode;
   double dt, time;
   FjFlow flow;
+  int n_lat;
+  int lat_off[FJ_TASK_MAX_LAT], lat_m[FJ_TASK_MAX_LAT], lat_n[FJ_TASK_MAX_LAT];
 };
-size_t fj_batched_ws_bytes(const FjTaskDesc* d, int n);
+size_t fj_batched_ws_bytes(const FjTaskDesc* d, int n, int precision);
 size_t fj_batched_desc_bytes(int n_tasks);
 int fj_batched_launch(const FjTaskDesc* d, int n_tasks, const double* seg, int kernel,
                       int precision, const FjKernelParams& kp, void* task_table,

@@ -18,7 +18,8 @@
 // one third-order Newton step.
 //
 // Data.  Everything one column of a strip needs (its vertices, chordwise and
-// spanwise filament data r0, eps |r0|^4 or 1/|r0|^2, g) is gathered from the
+// spanwise filament data: g r0 and eps |r0|^4, or r0, 1/|r0|^2 and g for the
+// cutoff kernel) is gathered from the
 // ordinary filament table (fj_segments_from_grid order) into one contiguous
 // strip-column record, so that a wave fetches it through the scalar unit in
 // a few wide loads and waits once per column (every lane of a wave works on
@@ -30,22 +31,12 @@
 // LDS in a fixed order and writes one partial per chunk; the partials are
 // summed in a fixed order by the common reduction (deterministic).
 #include "fj_common.hpp"
+#include "fj_lattice_dev.hpp"
 
-#ifndef FJ_LAT_ROWS
-#define FJ_LAT_ROWS 2
-#endif
 #ifndef FJ_LAT_MIN_WAVES
 #define FJ_LAT_MIN_WAVES 5
 #endif
 #define FJ_MAX_LAT 16
-
-// strip-column record: vertices (FJ_LAT_ROWS + 1) x 3, chordwise filaments
-// FJ_LAT_ROWS x 5, spanwise filaments (to the previous column)
-// (FJ_LAT_ROWS + 1) x 5; a filament's 5 are r0 (3), aux, g
-#define FJ_LAT_COFF (3 * (FJ_LAT_ROWS + 1))
-#define FJ_LAT_SOFF (FJ_LAT_COFF + 5 * FJ_LAT_ROWS)
-#define FJ_LAT_RAW (FJ_LAT_SOFF + 5 * (FJ_LAT_ROWS + 1))
-#define FJ_LAT_REC ((FJ_LAT_RAW + 3) / 4 * 4)
 
 struct FjLattice {
   int off;       // first filament of this lattice in the filament table
@@ -61,85 +52,8 @@ struct FjLatticeList {
   int n_tasks;
   int n_recs;
   int cb_width;  // chordwise-filament columns per block
+  int kernel;    // FJ_KERNEL_*: record layout of the filament data
 };
-
-struct FjVtx {
-  double rx, ry, rz, s, inv;
-};
-
-__device__ __forceinline__ FjVtx fj_lat_vertex(const double* __restrict__ v, double x, double y,
-                                               double z, double e2) {
-  FjVtx o;
-  o.rx = x - v[0];
-  o.ry = y - v[1];
-  o.rz = z - v[2];
-  o.s = fma(o.rz, o.rz, fma(o.ry, o.ry, fma(o.rx, o.rx, e2)));
-  o.inv = fj_rsqrt(o.s);
-  return o;
-}
-
-// filament A -> B with data f = (r0 = B - A, aux, g)
-template <int KIND>
-__device__ __forceinline__ void fj_lat_eval(const FjVtx& A, const FjVtx& B,
-                                            const double* __restrict__ f, double e2,
-                                            double rcut2, double& ax, double& ay, double& az) {
-  const double cx = A.ry * B.rz - A.rz * B.ry;
-  const double cy = A.rz * B.rx - A.rx * B.rz;
-  const double cz = A.rx * B.ry - A.ry * B.rx;
-  const double dx = fma(A.rx, A.inv, -(B.rx * B.inv));
-  const double dy = fma(A.ry, A.inv, -(B.ry * B.inv));
-  const double dz = fma(A.rz, A.inv, -(B.rz * B.inv));
-  const double a = f[0] * dx + f[1] * dy + f[2] * dz;
-  double D = fma(cz, cz, fma(cy, cy, cx * cx));
-  if (KIND == FJ_KERNEL_EPSILON) D += f[3];
-  // 1/D: hardware estimate, then y (1 + e + e^2) with e = 1 - D y
-  double w = __builtin_amdgcn_rcp(D);
-  const double e = fma(-D, w, 1.0);
-  w = fma(w, fma(e, e, e), w);
-  double q = (f[4] * a) * w;
-  if (KIND == FJ_KERNEL_CUTOFF) {
-    // squared distance from x to the filament line: |r1|^2 - (r0.r1)^2 / |r0|^2
-    const double a1 = f[0] * A.rx + f[1] * A.ry + f[2] * A.rz;
-    const double rad2 = (A.s - e2) - a1 * (a1 * f[3]);
-    q = (rad2 > rcut2) ? q : 0.0;
-  }
-  ax = fma(cx, q, ax);
-  ay = fma(cy, q, ay);
-  az = fma(cz, q, az);
-}
-
-// one task: R chordwise filament rows and SR spanwise rows (R, plus the
-// closing row on a lattice's last strip) of the strip whose column records
-// start at `col`, chordwise columns jb .. je-1
-template <int KIND, int R, int SR>
-__device__ __forceinline__ void fj_lat_task(const double* __restrict__ col, int jb, int je,
-                                            double x, double y, double z, double e2, double rcut2,
-                                            double& ax, double& ay, double& az) {
-  FjVtx P[R + 1], C[R + 1];
-  if (jb > 0) {
-    const double* r = col + (size_t)(jb - 1) * FJ_LAT_REC;
-#pragma unroll
-    for (int k = 0; k <= R; ++k) P[k] = fj_lat_vertex(r + 3 * k, x, y, z, e2);
-  }
-  for (int j = jb; j < je; ++j) {
-    const double* rp = col + (size_t)j * FJ_LAT_REC;
-    double r[FJ_LAT_RAW];
-#pragma unroll
-    for (int k = 0; k < FJ_LAT_RAW; ++k) r[k] = rp[k];
-#pragma unroll
-    for (int k = 0; k <= R; ++k) C[k] = fj_lat_vertex(r + 3 * k, x, y, z, e2);
-#pragma unroll
-    for (int k = 0; k < R; ++k)
-      fj_lat_eval<KIND>(C[k], C[k + 1], r + FJ_LAT_COFF + 5 * k, e2, rcut2, ax, ay, az);
-    if (j > 0) {
-#pragma unroll
-      for (int k = 0; k < SR; ++k)
-        fj_lat_eval<KIND>(P[k], C[k], r + FJ_LAT_SOFF + 5 * k, e2, rcut2, ax, ay, az);
-    }
-#pragma unroll
-    for (int k = 0; k <= R; ++k) P[k] = C[k];
-  }
-}
 
 // strip-column records from the filament table: one thread per record
 // word (consecutive threads write consecutive words)
@@ -153,27 +67,7 @@ __global__ void k_lattice_records(const double* __restrict__ seg, FjLatticeList 
   const FjLattice& L = LL.l[li];
   const int local = q - L.rec_off;
   const int s = local / (L.n + 1), j = local - s * (L.n + 1);
-  const int i0 = s * FJ_LAT_ROWS;
-  const int rows = min(FJ_LAT_ROWS, L.m - i0);
-  const int srows = rows + ((i0 + rows == L.m) ? 1 : 0);
-  const double* chord = seg + (size_t)L.off * FJ_SEG_STRIDE;
-  const double* span = seg + (size_t)(L.off + L.m * (L.n + 1)) * FJ_SEG_STRIDE;
-  double v = 0.0;
-  if (f < FJ_LAT_COFF) {
-    // vertex (i, j): start of chordwise filament (i, j), or the end of
-    // chordwise filament (m-1, j) on the last vertex row
-    const int k = f / 3, c = f - 3 * k, i = i0 + k;
-    if (k <= rows)
-      v = (i < L.m) ? chord[(size_t)(i * (L.n + 1) + j) * FJ_SEG_STRIDE + c]
-                    : chord[(size_t)((L.m - 1) * (L.n + 1) + j) * FJ_SEG_STRIDE + 3 + c];
-  } else if (f < FJ_LAT_SOFF) {
-    const int k = (f - FJ_LAT_COFF) / 5, c = f - FJ_LAT_COFF - 5 * k;
-    if (k < rows) v = chord[(size_t)((i0 + k) * (L.n + 1) + j) * FJ_SEG_STRIDE + 6 + c];
-  } else if (f < FJ_LAT_RAW) {
-    const int k = (f - FJ_LAT_SOFF) / 5, c = f - FJ_LAT_SOFF - 5 * k;
-    if (j > 0 && k < srows) v = span[(size_t)((i0 + k) * L.n + j - 1) * FJ_SEG_STRIDE + 6 + c];
-  }
-  tab[w] = v;
+  tab[w] = fj_lat_record_word(seg, L.off, L.m, L.n, s, j, f, LL.kernel);
 }
 
 template <int KIND>
@@ -205,21 +99,8 @@ __global__ __launch_bounds__(FJ_BS_THREADS, KIND == FJ_KERNEL_CUTOFF ? 4 : FJ_LA
     const int last = (i0 + rows == L.m) ? 1 : 0;
     const int jb = b * LL.cb_width;
     const int je = min(L.n + 1, jb + LL.cb_width);
-    // every strip but a lattice's last has FJ_LAT_ROWS rows
     const double* col = tab + (size_t)(L.rec_off + s * (L.n + 1)) * FJ_LAT_REC;
-#define FJ_LAT_ARGS col, jb, je, x, y, z, e2, rcut2, ax, ay, az
-    if (rows == FJ_LAT_ROWS && !last) fj_lat_task<KIND, FJ_LAT_ROWS, FJ_LAT_ROWS>(FJ_LAT_ARGS);
-    else if (rows == FJ_LAT_ROWS) fj_lat_task<KIND, FJ_LAT_ROWS, FJ_LAT_ROWS + 1>(FJ_LAT_ARGS);
-#if FJ_LAT_ROWS > 3
-    else if (rows == 3) fj_lat_task<KIND, 3, 4>(FJ_LAT_ARGS);
-#endif
-#if FJ_LAT_ROWS > 2
-    else if (rows == 2) fj_lat_task<KIND, 2, 3>(FJ_LAT_ARGS);
-#endif
-#if FJ_LAT_ROWS > 1
-    else if (rows == 1) fj_lat_task<KIND, 1, 2>(FJ_LAT_ARGS);
-#endif
-#undef FJ_LAT_ARGS
+    fj_lat_strip<KIND>(col, rows, last, jb, je, x, y, z, e2, rcut2, ax, ay, az);
   }
   if (wave > 0) {
     red[wave - 1][0][lane] = ax;
@@ -288,6 +169,7 @@ static int fj_lat_plan(int n_tgt, int n_lat, const fj_lattice_ref* lats, FjLatPl
   }
   L.n_tasks = tasks;
   L.n_recs = recs;
+  L.kernel = FJ_KERNEL_EPSILON;
   // one task per wave per workgroup, longer chunks when there is more work
   // than about FJ_BS_TARGET_BLOCKS workgroups
   int per_chunk = FJ_BS_WAVES;
@@ -331,6 +213,7 @@ extern "C" int fj_pair_sum_lattices(const double* seg, int n_lat, const fj_latti
   kp.eps = eps; kp.eps2 = eps * eps; kp.rcut = rcut;
   double* part = (double*)workspace;
   double* tab = (double*)((char*)workspace + fj_lat_part_bytes(p));
+  p.list.kernel = kernel;
   hipLaunchKernelGGL(k_lattice_records, dim3(fj_cdiv((long long)p.list.n_recs * FJ_LAT_REC, 256)),
                      dim3(256), 0, st, seg, p.list, tab);
   FJ_CHECK_LAUNCH();

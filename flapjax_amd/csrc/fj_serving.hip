@@ -376,6 +376,10 @@ static int fj_tick_layout(int n_cases, const fj_step_case* cases, int free_wake,
       d.n_tgt = (free_wake && K.m_star > 0) ? K.m_star * (K.n + 1) : 0;
       d.seg_off = L.roll_rows; d.n_seg = s_b + s_w;
       d.mode = FJ_OUT_CONVECT; d.dt = K.dt; d.time = K.time;
+      // sources: the bound and the wake lattice at n-1
+      d.n_lat = 2;
+      d.lat_off[0] = L.roll_rows; d.lat_m[0] = K.m; d.lat_n[0] = K.n;
+      d.lat_off[1] = L.roll_rows + s_b; d.lat_m[1] = K.m_star; d.lat_n[1] = K.n;
     }
     L.geo_tot += V + nb + s_b + K.m_star * K.n + (K.n + 1) + (free_wake ? 0 : K.m_star * (K.n + 1));
     L.roll_tot += s_b + s_w;
@@ -401,10 +405,17 @@ static void fj_tick_tasks(int n_cases, const fj_step_case* cases, const FjCaseDe
     u.tgt = c + 3 * (size_t)C.p_off; u.out = vw + 3 * (size_t)C.p_off;
     u.n_tgt = C.nb; u.seg_off = C.seg_frc + C.s_b; u.n_seg = C.s_w;
     u.mode = FJ_OUT_STORE; u.dt = 0.0; u.time = cases[i].time;
+    // sources: the wake lattice at n
+    u.n_lat = 1;
+    u.lat_off[0] = C.seg_frc + C.s_b; u.lat_m[0] = cases[i].m_star; u.lat_n[0] = cases[i].n;
     FjTaskDesc& f = frc[i];
     f.tgt = mids + 3 * (size_t)C.mid_off; f.out = vmid + 3 * (size_t)C.mid_off;
     f.n_tgt = C.s_b; f.seg_off = C.seg_frc; f.n_seg = C.s_b + C.s_w;
     f.mode = FJ_OUT_STORE; f.dt = 0.0; f.time = cases[i].time;
+    // sources: the bound and the wake lattice at n
+    f.n_lat = 2;
+    f.lat_off[0] = C.seg_frc; f.lat_m[0] = cases[i].m; f.lat_n[0] = cases[i].n;
+    f.lat_off[1] = C.seg_frc + C.s_b; f.lat_m[1] = cases[i].m_star; f.lat_n[1] = cases[i].n;
   }
 }
 
@@ -421,9 +432,16 @@ static void fj_tick_offsets(FjTickLayout& L, const FjTaskDesc* roll, const FjTas
   L.off_vmid = take(3 * sizeof(double) * (size_t)L.mid_tot);
   L.off_seg_roll = take(FJ_SEG_STRIDE * sizeof(double) * (size_t)L.roll_rows);
   L.off_seg_frc = take(FJ_SEG_STRIDE * sizeof(double) * (size_t)L.frc_rows);
-  L.off_ws_roll = take(fj_batched_ws_bytes(roll, n_cases));
-  L.off_ws_up = take(fj_batched_ws_bytes(up, n_cases));
-  L.off_ws_frc = take(fj_batched_ws_bytes(frc, n_cases));
+  // the rollup and force pair sums run in either form (FP64 lattice or
+  // mixed table), the upwash always in the FP64 lattice form
+  auto either = [&](const FjTaskDesc* d) {
+    const size_t a = fj_batched_ws_bytes(d, n_cases, FJ_PREC_FP64);
+    const size_t b = fj_batched_ws_bytes(d, n_cases, FJ_PREC_MIXED);
+    return a > b ? a : b;
+  };
+  L.off_ws_roll = take(either(roll));
+  L.off_ws_up = take(fj_batched_ws_bytes(up, n_cases, FJ_PREC_FP64));
+  L.off_ws_frc = take(either(frc));
   L.total = o;
 }
 
