@@ -33,12 +33,12 @@ N_TSTEP = 200
 ALPHA0, PITCH, PLUNGE, K_RED = 5.0, 3.0, 0.05, 0.2
 F_PAIR = 57  # algorithmic FLOP per pair interaction
 PEAK_FP64 = 78.6  # TFLOP/s, MI355X FP64 vector (datasheet)
-# HBM bytes per rollup launch (partial + reduce kernels), from rocprofv3
-# --pmc FETCH_SIZE and WRITE_SIZE runs of `bench.py --profile-rollup`, each
-# scaled by the same counter on a 256 MiB device copy (FETCH_SIZE reads 0.5x
-# the copied bytes on MI355X, WRITE_SIZE 1.0x); re-measured whenever the
-# pair-sum launch shape changes.
-ROLLUP_TRAFFIC_BYTES = 36_979_000
+# HBM bytes per FP64 rollup launch (records, partial and reduce kernels of
+# the lattice pair sum), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs
+# of `bench.py --profile-rollup`, each scaled by the same counter on a
+# 256 MiB device copy (FETCH_SIZE reads 0.5x the copied bytes on MI355X,
+# WRITE_SIZE 1.0x); re-measured whenever the pair-sum launch shape changes.
+ROLLUP_TRAFFIC_BYTES = 47_463_000
 
 
 def n_seg(m, n):

@@ -21,7 +21,7 @@
 #include <vector>
 
 #include "fj_common.hpp"
-#include "fj_aic_dev.hpp"
+#include "fj_lattice_dev.hpp"
 #include "fj_refine_dev.hpp"
 #include "fj_segments_dev.hpp"
 #include "fj_stencil_dev.hpp"
@@ -32,7 +32,7 @@ struct FjCaseDev {
   int p_off, mid_off;   // first panel / filament midpoint in the packed arrays
   int seg_roll, seg_frc;  // first row of the case's rollup / force filament tables
   // first work item (or workgroup) of the case in each batched kernel
-  int geo_off, roll_off, wk_off, bnd_off, aic_off, rhs_off, rblk_off, frc_off;
+  int geo_off, roll_off, wk_off, bnd_off, rhs_off, rblk_off, frc_off;
 };
 
 template <int FjCaseDev::*F>
@@ -169,25 +169,96 @@ __global__ void k_bt_segments(const FjCaseDev* __restrict__ cs, int ncase, int t
   }
 }
 
-// ---- AIC of every case (bound rings, no mirror, no wake lumping) ---------------
+// ---- AIC of every case (bound rings, no mirror, no wake lumping), by lattice
+// rows: one wave per (case, collocation point) walks the
+// bound lattice row by row, a lane per vertex column (64-column chunks
+// overlapping by one column), keeping its vertex of the current row
+// (r, 1/|r|) in registers: per row it evaluates the vertex below, the
+// chordwise filament between the two (same lane) and the spanwise filament
+// to the right neighbour's vertex (a wave shuffle), each once, and combines
+// them into the ring columns as the reference does,
+// A[t, (i, j)] = -(M[i, j+1] - M[i, j]) + (N[i+1, j] - N[i, j])
+// (aero/aic.py:55-57): ~1 vertex and 2 filament evaluations per ring
+// instead of 4 filament evaluations, no barriers.
+#define FJ_AIC_WAVES 4
+
 template <int KIND>
-__global__ __launch_bounds__(64) void k_bt_aic(const FjCaseDev* __restrict__ cs, int ncase,
-                                               const double* __restrict__ c,
-                                               const double* __restrict__ nc,
-                                               FjKernelParams kp) {
-  const int w = blockIdx.x;
-  const FjCaseDev& C = cs[fj_case_of<&FjCaseDev::aic_off>(cs, ncase, w)];
+__device__ __forceinline__ double fj_aic_fil(const FjVtx& A, const FjVtx& B,
+                                             const double* __restrict__ ya,
+                                             const double* __restrict__ yb, double nx, double ny,
+                                             double nz, double eps, double e2, double rcut2) {
+  const double r0x = yb[0] - ya[0], r0y = yb[1] - ya[1], r0z = yb[2] - ya[2];
+  const double cx = A.ry * B.rz - A.rz * B.ry;
+  const double cy = A.rz * B.rx - A.rx * B.rz;
+  const double cz = A.rx * B.ry - A.ry * B.rx;
+  const double dx = fma(A.rx, A.inv, -(B.rx * B.inv));
+  const double dy = fma(A.ry, A.inv, -(B.ry * B.inv));
+  const double dz = fma(A.rz, A.inv, -(B.rz * B.inv));
+  const double a = r0x * dx + r0y * dy + r0z * dz;
+  const double r0sq = r0x * r0x + r0y * r0y + r0z * r0z;
+  const double D = fma(cz, cz, fma(cy, cy, fma(cx, cx, KIND == FJ_KERNEL_EPSILON ? eps * r0sq * r0sq : 0.0)));
+  double w = __builtin_amdgcn_rcp(D);
+  const double e = fma(-D, w, 1.0);
+  w = fma(w, fma(e, e, e), w);
+  double q = a * w * FJ_INV_4PI;
+  if (KIND == FJ_KERNEL_CUTOFF) {
+    const double a1 = r0x * A.rx + r0y * A.ry + r0z * A.rz;
+    const double rad2 = (A.s - e2) - a1 * a1 / r0sq;
+    q = (rad2 > rcut2) ? q : 0.0;
+  }
+  return (cx * nx + cy * ny + cz * nz) * q;
+}
+
+__device__ __forceinline__ FjVtx fj_shfl_down1(const FjVtx& v) {
+  FjVtx o;
+  o.rx = __shfl_down(v.rx, 1);
+  o.ry = __shfl_down(v.ry, 1);
+  o.rz = __shfl_down(v.rz, 1);
+  o.inv = __shfl_down(v.inv, 1);
+  o.s = __shfl_down(v.s, 1);
+  return o;
+}
+
+template <int KIND>
+__global__ __launch_bounds__(64 * FJ_AIC_WAVES) void k_bt_aic_waves(
+    const FjCaseDev* __restrict__ cs, int ncase, int nb_tot, const double* __restrict__ c,
+    const double* __restrict__ nc, FjKernelParams kp) {
+  const int lane = threadIdx.x & 63;
+  const int g = blockIdx.x * FJ_AIC_WAVES + (threadIdx.x >> 6);  // packed collocation point
+  if (g >= nb_tot) return;  // whole wave
+  const FjCaseDev& C = cs[fj_case_of<&FjCaseDev::rhs_off>(cs, ncase, g)];
   const fj_step_case& K = C.c;
-  const int nrb = fj_cdiv(C.nb, 64);
-  const int local = w - C.aic_off;
-  const int t = local / nrb;
-  const int r = (local % nrb) * 64 + threadIdx.x;
-  if (r >= C.nb) return;
+  const int t = g - C.rhs_off;
   const size_t o = 3 * (size_t)(C.p_off + t);
-  const double mp[3] = {0, 0, 0}, mn[3] = {0, 0, 0};
-  K.aic[(size_t)t * C.nb + r] = fj_ring<KIND>(c[o], c[o + 1], c[o + 2], nc[o], nc[o + 1],
-                                              nc[o + 2], K.zeta_b, K.n, r / K.n, r % K.n, 0,
-                                              mp, mn, kp);
+  const double x = c[o], y = c[o + 1], zt = c[o + 2];
+  const double nx = nc[o], ny = nc[o + 1], nz = nc[o + 2];
+  const int m = K.m, n = K.n;
+  const double* zb = K.zeta_b;
+  const double e2 = (KIND == FJ_KERNEL_PLAIN) ? 0.0 : kp.eps2;
+  const double rcut2 = kp.rcut * kp.rcut;
+  double* arow = K.aic + (size_t)t * C.nb;
+  for (int j0 = 0; j0 < n; j0 += 63) {
+    // lane -> vertex column j (clamped: lanes past the last column repeat it
+    // and write nothing)
+    const int j = min(j0 + lane, n);
+    const bool ring = lane < 63 && j0 + lane < n;  // ring column j of this chunk
+    const double* ya = zb + 3 * j;                   // vertex (0, j)
+    FjVtx T = fj_lat_vertex(ya, x, y, zt, e2);
+    FjVtx Tr = fj_shfl_down1(T);
+    double n_top = ring ? fj_aic_fil<KIND>(T, Tr, ya, ya + 3, nx, ny, nz, kp.eps, e2, rcut2) : 0.0;
+    for (int i = 0; i < m; ++i) {
+      const double* yb = ya + 3 * (size_t)(n + 1);  // vertex (i + 1, j)
+      const FjVtx B = fj_lat_vertex(yb, x, y, zt, e2);
+      const double mj = fj_aic_fil<KIND>(T, B, ya, yb, nx, ny, nz, kp.eps, e2, rcut2);
+      const FjVtx Br = fj_shfl_down1(B);
+      const double n_bot = ring ? fj_aic_fil<KIND>(B, Br, yb, yb + 3, nx, ny, nz, kp.eps, e2, rcut2) : 0.0;
+      const double mj1 = __shfl_down(mj, 1);
+      if (ring) arow[i * n + j] = -(mj1 - mj) + (n_bot - n_top);
+      T = B;
+      n_top = n_bot;
+      ya = yb;
+    }
+  }
 }
 
 // ---- right-hand sides (aero/uvlm.py:961-985) -------------------------------------
@@ -338,7 +409,7 @@ static size_t fj_align256(size_t b) { return (b + 255) / 256 * 256; }
 
 struct FjTickLayout {
   int nb_tot = 0, mid_tot = 0, roll_rows = 0, frc_rows = 0;
-  int geo_tot = 0, roll_tot = 0, wk_tot = 0, bnd_tot = 0, aic_tot = 0, rblk_tot = 0, frc_tot = 0;
+  int geo_tot = 0, roll_tot = 0, wk_tot = 0, bnd_tot = 0, rblk_tot = 0, frc_tot = 0;
   size_t ws_roll = 0, ws_up = 0, ws_frc = 0;
   size_t off_c, off_nc, off_cd, off_rhs, off_vw, off_mid, off_vmid, off_seg_roll, off_seg_frc;
   size_t off_ws_roll, off_ws_up, off_ws_frc, total;
@@ -367,7 +438,7 @@ static int fj_tick_layout(int n_cases, const fj_step_case* cases, int free_wake,
       C.p_off = L.nb_tot; C.mid_off = L.mid_tot;
       C.seg_roll = L.roll_rows; C.seg_frc = L.frc_rows;
       C.geo_off = L.geo_tot; C.roll_off = L.roll_tot; C.wk_off = L.wk_tot; C.bnd_off = L.bnd_tot;
-      C.aic_off = L.aic_tot; C.rhs_off = L.nb_tot; C.rblk_off = L.rblk_tot; C.frc_off = L.frc_tot;
+      C.rhs_off = L.nb_tot; C.rblk_off = L.rblk_tot; C.frc_off = L.frc_tot;
     }
     if (roll) {
       FjTaskDesc& d = roll[i];
@@ -385,7 +456,6 @@ static int fj_tick_layout(int n_cases, const fj_step_case* cases, int free_wake,
     L.roll_tot += s_b + s_w;
     L.wk_tot += s_w;
     L.bnd_tot += s_b;
-    L.aic_tot += nb * fj_cdiv(nb, 64);
     L.rblk_tot += fj_cdiv(nb, FJ_RV_WAVES);
     L.frc_tot += V + nb;
     L.nb_tot += nb;
@@ -558,15 +628,18 @@ extern "C" int fj_step_batched(int n_cases, const fj_step_case* cases, const fj_
                          ws + L.off_ws_up, st);
   if (rc != FJ_OK) return rc;
   // 4. AIC, right-hand sides, refinement
-  if (kernel == FJ_KERNEL_EPSILON)
-    hipLaunchKernelGGL(k_bt_aic<FJ_KERNEL_EPSILON>, dim3(L.aic_tot), dim3(64), 0, st, dcs, n_cases,
-                       c, nc, kp);
-  else if (kernel == FJ_KERNEL_CUTOFF)
-    hipLaunchKernelGGL(k_bt_aic<FJ_KERNEL_CUTOFF>, dim3(L.aic_tot), dim3(64), 0, st, dcs, n_cases,
-                       c, nc, kp);
-  else
-    hipLaunchKernelGGL(k_bt_aic<FJ_KERNEL_PLAIN>, dim3(L.aic_tot), dim3(64), 0, st, dcs, n_cases, c,
-                       nc, kp);
+  {
+    const dim3 grid(fj_cdiv(L.nb_tot, FJ_AIC_WAVES)), blk(64 * FJ_AIC_WAVES);
+    if (kernel == FJ_KERNEL_EPSILON)
+      hipLaunchKernelGGL(k_bt_aic_waves<FJ_KERNEL_EPSILON>, grid, blk, 0, st, dcs, n_cases, L.nb_tot,
+                         c, nc, kp);
+    else if (kernel == FJ_KERNEL_CUTOFF)
+      hipLaunchKernelGGL(k_bt_aic_waves<FJ_KERNEL_CUTOFF>, grid, blk, 0, st, dcs, n_cases, L.nb_tot,
+                         c, nc, kp);
+    else
+      hipLaunchKernelGGL(k_bt_aic_waves<FJ_KERNEL_PLAIN>, grid, blk, 0, st, dcs, n_cases, L.nb_tot,
+                         c, nc, kp);
+  }
   FJ_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_bt_rhs, grid_of(L.nb_tot), dim3(FJ_SV_EW), 0, st, dcs, n_cases, L.nb_tot,
                      fl, c, nc, cd, vw, rhs);

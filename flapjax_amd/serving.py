@@ -295,11 +295,14 @@ class ServingEngine:
         self._keep = (table, ws)
 
     # ---- one engine tick: every active case advances one step ----------------
-    def _tick(self, slots):
-        """One time step of every active case in one native call
+    def _tick(self, slots, parity):
+        """Enqueue one time step of every active case in one native call
         (fj_step_batched): geometry, wake shift and rollup, upwash, AIC,
         right-hand side and kept-inverse refinement, forces and lift of all
-        cases, each stage one launch over all cases."""
+        cases, each stage one launch over all cases.  Returns (event, host
+        status, slots); nothing waits for the device here.  ``parity``
+        selects one of two host staging / status buffers: the tick enqueued
+        two ticks earlier has completed when this one is built."""
         nc = len(slots)
         cases = (L.FjStepCase * nc)()
         for k, sl in enumerate(slots):
@@ -325,7 +328,7 @@ class ServingEngine:
         tb = lib.fj_step_batched_table_bytes(nc)
         ws = self._grow("ws", wsb)
         table = self._grow("table", tb)
-        staging = self._grow("staging", tb, pinned=True)
+        staging = self._grow(f"staging{parity}", tb, pinned=True)
         fd = self.flow.descriptor()
         prec = L.PREC_MIXED if self.precision == "mixed" else L.PREC_FP64
         rc = lib.fj_step_batched(nc, cases, ctypes.byref(fd), self.flow.rho, 0.7, self.kind, EPSILON, R_CUTOFF,
@@ -337,9 +340,17 @@ class ServingEngine:
             sl.gw, sl.gw_next = sl.gw_next, sl.gw
             sl.i += 1
         status = torch.cat([sl.iters for sl in slots])
-        status_host = self._grow("status", 4 * nc, pinned=True)[:4 * nc].view(torch.int32)
+        status_host = self._grow(f"status{parity}", 4 * nc, pinned=True)[:4 * nc].view(torch.int32)
         status_host.copy_(status, non_blocking=True)
-        torch.cuda.current_stream().synchronize()
+        ev = torch.cuda.Event()
+        ev.record()
+        return ev, status_host, list(slots)
+
+    @staticmethod
+    def _check(tick):
+        """Wait for an enqueued tick and check every case's stop test."""
+        ev, status_host, slots = tick
+        ev.synchronize()
         bad = [sl.req.case_id for sl, it in zip(slots, status_host.tolist()) if it < 0]
         if bad:
             raise L.FlapjaxLibError(f"circulation solve failed its stop test for cases {bad}")
@@ -372,7 +383,8 @@ class ServingEngine:
         if self._adm is None:
             self._adm = torch.cuda.Stream(device=self.dev)
         adm = self._adm
-        while pending or active or warming:
+        inflight = None  # the enqueued tick not yet waited for, and the cases it finishes
+        while pending or active or warming or inflight is not None:
             now = time_source() - t0
             # admission: arrived, slot free, footprint fits.  A new case's
             # set-up (grids, AIC, factorisation and inverse, static solve) is
@@ -385,7 +397,7 @@ class ServingEngine:
             # them has been synchronised.
             while pending and pending[0].arrival <= now and len(active) + len(warming) < self.max_slots:
                 fp = footprint_bytes(pending[0], self.precision)
-                if used + fp > self.budget and (active or warming):
+                if used + fp > self.budget and (active or warming or inflight is not None):
                     break
                 req = pending.pop(0)
                 adm.wait_stream(main)
@@ -398,34 +410,40 @@ class ServingEngine:
                 used += fp
                 peak = max(peak, used)
                 warming.append(sl)
-            if warming and not active:
+            if warming and not active and inflight is None:
                 warming[0].ready.synchronize()
             for sl in [w for w in warming if w.ready.query()]:
                 main.wait_event(sl.ready)
                 warming.remove(sl)
                 active.append(sl)
-            if not active:
-                if not warming:
-                    wait = pending[0].arrival - (time_source() - t0)
-                    if wait > 0:
-                        time.sleep(min(wait, 0.01))
-                continue
-            self._tick(active)
-            ticks += 1
-            now = time_source() - t0
-            still = []
-            for sl in active:
-                r = sl.req
-                s_b, s_w = sl.s_b, sl.s_w
-                pairs += (r.m_star * (r.n + 1) * (s_b + s_w) if self.free_wake else 0) + r.m * r.n * s_w + \
-                    s_b * (s_b + s_w)
-                if sl.done:
+            # ticks are pipelined: tick k is enqueued before the host waits
+            # for tick k-1, so the device always has the next tick queued
+            # while the host checks the previous one and admits cases
+            cur = None
+            if active:
+                tick = self._tick(active, ticks % 2)
+                ticks += 1
+                for sl in active:
+                    r = sl.req
+                    s_b, s_w = sl.s_b, sl.s_w
+                    pairs += (r.m_star * (r.n + 1) * (s_b + s_w) if self.free_wake else 0) + \
+                        r.m * r.n * s_w + s_b * (s_b + s_w)
+                cur = (tick, [sl for sl in active if sl.done])
+                active = [sl for sl in active if not sl.done]
+            if inflight is not None:
+                tick, finished = inflight
+                self._check(tick)
+                now = time_source() - t0
+                for sl in finished:
+                    r = sl.req
                     results.append(CaseResult(r.case_id, r.arrival, sl.start, now, sl.lift.cpu().numpy(),
                                               sl.gb.cpu().numpy(), sl.zw.cpu().numpy()))
                     used -= sl.fp
-                else:
-                    still.append(sl)
-            active = still
+            inflight = cur
+            if cur is None and not active and not warming and pending:
+                wait = pending[0].arrival - (time_source() - t0)
+                if wait > 0:
+                    time.sleep(min(wait, 0.01))
             if log is not None and ticks % 50 == 0:
                 log(f"tick {ticks}: active {len(active)}, pending {len(pending)}, done {len(results)}")
         makespan = time_source() - t0
