@@ -32,7 +32,7 @@ struct FjCaseDev {
   int p_off, mid_off;   // first panel / filament midpoint in the packed arrays
   int seg_roll, seg_frc;  // first row of the case's rollup / force filament tables
   // first work item (or workgroup) of the case in each batched kernel
-  int geo_off, roll_off, wk_off, bnd_off, rhs_off, rblk_off, frc_off;
+  int geo_off, roll_off, wk_off, bnd_off, aic_off, rhs_off, rblk_off, frc_off;
 };
 
 template <int FjCaseDev::*F>
@@ -169,18 +169,19 @@ __global__ void k_bt_segments(const FjCaseDev* __restrict__ cs, int ncase, int t
   }
 }
 
-// ---- AIC of every case (bound rings, no mirror, no wake lumping), by lattice
-// rows: one wave per (case, collocation point) walks the
-// bound lattice row by row, a lane per vertex column (64-column chunks
-// overlapping by one column), keeping its vertex of the current row
-// (r, 1/|r|) in registers: per row it evaluates the vertex below, the
-// chordwise filament between the two (same lane) and the spanwise filament
-// to the right neighbour's vertex (a wave shuffle), each once, and combines
-// them into the ring columns as the reference does,
+// ---- AIC of every case (bound rings, no mirror, no wake lumping), by column
+// strips: one thread per (case, collocation point, block of FJ_AIC_W ring
+// columns) walks the bound lattice row by row, keeping the per-vertex terms
+// (r, 1/|r|) of its FJ_AIC_W + 1 vertices of the current row and the
+// spanwise filaments between them in registers; per row it evaluates the
+// vertices below, the chordwise filaments between the rows and the spanwise
+// filaments of the new row, each once, and combines them into the ring
+// columns as the reference does,
 // A[t, (i, j)] = -(M[i, j+1] - M[i, j]) + (N[i+1, j] - N[i, j])
-// (aero/aic.py:55-57): ~1 vertex and 2 filament evaluations per ring
-// instead of 4 filament evaluations, no barriers.
-#define FJ_AIC_WAVES 4
+// (aero/aic.py:55-57): (W+1) vertex and 2W+1 filament evaluations per W
+// rings and row instead of 4W filament evaluations; no barriers, no shuffles.
+#define FJ_AIC_W 2
+#define FJ_AIC_THREADS 256
 
 template <int KIND>
 __device__ __forceinline__ double fj_aic_fil(const FjVtx& A, const FjVtx& B,
@@ -209,55 +210,59 @@ __device__ __forceinline__ double fj_aic_fil(const FjVtx& A, const FjVtx& B,
   return (cx * nx + cy * ny + cz * nz) * q;
 }
 
-__device__ __forceinline__ FjVtx fj_shfl_down1(const FjVtx& v) {
-  FjVtx o;
-  o.rx = __shfl_down(v.rx, 1);
-  o.ry = __shfl_down(v.ry, 1);
-  o.rz = __shfl_down(v.rz, 1);
-  o.inv = __shfl_down(v.inv, 1);
-  o.s = __shfl_down(v.s, 1);
-  return o;
-}
-
 template <int KIND>
-__global__ __launch_bounds__(64 * FJ_AIC_WAVES) void k_bt_aic_waves(
-    const FjCaseDev* __restrict__ cs, int ncase, int nb_tot, const double* __restrict__ c,
+__global__ __launch_bounds__(FJ_AIC_THREADS) void k_bt_aic_strips(
+    const FjCaseDev* __restrict__ cs, int ncase, int total, const double* __restrict__ c,
     const double* __restrict__ nc, FjKernelParams kp) {
-  const int lane = threadIdx.x & 63;
-  const int g = blockIdx.x * FJ_AIC_WAVES + (threadIdx.x >> 6);  // packed collocation point
-  if (g >= nb_tot) return;  // whole wave
-  const FjCaseDev& C = cs[fj_case_of<&FjCaseDev::rhs_off>(cs, ncase, g)];
+  const int item = blockIdx.x * blockDim.x + threadIdx.x;
+  if (item >= total) return;
+  const FjCaseDev& C = cs[fj_case_of<&FjCaseDev::aic_off>(cs, ncase, item)];
   const fj_step_case& K = C.c;
-  const int t = g - C.rhs_off;
+  const int m = K.m, n = K.n;
+  const int blocks = fj_cdiv(n, FJ_AIC_W);
+  const int local = item - C.aic_off;
+  const int t = local / blocks, j0 = (local - t * blocks) * FJ_AIC_W;
   const size_t o = 3 * (size_t)(C.p_off + t);
   const double x = c[o], y = c[o + 1], zt = c[o + 2];
   const double nx = nc[o], ny = nc[o + 1], nz = nc[o + 2];
-  const int m = K.m, n = K.n;
   const double* zb = K.zeta_b;
   const double e2 = (KIND == FJ_KERNEL_PLAIN) ? 0.0 : kp.eps2;
   const double rcut2 = kp.rcut * kp.rcut;
+  // vertex columns j0 .. j0 + W (clamped to n: columns past the lattice
+  // repeat the last one and their rings are not written)
+  int col[FJ_AIC_W + 1];
+#pragma unroll
+  for (int k = 0; k <= FJ_AIC_W; ++k) col[k] = min(j0 + k, n);
+  FjVtx T[FJ_AIC_W + 1];
+  double ntop[FJ_AIC_W];
+  const double* ya = zb;  // vertex row 0
+#pragma unroll
+  for (int k = 0; k <= FJ_AIC_W; ++k) T[k] = fj_lat_vertex(ya + 3 * col[k], x, y, zt, e2);
+#pragma unroll
+  for (int k = 0; k < FJ_AIC_W; ++k)
+    ntop[k] = fj_aic_fil<KIND>(T[k], T[k + 1], ya + 3 * col[k], ya + 3 * col[k + 1], nx, ny, nz,
+                               kp.eps, e2, rcut2);
   double* arow = K.aic + (size_t)t * C.nb;
-  for (int j0 = 0; j0 < n; j0 += 63) {
-    // lane -> vertex column j (clamped: lanes past the last column repeat it
-    // and write nothing)
-    const int j = min(j0 + lane, n);
-    const bool ring = lane < 63 && j0 + lane < n;  // ring column j of this chunk
-    const double* ya = zb + 3 * j;                   // vertex (0, j)
-    FjVtx T = fj_lat_vertex(ya, x, y, zt, e2);
-    FjVtx Tr = fj_shfl_down1(T);
-    double n_top = ring ? fj_aic_fil<KIND>(T, Tr, ya, ya + 3, nx, ny, nz, kp.eps, e2, rcut2) : 0.0;
-    for (int i = 0; i < m; ++i) {
-      const double* yb = ya + 3 * (size_t)(n + 1);  // vertex (i + 1, j)
-      const FjVtx B = fj_lat_vertex(yb, x, y, zt, e2);
-      const double mj = fj_aic_fil<KIND>(T, B, ya, yb, nx, ny, nz, kp.eps, e2, rcut2);
-      const FjVtx Br = fj_shfl_down1(B);
-      const double n_bot = ring ? fj_aic_fil<KIND>(B, Br, yb, yb + 3, nx, ny, nz, kp.eps, e2, rcut2) : 0.0;
-      const double mj1 = __shfl_down(mj, 1);
-      if (ring) arow[i * n + j] = -(mj1 - mj) + (n_bot - n_top);
-      T = B;
-      n_top = n_bot;
-      ya = yb;
+  for (int i = 0; i < m; ++i) {
+    const double* yb = ya + 3 * (size_t)(n + 1);  // vertex row i + 1
+    FjVtx B[FJ_AIC_W + 1];
+    double mc[FJ_AIC_W + 1];
+#pragma unroll
+    for (int k = 0; k <= FJ_AIC_W; ++k) {
+      B[k] = fj_lat_vertex(yb + 3 * col[k], x, y, zt, e2);
+      mc[k] = fj_aic_fil<KIND>(T[k], B[k], ya + 3 * col[k], yb + 3 * col[k], nx, ny, nz, kp.eps, e2,
+                               rcut2);
     }
+#pragma unroll
+    for (int k = 0; k < FJ_AIC_W; ++k) {
+      const double nbot = fj_aic_fil<KIND>(B[k], B[k + 1], yb + 3 * col[k], yb + 3 * col[k + 1], nx,
+                                           ny, nz, kp.eps, e2, rcut2);
+      if (j0 + k < n) arow[i * n + j0 + k] = -(mc[k + 1] - mc[k]) + (nbot - ntop[k]);
+      ntop[k] = nbot;
+    }
+#pragma unroll
+    for (int k = 0; k <= FJ_AIC_W; ++k) T[k] = B[k];
+    ya = yb;
   }
 }
 
@@ -409,7 +414,7 @@ static size_t fj_align256(size_t b) { return (b + 255) / 256 * 256; }
 
 struct FjTickLayout {
   int nb_tot = 0, mid_tot = 0, roll_rows = 0, frc_rows = 0;
-  int geo_tot = 0, roll_tot = 0, wk_tot = 0, bnd_tot = 0, rblk_tot = 0, frc_tot = 0;
+  int geo_tot = 0, roll_tot = 0, wk_tot = 0, bnd_tot = 0, aic_tot = 0, rblk_tot = 0, frc_tot = 0;
   size_t ws_roll = 0, ws_up = 0, ws_frc = 0;
   size_t off_c, off_nc, off_cd, off_rhs, off_vw, off_mid, off_vmid, off_seg_roll, off_seg_frc;
   size_t off_ws_roll, off_ws_up, off_ws_frc, total;
@@ -438,7 +443,7 @@ static int fj_tick_layout(int n_cases, const fj_step_case* cases, int free_wake,
       C.p_off = L.nb_tot; C.mid_off = L.mid_tot;
       C.seg_roll = L.roll_rows; C.seg_frc = L.frc_rows;
       C.geo_off = L.geo_tot; C.roll_off = L.roll_tot; C.wk_off = L.wk_tot; C.bnd_off = L.bnd_tot;
-      C.rhs_off = L.nb_tot; C.rblk_off = L.rblk_tot; C.frc_off = L.frc_tot;
+      C.aic_off = L.aic_tot; C.rhs_off = L.nb_tot; C.rblk_off = L.rblk_tot; C.frc_off = L.frc_tot;
     }
     if (roll) {
       FjTaskDesc& d = roll[i];
@@ -456,6 +461,7 @@ static int fj_tick_layout(int n_cases, const fj_step_case* cases, int free_wake,
     L.roll_tot += s_b + s_w;
     L.wk_tot += s_w;
     L.bnd_tot += s_b;
+    L.aic_tot += nb * fj_cdiv(K.n, FJ_AIC_W);
     L.rblk_tot += fj_cdiv(nb, FJ_RV_WAVES);
     L.frc_tot += V + nb;
     L.nb_tot += nb;
@@ -629,16 +635,16 @@ extern "C" int fj_step_batched(int n_cases, const fj_step_case* cases, const fj_
   if (rc != FJ_OK) return rc;
   // 4. AIC, right-hand sides, refinement
   {
-    const dim3 grid(fj_cdiv(L.nb_tot, FJ_AIC_WAVES)), blk(64 * FJ_AIC_WAVES);
+    const dim3 grid(fj_cdiv(L.aic_tot, FJ_AIC_THREADS)), blk(FJ_AIC_THREADS);
     if (kernel == FJ_KERNEL_EPSILON)
-      hipLaunchKernelGGL(k_bt_aic_waves<FJ_KERNEL_EPSILON>, grid, blk, 0, st, dcs, n_cases, L.nb_tot,
-                         c, nc, kp);
+      hipLaunchKernelGGL(k_bt_aic_strips<FJ_KERNEL_EPSILON>, grid, blk, 0, st, dcs, n_cases,
+                         L.aic_tot, c, nc, kp);
     else if (kernel == FJ_KERNEL_CUTOFF)
-      hipLaunchKernelGGL(k_bt_aic_waves<FJ_KERNEL_CUTOFF>, grid, blk, 0, st, dcs, n_cases, L.nb_tot,
-                         c, nc, kp);
+      hipLaunchKernelGGL(k_bt_aic_strips<FJ_KERNEL_CUTOFF>, grid, blk, 0, st, dcs, n_cases,
+                         L.aic_tot, c, nc, kp);
     else
-      hipLaunchKernelGGL(k_bt_aic_waves<FJ_KERNEL_PLAIN>, grid, blk, 0, st, dcs, n_cases, L.nb_tot,
-                         c, nc, kp);
+      hipLaunchKernelGGL(k_bt_aic_strips<FJ_KERNEL_PLAIN>, grid, blk, 0, st, dcs, n_cases,
+                         L.aic_tot, c, nc, kp);
   }
   FJ_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_bt_rhs, grid_of(L.nb_tot), dim3(FJ_SV_EW), 0, st, dcs, n_cases, L.nb_tot,
