@@ -302,6 +302,9 @@ static void fj_task_tiling(int n_tgt, int n_seg, int& tiles, int& chunks, int& c
 // lattice form of a task: column blocks of 16 chordwise-filament columns
 // (about 64 filaments per wave task), widened until the task has at most
 // about FJ_BS_TARGET_BLOCKS work items; depends on the task's own sizes only
+#ifndef FJ_BATCH_LAT_WIDTH
+#define FJ_BATCH_LAT_WIDTH 16
+#endif
 struct FjLatTiling {
   int tiles, chunks, width, n_ltasks, n_recs;
   int cb[FJ_TASK_MAX_LAT], ltask[FJ_TASK_MAX_LAT], rec[FJ_TASK_MAX_LAT];
@@ -314,7 +317,7 @@ static FjLatTiling fj_task_lattice_tiling(const FjTaskDesc& d) {
   for (int k = 0; k < FJ_TASK_MAX_LAT; ++k) L.cb[k] = L.ltask[k] = L.rec[k] = 0;
   int max_cols = 1;
   for (int k = 0; k < n_lat; ++k) max_cols = d.lat_n[k] + 1 > max_cols ? d.lat_n[k] + 1 : max_cols;
-  L.width = max_cols < 16 ? max_cols : 16;
+  L.width = max_cols < FJ_BATCH_LAT_WIDTH ? max_cols : FJ_BATCH_LAT_WIDTH;
   for (;;) {
     L.n_ltasks = 0;
     L.n_recs = 0;

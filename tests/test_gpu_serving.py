@@ -20,9 +20,11 @@ pytestmark = pytest.mark.gpu
 def _stream():
     from flapjax_amd.serving import CaseRequest
 
-    # the last case shares the first one's lattice and initial pose (and so
-    # its kept inverse) but not its motion, steps or wake
-    specs = [(4, 8, 12, 6), (8, 16, 9, 8), (4, 16, 15, 5), (8, 8, 7, 7), (6, 12, 11, 4), (4, 8, 10, 5)]
+    # the sixth case shares the first one's lattice and initial pose (and so
+    # its kept inverse) but not its motion, steps or wake; the last has an
+    # odd number of spanwise panels (a partial column block in the AIC)
+    specs = [(4, 8, 12, 6), (8, 16, 9, 8), (4, 16, 15, 5), (8, 8, 7, 7), (6, 12, 11, 4), (4, 8, 10, 5),
+             (3, 7, 9, 4)]
     return [CaseRequest(i, m, n, st, ms, 2.0 + i, 0.1 + 0.05 * i, arrival=0.002 * i)
             for i, (m, n, st, ms) in enumerate(specs)]
 
