@@ -12,6 +12,7 @@ import ctypes
 from ctypes import POINTER, c_double, c_int, c_int32, c_size_t, c_void_p
 from pathlib import Path
 
+import numpy as np
 import torch
 
 LIB_PATH = Path(__file__).resolve().parent / "libflapjax_amd.so"
@@ -128,6 +129,21 @@ class FjStepCase(ctypes.Structure):
             "hg", "hg_dot", "dof", "zeta0", "zeta_b_nm1", "zeta_b", "zeta_b_dot", "gamma_b_nm1", "gamma_w_nm1",
             "zeta_w_nm1", "gamma_w", "zeta_w", "aic", "aic_inv", "gamma_b", "gamma_b_dot", "f_steady", "lift",
             "refine_ws", "iters")]
+
+
+def _step_case_dtype():
+    """NumPy structured dtype with the layout of fj_step_case (pointers as
+    uint64), so a tick's case records are filled with array operations."""
+    names, formats, offsets = [], [], []
+    for name, ct in FjStepCase._fields_:
+        names.append(name)
+        formats.append(np.int32 if ct is c_int else np.float64 if ct is c_double else np.uint64)
+        offsets.append(getattr(FjStepCase, name).offset)
+    return np.dtype({"names": names, "formats": formats, "offsets": offsets,
+                     "itemsize": ctypes.sizeof(FjStepCase)})
+
+
+STEP_CASE_DTYPE = _step_case_dtype()
 
 
 class FjLatticeRef(ctypes.Structure):

@@ -237,8 +237,20 @@ class _Slot:
         K.static_wake_gamma(self.gb, ms, out=self.gw)
         self.gb_prev = self.gb.clone()
         self.ws = torch.empty(L.lib().fj_refine_workspace_bytes(m * n) // 8 + 1, dtype=F64, device=dev)
-        self.iters = torch.zeros(1, dtype=torch.int32, device=dev)
         self.fs = torch.empty((m + 1, n + 1, 3), dtype=F64, device=dev)
+        # (A, B) buffer pairs of the step state: step i writes into B when i
+        # is odd and reads the previous step from A, and the other way round
+        # when i is even; the step-0 state is in A
+        self.zb_pair = (self.zb, self.zb_prev)
+        self.gb_pair = (self.gb, self.gb_prev)
+        self.zw_pair = (self.zw, self.zw_next)
+        self.gw_pair = (self.gw, self.gw_next)
+
+    def final_state(self):
+        """(circulation, wake grid) of the last step: the step-i state is in
+        the second buffer of each pair when i is odd."""
+        k = self.i & 1
+        return self.gb_pair[k], self.zw_pair[k]
 
     @property
     def done(self):
@@ -260,6 +272,7 @@ class ServingEngine:
         self._staging = {}
         self._bufs = {}
         self._adm = None
+        self._plan = None
         self._cache = _InverseCache(inverse_cache_bytes) if inverse_cache_bytes > 0 else None
 
     # ---- fused pair sums ---------------------------------------------------
@@ -295,6 +308,39 @@ class ServingEngine:
         self._keep = (table, ws)
 
     # ---- one engine tick: every active case advances one step ----------------
+    def _tick_plan(self, slots):
+        """Case records of a set of active cases: the fields that stay fixed
+        while the set does, the two buffers of each (previous, current) pair,
+        the per-step strides, and the workspace and table sizes.  Rebuilt
+        when the set changes."""
+        nc = len(slots)
+        rec = np.zeros(nc, dtype=L.STEP_CASE_DTYPE)
+        col = lambda f: np.array([f(sl) for sl in slots], dtype=np.uint64)  # noqa: E731
+        rec["m"] = [sl.req.m for sl in slots]
+        rec["n"] = [sl.req.n for sl in slots]
+        rec["m_star"] = [sl.req.m_star for sl in slots]
+        rec["dt"] = [sl.dt for sl in slots]
+        for field, f in (("dof", lambda sl: sl.dof.data_ptr()), ("zeta0", lambda sl: sl.zeta0.data_ptr()),
+                         ("zeta_b_dot", lambda sl: sl.zbd.data_ptr()), ("aic", lambda sl: sl.A.data_ptr()),
+                         ("aic_inv", lambda sl: sl.Ainv.data_ptr()), ("gamma_b_dot", lambda sl: sl.gbd.data_ptr()),
+                         ("f_steady", lambda sl: sl.fs.data_ptr()), ("refine_ws", lambda sl: sl.ws.data_ptr())):
+            rec[field] = col(f)
+        iters = self._grow("iters", 4 * nc)  # one stop-test word per case of the tick
+        rec["iters"] = iters.data_ptr() + 4 * np.arange(nc, dtype=np.uint64)
+        plan = {
+            "key": tuple(id(sl) for sl in slots), "rec": rec, "iters": iters,
+            "dt": rec["dt"].copy(),
+            "hg": col(lambda sl: sl.hg.data_ptr()), "hgd": col(lambda sl: sl.hgd.data_ptr()),
+            "hg_stride": col(lambda sl: sl.hg.stride(0) * 8), "lift": col(lambda sl: sl.lift.data_ptr()),
+        }
+        for name in ("zb", "gb", "zw", "gw"):
+            plan[name] = (col(lambda sl, nm=name: getattr(sl, nm + "_pair")[0].data_ptr()),
+                          col(lambda sl, nm=name: getattr(sl, nm + "_pair")[1].data_ptr()))
+        lib = L.lib()
+        plan["ws"] = lib.fj_step_batched_workspace_bytes(nc, ctypes.c_void_p(rec.ctypes.data))
+        plan["table"] = lib.fj_step_batched_table_bytes(nc)
+        return plan
+
     def _tick(self, slots, parity):
         """Enqueue one time step of every active case in one native call
         (fj_step_batched): geometry, wake shift and rollup, upwash, AIC,
@@ -304,47 +350,47 @@ class ServingEngine:
         selects one of two host staging / status buffers: the tick enqueued
         two ticks earlier has completed when this one is built."""
         nc = len(slots)
-        cases = (L.FjStepCase * nc)()
-        for k, sl in enumerate(slots):
-            i = sl.i + 1
-            # (previous step, this step) buffer pairs
-            sl.zb_prev, sl.zb = sl.zb, sl.zb_prev
-            sl.gb_prev, sl.gb = sl.gb, sl.gb_prev
-            c = cases[k]
-            c.m, c.n, c.m_star = sl.req.m, sl.req.n, sl.req.m_star
-            c.dt, c.time = sl.dt, i * sl.dt
-            c.hg, c.hg_dot = sl.hg[i].data_ptr(), sl.hgd[i].data_ptr()
-            c.dof, c.zeta0 = sl.dof.data_ptr(), sl.zeta0.data_ptr()
-            c.zeta_b_nm1, c.zeta_b, c.zeta_b_dot = sl.zb_prev.data_ptr(), sl.zb.data_ptr(), sl.zbd.data_ptr()
-            c.gamma_b_nm1, c.gamma_b = sl.gb_prev.data_ptr(), sl.gb.data_ptr()
-            c.gamma_w_nm1, c.zeta_w_nm1 = sl.gw.data_ptr(), sl.zw.data_ptr()
-            c.gamma_w, c.zeta_w = sl.gw_next.data_ptr(), sl.zw_next.data_ptr()
-            c.aic, c.aic_inv = sl.A.data_ptr(), sl.Ainv.data_ptr()
-            c.gamma_b_dot, c.f_steady = sl.gbd.data_ptr(), sl.fs.data_ptr()
-            c.lift = sl.lift.data_ptr() + 8 * i
-            c.refine_ws, c.iters = sl.ws.data_ptr(), sl.iters.data_ptr()
+        plan = self._plan
+        if plan is None or plan["key"] != tuple(id(sl) for sl in slots):
+            plan = self._plan = self._tick_plan(slots)
+        rec = self._fill_records(plan, slots)
         lib = L.lib()
-        wsb = lib.fj_step_batched_workspace_bytes(nc, cases)
-        tb = lib.fj_step_batched_table_bytes(nc)
-        ws = self._grow("ws", wsb)
-        table = self._grow("table", tb)
-        staging = self._grow(f"staging{parity}", tb, pinned=True)
+        ws = self._grow("ws", plan["ws"])
+        table = self._grow("table", plan["table"])
+        staging = self._grow(f"staging{parity}", plan["table"], pinned=True)
         fd = self.flow.descriptor()
         prec = L.PREC_MIXED if self.precision == "mixed" else L.PREC_FP64
-        rc = lib.fj_step_batched(nc, cases, ctypes.byref(fd), self.flow.rho, 0.7, self.kind, EPSILON, R_CUTOFF,
-                                 prec, int(self.free_wake), 3, 0.0, L.ptr(table), ctypes.c_void_p(staging.data_ptr()),
-                                 L.ptr(ws), L.stream())
+        rc = lib.fj_step_batched(nc, ctypes.c_void_p(rec.ctypes.data), ctypes.byref(fd), self.flow.rho, 0.7,
+                                 self.kind, EPSILON, R_CUTOFF, prec, int(self.free_wake), 3, 0.0, L.ptr(table),
+                                 ctypes.c_void_p(staging.data_ptr()), L.ptr(ws), L.stream())
         L.check(rc, "fj_step_batched")
         for sl in slots:
-            sl.zw, sl.zw_next = sl.zw_next, sl.zw
-            sl.gw, sl.gw_next = sl.gw_next, sl.gw
             sl.i += 1
-        status = torch.cat([sl.iters for sl in slots])
         status_host = self._grow(f"status{parity}", 4 * nc, pinned=True)[:4 * nc].view(torch.int32)
-        status_host.copy_(status, non_blocking=True)
+        status_host.copy_(plan["iters"][:4 * nc].view(torch.int32), non_blocking=True)
         ev = torch.cuda.Event()
         ev.record()
         return ev, status_host, list(slots)
+
+    @staticmethod
+    def _fill_records(plan, slots):
+        """The per-step fields of the case records for the next step of
+        every case (time, pose, lift slot, the state buffer pairs' roles)."""
+        rec = plan["rec"]
+        i = np.fromiter((sl.i + 1 for sl in slots), dtype=np.uint64, count=len(slots))
+        odd = (i & np.uint64(1)).astype(bool)
+        rec["time"] = i.astype(np.float64) * plan["dt"]
+        rec["hg"] = plan["hg"] + i * plan["hg_stride"]
+        rec["hg_dot"] = plan["hgd"] + i * plan["hg_stride"]
+        rec["lift"] = plan["lift"] + np.uint64(8) * i
+        # buffer pairs (A, B): step i writes its state into B and reads the
+        # previous step's from A when i is odd, the other way round when even
+        for cur, prev, name in (("zeta_b", "zeta_b_nm1", "zb"), ("gamma_b", "gamma_b_nm1", "gb"),
+                                ("zeta_w", "zeta_w_nm1", "zw"), ("gamma_w", "gamma_w_nm1", "gw")):
+            a, b = plan[name]
+            rec[cur] = np.where(odd, b, a)
+            rec[prev] = np.where(odd, a, b)
+        return rec
 
     @staticmethod
     def _check(tick):
@@ -436,8 +482,9 @@ class ServingEngine:
                 now = time_source() - t0
                 for sl in finished:
                     r = sl.req
+                    gb, zw = sl.final_state()
                     results.append(CaseResult(r.case_id, r.arrival, sl.start, now, sl.lift.cpu().numpy(),
-                                              sl.gb.cpu().numpy(), sl.zw.cpu().numpy()))
+                                              gb.cpu().numpy(), zw.cpu().numpy()))
                     used -= sl.fp
             inflight = cur
             if cur is None and not active and not warming and pending:
