@@ -93,6 +93,43 @@ def make_case(precision, aic_reuse=True):
     return uv, hg, hgd
 
 
+def static_bench(reps=20):
+    """The steady case (8 x 32 rings, span 10, 5 deg pitch, 20 wake rows,
+    dt = c/(8U)): one static solve (geometry, AIC with the lumped wake,
+    factorisation and refined solve, forces) from a fresh solver per case,
+    event-timed over ``reps`` cases; pairs per case = AIC + forces."""
+    import torch
+
+    import flapjax_amd as fj
+    from flapjax_amd.uvlm import UVLM
+    from flapjax_amd.utils import make_rectangular_grid, to_device
+
+    m, n, ms = 8, 32, 20
+    hg = to_device(pitch_plunge(n, SPAN, 1, 1.0 / m)[0][0])
+
+    def case():
+        uv = UVLM([(m, n, ms)], np.arange(n + 1), free_wake=False)
+        uv.set_design_variables(1.0 / m, fj.Constant(np.array([1.0, 0.0, 0.0]), 1.225, True),
+                                make_rectangular_grid(m, n, 1.0, 0.25), hg)
+        return uv.solve_static(hg=hg)
+
+    case()
+    torch.cuda.synchronize()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    s.record()
+    for _ in range(reps):
+        case()
+    e.record()
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / reps
+    s_b, s_w = n_seg(m, n), n_seg(ms, n)
+    pairs = m * n * (s_b + s_w) + s_b * (s_b + s_w)
+    return {"workload": "steady rigid wing, 8x32 rings, 20 wake rows, 5 deg", "ms_per_case": round(1e3 * wall, 4),
+            "device_ms_per_case": round(s.elapsed_time(e) / reps, 4), "cases_per_s": round(1.0 / wall, 2),
+            "pairs_per_case": pairs}
+
+
 def time_form(precision, steps, warmup, aic_reuse=True, whole_case=True):
     import torch
 
@@ -414,6 +451,7 @@ def main():
                          "p99_s": round(st["p99_s"], 4), "gpairs_per_s": round(st["gpairs_per_s"], 3),
                          "makespan_s": round(st["makespan_s"], 3)}
         srv["mixed_speedup_vs_fp64"] = round(srv["mixed"]["cases_per_s"] / srv["fp64"]["cases_per_s"], 3)
+    steady = static_bench()
     sweep = pair_sweep()
     conv = convection_bench()
     cpu = None if args.no_cpu else cpu_baseline(sample)
@@ -453,6 +491,7 @@ def main():
                               f"{args.serve_steps[0]}-{args.serve_steps[1]} steps, wake rows = "
                               + (f"min(steps, {args.serve_wake_cap})" if args.serve_wake_cap else "steps")
                               + ", pitch 0-5 deg, k 0.05-0.5", **srv},
+        "steady_case": steady,
         "pair_sweep_gpairs_s": sweep,
         "convection": conv,
         "cpu_baseline": cpu,

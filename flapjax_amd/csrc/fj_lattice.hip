@@ -33,6 +33,15 @@
 #include "fj_common.hpp"
 #include "fj_lattice_dev.hpp"
 
+// plan: column blocks are halved until about FJ_LAT_WANT_WAVES wave tasks
+// exist for the launch's target tiles, but not below FJ_LAT_MIN_WIDTH
+// columns (each task re-evaluates its first column's vertices)
+#ifndef FJ_LAT_WANT_WAVES
+#define FJ_LAT_WANT_WAVES 32768
+#endif
+#ifndef FJ_LAT_MIN_WIDTH
+#define FJ_LAT_MIN_WIDTH 16
+#endif
 #ifndef FJ_LAT_MIN_WAVES
 #define FJ_LAT_MIN_WAVES 5
 #endif
@@ -141,14 +150,14 @@ static int fj_lat_plan(int n_tgt, int n_lat, const fj_lattice_ref* lats, FjLatPl
     if (lats[k].m < 0 || lats[k].n < 0 || lats[k].seg_off < 0) return FJ_ERR_ARG;
     if (lats[k].m > 0 && lats[k].n > 0 && lats[k].n + 1 > max_cols) max_cols = lats[k].n + 1;
   }
-  const long long want = (long long)FJ_BS_TARGET_BLOCKS * FJ_BS_WAVES / p.tiles;
+  const long long want = (long long)FJ_LAT_WANT_WAVES / p.tiles;
   int width = max_cols;
   for (;;) {
     long long tasks = 0;
     for (int k = 0; k < n_lat; ++k)
       if (lats[k].m > 0 && lats[k].n > 0)
         tasks += (long long)fj_cdiv(lats[k].m, FJ_LAT_ROWS) * fj_cdiv(lats[k].n + 1, width);
-    if (tasks >= want || width <= 8) break;
+    if (tasks >= want || width <= FJ_LAT_MIN_WIDTH) break;
     width = fj_cdiv(width, 2);
   }
   FjLatticeList& L = p.list;
