@@ -1,0 +1,777 @@
+"""FP64 CPU oracle for the geometrically exact beam and the aeroelastic
+coupling that calls the UVLM hot path (test infrastructure only).
+
+A plain NumPy restatement of the reference's SE(3) two-node beam
+(Sonneville et al. 2014 formulation), its generalised-alpha Lie-group time
+integrator, and the fluid-structure iteration of the coupled solver.  Every
+function cites the reference file:line it follows (paths relative to
+``src/flapjax/`` of ben-l-p/flapjax).  Scope: the reference's default
+``OptionalJacobians`` (all off, structure/data_structures.py:26-32), no
+gravity, no lumped masses, no thrust, no control surfaces.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU baseline
+may import this module.  The product path (``flapjax_amd``) never calls it.
+
+Pinning status (tests/test_oracle_beam.py):
+  * SO(3)/SE(3) maps: exp against the truncated series to order 30, log as
+    the inverse of exp, T(h) against central differences of log(exp(h)^-1
+    exp(h + e)), T^-1 T = I, both branches of every small-angle switch.
+  * P(d): central differences of d(H_A, H_B) under local perturbations.
+  * element forces: the reference's axial / torsional strain tests
+    (tests/structure/static_primal/test_multi_element_beam.py).
+  * static solve: the reference's Geradin beam golden values
+    (tests/structure/static_primal/test_geradin_beam.py, 1 % tolerance) and
+    the Euler-Bernoulli + Timoshenko-shear closed form for a small tip load.
+  * mass matrix: kinetic energy 1/2 v^T M v of rigid translations and
+    rotations equals the closed form of a uniform bar.
+  * q_dot: the complex-step derivative equals central differences.
+  * dynamics: the reference's rigid rotation test (constant angular velocity
+    about the beam axis, tests/structure/dynamic_primal/
+    test_const_initial_rotational_velocity.py) and rigid translation.
+  * coupling: force projection closed form (sum of forces and moments about
+    the node), and equilibrium persistence of a steady aeroelastic state
+    (tests/coupled/dynamic/test_dynamic_equilibrium.py).
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+# utils/constants.py:13
+SMALL_ANG_THRESH = 1e-10
+I3 = np.eye(3)
+I6 = np.eye(6)
+
+
+# ---------------------------------------------------------------------------
+# SO(3), algebra/so3.py.  Written with analytic operations only (no abs, no
+# conjugation) so that complex-step differentiation goes through them.
+# ---------------------------------------------------------------------------
+def _re(x):
+    return np.real(x)
+
+
+def skew(v):
+    """algebra/so3.py:vec_to_skew."""
+    z = 0.0 * v[0]
+    return np.array(((z, -v[2], v[1]), (v[2], z, -v[0]), (-v[1], v[0], z)))
+
+
+def unskew(m):
+    """algebra/so3.py:skew_to_vec (both entries of each pair)."""
+    return np.array((0.5 * (m[2, 1] - m[1, 2]), 0.5 * (m[0, 2] - m[2, 0]), 0.5 * (m[1, 0] - m[0, 1])))
+
+
+def _norm2(v):
+    return v[0] * v[0] + v[1] * v[1] + v[2] * v[2]
+
+
+def alpha_fn(b):
+    """algebra/so3.py:alpha (switch on |b|)."""
+    nb = np.sqrt(_norm2(b))
+    if _re(nb) > SMALL_ANG_THRESH:
+        return np.sin(nb) / nb
+    return 1.0 - nb * nb / 6.0
+
+
+def beta_fn(b):
+    """algebra/so3.py:beta (switch on |b|)."""
+    nb2 = _norm2(b)
+    nb = np.sqrt(nb2)
+    if _re(nb) > SMALL_ANG_THRESH:
+        return 2.0 * (1.0 - np.cos(nb)) / nb2
+    return 0.5 - nb2 / 24.0
+
+
+def t_so3(w):
+    """algebra/so3.py:t_so3 (switch on |w|^2); small branch = t_sum order 2."""
+    s = skew(w)
+    n2 = _norm2(w)
+    if _re(n2) > SMALL_ANG_THRESH:
+        return I3 - 0.5 * beta_fn(w) * s + (1.0 - alpha_fn(w)) / n2 * (s @ s)
+    return I3 - s / 2.0 + (s @ s) / 6.0
+
+
+def t_inv_so3(w):
+    """algebra/so3.py:t_inv_so3 (switch on |w|); small branch = t_inv_sum
+    order 2 with the Bernoulli numbers B1 = -1/2, B2 = 1/6."""
+    s = skew(w)
+    n2 = _norm2(w)
+    if _re(np.sqrt(n2)) > SMALL_ANG_THRESH:
+        return I3 + 0.5 * s + (1.0 - alpha_fn(w) / beta_fn(w)) / n2 * (s @ s)
+    return I3 + s / 2.0 + (s @ s) / 12.0
+
+
+def exp_so3(w):
+    """algebra/so3.py:exp_so3; small branch = exp_sum order 2."""
+    s = skew(w)
+    n2 = _norm2(w)
+    if _re(n2) > SMALL_ANG_THRESH:
+        a = np.sqrt(n2)
+        return I3 + np.sin(a) / a * s + (1.0 - np.cos(a)) / (a * a) * (s @ s)
+    return I3 + s + (s @ s) / 2.0
+
+
+def log_so3(r):
+    """algebra/so3.py:log_so3 (switch on cos(theta) < cos(thresh))."""
+    ct = 0.5 * (np.trace(r) - 1.0)
+    if _re(ct) < np.cos(SMALL_ANG_THRESH):
+        th = np.arccos(ct)
+        return unskew(th / (2.0 * np.sin(th)) * (r - r.T))
+    g = r - I3
+    return unskew(g - (g @ g) / 2.0)
+
+
+# ---------------------------------------------------------------------------
+# SE(3), algebra/se3.py.  Vectors are h = (u, omega).
+# ---------------------------------------------------------------------------
+def hat6(h):
+    """algebra/se3.py:ha_to_ha_hat."""
+    out = np.zeros((6, 6), dtype=np.result_type(h, float))
+    out[:3, :3] = skew(h[3:])
+    out[:3, 3:] = skew(h[:3])
+    out[3:, 3:] = skew(h[3:])
+    return out
+
+
+def check6(h):
+    """algebra/se3.py:ha_to_ha_check."""
+    out = np.zeros((6, 6), dtype=np.result_type(h, float))
+    out[:3, 3:] = -skew(h[:3])
+    out[3:, :3] = -skew(h[:3])
+    out[3:, 3:] = -skew(h[3:])
+    return out
+
+
+def tilde4(h):
+    """algebra/se3.py:ha_to_ha_tilde."""
+    out = np.zeros((4, 4), dtype=np.result_type(h, float))
+    out[:3, :3] = skew(h[3:])
+    out[:3, 3] = h[:3]
+    return out
+
+
+def t_u_omega_plus(h):
+    """algebra/se3.py:t_u_omega_plus (Sonneville 2013 eq. A.12)."""
+    a, b = h[:3], h[3:]
+    n2 = _norm2(b)
+    sa, sb = skew(a), skew(b)
+    if _re(n2) > SMALL_ANG_THRESH:
+        al, be = alpha_fn(b), beta_fn(b)
+        ba = b[0] * a[0] + b[1] * a[1] + b[2] * a[2]
+        return (-0.5 * be * sa + (1.0 - al) / n2 * (sa @ sb + sb @ sa)
+                + ba / n2 * ((be - al) * sb + (0.5 * be - 3.0 * (1.0 - al) / n2) * (sb @ sb)))
+    return -0.5 * sa
+
+
+def t_se3(h):
+    """algebra/se3.py:t_se3 (eq. A.11); small branch = t_sum(hat, 2)."""
+    w = h[3:]
+    if _re(_norm2(w)) > SMALL_ANG_THRESH:
+        t = t_so3(w)
+        out = np.zeros((6, 6), dtype=np.result_type(h, float))
+        out[:3, :3] = t
+        out[3:, 3:] = t
+        out[:3, 3:] = t_u_omega_plus(h)
+        return out
+    a = hat6(h)
+    return I6 - a / 2.0 + (a @ a) / 6.0
+
+
+def t_inv_se3(h):
+    """algebra/se3.py:t_inv_se3 (eq. A.13/A.14); small branch = t_inv_sum(hat, 2)."""
+    w = h[3:]
+    if _re(_norm2(w)) > SMALL_ANG_THRESH:
+        ti = t_inv_so3(w)
+        out = np.zeros((6, 6), dtype=np.result_type(h, float))
+        out[:3, :3] = ti
+        out[3:, 3:] = ti
+        out[:3, 3:] = -ti @ t_u_omega_plus(h) @ ti
+        return out
+    a = hat6(h)
+    return I6 + a / 2.0 + (a @ a) / 12.0
+
+
+def exp_se3(h):
+    """algebra/se3.py:exp_se3 (eq. A.10); small branch = exp_sum(tilde, 2)."""
+    w = h[3:]
+    if _re(_norm2(w)) > SMALL_ANG_THRESH:
+        out = np.zeros((4, 4), dtype=np.result_type(h, float))
+        out[:3, :3] = exp_so3(w)
+        out[:3, 3] = t_so3(w).T @ h[:3]
+        out[3, 3] = 1.0
+        return out
+    x = tilde4(h)
+    return np.eye(4) + x + (x @ x) / 2.0
+
+
+def log_se3(g):
+    """algebra/se3.py:log_se3 (eq. A.15); small branch = log_sum order 2."""
+    w = log_so3(g[:3, :3])
+    if _re(_norm2(w)) > SMALL_ANG_THRESH:
+        return np.concatenate((t_inv_so3(w).T @ g[:3, 3], w))
+    e = g - np.eye(4)
+    lg = e - (e @ e) / 2.0
+    return np.concatenate((lg[:3, 3], unskew(lg[:3, :3])))
+
+
+def hg_inv(g):
+    """algebra/se3.py:hg_inv."""
+    out = np.eye(4)
+    out[:3, :3] = g[:3, :3].T
+    out[:3, 3] = -(g[:3, :3].T @ g[:3, 3])
+    return out
+
+
+def hg_to_d(g1, g2):
+    """algebra/se3.py:hg_to_d (Sonneville 2013 eq. 56)."""
+    return log_se3(hg_inv(g1) @ g2)
+
+
+def chi(r):
+    """algebra/base.py:chi."""
+    out = np.zeros((6, 6))
+    out[:3, :3] = r
+    out[3:, 3:] = r
+    return out
+
+
+def p_mat(d, ad_inv):
+    """algebra/se3.py:p: P(d) = [-T^-1(-d) Ad, T^-1(d) Ad], (6, 12)."""
+    tp = t_inv_se3(d)
+    tn = tp - hat6(d)
+    return np.concatenate((-tn @ ad_inv, tp @ ad_inv), axis=1)
+
+
+def q_mat(s, d, ad_inv):
+    """algebra/se3.py:t_star and q: T* = s T(s d) T^-1(d); Q = [(I - T*) Ad, T* Ad]."""
+    ts = s * (t_se3(s * d) @ t_inv_se3(d))
+    return np.concatenate(((I6 - ts) @ ad_inv, ts @ ad_inv), axis=1)
+
+
+def q_dot_mat(s, d, d_dot, ad_inv, h=1e-30):
+    """algebra/se3.py:q_dot, the JVP of q along d_dot, by complex step."""
+    return np.imag(q_mat(s, d + 1j * h * d_dot, ad_inv)) / h
+
+
+# ---------------------------------------------------------------------------
+# Quadrature, algebra/integration.py (orders 3 = constants.py BASE_*_ORDER)
+# ---------------------------------------------------------------------------
+LOBATTO3_MID = 0.5
+LEGENDRE3 = (0.5 - 0.5 * np.sqrt(3.0 / 5.0), 0.5, 0.5 + 0.5 * np.sqrt(3.0 / 5.0))
+LEGENDRE3_W = (5.0 / 9.0, 8.0 / 9.0, 5.0 / 9.0)
+
+
+# ---------------------------------------------------------------------------
+# Convergence bookkeeping, utils/data_structures.py:23-170
+# ---------------------------------------------------------------------------
+@dataclass(frozen=True)
+class ConvergenceSettings:
+    max_n_iter: int | None
+    rel_disp_tol: float | None
+    abs_disp_tol: float | None
+    rel_force_tol: float | None
+    abs_force_tol: float | None
+
+
+DEFAULT_STRUCT = ConvergenceSettings(25, 1e-5, 1e-7, 1e-6, 1e-8)   # structure/beam.py:71-77
+DEFAULT_FSI = ConvergenceSettings(25, 1e-3, 1e-5, 1e-3, 1e-5)      # coupled/coupled.py:38-44
+
+
+def _nan_to_inf(x):
+    return np.inf if np.isnan(x) else (np.finfo(float).max if np.isposinf(x) else x)
+
+
+class ConvergenceStatus:
+    """utils/data_structures.py:32-170."""
+
+    def __init__(self, settings):
+        self.s = settings
+        self.reset()
+
+    def reset(self):
+        self.i_iter = 0
+        self.abs_disp = self.rel_disp = self.abs_force = self.rel_force = 0.0
+        self.converged = self.has_nan = self.final_iter = False
+        self.c_ad = self.c_rd = self.c_af = self.c_rf = False
+
+    def update(self, delta_disp, total_disp, delta_force, total_force):
+        s = self.s
+        self.i_iter += 1
+        if delta_disp is not None:
+            self.abs_disp = float(np.linalg.norm(delta_disp))
+            self.has_nan = bool(np.isnan(delta_disp).any())
+        if s.abs_disp_tol is not None:
+            self.c_ad = self.abs_disp < s.abs_disp_tol
+        if s.rel_disp_tol is not None:
+            with np.errstate(divide="ignore", invalid="ignore"):
+                self.rel_disp = np.float64(self.abs_disp) / np.float64(np.linalg.norm(total_disp))
+            self.c_rd = _nan_to_inf(self.rel_disp) < s.rel_disp_tol
+        if s.abs_force_tol is not None:
+            self.abs_force = float(np.linalg.norm(delta_force))
+            self.c_af = self.abs_force < s.abs_force_tol
+        if s.rel_force_tol is not None:
+            with np.errstate(divide="ignore", invalid="ignore"):
+                self.rel_force = np.float64(self.abs_force) / np.float64(np.abs(total_force).max())
+            self.c_rf = _nan_to_inf(self.rel_force) < s.rel_force_tol
+        self.converged = (self.c_rd or self.c_ad or self.c_rf or self.c_af) and self.i_iter > 0
+        if s.max_n_iter is not None:
+            self.final_iter = self.i_iter >= s.max_n_iter
+
+    def done(self):
+        return self.converged or self.has_nan or self.final_iter
+
+
+# ---------------------------------------------------------------------------
+# Generalised-alpha integrator, structure/time_integration.py:10-174
+# ---------------------------------------------------------------------------
+class TimeIntegrator:
+    def __init__(self, spectral_radius, dt):
+        r = float(spectral_radius)
+        self.dt = float(dt)
+        self.alpha_m = (2.0 * r - 1.0) / (r + 1.0)
+        self.alpha_f = r / (r + 1.0)
+        self.gamma = (3.0 - r) / (2.0 + 2.0 * r)
+        self.beta = 1.0 / ((r + 1.0) ** 2)
+        self.gamma_prime = self.gamma / (self.beta * self.dt)
+        self.beta_prime = (1.0 - self.alpha_m) / (self.beta * self.dt * self.dt * (1.0 - self.alpha_f))
+
+    def predict(self, v, v_dot, a):
+        """predict_q + calculate_q_alpha (time_integration.py:86-150)."""
+        am, af, g, b, dt = self.alpha_m, self.alpha_f, self.gamma, self.beta, self.dt
+        a_n = (af * v_dot - am * a) / (1.0 - am)
+        phi_n = dt * v + dt * dt * ((0.5 - b) * a + b * a_n)
+        v_n = v + (1.0 - g) * dt * a + g * dt * a_n
+        v_dot_n = ((1.0 - am) * a_n + am * a - af * v_dot) / (1.0 - af)
+        phi_alpha = (1.0 - af) * phi_n
+        v_alpha = (1.0 - af) * v_n + af * v
+        v_dot_alpha = (1.0 - af) * v_dot_n + af * v_dot
+        return phi_alpha, v_alpha, v_dot_alpha
+
+    def f_alpha(self, f_nm1, f_n):
+        return (1.0 - self.alpha_f) * f_n + self.alpha_f * f_nm1
+
+
+# ---------------------------------------------------------------------------
+# Beam, structure/beam.py + structure/utils.py
+# ---------------------------------------------------------------------------
+class BeamOracle:
+    """BaseBeamStructure (structure/beam.py:79-2990) for chains and trees of
+    two-node elements with the default optional Jacobians."""
+
+    def __init__(self, num_nodes, connectivity, y_vector, k_cs_index=None, m_cs_index=None,
+                 relaxation_factor=1.0, spectral_radius=0.9, settings=DEFAULT_STRUCT):
+        self.n_nodes = int(num_nodes)
+        self.n_dof = 6 * self.n_nodes
+        self.conn = np.asarray(connectivity, dtype=np.int64).reshape(-1, 2)
+        self.n_elem = self.conn.shape[0]
+        y = np.asarray(y_vector, dtype=float)
+        self.y_vector = np.broadcast_to(y.reshape(-1, 3), (self.n_elem, 3)).copy()
+        self.k_cs_index = np.zeros(self.n_elem, int) if k_cs_index is None else np.asarray(k_cs_index)
+        self.m_cs_index = np.zeros(self.n_elem, int) if m_cs_index is None else np.asarray(m_cs_index)
+        self.relaxation_factor = float(relaxation_factor)
+        self.spectral_radius = float(spectral_radius)
+        self.settings = settings
+
+    # structure/beam.py:322-530
+    def set_design_variables(self, coords, k_cs, m_cs=None):
+        self.x0 = np.asarray(coords, float)
+        k = np.asarray(k_cs, float)
+        self.k_cs = k[None] if k.ndim == 2 else k
+        m = np.zeros((6, 6)) if m_cs is None else np.asarray(m_cs, float)
+        self.m_cs = m[None] if m.ndim == 2 else m
+        dx = self.x0[self.conn[:, 1]] - self.x0[self.conn[:, 0]]
+        self.l0 = np.linalg.norm(dx, axis=-1)
+        self.d0 = np.zeros((self.n_elem, 6))
+        self.d0[:, 0] = self.l0
+        ex = dx / self.l0[:, None]
+        ez = np.cross(ex, self.y_vector)
+        ez = ez / np.linalg.norm(ez, axis=-1)[:, None]
+        ey = np.cross(ez, ex)
+        self.o0 = np.stack((ex, ey, ez), axis=-1)  # columns
+        self.ad_inv = np.stack([chi(o.T) for o in self.o0])
+        self.hg0 = np.broadcast_to(np.eye(4), (self.n_nodes, 4, 4)).copy()
+        self.hg0[:, :3, 3] = self.x0
+
+    def kmat(self, e):
+        return self.k_cs[self.k_cs_index[e]]
+
+    def mmat(self, e):
+        return self.m_cs[self.m_cs_index[e]]
+
+    # structure/beam.py:1482-1500
+    def make_d(self, hg):
+        out = np.zeros((self.n_elem, 6))
+        for e, (a, b) in enumerate(self.conn):
+            base = np.eye(4)
+            base[:3, :3] = self.o0[e]
+            out[e] = hg_to_d(hg[a] @ base, hg[b] @ base)
+        return out
+
+    def make_eps(self, d):
+        """structure/beam.py:1463-1472."""
+        return (d - self.d0) / self.l0[:, None]
+
+    def elem_vec(self, x, e):
+        a, b = self.conn[e]
+        return np.concatenate((x[a], x[b]))
+
+    # structure/utils.py:_k_t_entry (no geometric term), _integrate_m_l, _integrate_c_t
+    def element(self, e, d, v=None, v_dot=None):
+        """Element quantities at configuration d: P, eps, K_t, f_int and, with
+        velocities, M_t, C_L, C_T, f_iner, f_gyr."""
+        ad, l = self.ad_inv[e], self.l0[e]
+        k = self.kmat(e)
+        p = p_mat(d, ad)
+        eps = (d - self.d0[e]) / l
+        out = dict(p=p, eps=eps, k_t=p.T @ k @ p / l, f_int=-(p.T @ (k @ eps)))
+        if v is None:
+            return out
+        m = self.mmat(e)
+        v_e = self.elem_vec(v, e)
+        vd_e = self.elem_vec(v_dot, e)
+        # mass: Lobatto order 3 (integration.py:gauss_lobatto), end values fixed
+        f0 = np.zeros((12, 12))
+        f0[:6, :6] = ad.T @ m @ ad
+        f1 = np.zeros((12, 12))
+        f1[6:, 6:] = ad.T @ m @ ad
+        qm = q_mat(LOBATTO3_MID, d, ad)
+        m_t = l * (0.5 * (2.0 / 6.0 * (f0 + f1) + 4.0 / 3.0 * (qm.T @ m @ qm)))
+        # gyroscopic: Legendre order 3
+        d_dot = p @ v_e
+        c_l = np.zeros((12, 12))
+        c_t = np.zeros((12, 12))
+        for s, w in zip(LEGENDRE3, LEGENDRE3_W):
+            q = q_mat(s, d, ad)
+            qd = q_dot_mat(s, d, d_dot, ad)
+            u = q @ v_e
+            cl = q.T @ (m @ qd - hat6(u).T @ m @ q)
+            c_l += w * cl
+            c_t += w * (cl - q.T @ check6(m @ u).T @ q)
+        c_l *= 0.5 * l
+        c_t *= 0.5 * l
+        out.update(m_t=m_t, c_l=c_l, c_t=c_t, f_iner=-(m_t @ vd_e), f_gyr=-(c_l @ v_e), d_dot=d_dot)
+        return out
+
+    def assemble_mat(self, blocks):
+        """structure/beam.py:603-620."""
+        out = np.zeros((self.n_dof, self.n_dof))
+        for e, (a, b) in enumerate(self.conn):
+            idx = np.concatenate((6 * a + np.arange(6), 6 * b + np.arange(6)))
+            out[np.ix_(idx, idx)] += blocks[e]
+        return out
+
+    def assemble_vec(self, vecs):
+        """structure/beam.py:622-631."""
+        out = np.zeros(self.n_dof)
+        for e, (a, b) in enumerate(self.conn):
+            out[6 * a:6 * a + 6] += vecs[e][:6]
+            out[6 * b:6 * b + 6] += vecs[e][6:]
+        return out
+
+    @staticmethod
+    def update_hg(hg, phi):
+        """structure/beam.py:1845-1856."""
+        phi = np.asarray(phi).reshape(-1, 6)
+        return np.stack([g @ exp_se3(p) for g, p in zip(hg, phi)])
+
+    def varphi_from_hg(self, hg):
+        return np.stack([hg_to_d(a, b) for a, b in zip(self.hg0, hg)])
+
+    def solve_dofs(self, prescribed_dofs):
+        pre = set(int(x) for x in np.atleast_1d(np.arange(self.n_dof)[prescribed_dofs]
+                                                  if isinstance(prescribed_dofs, slice) else
+                                                  (prescribed_dofs if prescribed_dofs is not None else [])))
+        return np.array(sorted(set(range(self.n_dof)) - pre), dtype=np.int64)
+
+    @staticmethod
+    def dead_to_local(f, hg):
+        """structure/beam.py:1387-1395 (R^T per node)."""
+        r = hg[:, :3, :3]
+        return np.concatenate((np.einsum("nji,nj->ni", r, f[:, :3]), np.einsum("nji,nj->ni", r, f[:, 3:])), -1)
+
+    def residual(self, hg, sd, f_follower=None, f_dead=None, dynamic=False, v=None, v_dot=None):
+        """make_f_res (structure/beam.py:1749-1843) plus the element data the
+        tangents need."""
+        d = self.make_d(hg)
+        els = [self.element(e, d[e], v, v_dot) if dynamic else self.element(e, d[e]) for e in range(self.n_elem)]
+        fr = [el["f_int"] for el in els]
+        fa = [np.abs(el["f_int"]) for el in els]
+        if dynamic:
+            fr = [f + el["f_iner"] + el["f_gyr"] for f, el in zip(fr, els)]
+            fa = [f + np.abs(el["f_iner"] + el["f_gyr"]) for f, el in zip(fa, els)]
+        f_res = self.assemble_vec(fr)
+        f_abs = self.assemble_vec(fa)
+        if f_follower is not None:
+            f_res = f_res + f_follower.ravel()
+            f_abs = f_abs + np.abs(f_follower.ravel())
+        if f_dead is not None:
+            fd = self.dead_to_local(f_dead, hg).ravel()
+            f_res = f_res + fd
+            f_abs = f_abs + np.abs(fd)
+        return f_res[sd], f_abs[sd], els, d
+
+    # structure/beam.py:1878-2092
+    def static_solve(self, prescribed_dofs, f_ext_follower=None, f_ext_dead=None, f_ext_aero=None, load_steps=1,
+                     hg_init=None):
+        sd = self.solve_dofs(prescribed_dofs)
+        w = np.linspace(0.0, 1.0, load_steps + 1)[1:]
+        hg = self.hg0.copy() if hg_init is None else np.array(hg_init, float)
+        iters = []
+        for i_ls in range(load_steps):
+            ffol = None if f_ext_follower is None else w[i_ls] * f_ext_follower
+            fdead = None
+            if f_ext_dead is not None or f_ext_aero is not None:
+                fdead = np.zeros((self.n_nodes, 6))
+                if f_ext_dead is not None:
+                    fdead = w[i_ls] * f_ext_dead
+                if f_ext_aero is not None:
+                    fdead = (fdead + w[i_ls] * f_ext_aero) if f_ext_dead is not None else w[i_ls] * f_ext_aero
+            st = ConvergenceStatus(self.settings)
+            while not st.done():
+                f_res, f_abs, els, _ = self.residual(hg, sd, ffol, fdead)
+                k = self.assemble_mat([el["k_t"] for el in els])[np.ix_(sd, sd)]
+                dx = np.linalg.solve(k, f_res) * self.relaxation_factor
+                full = np.zeros(self.n_dof)
+                full[sd] = dx
+                hg = self.update_hg(hg, full)
+                h_full = self.varphi_from_hg(hg).ravel()[sd] if self.settings.rel_disp_tol is not None else None
+                st.update(dx, h_full, f_res, f_abs)
+            iters.append(st.i_iter)
+        return dict(hg=hg, varphi=self.varphi_from_hg(hg), d=self.make_d(hg), iters=iters)
+
+    # structure/beam.py:2129-2861 (structure-only and coupled time stepping)
+    def dynamic_solve(self, init, prescribed_dofs, n_tstep, dt, f_ext_follower=None, f_ext_dead=None, load_steps=1,
+                      aero=None):
+        """init: dict(hg, v, v_dot) at t0 (a = v_dot, structure/beam.py:2970).
+        aero: optional coupling callback object (see CoupledOracle)."""
+        ti = TimeIntegrator(self.spectral_radius, dt)
+        sd = self.solve_dofs(prescribed_dofs)
+        nn = self.n_nodes
+        w = np.linspace(0.0, 1.0, load_steps + 1)[1:]
+
+        def series(f):
+            if f is None:
+                return None
+            f = np.asarray(f, float)
+            f = np.broadcast_to(f, (n_tstep, nn, 6)) if f.ndim == 2 else f
+            fa = np.zeros((n_tstep, nn, 6))
+            fa[1:] = ti.f_alpha(f[:-1], f[1:])
+            return fa
+
+        ffa, fda = series(f_ext_follower), series(f_ext_dead)
+        hg = np.zeros((n_tstep, nn, 4, 4))
+        v = np.zeros((n_tstep, nn, 6))
+        vd = np.zeros((n_tstep, nn, 6))
+        acc = np.zeros((n_tstep, nn, 6))
+        hg[0] = init["hg"]
+        v[0] = init.get("v", np.zeros((nn, 6)))
+        vd[0] = init.get("v_dot", np.zeros((nn, 6)))
+        acc[0] = vd[0]
+        f_aero_local = np.zeros((n_tstep, nn, 6))
+        if aero is not None:
+            f_aero_local[0] = aero.initial_local_force(hg[0])
+        info = dict(struct_iters=[], fsi_iters=[])
+
+        def load_steps_solve(i, phi_a, v_a, vd_a, f_aero_steps):
+            for i_ls in range(load_steps):
+                st = ConvergenceStatus(self.settings)
+                ffol = None if ffa is None else w[i_ls] * ffa[i]
+                fdead = None
+                if fda is not None or f_aero_steps is not None:
+                    if fda is not None and f_aero_steps is not None:
+                        fdead = w[i_ls] * fda[i] + f_aero_steps[i_ls]
+                    elif fda is not None:
+                        fdead = w[i_ls] * fda[i]
+                    else:
+                        fdead = f_aero_steps[i_ls]
+                while not st.done():
+                    hg_u = self.update_hg(hg[i - 1], phi_a)
+                    f_res, f_abs, els, _ = self.residual(hg_u, sd, ffol, fdead, True, v_a, vd_a)
+                    sys_e = [ti.beta_prime * el["m_t"] + ti.gamma_prime * el["c_t"] for el in els]
+                    k_t = self.assemble_mat([el["k_t"] for el in els])
+                    tn = np.zeros((self.n_dof, self.n_dof))
+                    for n in range(nn):
+                        tn[6 * n:6 * n + 6, 6 * n:6 * n + 6] = t_se3(phi_a[n])
+                    mat = (self.assemble_mat(sys_e) + k_t @ tn)[np.ix_(sd, sd)]
+                    dx = np.linalg.solve(mat, f_res) * self.relaxation_factor
+                    phi_a = phi_a.copy()
+                    v_a = v_a.copy()
+                    vd_a = vd_a.copy()
+                    phi_a.reshape(-1)[sd] += dx
+                    v_a.reshape(-1)[sd] += ti.gamma_prime * dx
+                    vd_a.reshape(-1)[sd] += ti.beta_prime * dx
+                    st.update(dx, phi_a, f_res, f_abs)
+                info["struct_iters"].append(st.i_iter)
+            return phi_a, v_a, vd_a
+
+        af = ti.alpha_f
+        for i in range(1, n_tstep):
+            phi_a, v_a, vd_a = ti.predict(v[i - 1], vd[i - 1], acc[i - 1])
+            if aero is None:
+                phi_a, v_a, vd_a = load_steps_solve(i, phi_a, v_a, vd_a, None)
+            else:
+                rot = hg[i - 1][:, :3, :3]
+                fl = f_aero_local[i - 1]
+                f_nm1 = np.concatenate((np.einsum("nij,nj->ni", rot, fl[:, :3]),
+                                        np.einsum("nij,nj->ni", rot, fl[:, 3:])), -1)
+                f_prev = f_nm1
+                fst = ConvergenceStatus(aero.fsi_settings)
+                while not fst.done():
+                    phi_n = phi_a / (1.0 - af)
+                    v_n = (v_a - af * v[i - 1]) / (1.0 - af)
+                    hg_n = self.update_hg(hg[i - 1], phi_n)
+                    hg_dot = np.einsum("nij,njk->nik", hg_n, np.stack([tilde4(x) for x in v_n]))
+                    f_n = aero.solve(i, hg_n, hg[i - 1], hg_dot)
+                    f_al = ti.f_alpha(f_nm1, f_n)
+                    steps = np.einsum("i,...->i...", w, f_al)
+                    phi_new, v_a, vd_a = load_steps_solve(i, phi_a, v_a, vd_a, steps)
+                    fst.update((phi_a - phi_new).ravel()[sd], phi_new.ravel()[sd],
+                               (f_al - f_prev).ravel()[sd], f_al.ravel()[sd])
+                    phi_a = phi_new
+                    f_prev = f_al
+                info["fsi_iters"].append(fst.i_iter)
+            # calculate_q_n_from_q_alpha + calculate_a_n (time_integration.py:53-84)
+            phi = phi_a / (1.0 - af)
+            v[i] = (v_a - af * v[i - 1]) / (1.0 - af)
+            vd[i] = (vd_a - af * vd[i - 1]) / (1.0 - af)
+            acc[i] = 1.0 / (1.0 - ti.alpha_m) * ((1.0 - af) * vd[i] + af * vd[i - 1] - ti.alpha_m * acc[i - 1])
+            hg[i] = self.update_hg(hg[i - 1], phi)
+            if aero is not None:
+                f_aero_local[i] = self.dead_to_local(aero.project(i, hg[i]), hg[i])
+        out = dict(hg=hg, v=v, v_dot=vd, a=acc, varphi=np.stack([self.varphi_from_hg(h) for h in hg]), info=info)
+        if aero is not None:
+            out["f_ext_aero"] = f_aero_local
+        return out
+
+
+# ---------------------------------------------------------------------------
+# Aero -> beam forcing and the coupled solver
+# ---------------------------------------------------------------------------
+def project_forcing_to_beam(f_total, rmat, dof_mapping, x0_aero):
+    """aero/utils.py:666-697: per beam node, the sum of the strip's vertex
+    forces and of (R x0) x f; later surfaces overwrite shared nodes."""
+    out = np.zeros((rmat.shape[0], 6))
+    for f, dm, x0 in zip(f_total, dof_mapping, x0_aero):
+        r = rmat[dm]
+        arm = np.einsum("ijk,lik->lij", r, x0)
+        out[dm, :3] = f.sum(axis=0)
+        out[dm, 3:] = np.cross(arm, f).sum(axis=0)
+    return out
+
+
+class CoupledOracle:
+    """coupled/coupled.py:46-486 for a UVLMOracle and a BeamOracle."""
+
+    def __init__(self, beam, uvlm, fsi_settings=DEFAULT_FSI, include_unsteady_force=True):
+        self.beam = beam
+        self.uvlm = uvlm
+        self.fsi_settings = fsi_settings
+        self.include_unsteady = include_unsteady_force
+
+    def _forces(self, out, hg, unsteady):
+        f = list(out["f_steady"])
+        if unsteady:
+            f = [a + b for a, b in zip(f, out["f_unsteady"])]
+        return project_forcing_to_beam(f, hg[:, :3, :3], self.uvlm.dof_mapping, self.uvlm.zeta_b0)
+
+    # coupled/coupled.py:158-305 (Aitken-relaxed static FSI)
+    def static_solve(self, prescribed_dofs, t=0.0, load_steps=1, fsi_relaxation=0.6, fsi_omega_min=1e-2):
+        beam, uv = self.beam, self.uvlm
+        sd = beam.solve_dofs(prescribed_dofs)
+        free = np.zeros(beam.n_dof, bool)
+        free[sd] = True
+        free = free.reshape(-1, 6)
+        hg = beam.hg0.copy()
+        f_applied_local = np.zeros((beam.n_nodes, 6))
+        aero = uv.base_solve_from_grid(None, t, uv.hg_to_zeta_b(hg), None, None, True, False)
+        st = ConvergenceStatus(self.fsi_settings)
+        omega_prev, r_prev = fsi_relaxation, np.zeros((beam.n_nodes, 6))
+        while not st.done():
+            f_new = self._forces(aero, hg, False)
+            rot = hg[:, :3, :3]
+            f_app = np.concatenate((np.einsum("nij,nj->ni", rot, f_applied_local[:, :3]),
+                                    np.einsum("nij,nj->ni", rot, f_applied_local[:, 3:])), -1)
+            r_curr = np.where(free, f_new - f_app, 0.0)
+            dr = r_curr - r_prev
+            den = float(np.vdot(dr, dr))
+            om_a = -omega_prev * float(np.vdot(r_prev, dr)) / (den if den > 0.0 else 1.0)
+            om_a = min(max(om_a, fsi_omega_min), 1.0)
+            omega = om_a if st.i_iter >= 2 else fsi_relaxation
+            f_aero = f_new if st.i_iter == 0 else f_app + omega * r_curr
+            res = beam.static_solve(prescribed_dofs, f_ext_aero=f_aero, load_steps=load_steps)
+            delta_raw = np.stack([hg_to_d(a, b) for a, b in zip(hg, res["hg"])])
+            delta = delta_raw if st.i_iter == 0 else delta_raw / omega
+            tot = np.stack([hg_to_d(a, b) for a, b in zip(res["hg"], beam.hg0)])
+            st.update(delta, tot, -r_curr, np.where(free, f_new, 0.0))
+            hg = res["hg"]
+            f_applied_local = beam.dead_to_local(f_aero, hg)
+            aero = uv.base_solve_from_grid(None, t, uv.hg_to_zeta_b(hg), None, None, True, False)
+            omega_prev, r_prev = omega, r_curr
+        return dict(hg=hg, aero=aero, iters=st.i_iter)
+
+    # coupled/coupled.py:307-433 + structure/beam.py:2319-2861
+    def dynamic_solve(self, init_hg, init_aero_case, prescribed_dofs, n_tstep, load_steps=1):
+        """init_aero_case: an OracleCase snapshot at t0 (e.g. from solve_static)."""
+        uv, beam = self.uvlm, self.beam
+        case = uv._empty_case(n_tstep)
+        for key in ("zeta_b", "zeta_b_dot", "zeta_w", "gamma_b", "gamma_b_dot", "gamma_w", "f_steady",
+                    "f_unsteady", "alpha", "c", "nc"):
+            for s in range(uv.n_surf):
+                getattr(case, key)[s][0] = getattr(init_aero_case, key)[s][0]
+        case.t[0] = init_aero_case.t[0]
+        outer = self
+
+        class _Aero:
+            fsi_settings = outer.fsi_settings
+
+            @staticmethod
+            def initial_local_force(hg0):
+                f = project_forcing_to_beam([x[0] for x in case.f_steady], hg0[:, :3, :3], uv.dof_mapping,
+                                            uv.zeta_b0)
+                return beam.dead_to_local(f, hg0)
+
+            @staticmethod
+            def solve(i, hg_n, hg_nm1, hg_dot):
+                case.t[i] = case.t[i - 1] + uv.dt
+                q = dict(gamma_b=[g[i - 1] for g in case.gamma_b], gamma_w=[g[i - 1] for g in case.gamma_w],
+                         gamma_b_dot=[g[i - 1] for g in case.gamma_b_dot], zeta_w=[z[i - 1] for z in case.zeta_w])
+                out = uv.base_solve_from_grid(q, case.t[i], uv.hg_to_zeta_b(hg_n), uv.hg_to_zeta_b(hg_nm1),
+                                              uv.hg_dot_to_zeta_b_dot(hg_dot), False, False)
+                uv._store(case, i, out, False)
+                return outer._forces(out, hg_n, outer.include_unsteady)
+
+            @staticmethod
+            def project(i, hg_n):
+                out = dict(f_steady=[x[i] for x in case.f_steady], f_unsteady=[x[i] for x in case.f_unsteady])
+                return outer._forces(out, hg_n, outer.include_unsteady)
+
+        res = beam.dynamic_solve(dict(hg=init_hg), prescribed_dofs, n_tstep, uv.dt, load_steps=load_steps,
+                                 aero=_Aero)
+        res["aero"] = case
+        return res
+
+
+# ---------------------------------------------------------------------------
+# Workload builders
+# ---------------------------------------------------------------------------
+def chain(n_nodes):
+    c = np.zeros((n_nodes - 1, 2), dtype=np.int64)
+    c[:, 0] = np.arange(n_nodes - 1)
+    c[:, 1] = np.arange(1, n_nodes)
+    return c
+
+
+def geradin_beam(n_nodes=20):
+    """models/geradin_beam/geradin_beam.py: length 5 along x, EA = GJ = 1e12,
+    GA = 3.231e8, EI = 9.345e6."""
+    b = BeamOracle(n_nodes, chain(n_nodes), np.array((0.0, 1.0, 0.0)))
+    coords = np.zeros((n_nodes, 3))
+    coords[:, 0] = np.linspace(0.0, 5.0, n_nodes)
+    k = np.full(6, 1e12)
+    k[1:3] = 3.231e8
+    k[4:6] = 9.345e6
+    b.set_design_variables(coords, np.diag(k), None)
+    return b
