@@ -117,7 +117,25 @@ _SIGS = {
     "fj_step_batched": ([c_int, c_void_p, POINTER(FjFlowfield), c_double, c_double, c_int, c_double, c_double,
                          c_int, c_int, c_int, c_double, c_void_p, c_void_p, c_void_p, c_void_p], c_int),
     "fj_flow_eval": ([POINTER(FjFlowfield), c_void_p, c_int, c_double, c_void_p, c_void_p], c_int),
+    "fj_beam_workspace_bytes": ([c_int], c_size_t),
+    "fj_beam_system": ([c_void_p, c_int] + [c_void_p] * 6 + [c_double, c_double, c_void_p, c_int] + [c_void_p] * 5,
+                       c_int),
+    "fj_beam_update": ([c_int, c_int, c_void_p, c_int, c_void_p, c_double, c_double, c_double] + [c_void_p] * 7,
+                       c_int),
+    "fj_beam_subnorm": ([c_void_p, c_void_p, c_int, c_void_p, c_void_p], c_int),
+    "fj_beam_nodes": ([c_int, c_int] + [c_void_p] * 7, c_int),
+    "fj_beam_project_forces": ([c_int, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
+                                c_void_p, c_void_p], c_int),
 }
+
+BEAM_NODE_UPDATE, BEAM_NODE_REL, BEAM_NODE_RATE = 0, 1, 2
+
+
+class FjBeamModel(ctypes.Structure):
+    """fj_beam_model: device element tables of a beam."""
+
+    _fields_ = [("n_nodes", c_int), ("n_elem", c_int)] + [
+        (name, c_void_p) for name in ("conn", "o0", "l0", "d0", "k_cs", "m_cs", "node_off", "node_elem")]
 
 
 

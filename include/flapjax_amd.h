@@ -422,6 +422,76 @@ int fj_step_batched(int n_cases, const fj_step_case* cases, const fj_flowfield* 
                     int free_wake, int max_iter, double tol, void* device_table,
                     void* host_staging, void* workspace, void* stream);
 
+/* ---- Structure side of the coupling: the SE(3) two-node beam -------------
+ * The Newton iterations of BaseBeamStructure.static_solve and
+ * base_dynamic_solve (structure/beam.py:1878-2092, 2129-2861) with the
+ * element formulas of structure/utils.py:_k_t_entry, _integrate_m_l and
+ * _integrate_c_t (the reference's default OptionalJacobians: no geometric,
+ * dead-load or gravity stiffness, no q_dot term in C_T), and the coupling
+ * seam of coupled/coupled.py.  Node frames are row-major 4x4 homogeneous
+ * transforms (16 doubles), nodal vectors 6 doubles (u, omega). */
+
+/* Element tables of a beam (device pointers), set up once per design. */
+typedef struct fj_beam_model {
+  int n_nodes, n_elem;
+  const int32_t* conn;      /* (n_elem, 2) node indices A, B */
+  const double* o0;         /* (n_elem, 3, 3) reference element triads, row-major, columns x y z */
+  const double* l0;         /* (n_elem,) reference lengths */
+  const double* d0;         /* (n_elem, 6) reference configurations (l0, 0, 0, 0, 0, 0) */
+  const double* k_cs;       /* (n_elem, 6, 6) cross-section stiffness of each element */
+  const double* m_cs;       /* (n_elem, 6, 6) cross-section mass of each element */
+  const int32_t* node_off;  /* (n_nodes + 1) CSR offsets of the elements at each node */
+  const int32_t* node_elem; /* (2 n_elem) entries 2 e + end, end 0 for node A and 1 for node B */
+} fj_beam_model;
+
+/* Device workspace of fj_beam_system for n_elem elements. */
+size_t fj_beam_workspace_bytes(int n_elem);
+
+/* One Newton system: the reduced matrix A (n_solve x n_solve, row-major) and
+ * residual r with the absolute-force sums f_abs (n_solve) over the solve DOFs
+ * solve_dofs (ascending, 6 node + component).  Replaces the assembly in
+ * structure/beam.py:1934-1995 (dynamic = 0: A = K_t, r = f_int + follower +
+ * R^T dead) and :2170-2290 (dynamic = 1: A = beta' M + gamma' C_T + K_t T(phi)
+ * with t_phi (n_nodes, 6, 6) = T(phi_alpha); r adds the inertial and
+ * gyroscopic forces of velocities v and accelerations v_dot, all local).
+ * hg: the configuration the iteration linearises about; f_follower (local)
+ * and f_dead (global, dead and aerodynamic loads) may be NULL. */
+int fj_beam_system(const fj_beam_model* model, int dynamic, const double* hg, const double* v,
+                   const double* v_dot, const double* t_phi, const double* f_follower,
+                   const double* f_dead, double beta_p, double gamma_p, const int32_t* solve_dofs,
+                   int n_solve, double* A, double* r, double* f_abs, void* workspace, void* stream);
+
+/* Newton update with dx = relax x (structure/beam.py:1974-1990, 2274-2296):
+ * dynamic = 1 adds dx, gamma' dx and beta' dx to phi, v and v_dot at the solve
+ * DOFs; dynamic = 0 writes the full increment (zero at prescribed DOFs) to
+ * phi.  norms (5 doubles) = |dx|, |phi| (dynamic; 0 when static), |r|,
+ * max |f_abs|, 1 if dx holds a NaN: the inputs of ConvergenceStatus.update
+ * (utils/data_structures.py:81-151). */
+int fj_beam_update(int dynamic, int n_nodes, const int32_t* solve_dofs, int n_solve, const double* x,
+                   double relax, double gamma_p, double beta_p, double* phi, double* v, double* v_dot,
+                   const double* r, const double* f_abs, double* norms, void* stream);
+
+/* out = |x[idx]| (the static solve's total-displacement norm). */
+int fj_beam_subnorm(const double* x, const int32_t* idx, int n, double* out, void* stream);
+
+#define FJ_BEAM_NODE_UPDATE 0 /* hg_out = hg exp(x) (structure/beam.py:1845-1856); t_out = T(x)
+                                 and h_out = log(hg_ref^-1 hg_out) when not NULL */
+#define FJ_BEAM_NODE_REL 1    /* h_out = log(hg^-1 hg_ref) (algebra/se3.py:hg_to_d) */
+#define FJ_BEAM_NODE_RATE 2   /* hg_out = hg tilde(x) (structure/beam.py:1521-1531) */
+
+/* Per-node SE(3) maps over n_nodes frames. */
+int fj_beam_nodes(int op, int n_nodes, const double* hg, const double* x, const double* hg_ref,
+                  double* hg_out, double* t_out, double* h_out, void* stream);
+
+/* Aerodynamic vertex forces of one surface onto the beam nodes
+ * (aero/utils.py:666-697): node dof_mapping[j] receives the sum over the
+ * m1 chordwise vertices of column j of f = f_a (+ f_b) and of (R x0) x f,
+ * R = rmat[node] (n_nodes, 3, 3); global frame.  clear zeroes all n_nodes
+ * rows first (the first surface); later surfaces overwrite shared nodes. */
+int fj_beam_project_forces(int n_nodes, const double* rmat, int m1, int n1, const double* f_a,
+                           const double* f_b, const double* x0, const int32_t* dof_mapping, int clear,
+                           double* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -38,6 +38,9 @@ from __future__ import annotations
 from dataclasses import dataclass
 
 import numpy as np
+import numpy as _numpy
+
+from . import uvlm_oracle as _U
 
 # utils/constants.py:13
 SMALL_ANG_THRESH = 1e-10
@@ -755,6 +758,45 @@ class CoupledOracle:
 
 
 # ---------------------------------------------------------------------------
+# Rounding sensitivity of the reference formulation
+# ---------------------------------------------------------------------------
+class _UlpShiftedNumpy:
+    """numpy with sin and cos of real arguments moved by one ulp (sin up,
+    cos down): a stand-in for another correct math library."""
+
+    def __getattr__(self, name):
+        return getattr(_numpy, name)
+
+    @staticmethod
+    def sin(x):
+        r = _numpy.sin(x)
+        return _numpy.nextafter(r, _numpy.inf) if _numpy.isrealobj(r) else r
+
+    @staticmethod
+    def cos(x):
+        r = _numpy.cos(x)
+        return _numpy.nextafter(r, -_numpy.inf) if _numpy.isrealobj(r) else r
+
+
+class ulp_shifted_trig:
+    """Context manager: inside it, this module's SO(3)/SE(3) maps use sin and
+    cos moved by one ulp.  Running a solve twice, with and without it, gives
+    the size of the differences between two correct FP64 evaluations of the
+    reference's formulas (whose small-angle closed forms cancel); device
+    results are held to a small multiple of it."""
+
+    def __enter__(self):
+        global np
+        np = _UlpShiftedNumpy()
+        return self
+
+    def __exit__(self, *exc):
+        global np
+        np = _numpy
+        return False
+
+
+# ---------------------------------------------------------------------------
 # Workload builders
 # ---------------------------------------------------------------------------
 def chain(n_nodes):
@@ -775,3 +817,20 @@ def geradin_beam(n_nodes=20):
     k[4:6] = 9.345e6
     b.set_design_variables(coords, np.diag(k), None)
     return b
+
+
+def cantilever_wing(m=2, n_nodes=5, m_star=4, k_diag=(1e6, 1e6, 1e6, 1e3, 1e3, 1e3), u_inf=(10.0, 0.0, 1.5),
+                    settings=DEFAULT_STRUCT, fsi=DEFAULT_FSI, spectral_radius=0.8, flow=None):
+    """models/cantilever_wing.py: beam along y, mirror plane y = 0."""
+    n = n_nodes - 1
+    beam = BeamOracle(n_nodes, chain(n_nodes), np.array((0.0, 0.0, 1.0)), settings=settings,
+                        spectral_radius=spectral_radius)
+    coords = np.zeros((n_nodes, 3))
+    coords[:, 1] = np.linspace(0.0, 5.0, n_nodes)
+    beam.set_design_variables(coords, np.diag(k_diag), np.eye(6))
+    uv = _U.UVLMOracle([(m, n, m_star)], np.arange(n_nodes), mirror_point=np.zeros(3),
+                      mirror_normal=np.array((0.0, 1.0, 0.0)))
+    ff = flow if flow is not None else _U.Constant(np.array(u_inf), 1.225, True)
+    grid = _U.make_rectangular_grid(m, n, 1.0, 0.25)
+    uv.set_design_variables(1.0 / (np.linalg.norm(u_inf) * m), ff, grid, beam.hg0)
+    return CoupledOracle(beam, uv, fsi)

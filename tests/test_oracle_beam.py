@@ -7,7 +7,6 @@ import numpy as np
 import pytest
 
 from oracle import beam_oracle as B
-from oracle import uvlm_oracle as U
 
 RNG = np.random.default_rng(7)
 
@@ -217,23 +216,6 @@ def test_projection_closed_form():
         assert np.allclose(out[j, 3:], mom)
 
 
-def cantilever_wing(m=2, n_nodes=5, m_star=4, k_diag=(1e6, 1e6, 1e6, 1e3, 1e3, 1e3), u_inf=(10.0, 0.0, 1.5),
-                    settings=B.DEFAULT_STRUCT, fsi=B.DEFAULT_FSI, spectral_radius=0.8, flow=None):
-    """models/cantilever_wing.py: beam along y, mirror plane y = 0."""
-    n = n_nodes - 1
-    beam = B.BeamOracle(n_nodes, B.chain(n_nodes), np.array((0.0, 0.0, 1.0)), settings=settings,
-                        spectral_radius=spectral_radius)
-    coords = np.zeros((n_nodes, 3))
-    coords[:, 1] = np.linspace(0.0, 5.0, n_nodes)
-    beam.set_design_variables(coords, np.diag(k_diag), np.eye(6))
-    uv = U.UVLMOracle([(m, n, m_star)], np.arange(n_nodes), mirror_point=np.zeros(3),
-                      mirror_normal=np.array((0.0, 1.0, 0.0)))
-    ff = flow if flow is not None else U.Constant(np.array(u_inf), 1.225, True)
-    grid = U.make_rectangular_grid(m, n, 1.0, 0.25)
-    uv.set_design_variables(1.0 / (np.linalg.norm(u_inf) * m), ff, grid, beam.hg0)
-    return B.CoupledOracle(beam, uv, fsi)
-
-
 def test_static_aeroelastic_equilibrium_and_persistence():
     """coupled/coupled.py static FSI converges to a state whose aero load
     balances the internal forces; time stepping from it keeps it
@@ -242,7 +224,7 @@ def test_static_aeroelastic_equilibrium_and_persistence():
     tightened 2x on this coarser wing: the drift is the equilibrium's
     convergence error)."""
     strict = B.ConvergenceSettings(30, 1e-9, 1e-11, 1e-9, 1e-11)
-    cw = cantilever_wing(settings=strict, fsi=strict)
+    cw = B.cantilever_wing(settings=strict, fsi=strict)
     st = cw.static_solve(np.arange(6))
     hg = st["hg"]
     assert hg[-1, 2, 3] > 1e-4  # the wing bends up
