@@ -177,6 +177,93 @@ def time_form(precision, steps, warmup, aic_reuse=True, whole_case=True):
             "p99_ms": float(np.percentile(lat, 99)), "case_s": case_s}
 
 
+# Coupled gust response: a cantilever beam along y (span 10) of 40 stiffness
+# stations, each a pair of two-node SE(3) elements (81 nodes, one per
+# spanwise station of the 16 x 80 lattice), mirrored about y = 0, freestream
+# U = 10 at 5 deg, 300 wake rows, dt = c/(16U), 1-cos vertical gust of
+# 0.05 U with L = 5 chords (the profile spans 2L = 10 chords) reaching the
+# wing after 1 chord of travel, 300 steps, from the static aeroelastic
+# equilibrium.  Cross-section mass 10 per unit length and unit inertias 10,
+# as in the reference's adjoint wing model: at 0.75 per unit length the
+# fluid's added mass (pi rho c^2 / 4 = 0.96) exceeds the structure's and
+# the unrelaxed partitioned FSI iteration of the dynamic solve diverges.
+CPL_M, CPL_N, CPL_M_STAR, CPL_STEPS, CPL_STATIONS = 16, 80, 300, 300, 40
+CPL_EI_FLAP, CPL_EI_LAG, CPL_GJ, CPL_EA, CPL_GA = 2e4, 4e6, 1e4, 1e8, 1e8
+CPL_MASS, CPL_INERTIA, CPL_U = 10.0, (1.0, 1.0, 1.0), 10.0
+
+
+def coupled_design(seed=3, stations=CPL_STATIONS):
+    """Cross-section tables of the coupled wing: k_cs (stations, 6, 6) with
+    each stiffness scaled by 1 + U(-0.1, 0.1) (seeded), m_cs (6, 6).  Local
+    axes of an element along +y with y_vector = z: x spanwise, y = global z,
+    z = global x, so component 4 is in-plane (lag) and 5 out-of-plane (flap)
+    bending."""
+    rng = np.random.default_rng(seed)
+    base = np.array((CPL_EA, CPL_GA, CPL_GA, CPL_GJ, CPL_EI_LAG, CPL_EI_FLAP))
+    k = np.stack([np.diag(base * (1.0 + 0.1 * rng.uniform(-1.0, 1.0, 6))) for _ in range(stations)])
+    m = np.diag((CPL_MASS, CPL_MASS, CPL_MASS) + tuple(CPL_MASS * np.array(CPL_INERTIA)))
+    return k, m
+
+
+def make_coupled(precision="fp64", m=CPL_M, n=CPL_N, m_star=CPL_M_STAR, stations=CPL_STATIONS, span=10.0,
+                 free_wake=False):
+    """The coupled gust model on the device (n elements, n + 1 nodes, two
+    elements per stiffness station)."""
+    import flapjax_amd as fj
+    from flapjax_amd.beam import BeamStructure
+    from flapjax_amd.coupled import CoupledAeroelastic
+    from flapjax_amd.uvlm import UVLM
+    from flapjax_amd.utils import make_rectangular_grid
+
+    nn = n + 1
+    conn = np.stack((np.arange(n), np.arange(1, nn)), axis=1)
+    beam = BeamStructure(nn, conn, np.array((0.0, 0.0, 1.0)), k_cs_index=np.arange(n) * stations // n)
+    coords = np.zeros((nn, 3))
+    coords[:, 1] = np.linspace(0.0, span, nn)
+    uv = UVLM([(m, n, m_star)], np.arange(nn), mirror_point=np.zeros(3), mirror_normal=np.array((0.0, 1.0, 0.0)),
+              free_wake=free_wake, precision=precision)
+    cw = CoupledAeroelastic(beam, uv)
+    a = np.deg2rad(5.0)
+    u_inf = CPL_U * np.array((np.cos(a), 0.0, np.sin(a)))
+    gust = fj.OneMinusCosine(u_inf, 1.225, True, 5.0, 0.05 * CPL_U, gust_x0=(-11.0, 0.0, 0.0))
+    k, mm = coupled_design(stations=stations)
+    cw.set_design_variables(coords, k, mm, None, 1.0 / (m * CPL_U), gust, make_rectangular_grid(m, n, 1.0, 0.25))
+    return cw
+
+
+def coupled_bench(precision, steps=CPL_STEPS):
+    """Static aeroelastic equilibrium, then ``steps`` coupled time steps
+    (FSI iterations of beam Newton solves and UVLM steps) end to end."""
+    import torch
+
+    cw = make_coupled(precision)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st = cw.static_solve(np.arange(6))
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    dyn = cw.dynamic_solve(st, np.arange(6), steps)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    fsi = np.asarray(dyn.structure.iters["fsi"])
+    tip = dyn.structure.hg[:, -1, 2, 3].cpu().numpy()
+    lift = dyn.aero.f_steady[0][..., 2].sum(dim=(1, 2)).cpu().numpy()
+    s_b, s_w = n_seg(CPL_M, CPL_N), n_seg(CPL_M_STAR, CPL_N)
+    # per FSI iteration (mirrored lattices: 2 x the filaments): AIC, wake
+    # upwash at the collocation points, forces at the filament midpoints
+    pairs_it = 2 * (CPL_M * CPL_N * s_b + CPL_M * CPL_N * s_w + s_b * (s_b + s_w))
+    dyn_s = t2 - t1
+    return {"static_s": round(t1 - t0, 3), "static_fsi_iters": int(st.structure.iters["fsi"]),
+            "dynamic_s": round(dyn_s, 3), "ms_per_step": round(1e3 * dyn_s / (steps - 1), 3),
+            "fsi_iters_mean": round(float(fsi.mean()), 3), "fsi_iters_max": int(fsi.max()),
+            "newton_iters_per_step": round(len(dyn.structure.iters["struct"]) / (steps - 1), 3),
+            "pairs_per_fsi_iter": int(pairs_it),
+            "aero_gpairs_per_s": round(pairs_it * fsi.sum() / dyn_s / 1e9, 3),
+            "cases_per_s": round(1.0 / (t2 - t0), 4), "tip_z_min_max": [round(float(tip.min()), 6),
+                                                                        round(float(tip.max()), 6)],
+            "lift_max": round(float(lift.max()), 6)}
+
+
 def rollup_roofline(reps=20):
     """Time the dominant launch alone (rollup pair sum, 13,000 x 27,992) with
     HIP events on the stream it runs on."""
@@ -419,6 +506,9 @@ def main():
                     help="wake rows per served case = min(steps, cap); 0: wake rows = steps")
     ap.add_argument("--profile-rollup", action="store_true",
                     help="only the rollup launch and a copy of known size (for counter runs)")
+    ap.add_argument("--coupled-only", action="store_true",
+                    help="only the coupled gust response (beam + UVLM FSI), both precision forms")
+    ap.add_argument("--coupled-steps", type=int, default=CPL_STEPS)
     ap.add_argument("--profile-kernels", action="store_true",
                     help="rollup, LU, inverse, convection and copy kernels a fixed number of times (for counter runs)")
     args = ap.parse_args()
@@ -436,6 +526,9 @@ def main():
         return
     if args.profile_kernels:
         profile_kernels()
+        return
+    if args.coupled_only:
+        print(json.dumps({"coupled_gust": {p: coupled_bench(p, args.coupled_steps) for p in ("fp64", "mixed")}}))
         return
     pairs, rollup_pairs = pairs_per_step()
     f64 = time_form("fp64", args.steps, args.warmup)
