@@ -343,6 +343,23 @@ def lu_solve(fac, b, trans=False):
     return x.t().contiguous() if b.dim() == 2 else x.reshape(n)
 
 
+def band_fits(n, kl, ku):
+    """Whether fj_band_solve takes an n x n system of bandwidths (kl, ku)."""
+    return 0 <= kl <= 63 and ku >= 0 and n * (2 * kl + ku + 2) <= L.lib().fj_band_solve_capacity()
+
+
+def band_solve(A, b, kl, ku, out=None):
+    """x = A^-1 b for a banded A stored dense (LAPACK dgbtf2/dgbtrs pivoting;
+    same pivot rows as lu_factor / lu_solve)."""
+    A = _c64(A, None, "A")
+    n = A.shape[0]
+    b = _c64(b, (n,), "b")
+    x = torch.empty_like(b) if out is None else out
+    L.check(L.lib().fj_band_solve(L.ptr(A), n, n, int(kl), int(ku), L.ptr(b), L.ptr(x), None, L.stream()),
+            "fj_band_solve")
+    return x
+
+
 def _refine_ws(n, device):
     return torch.empty(L.lib().fj_refine_workspace_bytes(n) // 8 + 1, dtype=F64, device=device)
 

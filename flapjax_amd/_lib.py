@@ -71,6 +71,8 @@ _SIGS = {
     "fj_lu_factor": ([c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p], c_int),
     "fj_lu_solve": ([c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p], c_int),
     "fj_lu_factor_f32": ([c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p], c_int),
+    "fj_band_solve_capacity": ([], c_size_t),
+    "fj_band_solve": ([c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p], c_int),
     "fj_refine_workspace_bytes": ([c_int], c_size_t),
     "fj_solve_refined": ([c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int,
                           c_double, c_void_p, c_void_p, c_void_p], c_int),

@@ -3,7 +3,8 @@
 ``BeamStructure`` keeps the reference's constructor, ``set_design_variables``,
 ``static_solve`` and ``dynamic_solve`` signatures.  Its Newton iterations run
 on the GPU through the C-ABI (``fj_beam_system`` assembles the reduced
-system from per-element kernels, ``fj_lu_factor``/``fj_lu_solve`` solve it,
+system from per-element kernels, ``fj_band_solve`` (banded LU, or the dense
+``fj_lu_factor``/``fj_lu_solve`` when the band does not fit) solves it,
 ``fj_beam_update`` applies the increment and returns the convergence norms);
 the host loop reads five norms per iteration and applies the reference's
 stop test (``ConvergenceStatus``).  State (frames, velocities, accelerations)
@@ -350,7 +351,34 @@ class BeamStructure:
     def _solve_dofs(self, prescribed):
         pre = set(prescribed)
         sd = np.array(sorted(set(range(self.n_dof)) - pre), np.int32)
+        self._bandwidth = self._reduced_bandwidth(sd)
         return torch.as_tensor(sd, device=self.hg0.device), len(sd)
+
+    def _reduced_bandwidth(self, sd):
+        """Half-bandwidth of the Newton matrix over the solve DOFs: the 12
+        DOFs of an element couple with each other, a node's 6 with each
+        other (T(phi)), nothing else."""
+        pos = np.full(self.n_dof, -1, np.int64)
+        pos[sd] = np.arange(len(sd))
+        bw = 0
+        groups = [np.arange(6 * a, 6 * a + 6) for a in range(self.n_nodes)]
+        groups += [np.concatenate((np.arange(6 * a, 6 * a + 6), np.arange(6 * b, 6 * b + 6))) for a, b in self._conn]
+        for g in groups:
+            p = pos[g]
+            p = p[p >= 0]
+            if p.size:
+                bw = max(bw, int(p.max() - p.min()))
+        return bw
+
+    def _newton_solve(self, A, r, n_s):
+        """Newton increment A^-1 r: banded LU when the band fits one
+        workgroup's LDS, else the dense LU (same pivot rows either way)."""
+        if not n_s:
+            return r
+        bw = self._bandwidth
+        if K.band_fits(n_s, bw, bw):
+            return K.band_solve(A, r, bw, bw)
+        return K.lu_solve(K.lu_factor(A, overwrite=True), r)
 
     def _force(self, f, name, shape):
         if f is None:
@@ -438,7 +466,7 @@ class BeamStructure:
             st = ConvergenceStatus(self.struct_convergence_settings)
             while not st.get_status():
                 A, r, fa = self._system(0, hg, None, None, None, ffol, fdead, sd, n_s)
-                x = K.lu_solve(K.lu_factor(A, overwrite=True), r) if n_s else r
+                x = self._newton_solve(A, r, n_s)
                 L.check(lib.fj_beam_update(0, self.n_nodes, L.ptr(sd), n_s, L.ptr(x), self.relaxation_factor, 0.0,
                                            0.0, L.ptr(dphi), None, None, L.ptr(r), L.ptr(fa), L.ptr(self._norms),
                                            L.stream()), "fj_beam_update")
@@ -478,7 +506,7 @@ class BeamStructure:
         while not st.get_status():
             self._nodes(L.BEAM_NODE_UPDATE, hg_prev, x=phi_a, hg_out=hg_u, t_out=t_phi)
             A, r, fa = self._system(1, hg_u, v_a, vd_a, t_phi, f_fol, f_dead, sd, n_s, ti)
-            x = K.lu_solve(K.lu_factor(A, overwrite=True), r) if n_s else r
+            x = self._newton_solve(A, r, n_s)
             L.check(lib.fj_beam_update(1, self.n_nodes, L.ptr(sd), n_s, L.ptr(x), self.relaxation_factor,
                                        ti.gamma_prime, ti.beta_prime, L.ptr(phi_a), L.ptr(v_a), L.ptr(vd_a), L.ptr(r),
                                        L.ptr(fa), L.ptr(self._norms), L.stream()), "fj_beam_update")

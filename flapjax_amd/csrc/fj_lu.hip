@@ -898,3 +898,111 @@ extern "C" int fj_lu_inverse_f32(const float* LU, int n, int lda, const int32_t*
   return fj_lu_inverse_t<float>(LU, n, lda, ipiv, (const float*)aux, Ainv, ldi,
                                 (hipStream_t)stream);
 }
+
+// ---- Banded solve (the beam's Newton systems) ---------------------------------
+// A x = b for a matrix whose nonzeros lie within kl sub- and ku super-diagonals
+// (a chain of two-node beam elements couples each node's 6 DOFs with its
+// neighbours' only: kl = ku = 11), read from a dense row-major array.  The
+// arithmetic is LAPACK's banded LU with partial pivoting (dgbtf2: first
+// maximal |a| of the column, scaling by the reciprocal pivot, rank-1 update of
+// the band, fill confined to kl extra super-diagonals) followed by dgbtrs's
+// substitutions (row exchanges and L applied to b column by column, then the
+// column-oriented upper-triangular sweep of dtbsv).  Partial pivoting on a
+// band picks the same rows as on the dense matrix, because the entries of a
+// column outside the band are zero.
+// One wave does the whole solve with the band in LDS (row i holds columns
+// i - kl .. i + kl + ku), so each of the n sequential pivot steps is a few
+// LDS round trips instead of a dense panel launch.
+#define FJ_BAND_LDS_DOUBLES 19968  // 156 KiB of the 160 KiB LDS
+
+__global__ __launch_bounds__(64) void k_band_solve(const double* __restrict__ A, int n, int lda,
+                                                   int kl, int ku, const double* __restrict__ b,
+                                                   double* __restrict__ x,
+                                                   int32_t* __restrict__ info) {
+  __shared__ double s[FJ_BAND_LDS_DOUBLES];
+  const int lane = threadIdx.x;
+  const int W = 2 * kl + ku + 1;
+  double* xs = s + (size_t)n * W;
+  for (int q = lane; q < n * W; q += 64) {
+    const int i = q / W, t = q - i * W;
+    const int c = i - kl + t;
+    s[q] = (c >= 0 && c < n && t <= kl + ku) ? A[(size_t)i * lda + c] : 0.0;
+  }
+  for (int i = lane; i < n; i += 64) xs[i] = b[i];
+  int first_zero = 0;
+  int ju = 0;  // last column reached by fill so far (dgbtf2's JU)
+  __syncthreads();
+  for (int k = 0; k < n; ++k) {
+    // pivot: first row of the largest |a| among rows k .. k + kl of column k
+    const int r = k + lane;
+    const double a = (lane <= kl && r < n) ? fabs(s[(size_t)r * W + (kl - lane)]) : -1.0;
+    double amax = a;
+    for (int off = 32; off > 0; off >>= 1) amax = fmax(amax, __shfl_xor(amax, off));
+    const unsigned long long mask = __ballot(a == amax && lane <= kl && r < n);
+    const int p = mask ? k + (int)__builtin_ctzll(mask) : k;
+    __syncthreads();
+    if (amax != 0.0) {
+      ju = max(ju, min(p + ku, n - 1));
+      if (p != k) {
+        for (int j = lane; j <= ju - k; j += 64) {
+          const int c = k + j;
+          {
+            double* u = s + (size_t)k * W + (c - k + kl);
+            double* v = s + (size_t)p * W + (c - p + kl);
+            const double tmp = *u;
+            *u = *v;
+            *v = tmp;
+          }
+        }
+        if (lane == 0) {
+          const double tmp = xs[k];
+          xs[k] = xs[p];
+          xs[p] = tmp;
+        }
+      }
+      __syncthreads();
+      const double rp = 1.0 / s[(size_t)k * W + kl];
+      const int km = min(kl, n - 1 - k);
+      for (int i = 1 + lane; i <= km; i += 64) s[(size_t)(k + i) * W + (kl - i)] *= rp;
+      __syncthreads();
+      const int nj = ju - k;  // columns k + 1 .. ju
+      const double bk = xs[k];
+      for (int q = lane; q < km * nj; q += 64) {
+        const int i = 1 + q / nj, j = 1 + q % nj;
+        const double l = s[(size_t)(k + i) * W + (kl - i)];
+        s[(size_t)(k + i) * W + (j - i + kl)] -= l * s[(size_t)k * W + (j + kl)];
+      }
+      for (int i = 1 + lane; i <= km; i += 64) xs[k + i] -= s[(size_t)(k + i) * W + (kl - i)] * bk;
+    } else if (first_zero == 0) {
+      first_zero = k + 1;
+    }
+    __syncthreads();
+  }
+  // U x = y, column by column (dtbsv, upper, no transpose)
+  for (int j = n - 1; j >= 0; --j) {
+    const double xj = xs[j];
+    double v = xj;
+    if (xj != 0.0) v = xj / s[(size_t)j * W + kl];
+    __syncthreads();
+    if (xj != 0.0) {
+      const int kk = min(kl + ku, j);
+      for (int i = 1 + lane; i <= kk; i += 64) xs[j - i] -= v * s[(size_t)(j - i) * W + (i + kl)];
+      if (lane == 0) xs[j] = v;
+    }
+    __syncthreads();
+  }
+  for (int i = lane; i < n; i += 64) x[i] = xs[i];
+  if (lane == 0 && info) *info = first_zero;
+}
+
+extern "C" size_t fj_band_solve_capacity(void) { return FJ_BAND_LDS_DOUBLES; }
+
+extern "C" int fj_band_solve(const double* A, int n, int lda, int kl, int ku, const double* b,
+                             double* x, int32_t* info, void* stream) {
+  if (n <= 0 || lda < n || kl < 0 || ku < 0 || kl > 63 || !A || !b || !x) return FJ_ERR_ARG;
+  if ((size_t)n * (size_t)(2 * kl + ku + 2) > (size_t)FJ_BAND_LDS_DOUBLES) return FJ_ERR_ARG;
+  hipLaunchKernelGGL(k_band_solve, dim3(1), dim3(64), 0, (hipStream_t)stream, A, n, lda, kl, ku, b,
+                     x, info);
+  FJ_CHECK_LAUNCH();
+  return FJ_OK;
+}

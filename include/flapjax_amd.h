@@ -217,6 +217,19 @@ int fj_lu_factor(double* A, int n, int lda, int32_t* ipiv, int32_t* info, void* 
 int fj_lu_solve(const double* LU, int n, int lda, const int32_t* ipiv, const void* aux,
                 double* b, int nrhs, int trans, void* stream);
 
+/* Banded solve of A x = b (the beam's Newton systems, which
+ * jnp.linalg.solve solves at structure/beam.py:2273): A is read from a dense
+ * row-major n x n array whose nonzeros lie within kl sub- and ku
+ * super-diagonals (kl <= 63); LAPACK's banded LU with partial pivoting
+ * (dgbtf2) and substitutions (dgbtrs), which pick the same pivot rows as the
+ * dense LU.  A and b are not modified; x may alias b.  info (device, 1 int32,
+ * may be NULL) receives 0 or 1 + the first column without a nonzero pivot.
+ * Needs n (2 kl + ku + 2) <= fj_band_solve_capacity() doubles; otherwise
+ * FJ_ERR_ARG (use fj_lu_factor). */
+size_t fj_band_solve_capacity(void);
+int fj_band_solve(const double* A, int n, int lda, int kl, int ku, const double* b, double* x,
+                  int32_t* info, void* stream);
+
 /* FP32 LU factor (row-major, in place; aux holds float block inverses). */
 int fj_lu_factor_f32(float* A, int n, int lda, int32_t* ipiv, int32_t* info, void* aux,
                      void* stream);
