@@ -915,6 +915,7 @@ extern "C" int fj_lu_inverse_f32(const float* LU, int n, int lda, const int32_t*
 // LDS round trips instead of a dense panel launch.
 #define FJ_BAND_LDS_DOUBLES 19968  // 156 KiB of the 160 KiB LDS
 
+template <int KLMAX>
 __global__ __launch_bounds__(64) void k_band_solve(const double* __restrict__ A, int n, int lda,
                                                    int kl, int ku, const double* __restrict__ b,
                                                    double* __restrict__ x,
@@ -923,56 +924,108 @@ __global__ __launch_bounds__(64) void k_band_solve(const double* __restrict__ A,
   const int lane = threadIdx.x;
   const int W = 2 * kl + ku + 1;
   double* xs = s + (size_t)n * W;
-  for (int q = lane; q < n * W; q += 64) {
-    const int i = q / W, t = q - i * W;
-    const int c = i - kl + t;
-    s[q] = (c >= 0 && c < n && t <= kl + ku) ? A[(size_t)i * lda + c] : 0.0;
+  // band -> LDS in batches of 32 independent loads per lane (one wave has
+  // to keep many loads in flight to hide the memory latency)
+  const int total = n * W;
+  for (int base = 0; base < total; base += 64 * 32) {
+    double v[32];
+#pragma unroll
+    for (int u = 0; u < 32; ++u) {
+      const int q = base + 64 * u + lane;
+      const int i = q / W, t = q - i * W;
+      const int c = i - kl + t;
+      v[u] = (q < total && c >= 0 && c < n && t <= kl + ku) ? A[(size_t)i * lda + c] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 32; ++u) {
+      const int q = base + 64 * u + lane;
+      if (q < total) s[q] = v[u];
+    }
   }
   for (int i = lane; i < n; i += 64) xs[i] = b[i];
   int first_zero = 0;
   int ju = 0;  // last column reached by fill so far (dgbtf2's JU)
   __syncthreads();
+  // Each column step is written without data-dependent branches (clamped
+  // addresses and selects): a taken branch costs a single wave more than
+  // the arithmetic of a whole step.
   for (int k = 0; k < n; ++k) {
-    // pivot: first row of the largest |a| among rows k .. k + kl of column k
-    const int r = k + lane;
-    const double a = (lane <= kl && r < n) ? fabs(s[(size_t)r * W + (kl - lane)]) : -1.0;
-    double amax = a;
-    for (int off = 32; off > 0; off >>= 1) amax = fmax(amax, __shfl_xor(amax, off));
-    const unsigned long long mask = __ballot(a == amax && lane <= kl && r < n);
-    const int p = mask ? k + (int)__builtin_ctzll(mask) : k;
-    __syncthreads();
+    const int km = min(kl, n - 1 - k);
+    // column k, rows k .. k + km, read by every lane (broadcast reads; rows
+    // past k + km read row k + km again and are ignored)
+    double a[KLMAX + 1];
+#pragma unroll
+    for (int i = 0; i <= KLMAX; ++i) {
+      const int ii = min(i, km);
+      a[i] = s[(size_t)(k + ii) * W + (kl - ii)];
+    }
+    // pivot: the first largest |a| (idamax)
+    double amax = fabs(a[0]);
+    int ip = 0;
+#pragma unroll
+    for (int i = 1; i <= KLMAX; ++i) {
+      const double t = fabs(a[i]);
+      const bool take = i <= km && t > amax;
+      amax = take ? t : amax;
+      ip = take ? i : ip;
+    }
     if (amax != 0.0) {
+      const int p = k + ip;
       ju = max(ju, min(p + ku, n - 1));
-      if (p != k) {
-        for (int j = lane; j <= ju - k; j += 64) {
-          const int c = k + j;
-          {
-            double* u = s + (size_t)k * W + (c - k + kl);
-            double* v = s + (size_t)p * W + (c - p + kl);
-            const double tmp = *u;
-            *u = *v;
-            *v = tmp;
+      const int nj = ju - k;
+      // rows k and p exchanged: multipliers l_i = a'_i / a'_0 (dgbtf2 scales
+      // by the reciprocal pivot); l_i for i > km repeats l_km
+      double l[KLMAX + 1];
+      double piv = a[0];
+#pragma unroll
+      for (int i = 1; i <= KLMAX; ++i) piv = i == ip ? a[i] : piv;
+      const double rp = 1.0 / piv;
+      double lkm = 0.0, lme = 0.0;
+#pragma unroll
+      for (int i = 1; i <= KLMAX; ++i) {
+        l[i] = (i == ip ? a[0] : a[i]) * rp;
+        lkm = i == km ? l[i] : lkm;
+        lme = i == lane ? l[i] : lme;
+      }
+#pragma unroll
+      for (int i = 1; i <= KLMAX; ++i) l[i] = i <= km ? l[i] : lkm;
+      // each lane owns columns k + j: exchange, then the rank-1 update of
+      // rows k + 1 .. k + km (all reads first, then the writes; rows past
+      // k + km rewrite row k + km with the same value)
+      if (km > 0) {
+        for (int j = 1 + lane; j <= nj; j += 64) {
+          double* uk = s + (size_t)k * W + (j + kl);
+          double* up = s + (size_t)p * W + (j - ip + kl);
+          const double u = *up;
+          const double t = *uk;
+          *uk = u;
+          *up = t;
+          double r[KLMAX + 1];
+#pragma unroll
+          for (int i = 1; i <= KLMAX; ++i) {
+            const int ii = min(i, km);
+            r[i] = s[(size_t)(k + ii) * W + (j - ii + kl)];
+          }
+#pragma unroll
+          for (int i = 1; i <= KLMAX; ++i) {
+            const int ii = min(i, km);
+            s[(size_t)(k + ii) * W + (j - ii + kl)] = r[i] - l[i] * u;
           }
         }
-        if (lane == 0) {
-          const double tmp = xs[k];
-          xs[k] = xs[p];
-          xs[p] = tmp;
-        }
+      }  // km == 0: no rows below, so p == k and there is nothing to update
+      // column k: pivot and multipliers; b: exchange and update
+      const double bk = xs[p];
+      const bool mine = lane >= 1 && lane <= km;
+      const int rl = mine ? k + lane : k;
+      const double bi = (mine && lane == ip) ? xs[k] : xs[rl];
+      if (lane == 0) {
+        s[(size_t)k * W + kl] = piv;
+        xs[k] = bk;
       }
-      __syncthreads();
-      const double rp = 1.0 / s[(size_t)k * W + kl];
-      const int km = min(kl, n - 1 - k);
-      for (int i = 1 + lane; i <= km; i += 64) s[(size_t)(k + i) * W + (kl - i)] *= rp;
-      __syncthreads();
-      const int nj = ju - k;  // columns k + 1 .. ju
-      const double bk = xs[k];
-      for (int q = lane; q < km * nj; q += 64) {
-        const int i = 1 + q / nj, j = 1 + q % nj;
-        const double l = s[(size_t)(k + i) * W + (kl - i)];
-        s[(size_t)(k + i) * W + (j - i + kl)] -= l * s[(size_t)k * W + (j + kl)];
+      if (mine) {
+        s[(size_t)rl * W + (kl - lane)] = lme;
+        xs[rl] = bi - lme * bk;
       }
-      for (int i = 1 + lane; i <= km; i += 64) xs[k + i] -= s[(size_t)(k + i) * W + (kl - i)] * bk;
     } else if (first_zero == 0) {
       first_zero = k + 1;
     }
@@ -995,14 +1048,135 @@ __global__ __launch_bounds__(64) void k_band_solve(const double* __restrict__ A,
   if (lane == 0 && info) *info = first_zero;
 }
 
+// The same solve with the bandwidths fixed at compile time (kl = ku = 11:
+// any chain of two-node beam elements).  KL zero rows below the matrix let
+// every column step read and update rows k .. k + KL unconditionally, and
+// the constant row stride turns every band address into an immediate
+// offset, so a column step is about a sixth of the generic kernel's
+// instructions (a single wave is bound by its instruction latency here).
+// Zero padding rows are never chosen as pivots (a zero is never strictly
+// larger than the current maximum) and never reach the solution.
+template <int KL, int KU>
+__global__ __launch_bounds__(64) void k_band_solve_fixed(const double* __restrict__ A, int n,
+                                                         int lda, const double* __restrict__ b,
+                                                         double* __restrict__ x,
+                                                         int32_t* __restrict__ info) {
+  constexpr int W = 2 * KL + KU + 1;
+  constexpr int D = W - 1;  // from (row i, column c) to (row i + 1, column c)
+  __shared__ double s[FJ_BAND_LDS_DOUBLES];
+  const int lane = threadIdx.x;
+  const int rows = n + KL;
+  double* xs = s + (size_t)rows * W;
+  const int total = rows * W;
+  for (int base = 0; base < total; base += 64 * 32) {
+    double v[32];
+#pragma unroll
+    for (int u = 0; u < 32; ++u) {
+      const int q = base + 64 * u + lane;
+      const int i = q / W, t = q - i * W;
+      const int c = i - KL + t;
+      v[u] = (q < total && i < n && c >= 0 && c < n && t <= KL + KU) ? A[(size_t)i * lda + c] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 32; ++u) {
+      const int q = base + 64 * u + lane;
+      if (q < total) s[q] = v[u];
+    }
+  }
+  for (int i = lane; i < rows; i += 64) xs[i] = i < n ? b[i] : 0.0;
+  int first_zero = 0;
+  int ju = 0;
+  __syncthreads();
+  for (int k = 0; k < n; ++k) {
+    double* colk = s + (size_t)k * W + KL;  // (row k, column k)
+    double a[KL + 1];
+#pragma unroll
+    for (int i = 0; i <= KL; ++i) a[i] = colk[i * D];
+    double amax = fabs(a[0]);
+    int ip = 0;
+#pragma unroll
+    for (int i = 1; i <= KL; ++i) {
+      const double t = fabs(a[i]);
+      const bool take = t > amax;
+      amax = take ? t : amax;
+      ip = take ? i : ip;
+    }
+    if (amax != 0.0) {
+      const int p = k + ip;
+      ju = max(ju, min(p + KU, n - 1));
+      const int nj = ju - k;
+      double piv = a[0];
+#pragma unroll
+      for (int i = 1; i <= KL; ++i) piv = i == ip ? a[i] : piv;
+      const double rp = 1.0 / piv;
+      double l[KL + 1];
+      double lme = 0.0;
+#pragma unroll
+      for (int i = 1; i <= KL; ++i) {
+        l[i] = (i == ip ? a[0] : a[i]) * rp;
+        lme = i == lane ? l[i] : lme;
+      }
+      if (lane >= 1 && lane <= nj) {
+        double* uk = colk + lane;        // (row k, column k + j)
+        double* up = uk + ip * D;        // (row p, column k + j)
+        const double u = *up;
+        const double t = *uk;
+        *uk = u;
+        *up = t;
+        double r[KL + 1];
+#pragma unroll
+        for (int i = 1; i <= KL; ++i) r[i] = uk[i * D];
+#pragma unroll
+        for (int i = 1; i <= KL; ++i) uk[i * D] = r[i] - l[i] * u;
+      }
+      const double bk = xs[p];
+      const bool mine = lane >= 1 && lane <= KL;
+      const int rl = mine ? k + lane : k;
+      const double bi = (mine && lane == ip) ? xs[k] : xs[rl];
+      if (lane == 0) {
+        colk[0] = piv;
+        xs[k] = bk;
+      }
+      if (mine) {
+        colk[lane * D] = lme;
+        xs[rl] = bi - lme * bk;
+      }
+    } else if (first_zero == 0) {
+      first_zero = k + 1;
+    }
+    __syncthreads();
+  }
+  for (int j = n - 1; j >= 0; --j) {
+    const double xj = xs[j];
+    double v = xj;
+    if (xj != 0.0) v = xj / s[(size_t)j * W + KL];
+    __syncthreads();
+    if (xj != 0.0) {
+      const int kk = min(KL + KU, j);
+      if (lane >= 1 && lane <= kk) xs[j - lane] -= v * s[(size_t)(j - lane) * W + (lane + KL)];
+      if (lane == 0) xs[j] = v;
+    }
+    __syncthreads();
+  }
+  for (int i = lane; i < n; i += 64) x[i] = xs[i];
+  if (lane == 0 && info) *info = first_zero;
+}
+
 extern "C" size_t fj_band_solve_capacity(void) { return FJ_BAND_LDS_DOUBLES; }
 
 extern "C" int fj_band_solve(const double* A, int n, int lda, int kl, int ku, const double* b,
                              double* x, int32_t* info, void* stream) {
   if (n <= 0 || lda < n || kl < 0 || ku < 0 || kl > 63 || !A || !b || !x) return FJ_ERR_ARG;
   if ((size_t)n * (size_t)(2 * kl + ku + 2) > (size_t)FJ_BAND_LDS_DOUBLES) return FJ_ERR_ARG;
-  hipLaunchKernelGGL(k_band_solve, dim3(1), dim3(64), 0, (hipStream_t)stream, A, n, lda, kl, ku, b,
-                     x, info);
+  if (kl == 11 && ku == 11 && (size_t)(n + 11) * 36 <= (size_t)FJ_BAND_LDS_DOUBLES)
+    hipLaunchKernelGGL((k_band_solve_fixed<11, 11>), dim3(1), dim3(64), 0, (hipStream_t)stream, A,
+                       n, lda, b, x, info);
+  else if (kl <= 15)
+    hipLaunchKernelGGL(k_band_solve<15>, dim3(1), dim3(64), 0, (hipStream_t)stream, A, n, lda, kl, ku,
+                       b, x, info);
+  else
+    hipLaunchKernelGGL(k_band_solve<63>, dim3(1), dim3(64), 0, (hipStream_t)stream, A, n, lda, kl, ku,
+                       b, x, info);
   FJ_CHECK_LAUNCH();
   return FJ_OK;
 }
