@@ -331,6 +331,17 @@ class BeamStructure:
         self._norms = torch.zeros(8, dtype=F64, device=dev)
         self._norms_host = torch.zeros(8, dtype=F64).pin_memory()
 
+    def set_stiffness_scaling(self, scale):
+        """Scale each cross-section stiffness of the k_cs table (one factor
+        per table entry) for the following solves; the design's k_cs is
+        kept.  The element table is rewritten in place on the device."""
+        self._need_design()
+        sc = _host(scale).reshape(-1)
+        if sc.shape != (self.k_cs.shape[0],):
+            raise ValueError(f"scale must have shape ({self.k_cs.shape[0]},)")
+        k = self.k_cs * sc[:, None, None]
+        self._tables["k_cs"].copy_(to_device(k[self.k_cs_index].reshape(-1) if self.n_elem else np.zeros(36)))
+
     # ------------------------------------------------------------------ helpers
     def _need_design(self):
         if self._model is None:

@@ -1,0 +1,472 @@
+"""Differentiable beam residual and the implicit derivatives of the device
+beam's Newton solves (the structural half of the reference's coupled
+adjoint, coupled/gradients/coupled.py:195-353 and :540-629, which apply
+jax.vjp to the beam residual of structure/beam.py:1749-1843).
+
+The forward solves run on the device (``BeamStructure``'s Newton loops); this
+module restates the residual they drive to zero as batched torch expressions
+so that torch's autograd gives its exact derivatives:
+
+* SE(3) maps of algebra/so3.py and algebra/se3.py with the reference's
+  small-angle switches (the value of each branch where it is used; the
+  exact branch is evaluated on a guarded argument elsewhere so that its
+  gradient stays finite);
+* element configuration d = log(H_A^-1 H_B) (structure/beam.py:1482-1500),
+  strain, internal forces -P^T K eps (structure/beam.py:1329-1345), the
+  inertial and gyroscopic forces -M_t v_dot - C_L v (structure/utils.py
+  _integrate_m_l, _integrate_c_t; C_L needs dQ/dd along d_dot, taken here
+  with torch.func.jvp), dead loads rotated into the node frames
+  (structure/beam.py:1387-1395);
+* ``StaticBeamSolve`` / ``DynamicBeamSolve``: autograd Functions whose
+  forward is the device Newton solve and whose backward solves the
+  transposed Jacobian of the residual at the converged state (implicit
+  function theorem) and pulls the adjoint back through the residual's
+  inputs (loads, stiffness scalings, previous-step state).
+
+The implicit derivative is exact for a converged solve; run the forward with
+tight structural tolerances when gradients are wanted.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+F64 = torch.float64
+SMALL = 1e-10  # utils/constants.py:13 (SMALL_ANG_THRESH)
+COS_SMALL = float(np.cos(SMALL))
+LOBATTO_MID = 0.5
+LEGENDRE_S = (0.5 - 0.5 * np.sqrt(3.0 / 5.0), 0.5, 0.5 + 0.5 * np.sqrt(3.0 / 5.0))
+LEGENDRE_W = (5.0 / 9.0, 8.0 / 9.0, 5.0 / 9.0)
+
+
+# ---------------------------------------------------------------------------
+# so(3) / se(3), algebra/so3.py and algebra/se3.py; vectors h = (u, omega)
+# ---------------------------------------------------------------------------
+def skew(v):
+    z = torch.zeros_like(v[..., 0])
+    return torch.stack((torch.stack((z, -v[..., 2], v[..., 1]), -1),
+                        torch.stack((v[..., 2], z, -v[..., 0]), -1),
+                        torch.stack((-v[..., 1], v[..., 0], z), -1)), -2)
+
+
+def vee(m):
+    return 0.5 * torch.stack((m[..., 2, 1] - m[..., 1, 2], m[..., 0, 2] - m[..., 2, 0], m[..., 1, 0] - m[..., 0, 1]),
+                             -1)
+
+
+def _n2(v):
+    return (v * v).sum(-1)
+
+
+def _eye(n, like):
+    return torch.eye(n, dtype=like.dtype, device=like.device).expand(*like.shape[:-1], n, n)
+
+
+def _guard(x, cond):
+    """x where cond holds, 1 elsewhere (keeps the unused branch finite)."""
+    return torch.where(cond, x, torch.ones_like(x))
+
+
+def alpha_fn(b):
+    n2 = _n2(b)
+    big = n2 > SMALL * SMALL  # |b| > SMALL
+    nb = torch.sqrt(_guard(n2, big))
+    return torch.where(big, torch.sin(nb) / nb, 1.0 - n2 / 6.0)
+
+
+def beta_fn(b):
+    n2 = _n2(b)
+    big = n2 > SMALL * SMALL
+    n2g = _guard(n2, big)
+    return torch.where(big, 2.0 * (1.0 - torch.cos(torch.sqrt(n2g))) / n2g, 0.5 - n2 / 24.0)
+
+
+def t_so3(w):
+    s = skew(w)
+    n2 = _n2(w)
+    big = n2 > SMALL
+    n2g = _guard(n2, big)
+    wg = torch.where(big[..., None], w, torch.ones_like(w))
+    ss = s @ s
+    exact = _eye(3, w) - 0.5 * beta_fn(wg)[..., None, None] * s + ((1.0 - alpha_fn(wg)) / n2g)[..., None, None] * ss
+    return torch.where(big[..., None, None], exact, _eye(3, w) - s / 2.0 + ss / 6.0)
+
+
+def t_inv_so3(w):
+    s = skew(w)
+    n2 = _n2(w)
+    big = n2 > SMALL * SMALL  # |w| > SMALL
+    n2g = _guard(n2, big)
+    wg = torch.where(big[..., None], w, torch.ones_like(w))
+    ss = s @ s
+    exact = _eye(3, w) + 0.5 * s + ((1.0 - alpha_fn(wg) / beta_fn(wg)) / n2g)[..., None, None] * ss
+    return torch.where(big[..., None, None], exact, _eye(3, w) + s / 2.0 + ss / 12.0)
+
+
+def exp_so3(w):
+    s = skew(w)
+    n2 = _n2(w)
+    big = n2 > SMALL
+    n2g = _guard(n2, big)
+    a = torch.sqrt(n2g)
+    ss = s @ s
+    exact = _eye(3, w) + (torch.sin(a) / a)[..., None, None] * s + ((1.0 - torch.cos(a)) / n2g)[..., None, None] * ss
+    return torch.where(big[..., None, None], exact, _eye(3, w) + s + ss / 2.0)
+
+
+def log_so3(r):
+    """algebra/so3.py:log_so3, switch on cos(theta) < cos(SMALL).  The angle
+    is taken as atan2(|vee(R - R^T)|, cos theta), the same angle as the
+    reference's arccos(cos theta) but with a derivative that stays accurate
+    for small rotations."""
+    ct = 0.5 * (r[..., 0, 0] + r[..., 1, 1] + r[..., 2, 2] - 1.0)
+    big = ct < COS_SMALL
+    sv = 0.5 * vee(r - r.transpose(-1, -2))  # sin(theta) n
+    s2 = _n2(sv)
+    sg = torch.sqrt(_guard(s2, big & (s2 > 0.0)))
+    th = torch.atan2(sg, ct)
+    fac = torch.where(big & (s2 > 0.0), th / sg, torch.ones_like(th))
+    exact = fac[..., None] * sv
+    g = r - _eye(3, r[..., 0])
+    small = vee(g - (g @ g) / 2.0)
+    return torch.where(big[..., None], exact, small)
+
+
+def hat6(h):
+    z = torch.zeros_like(skew(h[..., :3]))
+    su, sw = skew(h[..., :3]), skew(h[..., 3:])
+    return torch.cat((torch.cat((sw, su), -1), torch.cat((z, sw), -1)), -2)
+
+
+def check6(h):
+    z = torch.zeros_like(skew(h[..., :3]))
+    su, sw = skew(h[..., :3]), skew(h[..., 3:])
+    return torch.cat((torch.cat((z, -su), -1), torch.cat((-su, -sw), -1)), -2)
+
+
+def tilde4(h):
+    sw = skew(h[..., 3:])
+    top = torch.cat((sw, h[..., :3, None]), -1)
+    bottom = torch.zeros_like(top[..., :1, :])
+    return torch.cat((top, bottom), -2)
+
+
+def t_u_omega_plus(h):
+    """algebra/se3.py:t_u_omega_plus (Sonneville 2013 eq. A.12)."""
+    a, b = h[..., :3], h[..., 3:]
+    n2 = _n2(b)
+    big = n2 > SMALL
+    n2g = _guard(n2, big)
+    bg = torch.where(big[..., None], b, torch.ones_like(b))
+    sa, sb = skew(a), skew(b)
+    al, be = alpha_fn(bg), beta_fn(bg)
+    ba = (b * a).sum(-1)
+    c = lambda x: x[..., None, None]  # noqa: E731
+    exact = (-0.5 * c(be) * sa + c((1.0 - al) / n2g) * (sa @ sb + sb @ sa)
+             + c(ba / n2g) * (c(be - al) * sb + c(0.5 * be - 3.0 * (1.0 - al) / n2g) * (sb @ sb)))
+    return torch.where(c(big), exact, -0.5 * sa)
+
+
+def _block(tl, tr, br):
+    z = torch.zeros_like(tl)
+    return torch.cat((torch.cat((tl, tr), -1), torch.cat((z, br), -1)), -2)
+
+
+def t_se3(h):
+    big = _n2(h[..., 3:]) > SMALL
+    hg = torch.where(big[..., None], h, torch.ones_like(h))
+    t = t_so3(hg[..., 3:])
+    exact = _block(t, t_u_omega_plus(hg), t)
+    a = hat6(h)
+    return torch.where(big[..., None, None], exact, _eye(6, h) - a / 2.0 + (a @ a) / 6.0)
+
+
+def t_inv_se3(h):
+    big = _n2(h[..., 3:]) > SMALL
+    hg = torch.where(big[..., None], h, torch.ones_like(h))
+    ti = t_inv_so3(hg[..., 3:])
+    exact = _block(ti, -ti @ t_u_omega_plus(hg) @ ti, ti)
+    a = hat6(h)
+    return torch.where(big[..., None, None], exact, _eye(6, h) + a / 2.0 + (a @ a) / 12.0)
+
+
+def exp_se3(h):
+    big = _n2(h[..., 3:]) > SMALL
+    hg = torch.where(big[..., None], h, torch.ones_like(h))
+    r = exp_so3(hg[..., 3:])
+    p = (t_so3(hg[..., 3:]).transpose(-1, -2) @ hg[..., :3, None])
+    top = torch.cat((r, p), -1)
+    bottom = torch.zeros_like(top[..., :1, :])
+    bottom = torch.cat((bottom[..., :3], torch.ones_like(bottom[..., 3:])), -1)
+    exact = torch.cat((top, bottom), -2)
+    x = tilde4(h)
+    return torch.where(big[..., None, None], exact, _eye(4, h[..., :4]) + x + (x @ x) / 2.0)
+
+
+def log_se3(g):
+    w = log_so3(g[..., :3, :3])
+    big = _n2(w) > SMALL
+    wg = torch.where(big[..., None], w, torch.ones_like(w))
+    exact = torch.cat(((t_inv_so3(wg).transpose(-1, -2) @ g[..., :3, 3, None])[..., 0], w), -1)
+    e = g - _eye(4, g[..., 0])
+    lg = e - (e @ e) / 2.0
+    small = torch.cat((lg[..., :3, 3], vee(lg[..., :3, :3])), -1)
+    return torch.where(big[..., None], exact, small)
+
+
+def hg_inv(g):
+    rt = g[..., :3, :3].transpose(-1, -2)
+    p = -(rt @ g[..., :3, 3, None])
+    top = torch.cat((rt, p), -1)
+    return torch.cat((top, g[..., 3:, :]), -2)
+
+
+def chi(r):
+    z = torch.zeros_like(r)
+    return torch.cat((torch.cat((r, z), -1), torch.cat((z, r), -1)), -2)
+
+
+def p_mat(d, ad):
+    """algebra/se3.py:p: [-T^-1(-d) Ad, T^-1(d) Ad] (6 x 12)."""
+    tp = t_inv_se3(d)
+    tn = tp - hat6(d)
+    return torch.cat((-(tn @ ad), tp @ ad), -1)
+
+
+def q_mat(s, d, ad):
+    """algebra/se3.py:q with t_star = s T(s d) T^-1(d)."""
+    ts = s * (t_se3(s * d) @ t_inv_se3(d))
+    return torch.cat(((_eye(6, d) - ts) @ ad, ts @ ad), -1)
+
+
+# ---------------------------------------------------------------------------
+# Beam residual, structure/beam.py:1749-1843 with structure/utils.py
+# ---------------------------------------------------------------------------
+class BeamTables:
+    """The element data of a beam as torch tensors: connectivity (n_elem, 2),
+    element triads o0 (n_elem, 3, 3, columns x y z), lengths l0, reference
+    configurations d0, stiffness and mass tables k_cs / m_cs (n, 6, 6) with
+    the per-element indices into them."""
+
+    def __init__(self, n_nodes, conn, o0, l0, d0, k_cs, m_cs, k_index, m_index, device):
+        dev = torch.device(device)
+        t = lambda x: torch.as_tensor(np.asarray(x, dtype=np.float64), device=dev)  # noqa: E731
+        self.n_nodes = int(n_nodes)
+        self.conn = torch.as_tensor(np.asarray(conn, dtype=np.int64).reshape(-1, 2), device=dev)
+        self.n_elem = self.conn.shape[0]
+        o0 = t(o0)
+        self.base = torch.zeros((self.n_elem, 4, 4), dtype=F64, device=dev)
+        self.base[:, :3, :3] = o0
+        self.base[:, 3, 3] = 1.0
+        self.ad = chi(o0.transpose(-1, -2))
+        self.l0, self.d0 = t(l0), t(d0)
+        self.k_cs, self.m_cs = t(k_cs), t(m_cs)
+        self.k_index = torch.as_tensor(np.asarray(k_index, dtype=np.int64), device=dev)
+        self.m_index = torch.as_tensor(np.asarray(m_index, dtype=np.int64), device=dev)
+        idx = torch.cat((6 * self.conn[:, :1] + torch.arange(6, device=dev), 6 * self.conn[:, 1:] + torch.arange(6, device=dev)),
+                        1)
+        self.elem_dofs = idx  # (n_elem, 12)
+
+    @classmethod
+    def from_beam(cls, beam):
+        """From a device ``BeamStructure`` with its design variables set."""
+        return cls(beam.n_nodes, beam._conn, beam.o0, beam.l0, beam.d0, beam.k_cs, beam.m_cs, beam.k_cs_index,
+                   beam.m_cs_index, beam.hg0.device)
+
+    def elem_vec(self, x):
+        return x.reshape(-1)[self.elem_dofs]
+
+    def assemble(self, fe):
+        out = torch.zeros(6 * self.n_nodes, dtype=fe.dtype, device=fe.device)
+        return out.index_add(0, self.elem_dofs.reshape(-1), fe.reshape(-1))
+
+
+def element_d(tb: BeamTables, hg):
+    """Element configurations d = log(H_A^-1 H_B) (structure/beam.py:1482-1500)."""
+    ha = hg[tb.conn[:, 0]] @ tb.base
+    hb = hg[tb.conn[:, 1]] @ tb.base
+    return log_se3(hg_inv(ha) @ hb)
+
+
+def dead_to_local(f, hg):
+    r = hg[..., :3, :3]
+    return torch.cat(((r.transpose(-1, -2) @ f[..., :3, None])[..., 0], (r.transpose(-1, -2) @ f[..., 3:, None])[..., 0]),
+                     -1)
+
+
+def beam_residual(tb: BeamTables, hg, k_scale=None, f_follower=None, f_dead=None, v=None, v_dot=None):
+    """Full residual vector (6 n_nodes) f_int (+ f_iner + f_gyr) + follower +
+    R^T dead (structure/beam.py:1749-1843); zero at a converged solve's
+    solve DOFs.  ``k_scale`` multiplies each cross-section stiffness of the
+    k_cs table (None: 1); v, v_dot (n_nodes, 6) select the dynamic form."""
+    d = element_d(tb, hg)
+    k = tb.k_cs if k_scale is None else tb.k_cs * k_scale[:, None, None]
+    ke = k[tb.k_index]
+    p = p_mat(d, tb.ad)
+    eps = (d - tb.d0) / tb.l0[:, None]
+    fe = -(p.transpose(-1, -2) @ (ke @ eps[..., None]))[..., 0]
+    if v is not None:
+        me = tb.m_cs[tb.m_index]
+        ve = tb.elem_vec(v)
+        vde = tb.elem_vec(v_dot)
+        ad, l = tb.ad, tb.l0
+        mad = ad.transpose(-1, -2) @ me @ ad
+        z = torch.zeros_like(mad)
+        f0 = torch.cat((torch.cat((mad, z), -1), torch.cat((z, z), -1)), -2)
+        f1 = torch.cat((torch.cat((z, z), -1), torch.cat((z, mad), -1)), -2)
+        qm = q_mat(LOBATTO_MID, d, ad)
+        m_t = l[:, None, None] * (0.5 * (2.0 / 6.0 * (f0 + f1) + 4.0 / 3.0 * (qm.transpose(-1, -2) @ me @ qm)))
+        d_dot = (p @ ve[..., None])[..., 0]
+        c_l = torch.zeros_like(m_t)
+        for s, w in zip(LEGENDRE_S, LEGENDRE_W):
+            q, qd = torch.func.jvp(lambda dd, s=s: q_mat(s, dd, ad), (d,), (d_dot,))
+            u = (q @ ve[..., None])[..., 0]
+            c_l = c_l + w * (q.transpose(-1, -2) @ (me @ qd - hat6(u).transpose(-1, -2) @ me @ q))
+        c_l = 0.5 * l[:, None, None] * c_l
+        fe = fe - (m_t @ vde[..., None])[..., 0] - (c_l @ ve[..., None])[..., 0]
+    f = tb.assemble(fe)
+    if f_follower is not None:
+        f = f + f_follower.reshape(-1)
+    if f_dead is not None:
+        f = f + dead_to_local(f_dead, hg).reshape(-1)
+    return f
+
+
+# ---------------------------------------------------------------------------
+# Implicit derivatives of the device Newton solves
+# ---------------------------------------------------------------------------
+def lie_gradient(hg, g_hg):
+    """The gradient of a scalar with matrix gradient g_hg at hg, expressed in
+    the right-trivialised increments xi of hg exp(xi) (n_nodes, 6)."""
+    with torch.enable_grad():
+        xi = torch.zeros(hg.shape[:-2] + (6,), dtype=F64, device=hg.device, requires_grad=True)
+        (g,) = torch.autograd.grad((hg.detach() @ exp_se3(xi) * g_hg).sum(), (xi,))
+    return g
+
+
+class BeamSolver:
+    """What the implicit Functions need from a device beam: the element tables,
+    the solve DOFs and the two device Newton solves."""
+
+    def __init__(self, beam, prescribed_dofs=(), time_integrator=None):
+        self.beam = beam
+        self.tables = BeamTables.from_beam(beam)
+        pre = beam.make_prescribed_dofs_tuple(prescribed_dofs)
+        self.prescribed = pre
+        self.sd, self.n_s = beam._solve_dofs(pre)
+        self.sd_long = self.sd.long()
+        self.ti = time_integrator
+        n = beam.n_nodes
+        dev = beam.hg0.device
+        self._bufs = (torch.empty((n, 4, 4), dtype=F64, device=dev), torch.empty((n, 6, 6), dtype=F64, device=dev))
+        self.iters = []
+
+    def static(self, f_dead, k_scale):
+        self.beam.set_stiffness_scaling(k_scale)
+        out = self.beam.static_solve(self.prescribed, f_ext_aero=f_dead.contiguous())
+        self.iters.append(out.iters[0])
+        return out.hg
+
+    def dynamic(self, hg_prev, phi, v, v_dot, f_dead, k_scale):
+        b = self.beam
+        b.set_stiffness_scaling(k_scale)
+        phi, v, v_dot = phi.clone().contiguous(), v.clone().contiguous(), v_dot.clone().contiguous()
+        self.iters.append(b._newton_dynamic(hg_prev.contiguous(), phi, v, v_dot, None, f_dead.contiguous(), self.sd,
+                                            self.n_s, self.ti, self._bufs))
+        return phi
+
+    def reduce(self, x):
+        return x.reshape(-1)[self.sd_long]
+
+    def expand(self, xs, like):
+        out = torch.zeros(like.numel(), dtype=F64, device=like.device)
+        return out.index_put((self.sd_long,), xs).reshape(like.shape)
+
+    def solve_transposed(self, jac, g):
+        return torch.linalg.solve(jac.transpose(0, 1), g)
+
+
+class StaticBeamSolve(torch.autograd.Function):
+    """hg = the static Newton solution under global dead loads f_dead
+    (n_nodes, 6) with stiffness scalings k_scale (structure/beam.py:1878-2092
+    on the device).  Backward: J^T mu = dL/dxi at the solution, J = dR/dxi of
+    R(hg exp(xi)) over the solve DOFs, then -mu^T dR/d(f_dead, k_scale)
+    (coupled/gradients/coupled.py:195-353)."""
+
+    @staticmethod
+    def forward(ctx, f_dead, k_scale, solver):
+        hg = solver.static(f_dead.detach(), k_scale.detach())
+        ctx.solver = solver
+        ctx.save_for_backward(hg, f_dead, k_scale)
+        return hg
+
+    @staticmethod
+    def backward(ctx, g_hg):
+        hg, f_dead, k_scale = ctx.saved_tensors
+        sv = ctx.solver
+        tb = sv.tables
+        g = sv.reduce(lie_gradient(hg, g_hg))
+        fd, ks = f_dead.detach(), k_scale.detach()
+
+        def res(xs):
+            xi = sv.expand(xs, fd)
+            return sv.reduce(beam_residual(tb, hg @ exp_se3(xi), ks, None, fd))
+
+        jac = torch.func.jacrev(res)(torch.zeros(sv.n_s, dtype=F64, device=hg.device))
+        mu = sv.solve_transposed(jac, g)
+        with torch.enable_grad():
+            fd_ = fd.clone().requires_grad_(True)
+            ks_ = ks.clone().requires_grad_(True)
+            r = sv.reduce(beam_residual(tb, hg, ks_, None, fd_))
+            gf, gk = torch.autograd.grad(r, (fd_, ks_), -mu)
+        return gf, gk, None
+
+
+class DynamicBeamSolve(torch.autograd.Function):
+    """The solve-DOF components of phi_a, the generalised-alpha Newton
+    solution of one load step from the start values (phi0, v0, v_dot0) at
+    the alpha point (structure/beam.py:2170-2316): R = 0 with
+    phi_a = phi0 except at the solve DOFs, hg_u = hg_prev exp(phi_a),
+    v_a = v0 + gamma' (phi_a - phi0), v_dot_a = v_dot0 + beta' (phi_a - phi0)
+    and global dead loads f_dead.  Backward as StaticBeamSolve, with the
+    Jacobian over the solve DOFs including the velocity and acceleration
+    terms (coupled/gradients/coupled.py:540-629).  ``full_state`` composes
+    phi_a, v_a and v_dot_a from the result differentiably."""
+
+    @staticmethod
+    def forward(ctx, hg_prev, phi0, v0, vd0, f_dead, k_scale, solver):
+        phi = solver.dynamic(hg_prev.detach(), phi0.detach(), v0.detach(), vd0.detach(), f_dead.detach(),
+                             k_scale.detach())
+        phi_s = solver.reduce(phi)
+        ctx.solver = solver
+        ctx.save_for_backward(phi_s, hg_prev, phi0, v0, vd0, f_dead, k_scale)
+        return phi_s
+
+    @staticmethod
+    def full_state(sv, phi_s, phi0, v0, vd0):
+        phi = phi0 + sv.expand(phi_s - sv.reduce(phi0), phi0)
+        dphi = phi - phi0
+        return phi, v0 + sv.ti.gamma_prime * dphi, vd0 + sv.ti.beta_prime * dphi
+
+    @staticmethod
+    def residual(sv, phi_s, hg_prev, phi0, v0, vd0, f_dead, k_scale):
+        phi, v, vd = DynamicBeamSolve.full_state(sv, phi_s, phi0, v0, vd0)
+        return sv.reduce(beam_residual(sv.tables, hg_prev @ exp_se3(phi), k_scale, None, f_dead, v, vd))
+
+    @staticmethod
+    def backward(ctx, g_phi_s):
+        phi_s, *inputs = ctx.saved_tensors
+        sv = ctx.solver
+        inputs = [x.detach() for x in inputs]
+
+        def res(xs):
+            return DynamicBeamSolve.residual(sv, phi_s + xs, *inputs)
+
+        jac = torch.func.jacrev(res)(torch.zeros(sv.n_s, dtype=F64, device=phi_s.device))
+        mu = sv.solve_transposed(jac, g_phi_s)
+        with torch.enable_grad():
+            leaves = [x.clone().requires_grad_(True) for x in inputs]
+            r = DynamicBeamSolve.residual(sv, phi_s, *leaves)
+            grads = torch.autograd.grad(r, leaves, -mu, allow_unused=True)
+        return (*grads, None)
