@@ -108,7 +108,8 @@ _SIGS = {
     "fj_strip_alpha": ([c_void_p, c_void_p, c_int, c_int, POINTER(FjFlowfield), c_double, c_double, c_void_p,
                         c_void_p], c_int),
     "fj_pair_sum_backward": ([c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, c_double, c_double, c_void_p,
-                              c_int, c_void_p, c_void_p], c_int),
+                              c_int, c_void_p, c_void_p, c_void_p], c_int),
+    "fj_pair_sum_backward_workspace_bytes": ([c_int, c_int], c_size_t),
     "fj_batched_task_bytes": ([c_int], c_size_t),
     "fj_batched_workspace_bytes": ([c_int, c_void_p, c_void_p], c_size_t),
     "fj_pair_sum_batched": ([c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,

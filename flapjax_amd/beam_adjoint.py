@@ -383,8 +383,54 @@ class BeamSolver:
         out = torch.zeros(like.numel(), dtype=F64, device=like.device)
         return out.index_put((self.sd_long,), xs).reshape(like.shape)
 
+    def _node_coloring(self):
+        """Colours of the nodes such that no two nodes of one colour share an
+        element neighbour (distance-2 colouring of the element graph): a
+        residual row of node i depends only on the states of i and its
+        element neighbours, so the columns of one colour never meet in a row."""
+        n = self.tables.n_nodes
+        nb = [{i} for i in range(n)]
+        for a, b in self.tables.conn.tolist():
+            nb[a].add(b)
+            nb[b].add(a)
+        colour = [-1] * n
+        for i in range(n):
+            used = {colour[k] for j in nb[i] for k in nb[j] if colour[k] >= 0}
+            c = 0
+            while c in used:
+                c += 1
+            colour[i] = c
+        near = torch.zeros((n, n), dtype=torch.bool)
+        for i in range(n):
+            near[i, list(nb[i])] = True
+        return torch.as_tensor(colour), near
+
+    def jacobian(self, res, x):
+        """d res / d x over the solve DOFs (n_s x n_s), exact, from one
+        forward-mode product per (colour, DOF component) instead of one per
+        column."""
+        if getattr(self, "_colour", None) is None:
+            colour, near = self._node_coloring()
+            sd = self.sd_long.cpu()
+            node, comp = sd // 6, sd % 6
+            seed = colour[node] * 6 + comp
+            n_col = int(seed.max()) + 1 if len(sd) else 0
+            dev = x.device
+            self._colour = (seed.to(dev), n_col, near[node][:, node].to(dev))
+        seed, n_col, mask = self._colour
+        v = torch.zeros((self.n_s, n_col), dtype=F64, device=x.device)
+        v[torch.arange(self.n_s, device=x.device), seed] = 1.0
+        jv = torch.func.vmap(lambda t: torch.func.jvp(res, (x,), (t,))[1], in_dims=1, out_dims=1)(v)
+        return torch.where(mask, jv[:, seed], torch.zeros((), dtype=F64, device=x.device))
+
     def solve_transposed(self, jac, g):
-        return torch.linalg.solve(jac.transpose(0, 1), g)
+        from . import _kernels as K
+
+        bw = getattr(self.beam, "_bandwidth", None) if hasattr(self, "beam") else None
+        jt = jac.transpose(0, 1).contiguous()
+        if jac.is_cuda and bw is not None and K.band_fits(self.n_s, bw, bw):
+            return K.band_solve(jt, g.contiguous(), bw, bw)
+        return torch.linalg.solve(jt, g)
 
 
 class StaticBeamSolve(torch.autograd.Function):
@@ -413,8 +459,7 @@ class StaticBeamSolve(torch.autograd.Function):
             xi = sv.expand(xs, fd)
             return sv.reduce(beam_residual(tb, hg @ exp_se3(xi), ks, None, fd))
 
-        jac = torch.func.jacrev(res)(torch.zeros(sv.n_s, dtype=F64, device=hg.device))
-        mu = sv.solve_transposed(jac, g)
+        mu = sv.solve_transposed(sv.jacobian(res, torch.zeros(sv.n_s, dtype=F64, device=hg.device)), g)
         with torch.enable_grad():
             fd_ = fd.clone().requires_grad_(True)
             ks_ = ks.clone().requires_grad_(True)
@@ -463,7 +508,7 @@ class DynamicBeamSolve(torch.autograd.Function):
         def res(xs):
             return DynamicBeamSolve.residual(sv, phi_s + xs, *inputs)
 
-        jac = torch.func.jacrev(res)(torch.zeros(sv.n_s, dtype=F64, device=phi_s.device))
+        jac = sv.jacobian(res, torch.zeros(sv.n_s, dtype=F64, device=phi_s.device))
         mu = sv.solve_transposed(jac, g_phi_s)
         with torch.enable_grad():
             leaves = [x.clone().requires_grad_(True) for x in inputs]

@@ -107,6 +107,78 @@ __global__ __launch_bounds__(FJ_BWD_THREADS) void k_bwd_targets(
   }
 }
 
+// The same sum spread over the whole chip: workgroup (tile, chunk) takes 64
+// targets (a lane each) against one contiguous chunk of filaments, its four
+// waves taking every fourth filament of the chunk; the waves' sums meet in LDS
+// in wave order and the chunk's partial (3 x n_tgt per chunk) is summed over
+// chunks in chunk order by k_bwd_targets_reduce, so the result does not depend
+// on scheduling.  A few thousand targets (the force midpoints of a lattice)
+// otherwise occupy only a few dozen workgroups.
+#define FJ_BWD_PART_WAVES 4
+#define FJ_BWD_TARGET_BLOCKS 2048
+#define FJ_BWD_MIN_CHUNK 256
+
+template <int KIND>
+__global__ __launch_bounds__(64 * FJ_BWD_PART_WAVES) void k_bwd_targets_part(
+    const double* __restrict__ seg, int n_seg, const double* __restrict__ tgt,
+    const double* __restrict__ w, int n_tgt, FjKernelParams kp, int chunk_len,
+    double* __restrict__ part) {
+  __shared__ double red[FJ_BWD_PART_WAVES][3][64];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int t = blockIdx.x * 64 + lane;
+  const int tc = t < n_tgt ? t : n_tgt - 1;
+  const double x = tgt[3 * tc], y = tgt[3 * tc + 1], z = tgt[3 * tc + 2];
+  const double wx = w[3 * tc], wy = w[3 * tc + 1], wz = w[3 * tc + 2];
+  const int s0 = blockIdx.y * chunk_len;
+  const int s1 = min(n_seg, s0 + chunk_len);
+  double ax = 0.0, ay = 0.0, az = 0.0;
+  for (int s = s0 + wave; s < s1; s += FJ_BWD_PART_WAVES) {
+    const double* sp = seg + (size_t)s * FJ_SEG_STRIDE;
+    const double g = sp[10];
+    double G1[3], G2[3], wK;
+    fj_bs_vjp<KIND>(x - sp[0], y - sp[1], z - sp[2], x - sp[3], y - sp[4], z - sp[5], wx, wy, wz,
+                    kp, G1, G2, wK);
+    ax += g * (G1[0] + G2[0]);
+    ay += g * (G1[1] + G2[1]);
+    az += g * (G1[2] + G2[2]);
+  }
+  red[wave][0][lane] = ax;
+  red[wave][1][lane] = ay;
+  red[wave][2][lane] = az;
+  __syncthreads();
+  if (wave == 0 && t < n_tgt) {
+    for (int k = 0; k < 3; ++k) {
+      double a = red[0][k][lane];
+      for (int q = 1; q < FJ_BWD_PART_WAVES; ++q) a += red[q][k][lane];
+      part[((size_t)blockIdx.y * 3 + k) * n_tgt + t] = a;
+    }
+  }
+}
+
+__global__ void k_bwd_targets_reduce(const double* __restrict__ part, int n_chunk, int n_tgt,
+                                     double* __restrict__ gx, int accumulate) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;  // (target, component)
+  if (i >= 3 * n_tgt) return;
+  const int t = i / 3, k = i - 3 * t;
+  double a = 0.0;
+  for (int c = 0; c < n_chunk; ++c) a += part[((size_t)c * 3 + k) * n_tgt + t];
+  a *= FJ_INV_4PI;
+  gx[i] = accumulate ? gx[i] + a : a;
+}
+
+static int fj_bwd_chunks(int n_tgt, int n_seg) {
+  const int tiles = fj_cdiv(n_tgt, 64);
+  int chunks = fj_cdiv(FJ_BWD_TARGET_BLOCKS, tiles);
+  chunks = min(chunks, max(1, fj_cdiv(n_seg, FJ_BWD_MIN_CHUNK)));
+  return max(1, min(chunks, 65535));
+}
+
+extern "C" size_t fj_pair_sum_backward_workspace_bytes(int n_tgt, int n_seg) {
+  if (n_tgt <= 0 || n_seg <= 0) return 256;
+  return ((size_t)fj_bwd_chunks(n_tgt, n_seg) * 3 * n_tgt * sizeof(double) + 255) / 256 * 256;
+}
+
 // dL/dy0_s, dL/dy1_s (3 each) and dL/dg_s, one filament per lane
 template <int KIND>
 __global__ __launch_bounds__(FJ_BWD_THREADS) void k_bwd_segments(
@@ -139,7 +211,7 @@ __global__ __launch_bounds__(FJ_BWD_THREADS) void k_bwd_segments(
 extern "C" int fj_pair_sum_backward(const double* seg, int n_seg, const double* targets,
                                     int n_tgt, const double* grad_v, int kernel, double eps,
                                     double rcut, double* grad_targets, int accumulate,
-                                    double* grad_seg, void* stream) {
+                                    double* grad_seg, void* workspace, void* stream) {
   if (n_seg < 0 || n_tgt < 0) return FJ_ERR_ARG;
   if (kernel < FJ_KERNEL_EPSILON || kernel > FJ_KERNEL_PLAIN) return FJ_ERR_ARG;
   if (n_seg == 0 || n_tgt == 0) return FJ_OK;
@@ -147,9 +219,18 @@ extern "C" int fj_pair_sum_backward(const double* seg, int n_seg, const double* 
   hipStream_t st = (hipStream_t)stream;
   FjKernelParams kp;
   kp.eps = eps; kp.eps2 = eps * eps; kp.rcut = rcut;
+  const int n_chunk = fj_bwd_chunks(n_tgt, n_seg);
+  const int chunk_len = fj_cdiv(n_seg, n_chunk);
 #define FJ_BWD(K)                                                                                  \
   do {                                                                                             \
-    if (grad_targets)                                                                              \
+    if (grad_targets && workspace) {                                                               \
+      hipLaunchKernelGGL(k_bwd_targets_part<K>, dim3(fj_cdiv(n_tgt, 64), fj_cdiv(n_seg, chunk_len)),\
+                         dim3(64 * FJ_BWD_PART_WAVES), 0, st, seg, n_seg, targets, grad_v, n_tgt,  \
+                         kp, chunk_len, (double*)workspace);                                       \
+      hipLaunchKernelGGL(k_bwd_targets_reduce, dim3(fj_cdiv(3 * n_tgt, 256)), dim3(256), 0, st,    \
+                         (const double*)workspace, fj_cdiv(n_seg, chunk_len), n_tgt, grad_targets, \
+                         accumulate);                                                              \
+    } else if (grad_targets)                                                                       \
       hipLaunchKernelGGL(k_bwd_targets<K>, dim3(fj_cdiv(n_tgt, FJ_BWD_THREADS)),                   \
                          dim3(FJ_BWD_THREADS), 0, st, seg, n_seg, targets, grad_v, n_tgt, kp,      \
                          grad_targets, accumulate);                                                \

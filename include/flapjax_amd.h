@@ -157,7 +157,12 @@ int fj_pair_sum_lattices(const double* seg, int n_lat, const fj_lattice_ref* lat
  * this from jax.vjp of compute_v_ind, aero/aic.py:248-310). */
 int fj_pair_sum_backward(const double* seg, int n_seg, const double* targets, int n_tgt,
                          const double* grad_v, int kernel, double eps, double rcut,
-                         double* grad_targets, int accumulate, double* grad_seg, void* stream);
+                         double* grad_targets, int accumulate, double* grad_seg, void* workspace,
+                         void* stream);
+/* workspace of fj_pair_sum_backward (may be NULL: dL/dx then takes one pass
+ * of a thread per target, which leaves most of the chip idle for a few
+ * thousand targets). */
+size_t fj_pair_sum_backward_workspace_bytes(int n_tgt, int n_seg);
 
 /* Fused pair sums of many independent cases in one launch (serving loop).
  * Per task i (HOST arrays of n_tasks): targets rows [tgt_off, tgt_off+n_tgt)

@@ -129,6 +129,26 @@ def _fd_check(fun, args, idx, rng, h=1e-6, tol=1e-6, manifold=False):
         assert abs(float(fd - an)) <= tol * max(1.0, abs(float(an))), (i, float(fd), float(an))
 
 
+def test_coloured_jacobian_equals_dense():
+    """The coloured forward-mode Jacobian equals torch.func.jacrev."""
+    b, tb, rng = _beam(6)
+    n = b.n_nodes
+    ti = TimeIntegrator(0.9, 0.01)
+    hgp = torch.as_tensor(np.stack([b.hg0[i] @ B.exp_se3(rng.normal(size=6) * 0.1) for i in range(n)]))
+    sv = _TorchSolver(tb, hgp, range(6), ti)
+    phi0, v0, vd0, f = (torch.as_tensor(rng.normal(size=(n, 6)) * s) for s in (0.05, 0.5, 5.0, 20.0))
+    ks = torch.as_tensor(rng.uniform(0.8, 1.2, b.k_cs.shape[0]))
+
+    def res(xs):
+        return BA.DynamicBeamSolve.residual(sv, sv.reduce(phi0) + xs, hgp, phi0, v0, vd0, f, ks)
+
+    x = torch.zeros(sv.n_s, dtype=F64)
+    jc = sv.jacobian(res, x)
+    jd = torch.func.jacrev(res)(x)
+    assert torch.allclose(jc, jd, rtol=1e-12, atol=1e-12 * float(jd.abs().max()))
+    assert sv._colour[1] == 18  # a chain needs three node colours
+
+
 def test_static_solve_gradient():
     b, tb, rng = _beam(5)
     n = b.n_nodes
