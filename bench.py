@@ -206,7 +206,7 @@ def coupled_design(seed=3, stations=CPL_STATIONS):
 
 
 def make_coupled(precision="fp64", m=CPL_M, n=CPL_N, m_star=CPL_M_STAR, stations=CPL_STATIONS, span=10.0,
-                 free_wake=False):
+                 free_wake=False, aic_reuse=None):
     """The coupled gust model on the device (n elements, n + 1 nodes, two
     elements per stiffness station)."""
     import flapjax_amd as fj
@@ -221,7 +221,7 @@ def make_coupled(precision="fp64", m=CPL_M, n=CPL_N, m_star=CPL_M_STAR, stations
     coords = np.zeros((nn, 3))
     coords[:, 1] = np.linspace(0.0, span, nn)
     uv = UVLM([(m, n, m_star)], np.arange(nn), mirror_point=np.zeros(3), mirror_normal=np.array((0.0, 1.0, 0.0)),
-              free_wake=free_wake, precision=precision)
+              free_wake=free_wake, precision=precision, aic_reuse=aic_reuse)
     cw = CoupledAeroelastic(beam, uv)
     a = np.deg2rad(5.0)
     u_inf = CPL_U * np.array((np.cos(a), 0.0, np.sin(a)))
@@ -262,6 +262,103 @@ def coupled_bench(precision, steps=CPL_STEPS):
             "cases_per_s": round(1.0 / (t2 - t0), 4), "tip_z_min_max": [round(float(tip.min()), 6),
                                                                         round(float(tip.max()), 6)],
             "lift_max": round(float(lift.max()), 6)}
+
+
+ADJ_STEPS, ADJ_TIP_WEIGHT, ADJ_CHECKPOINT = 100, 1.0, 10
+ADJ_CHECK_STEPS, ADJ_CHECK_NEWTON, ADJ_CHECK_FSI = 10, 8, 10
+
+
+def _adjoint_case(steps, fixed_iterations):
+    """The coupled gust model, its differentiable rerun and the production
+    objective as a function of (stiffness scalings, twist angles)."""
+    import torch
+
+    from flapjax_amd.adjoint import twisted_grid
+    from flapjax_amd.beam import ConvergenceSettings
+    from flapjax_amd.coupled_adjoint import DifferentiableAeroelastic
+
+    # the check factors every circulation system afresh (as jnp.linalg.solve
+    # does): the kept-inverse refinement stops at dsgesv's backward-error
+    # bound, which leaves ~cond(A) sqrt(N) eps (~1e-8 here) of noise in the
+    # circulation, too much for differences
+    cw = make_coupled("fp64", aic_reuse=not fixed_iterations)
+    if fixed_iterations:
+        # zero tolerances: every Newton and FSI loop runs its full count, far
+        # past convergence, so the objective is smooth for the differences
+        cw.structure.struct_convergence_settings = ConvergenceSettings(ADJ_CHECK_NEWTON, 0.0, 0.0, 0.0, 0.0)
+        cw.fsi_convergence_settings = ConvergenceSettings(ADJ_CHECK_FSI, 0.0, 0.0, 0.0, 0.0)
+    zeta0 = cw.aero.zeta_b0[0]
+    model = DifferentiableAeroelastic(cw, np.arange(6), checkpoint_every=ADJ_CHECKPOINT)
+
+    def production(k_, t_):
+        z0 = twisted_grid(zeta0, torch.cat((torch.zeros(1, dtype=torch.float64, device="cuda"), t_))).contiguous()
+        cw.aero.set_design_variables(cw.aero.dt, cw.aero.flowfield, z0, cw.structure.hg0)
+        cw.structure.set_stiffness_scaling(k_)
+        st = cw.static_solve(np.arange(6))
+        dyn = cw.dynamic_solve(st, np.arange(6), steps)
+        f = dyn.aero.f_steady[0] + dyn.aero.f_unsteady[0]
+        zt = dyn.structure.hg[:, -1, 2, 3]
+        return float(cw.aero.dt * (f[1:, ..., 2].sum(dim=(1, 2)) + ADJ_TIP_WEIGHT * (zt[1:] - zt[0]) ** 2).sum())
+
+    return model, production
+
+
+def adjoint_bench(steps=ADJ_STEPS, n_dir=2, seed=5):
+    """Gradient of J = dt sum_n [lift_n + w (z_tip,n - z_tip,0)^2] over the
+    coupled gust case shortened to ``steps`` steps, with respect to the 40
+    stiffness scalings and 80 spanwise twist angles: one reverse pass
+    (checkpoint every 10 steps) at the solvers' default tolerances, timed end
+    to end beside the production forward solve.  The check: the same case
+    over its first 10 steps with every Newton and FSI loop run a fixed number
+    of iterations, gradient against fourth-order central differences of the
+    production solve along ``n_dir`` seeded random directions."""
+    import torch
+
+    ks = torch.ones(CPL_STATIONS, dtype=torch.float64, device="cuda")
+    tw = torch.zeros(CPL_N, dtype=torch.float64, device="cuda")
+    model, production = _adjoint_case(steps, False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    J, g_k, g_t = model.gradient(ks, tw, steps, ADJ_TIP_WEIGHT)
+    torch.cuda.synchronize()
+    t_grad = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    J_prod = production(ks, tw)
+    t_fwd = time.perf_counter() - t1
+    out = {"workload": f"coupled gust case, {steps} steps; d/d(40 stiffness scalings, 80 twist angles) of "
+                       f"dt sum (lift + {ADJ_TIP_WEIGHT} (z_tip - z_tip0)^2), default solver tolerances",
+           "gradient_s": round(t_grad, 3), "production_forward_s": round(t_fwd, 3),
+           "gradient_over_forward": round(t_grad / t_fwd, 2), "J": float(J),
+           "J_production_rel_diff": float("%.3g" % (abs(float(J) - J_prod) / abs(J_prod))),
+           "grad_norm_k": float(g_k.norm()), "grad_norm_twist": float(g_t.norm()),
+           "checkpoint_every": ADJ_CHECKPOINT, "fsi_iters_mean": round(float(np.mean(model.fsi_iters)), 3)}
+    model, production = _adjoint_case(ADJ_CHECK_STEPS, True)
+    t2 = time.perf_counter()
+    J, g_k, g_t = model.gradient(ks, tw, ADJ_CHECK_STEPS, ADJ_TIP_WEIGHT)
+    torch.cuda.synchronize()
+    t_chk = time.perf_counter() - t2
+    # noise floor of the production objective: two designs 1e-13 apart
+    j_a, j_b = production(ks, tw), production(ks * (1.0 + 1e-13), tw)
+    noise = abs(j_a - j_b) / abs(j_a)
+    rng = np.random.default_rng(seed)
+    errs = []
+    for _ in range(n_dir):
+        dk = torch.as_tensor(rng.normal(size=CPL_STATIONS), device="cuda")
+        dt_ = torch.as_tensor(rng.normal(size=CPL_N), device="cuda")
+        hk, ht = 1e-2, 1e-3
+        at = lambda s: production(ks + s * hk * dk, tw + s * ht * dt_)  # noqa: E731
+        fd = (-at(2) + 8 * at(1) - 8 * at(-1) + at(-2)) / 12.0
+        an = float((g_k * dk).sum()) * hk + float((g_t * dt_).sum()) * ht
+        errs.append(abs(fd - an) / max(abs(fd), 1e-300))
+    out["check"] = {"steps": ADJ_CHECK_STEPS, "newton_iters": ADJ_CHECK_NEWTON, "fsi_iters": ADJ_CHECK_FSI,
+                    "gradient_s": round(t_chk, 3), "J_production_rel_diff":
+                        float("%.3g" % (abs(float(J) - production(ks, tw)) / abs(float(J)))),
+                    "fd_rel_err": [float("%.3g" % e) for e in errs], "J_noise_rel": float("%.3g" % noise),
+                    "note": "differences resolve the gradient only to ~J_noise / h: the unsteady force "
+                            "amplifies circulation rounding by relax/dt, which keeps the production "
+                            "objective noisy at this size"}
+    out["peak_mem_gb"] = round(torch.cuda.max_memory_allocated() / 1e9, 3)
+    return out
 
 
 def rollup_roofline(reps=20):
@@ -509,6 +606,9 @@ def main():
     ap.add_argument("--coupled-only", action="store_true",
                     help="only the coupled gust response (beam + UVLM FSI), both precision forms")
     ap.add_argument("--coupled-steps", type=int, default=CPL_STEPS)
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="only the coupled adjoint gradient (shortened gust case)")
+    ap.add_argument("--adjoint-steps", type=int, default=ADJ_STEPS)
     ap.add_argument("--profile-kernels", action="store_true",
                     help="rollup, LU, inverse, convection and copy kernels a fixed number of times (for counter runs)")
     args = ap.parse_args()
@@ -530,6 +630,9 @@ def main():
     if args.coupled_only:
         print(json.dumps({"coupled_gust": {p: coupled_bench(p, args.coupled_steps) for p in ("fp64", "mixed")}}))
         return
+    if args.adjoint_only:
+        print(json.dumps({"coupled_adjoint": adjoint_bench(args.adjoint_steps)}))
+        return
     pairs, rollup_pairs = pairs_per_step()
     f64 = time_form("fp64", args.steps, args.warmup)
     mix = time_form("mixed", args.steps, args.warmup)
@@ -545,6 +648,8 @@ def main():
                          "makespan_s": round(st["makespan_s"], 3)}
         srv["mixed_speedup_vs_fp64"] = round(srv["mixed"]["cases_per_s"] / srv["fp64"]["cases_per_s"], 3)
     steady = static_bench()
+    coupled = {p: coupled_bench(p, args.coupled_steps) for p in ("fp64", "mixed")} if args.coupled_steps > 0 else None
+    adjoint = adjoint_bench(args.adjoint_steps) if args.adjoint_steps > 0 else None
     sweep = pair_sweep()
     conv = convection_bench()
     cpu = None if args.no_cpu else cpu_baseline(sample)
@@ -585,6 +690,8 @@ def main():
                               + (f"min(steps, {args.serve_wake_cap})" if args.serve_wake_cap else "steps")
                               + ", pitch 0-5 deg, k 0.05-0.5", **srv},
         "steady_case": steady,
+        "coupled_gust": coupled,
+        "coupled_adjoint": adjoint,
         "pair_sweep_gpairs_s": sweep,
         "convection": conv,
         "cpu_baseline": cpu,

@@ -303,27 +303,38 @@ def beam_residual(tb: BeamTables, hg, k_scale=None, f_follower=None, f_dead=None
     d = element_d(tb, hg)
     k = tb.k_cs if k_scale is None else tb.k_cs * k_scale[:, None, None]
     ke = k[tb.k_index]
-    p = p_mat(d, tb.ad)
+    ad = tb.ad
+    tinv = t_inv_se3(d)
+    p = torch.cat((-((tinv - hat6(d)) @ ad), tinv @ ad), -1)  # p_mat with T^-1(d) shared
     eps = (d - tb.d0) / tb.l0[:, None]
     fe = -(p.transpose(-1, -2) @ (ke @ eps[..., None]))[..., 0]
     if v is not None:
         me = tb.m_cs[tb.m_index]
         ve = tb.elem_vec(v)
         vde = tb.elem_vec(v_dot)
-        ad, l = tb.ad, tb.l0
+        l = tb.l0
         mad = ad.transpose(-1, -2) @ me @ ad
         z = torch.zeros_like(mad)
         f0 = torch.cat((torch.cat((mad, z), -1), torch.cat((z, z), -1)), -2)
         f1 = torch.cat((torch.cat((z, z), -1), torch.cat((z, mad), -1)), -2)
-        qm = q_mat(LOBATTO_MID, d, ad)
-        m_t = l[:, None, None] * (0.5 * (2.0 / 6.0 * (f0 + f1) + 4.0 / 3.0 * (qm.transpose(-1, -2) @ me @ qm)))
+        # Q at the Lobatto midpoint and the three Legendre points in one batch,
+        # with its derivative along d_dot (the reference's q_dot JVP)
+        s_all = torch.tensor((LOBATTO_MID,) + tuple(LEGENDRE_S), dtype=F64, device=d.device)[:, None, None]
+        eye = torch.eye(6, dtype=F64, device=d.device)
+
+        def qs(dd):
+            ts = s_all[..., None] * (t_se3(s_all * dd[None]) @ t_inv_se3(dd)[None])
+            return torch.cat(((eye - ts) @ ad, ts @ ad), -1)
+
         d_dot = (p @ ve[..., None])[..., 0]
-        c_l = torch.zeros_like(m_t)
-        for s, w in zip(LEGENDRE_S, LEGENDRE_W):
-            q, qd = torch.func.jvp(lambda dd, s=s: q_mat(s, dd, ad), (d,), (d_dot,))
-            u = (q @ ve[..., None])[..., 0]
-            c_l = c_l + w * (q.transpose(-1, -2) @ (me @ qd - hat6(u).transpose(-1, -2) @ me @ q))
-        c_l = 0.5 * l[:, None, None] * c_l
+        q, qd = torch.func.jvp(qs, (d,), (d_dot,))
+        qm = q[0]
+        m_t = l[:, None, None] * (0.5 * (2.0 / 6.0 * (f0 + f1) + 4.0 / 3.0 * (qm.transpose(-1, -2) @ me @ qm)))
+        ql, qdl = q[1:], qd[1:]
+        u = (ql @ ve[None, ..., None])[..., 0]
+        terms = ql.transpose(-1, -2) @ (me @ qdl - hat6(u).transpose(-1, -2) @ me @ ql)
+        w = torch.tensor(LEGENDRE_W, dtype=F64, device=d.device)[:, None, None, None]
+        c_l = 0.5 * l[:, None, None] * (w * terms).sum(0)
         fe = fe - (m_t @ vde[..., None])[..., 0] - (c_l @ ve[..., None])[..., 0]
     f = tb.assemble(fe)
     if f_follower is not None:
