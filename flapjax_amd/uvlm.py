@@ -135,7 +135,6 @@ class UVLM:
         # wake, the dynamic one does not
         self._solvers = {True: RefinedSolver(), False: RefinedSolver()} if self.aic_reuse else None
         self._refactor = False
-        self._stale = {True: False, False: False}
         self._aic = None
         self._bufs = {}
 
@@ -275,9 +274,8 @@ class UVLM:
         if self._solvers is None:
             return K.lu_solve(K.lu_factor(A, overwrite=True), rhs)
         solver = self._solvers[bool(static)]
-        if self._refactor or self._stale[bool(static)]:
+        if self._refactor:
             solver.factor(A)
-            self._stale[bool(static)] = False
         self._last_system = (A, rhs)
         return solver.submit(A, rhs)
 
@@ -288,12 +286,6 @@ class UVLM:
         the step again."""
         res = step()
         if self._confirm(static):
-            # a solve that needed every correction the refinement allows is
-            # about to fail: refactor before the next one instead of after it
-            # (a failed solve costs the whole step again)
-            sv = self._solvers[bool(static)] if self._solvers is not None else None
-            if sv is not None and sv.last_iters >= sv.max_iter:
-                self._stale[bool(static)] = True
             return res
         self._refactor = True
         try:
@@ -545,9 +537,8 @@ class UVLM:
             gout.copy_(K.lu_solve(K.lu_factor(A, overwrite=True), rhs))
         else:
             solver = self._solvers[False]
-            if self._refactor or self._stale[False]:
+            if self._refactor:
                 solver.factor(A)
-                self._stale[False] = False
             self._last_system = (A, rhs)
             solver.submit(A, rhs, out=gout)
         if not single:
