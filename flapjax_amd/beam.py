@@ -341,6 +341,7 @@ class BeamStructure:
             raise ValueError(f"scale must have shape ({self.k_cs.shape[0]},)")
         k = self.k_cs * sc[:, None, None]
         self._tables["k_cs"].copy_(to_device(k[self.k_cs_index].reshape(-1) if self.n_elem else np.zeros(36)))
+        self.stiffness_scaling = sc.copy()
 
     # ------------------------------------------------------------------ helpers
     def _need_design(self):

@@ -303,6 +303,11 @@ class DifferentiableAeroelastic:
         """(J, dJ/dk_scale, dJ/dtwist) by the reverse pass."""
         ks = torch.as_tensor(k_scale, dtype=F64, device=self.beam.hg0.device).detach().clone().requires_grad_(True)
         tw = torch.as_tensor(twist, dtype=F64, device=self.beam.hg0.device).detach().clone().requires_grad_(True)
-        J = self.objective(ks, tw, n_tstep, tip_weight)
-        g_k, g_t = torch.autograd.grad(J, (ks, tw))
+        before = getattr(self.beam, "stiffness_scaling", None)
+        try:
+            J = self.objective(ks, tw, n_tstep, tip_weight)
+            g_k, g_t = torch.autograd.grad(J, (ks, tw))
+        finally:
+            # the solves rescale the beam's device tables; leave them as found
+            self.beam.set_stiffness_scaling(np.ones(self.beam.k_cs.shape[0]) if before is None else before)
         return J.detach(), g_k, g_t
