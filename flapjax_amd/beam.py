@@ -388,8 +388,11 @@ class BeamStructure:
         if not n_s:
             return r
         bw = self._bandwidth
-        if K.band_fits(n_s, bw, bw):
-            return K.band_solve(A, r, bw, bw)
+        key = (n_s, bw)
+        if getattr(self, "_band_key", None) != key:
+            self._band_key, self._band_ok = key, K.band_fits(n_s, bw, bw)
+        if self._band_ok:
+            return K.band_solve(A, r, bw, bw, out=self._sys_bufs[n_s][3])
         return K.lu_solve(K.lu_factor(A, overwrite=True), r)
 
     def _force(self, f, name, shape):
@@ -421,9 +424,16 @@ class BeamStructure:
         return out
 
     def _system(self, dynamic, hg, v, v_dot, t_phi, f_fol, f_dead, sd, n_s, ti=None):
-        A = torch.empty((n_s, n_s), dtype=F64, device=hg.device)
-        r = torch.empty(n_s, dtype=F64, device=hg.device)
-        fa = torch.empty(n_s, dtype=F64, device=hg.device)
+        # one set of system buffers per size, reused by every Newton iteration
+        # (the previous iteration's system is consumed before the next is built)
+        buf = self._sys_bufs.get(n_s) if hasattr(self, "_sys_bufs") else None
+        if buf is None or buf[0].device != hg.device:
+            buf = (torch.empty((n_s, n_s), dtype=F64, device=hg.device), torch.empty(n_s, dtype=F64, device=hg.device),
+                   torch.empty(n_s, dtype=F64, device=hg.device), torch.empty(n_s, dtype=F64, device=hg.device))
+            if not hasattr(self, "_sys_bufs"):
+                self._sys_bufs = {}
+            self._sys_bufs[n_s] = buf
+        A, r, fa = buf[0], buf[1], buf[2]
         bp = ti.beta_prime if ti is not None else 0.0
         gp = ti.gamma_prime if ti is not None else 0.0
         rc = L.lib().fj_beam_system(ctypes.c_void_p(ctypes.addressof(self._model)), int(dynamic), L.ptr(hg), L.ptr(v), L.ptr(v_dot),
