@@ -163,9 +163,14 @@ __global__ __launch_bounds__(FJ_PANEL_THREADS) void k_lu_panel(T* __restrict__ A
 
 // Columns outside the panel, one thread each: the panel's row exchanges
 // (laswp), then U12 = L11^{-1} A12 for the columns right of the panel (trsm).
+// With a two-level blocking (below) the exchanges of a sub-panel reach every
+// column while its U12 is only formed inside the current outer block
+// (trsm_end); the outer block's own U12 is then formed once, without the
+// exchanges (do_swap = 0).
 template <typename T, int JB>
 __global__ __launch_bounds__(256) void k_lu_swap_trsm(T* __restrict__ A, int n, int lda, int k,
-                                                      int jb, const int32_t* __restrict__ ipiv) {
+                                                      int jb, const int32_t* __restrict__ ipiv,
+                                                      int trsm_end, int do_swap) {
   __shared__ T s_L11[JB][JB + 1];
   __shared__ T s_u[JB][256];
   __shared__ int s_ipiv[JB];
@@ -179,15 +184,17 @@ __global__ __launch_bounds__(256) void k_lu_swap_trsm(T* __restrict__ A, int n, 
   int c = blockIdx.x * blockDim.x + tid;
   if (c >= n - jb) return;
   if (c >= k) c += jb;  // skip the panel columns
-  for (int jj = 0; jj < jb; ++jj) {
-    const int p = s_ipiv[jj];
-    if (p != k + jj) {
-      const T t0 = A[(size_t)(k + jj) * lda + c];
-      A[(size_t)(k + jj) * lda + c] = A[(size_t)p * lda + c];
-      A[(size_t)p * lda + c] = t0;
+  if (do_swap) {
+    for (int jj = 0; jj < jb; ++jj) {
+      const int p = s_ipiv[jj];
+      if (p != k + jj) {
+        const T t0 = A[(size_t)(k + jj) * lda + c];
+        A[(size_t)(k + jj) * lda + c] = A[(size_t)p * lda + c];
+        A[(size_t)p * lda + c] = t0;
+      }
     }
   }
-  if (c >= k + jb) {
+  if (c >= k + jb && c < trsm_end) {
     // forward substitution with the column staged in this thread's LDS slot
     for (int r = 0; r < jb; ++r) s_u[r][tid] = A[(size_t)(k + r) * lda + c];
     for (int r = 1; r < jb; ++r) {
@@ -234,9 +241,11 @@ struct FjMfma<float> {
 #define FJ_GT 64  // C tile edge
 #define FJ_GK 32  // max K
 
+// Columns limited to [k + kb, c_end) (the outer block during a two-level
+// factorisation), rows to [k + kb, n).
 template <typename T>
 __global__ __launch_bounds__(256) void k_lu_update(T* __restrict__ A, int n, int lda, int k,
-                                                   int kb) {
+                                                   int kb, int c_end) {
   __shared__ T sL[FJ_GT][FJ_GK + 1];
   __shared__ T sU[FJ_GK][FJ_GT + 1];
   const int base = k + kb;
@@ -249,7 +258,7 @@ __global__ __launch_bounds__(256) void k_lu_update(T* __restrict__ A, int n, int
   }
   for (int e = tid; e < FJ_GK * FJ_GT; e += 256) {
     const int q = e / FJ_GT, j = e % FJ_GT;
-    sU[q][j] = (c0 + j < n && q < kb) ? A[(size_t)(k + q) * lda + c0 + j] : T(0);
+    sU[q][j] = (c0 + j < c_end && q < kb) ? A[(size_t)(k + q) * lda + c0 + j] : T(0);
   }
   __syncthreads();
   const int lane = tid & 63, wave = tid >> 6;
@@ -280,7 +289,7 @@ __global__ __launch_bounds__(256) void k_lu_update(T* __restrict__ A, int n, int
       for (int v = 0; v < 4; ++v) {
         const int i = r0 + wr + a * 16 + FjMfma<T>::row(lk, v);
         const int j = c0 + wc + b * 16 + li;
-        if (i < n && j < n) A[(size_t)i * lda + j] -= acc[a][b][v];
+        if (i < n && j < c_end) A[(size_t)i * lda + j] -= acc[a][b][v];
       }
 }
 
@@ -294,20 +303,42 @@ static int fj_lu_factor_t(T* A, int n, int lda, int32_t* ipiv, int32_t* info, T*
                           hipStream_t st) {
   hipLaunchKernelGGL(k_perm_init, dim3(fj_cdiv(n, 256)), dim3(256), 0, st, ipiv, n, info);
   FJ_CHECK_LAUNCH();
-  for (int k = 0; k < n; k += JB) {
-    const int jb = (n - k < JB) ? n - k : JB;
-    hipLaunchKernelGGL((k_lu_panel<T, R, JB>), dim3(1), dim3(FJ_PANEL_THREADS), 0, st, A, n, lda, k,
-                       ipiv, info);
-    FJ_CHECK_LAUNCH();
-    if (n - jb > 0) {
-      hipLaunchKernelGGL((k_lu_swap_trsm<T, JB>), dim3(fj_cdiv(n - jb, 256)), dim3(256), 0, st, A, n,
-                         lda, k, jb, ipiv);
+  // Two-level blocking: register panels of JB columns (as wide as one
+  // workgroup's registers allow at this n) inside outer blocks of FJ_GK
+  // columns.  Sub-panel updates stay inside the outer block; the trailing
+  // matrix gets one rank-FJ_GK matrix-core update per outer block instead of
+  // FJ_GK / JB narrow ones, each of which streams the whole trailing matrix.
+  constexpr int NB = FJ_GK;
+  for (int k0 = 0; k0 < n; k0 += NB) {
+    const int nb = (n - k0 < NB) ? n - k0 : NB;
+    const int kend = k0 + nb;
+    for (int k = k0; k < kend; k += JB) {
+      const int jb = (n - k < JB) ? n - k : JB;
+      hipLaunchKernelGGL((k_lu_panel<T, R, JB>), dim3(1), dim3(FJ_PANEL_THREADS), 0, st, A, n, lda,
+                         k, ipiv, info);
       FJ_CHECK_LAUNCH();
+      if (n - jb > 0) {
+        hipLaunchKernelGGL((k_lu_swap_trsm<T, JB>), dim3(fj_cdiv(n - jb, 256)), dim3(256), 0, st, A,
+                           n, lda, k, jb, ipiv, JB == NB ? n : kend, 1);
+        FJ_CHECK_LAUNCH();
+      }
+      const int c_end = JB == NB ? n : kend;
+      const int rows = n - k - jb, cols = c_end - k - jb;
+      if (rows > 0 && cols > 0) {
+        const dim3 g(fj_cdiv(cols, FJ_GT), fj_cdiv(rows, FJ_GT));
+        hipLaunchKernelGGL(k_lu_update<T>, g, dim3(256), 0, st, A, n, lda, k, jb, c_end);
+        FJ_CHECK_LAUNCH();
+      }
     }
-    const int rest = n - k - jb;
-    if (rest > 0) {
+    if (JB < NB && kend < n) {
+      // the outer block's U12 (its exchanges are already applied) and the
+      // rank-nb trailing update
+      hipLaunchKernelGGL((k_lu_swap_trsm<T, NB>), dim3(fj_cdiv(n - nb, 256)), dim3(256), 0, st, A, n,
+                         lda, k0, nb, ipiv, n, 0);
+      FJ_CHECK_LAUNCH();
+      const int rest = n - kend;
       const dim3 g(fj_cdiv(rest, FJ_GT), fj_cdiv(rest, FJ_GT));
-      hipLaunchKernelGGL(k_lu_update<T>, g, dim3(256), 0, st, A, n, lda, k, jb);
+      hipLaunchKernelGGL(k_lu_update<T>, g, dim3(256), 0, st, A, n, lda, k0, nb, n);
       FJ_CHECK_LAUNCH();
     }
   }
