@@ -92,7 +92,7 @@ __device__ __forceinline__ void fj_beam_elem_vec(const FjBeamDev& bm, const doub
 }
 
 template <int DYN>
-__global__ void k_beam_points(FjBeamDev bm, const double* __restrict__ hg, const double* __restrict__ v,
+__global__ __launch_bounds__(64) void k_beam_points(FjBeamDev bm, const double* __restrict__ hg, const double* __restrict__ v,
                               double* __restrict__ ws) {
   const int slots = DYN ? 5 : 1;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
