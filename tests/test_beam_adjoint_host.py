@@ -184,3 +184,24 @@ def test_dynamic_solve_gradient():
     # the step's accelerations scale rounding in phi by beta' ~ 1/dt^2, so
     # the differences take a wider step (fourth-order stencil)
     _fd_check(fun, [hgp, phi0, v0, vd0, f, ks], [0, 1, 2, 3, 4, 5], rng, h=1e-3, manifold=True)
+
+
+@pytest.mark.parametrize("relative", [True, False])
+def test_flow_expression_matches_oracle(relative):
+    """The torch flowfield of the coupled adjoint (coupled_adjoint.flow_at)
+    equals the oracle's 1-cos gust (aero/flowfields.py:101-181) and constant
+    flow at points inside, outside and on both sides of the gust."""
+    from flapjax_amd.coupled_adjoint import flow_at
+    from flapjax_amd.flowfields import Constant, OneMinusCosine
+    from oracle import uvlm_oracle as U
+
+    rng = np.random.default_rng(3)
+    u = np.array((10.0, 0.0, 1.0))
+    x = rng.uniform(-8.0, 8.0, size=(200, 3))
+    for t in (0.0, 0.3, 0.9):
+        g = OneMinusCosine(u, 1.225, relative, 2.5, 0.7, gust_x0=(-4.0, 0.0, 0.0))
+        og = U.OneMinusCosine(u, 1.225, relative, gust_length=2.5, gust_amplitude=0.7,
+                              gust_x0=np.array((-4.0, 0.0, 0.0)))
+        assert np.allclose(flow_at(g, torch.as_tensor(x), t).numpy(), og(x, t), rtol=0, atol=1e-14)
+        c = Constant(u, 1.225, relative)
+        assert np.allclose(flow_at(c, torch.as_tensor(x), t).numpy(), U.Constant(u, 1.225, relative)(x, t))
