@@ -55,8 +55,8 @@ def pairs_per_step(m=M, n=N, m_star=M_STAR):
 
 
 def pitch_plunge(n, span, n_tstep, dt):
-    """hg (n_tstep, n+1, 4, 4) and hg_dot (skew rate convention of the
-    reference's prescribed-motion tests) for the pitch-plunge motion."""
+    """hg (n_tstep, n+1, 4, 4) and hg_dot = d(hg)/dt (exact rotation rate
+    theta_dot dR/dtheta plus the plunge rate) for the pitch-plunge motion."""
     omega = 2.0 * K_RED  # k = omega c / (2 U), c = U = 1
     t = np.arange(n_tstep) * dt
     th = np.deg2rad(ALPHA0) + np.deg2rad(PITCH) * np.sin(omega * t)
@@ -74,8 +74,10 @@ def pitch_plunge(n, span, n_tstep, dt):
     hg[:, :, 2, 3] = h[:, None]
     hg[:, :, 3, 3] = 1.0
     hgd = np.zeros_like(hg)
-    hgd[:, :, 0, 2] = thd[:, None]
-    hgd[:, :, 2, 0] = -thd[:, None]
+    hgd[:, :, 0, 0] = (-s * thd)[:, None]
+    hgd[:, :, 0, 2] = (c * thd)[:, None]
+    hgd[:, :, 2, 0] = (-c * thd)[:, None]
+    hgd[:, :, 2, 2] = (-s * thd)[:, None]
     hgd[:, :, 2, 3] = hd[:, None]
     return hg, hgd
 
@@ -174,7 +176,43 @@ def time_form(precision, steps, warmup, aic_reuse=True, whole_case=True):
     torch.cuda.synchronize()
     case_s = time.perf_counter() - tc
     return {"wall_s": wall, "ms_per_step": 1e3 * wall / steps, "p50_ms": float(np.percentile(lat, 50)),
-            "p99_ms": float(np.percentile(lat, 99)), "case_s": case_s}
+            "p99_ms": float(np.percentile(lat, 99)), "case_s": case_s, "case": c2}
+
+
+FIXTURE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden", "cfg1_free_wake_16x64.npz")
+
+
+def errors_vs_oracle(case):
+    """Per-result errors (max |a - b| / max |b|) of a whole 200-step case
+    against the oracle's FP64 run of the same workload (the committed
+    fixture tests/golden/cfg1_free_wake_16x64.npz), beside the oracle's own
+    rounding floor (its run with every pair sum summed in reverse order):
+    gamma_b over steps 1-50 (max) and at step 199, and gamma_w, zeta_w,
+    f_steady, f_unsteady, alpha and the induced velocity at the collocation
+    points at steps 50 and 199."""
+    if not os.path.exists(FIXTURE):
+        return None
+    ref = np.load(FIXTURE)
+
+    def rel(a, b):
+        return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+    kept = [int(k) for k in ref["steps_kept"]]
+    g = case.gamma_b[0][kept].cpu().numpy()
+    eg = [rel(g[i], ref["gamma_b"][i]) for i in range(len(kept))]
+    early = [i for i, k in enumerate(kept) if k <= 50]
+    out = {"gamma_b_steps_1_50": float("%.3g" % max(eg[i] for i in early)),
+           "gamma_b_199": float("%.3g" % eg[-1])}
+    floor = {"gamma_b_steps_1_50": float("%.3g" % max(ref["floor_gamma_b"][i] for i in early)),
+             "gamma_b_199": float("%.3g" % ref["floor_gamma_b"][-1])}
+    for st in (int(x) for x in ref["state_steps"]):
+        for key in ("gamma_w", "zeta_w", "f_steady", "f_unsteady", "alpha"):
+            out[f"{key}_{st}"] = float("%.3g" % rel(getattr(case, key)[0][st].cpu().numpy(), ref[f"{key}_{st}"]))
+            floor[f"{key}_{st}"] = float("%.3g" % ref[f"floor_{key}_{st}"])
+        v = case.get_v_ind(case.c[0][st], st).cpu().numpy()
+        out[f"v_ind_c_{st}"] = float("%.3g" % rel(v, ref[f"v_ind_c_{st}"]))
+        floor[f"v_ind_c_{st}"] = float("%.3g" % ref[f"floor_v_ind_c_{st}"])
+    return {"err": out, "oracle_floor": floor}
 
 
 # Coupled gust response: a cantilever beam along y (span 10) of 40 stiffness
@@ -236,6 +274,11 @@ def coupled_bench(precision, steps=CPL_STEPS):
     (FSI iterations of beam Newton solves and UVLM steps) end to end."""
     import torch
 
+    # warm-up on an instance of its own (first-use costs of the beam and
+    # coupling kernels and of the caching allocator stay out of the timing)
+    warm = make_coupled(precision)
+    warm.dynamic_solve(warm.static_solve(np.arange(6)), np.arange(6), 3)
+    del warm
     cw = make_coupled(precision)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -548,6 +591,11 @@ def serve(precision, n_cases, rate, m_star_cap, steps_range):
     return stats
 
 
+def progress(msg):
+    """A progress line on stderr (stdout carries only the JSON line)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def _timed(fn, min_seconds, max_reps=50):
     t0 = time.perf_counter()
     reps = 0
@@ -597,10 +645,15 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-cases", type=int, default=256)
-    ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-steps", type=int, nargs=2, default=(50, 400))
+    ap.add_argument("--serve-full-rate", type=float, default=2.0,
+                    help="arrival rate of the stream with wake rows = steps (below its ~2.5 cases/s capacity)")
+    ap.add_argument("--serve-full-mixed", action="store_true",
+                    help="also serve the wake rows = steps stream in the mixed form (another ~2 minutes)")
+    ap.add_argument("--serve-capped-rate", type=float, default=30.0,
+                    help="arrival rate of the capped-wake stream below its capacity (~44 cases/s)")
     ap.add_argument("--serve-wake-cap", type=int, default=32,
-                    help="wake rows per served case = min(steps, cap); 0: wake rows = steps")
+                    help="wake rows per case of the capped stream = min(steps, cap)")
     ap.add_argument("--profile-rollup", action="store_true",
                     help="only the rollup launch and a copy of known size (for counter runs)")
     ap.add_argument("--coupled-only", action="store_true",
@@ -634,22 +687,46 @@ def main():
         print(json.dumps({"coupled_adjoint": adjoint_bench(args.adjoint_steps)}))
         return
     pairs, rollup_pairs = pairs_per_step()
+    progress("cfg1 step, fp64")
     f64 = time_form("fp64", args.steps, args.warmup)
+    progress("cfg1 step, mixed")
     mix = time_form("mixed", args.steps, args.warmup)
+    par_f64 = errors_vs_oracle(f64.pop("case"))
+    par_mix = errors_vs_oracle(mix.pop("case"))
     lit = time_form("fp64", max(3, args.steps // 4), 1, aic_reuse=False, whole_case=False)
     roof, sample = rollup_roofline()
     srv = {}
     if args.serve_cases > 0:
+        def row(st):
+            return {"cases_per_s": round(st["cases_per_s"], 3), "p50_s": round(st["p50_s"], 4),
+                    "p99_s": round(st["p99_s"], 4), "gpairs_per_s": round(st["gpairs_per_s"], 3),
+                    "makespan_s": round(st["makespan_s"], 3)}
+
+        base = (f"{args.serve_cases} Poisson arrivals (seed 2), M in {{4,8,16,32}}, N in {{16,32,64,128}}, "
+                f"{args.serve_steps[0]}-{args.serve_steps[1]} steps, pitch 0-5 deg, k 0.05-0.5, free wake")
+        full = {"stream": base + f", wake rows = steps, arrivals at {args.serve_full_rate}/s (below capacity: "
+                                 "cases/s is arrival-bound, p50/p99 are service latencies)"}
+        for prec in ("fp64", "mixed") if args.serve_full_mixed else ("fp64",):
+            progress(f"serving: wake rows = steps, {prec}")
+            full[prec] = row(serve(prec, args.serve_cases, args.serve_full_rate, None, tuple(args.serve_steps)))
+        cap = {"stream": base + f", wake rows = min(steps, {args.serve_wake_cap}) (a capped stream)"}
         for prec in ("fp64", "mixed"):
-            st = serve(prec, args.serve_cases, args.serve_rate, args.serve_wake_cap or None,
-                       tuple(args.serve_steps))
-            srv[prec] = {"cases_per_s": round(st["cases_per_s"], 3), "p50_s": round(st["p50_s"], 4),
-                         "p99_s": round(st["p99_s"], 4), "gpairs_per_s": round(st["gpairs_per_s"], 3),
-                         "makespan_s": round(st["makespan_s"], 3)}
-        srv["mixed_speedup_vs_fp64"] = round(srv["mixed"]["cases_per_s"] / srv["fp64"]["cases_per_s"], 3)
+            progress(f"serving: capped wake, saturated, {prec}")
+            cap[prec + "_saturated"] = row(serve(prec, args.serve_cases, 1e4, args.serve_wake_cap,
+                                                 tuple(args.serve_steps)))
+        progress("serving: capped wake, below capacity, fp64")
+        cap[f"fp64_at_{args.serve_capped_rate:g}_per_s"] = row(
+            serve("fp64", args.serve_cases, args.serve_capped_rate, args.serve_wake_cap, tuple(args.serve_steps)))
+        cap["mixed_speedup_vs_fp64"] = round(cap["mixed_saturated"]["cases_per_s"]
+                                             / cap["fp64_saturated"]["cases_per_s"], 3)
+        srv = {"wake_rows_eq_steps": full, "capped_wake": cap}
+    progress("steady case")
     steady = static_bench()
+    progress("coupled gust")
     coupled = {p: coupled_bench(p, args.coupled_steps) for p in ("fp64", "mixed")} if args.coupled_steps > 0 else None
+    progress("coupled adjoint")
     adjoint = adjoint_bench(args.adjoint_steps) if args.adjoint_steps > 0 else None
+    progress("pair sweep, convection, cpu baseline")
     sweep = pair_sweep()
     conv = convection_bench()
     cpu = None if args.no_cpu else cpu_baseline(sample)
@@ -675,20 +752,20 @@ def main():
         "cases_per_s": round(1.0 / f64["case_s"], 3),
         "fp64_refactor_each_step": {"ms_per_step": round(lit["ms_per_step"], 4),
                                     "value": round(pairs / lit["ms_per_step"] / 1e6, 3)},
+        "parity": {"fp64": par_f64, "note": "errors of a whole 200-step case against the oracle's FP64 run "
+                   "(max |a - b| / max |b|); oracle_floor: the oracle against itself with the pair sums summed "
+                   "in reverse order -- the free-wake rollup amplifies rounding past step ~50"},
         "mixed": {"value": round(value_mixed, 3), "unit": "Gpairs/s", "dtype": "f32+f64",
                   "ms_per_step": round(mix["ms_per_step"], 4),
                   "latency_ms": {"p50": round(mix["p50_ms"], 4), "p99": round(mix["p99_ms"], 4)},
-                  "cases_per_s": round(1.0 / mix["case_s"], 3), "speedup_vs_fp64": round(value_mixed / value, 3)},
+                  "cases_per_s": round(1.0 / mix["case_s"], 3), "speedup_vs_fp64": round(value_mixed / value, 3),
+                  "err": par_mix["err"] if par_mix else None},
         "roofline": {"bound": "valu", "achieved": round(roof["fp64"]["tflops"], 3), "peak": PEAK_FP64,
                      "unit": "TFLOP/s", "frac": round(roof["fp64"]["tflops"] / PEAK_FP64, 4), "traffic": ROLLUP_TRAFFIC_BYTES,
                      "kernel": "k_lattice_records + k_lattice_partial + k_pair_reduce (rollup pair sum)",
                      "launch_ms": round(roof["fp64"]["ms"], 4), "pairs_per_launch": roof["fp64"]["pairs"],
                      "mixed_launch_ms": round(roof["mixed"]["ms"], 4)},
-        "serving": {"stream": f"{args.serve_cases} Poisson arrivals at {args.serve_rate}/s (seed 2), "
-                              "M in {4,8,16,32}, N in {16,32,64,128}, "
-                              f"{args.serve_steps[0]}-{args.serve_steps[1]} steps, wake rows = "
-                              + (f"min(steps, {args.serve_wake_cap})" if args.serve_wake_cap else "steps")
-                              + ", pitch 0-5 deg, k 0.05-0.5", **srv},
+        "serving": srv,
         "steady_case": steady,
         "coupled_gust": coupled,
         "coupled_adjoint": adjoint,

@@ -138,6 +138,15 @@ class UVLM:
         self._aic = None
         self._bufs = {}
 
+    def linearise(self, reference, wake_type, bound_upwash: bool = True, wake_upwash: bool = True,
+                  unsteady_force: bool = True):
+        """aero/uvlm.py:356-385: the linearised model about a steady reference
+        case (a static solve of this UVLM)."""
+        from .linear_uvlm import LinearUVLM
+
+        return LinearUVLM(self, reference=reference, wake_type=wake_type, bound_upwash=bound_upwash,
+                          wake_upwash=wake_upwash, unsteady_force=unsteady_force)
+
     # ------------------------------------------------------------------ setup
     @property
     def dt(self):
@@ -620,6 +629,15 @@ class UVLM:
             warn("Horseshoe wake requested but m_star == 0 for at least one surface, skipping horseshoe wake.")
         case = AeroCase.zeros(self.grid_disc, 1, self.hg_ref.device)
         case.t[0] = float(t)
+        # settings the AeroCase accessors use (aero/uvlm.py:1300-1326)
+        case.kernels = list(self.kernels_b) + list(self.kernels_w)
+        case.mirror_point, case.mirror_normal = self.mirror_point, self.mirror_normal
+        case.flowfield = self.flowfield
+        case.dof_mapping = list(self._dof_host)
+        case.static_horseshoe = bool(horseshoe)
+        case.free_wake = self.free_wake
+        case.gamma_dot_relaxation = self.gamma_dot_relaxation
+        case.batch_size = self.batch_size
         return self.case_solve(case, 0, hg, None, None, True, horseshoe)
 
     def solve_prescribed_dynamic(self, init_case, hg_t, hg_dot_t, cs_ang_t=None, cs_vel_t=None, callback=None):

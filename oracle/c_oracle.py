@@ -41,3 +41,13 @@ def pair_sum_eps(targets, y0, y1, g, eps=1e-7):
     p = lambda arr: arr.ctypes.data_as(ctypes.POINTER(ctypes.c_double))  # noqa: E731
     _lib().fj_oracle_pair_sum_eps(p(x), x.shape[0], p(a), p(b), p(g), a.shape[0], float(eps), p(out))
     return out
+
+
+def oracle_pair_sum(targets, y0, y1, g, kernel="epsilon", chunk=256):
+    """Drop-in for uvlm_oracle.pair_sum (install with uvlm_oracle.set_pair_sum):
+    epsilon-kernel sums go to the C restatement, the other kernels to NumPy."""
+    if kernel != "epsilon":
+        from oracle import uvlm_oracle
+
+        return uvlm_oracle.pair_sum(targets, y0, y1, g, kernel, chunk)
+    return pair_sum_eps(targets, y0, y1, g)

@@ -428,6 +428,19 @@ def strip_alpha(zeta_b, f_steady, v_func, rho):
     return out
 
 
+def compute_gamma_dot(gamma_b_n, gamma_b_nm1, gamma_b_dot_nm1, dt, relaxation):
+    """aero/uvlm.py:691-715: filtered time derivative of the ring strengths,
+    g (gamma_n - gamma_nm1) / dt + (1 - g) gamma_dot_nm1, per surface."""
+    return [(gn - gp) / dt * relaxation + gdp * (1.0 - relaxation)
+            for gn, gp, gdp in zip(gamma_b_n, gamma_b_nm1, gamma_b_dot_nm1)]
+
+
+def unsteady_forcing(gamma_b_dot, nc, rho):
+    """aero/uvlm.py:1093-1103: split_to_vertex(rho * gamma_dot * n) over both
+    grid axes, with the panel normal n of compute_surf_nc (|n| = 2 * area)."""
+    return [split_to_vertex(rho * gd[..., None] * n, (0, 1)) for gd, n in zip(gamma_b_dot, nc)]
+
+
 # ---------------------------------------------------------------------------
 # Wake convection, aero/utils.py:400-577
 # ---------------------------------------------------------------------------
@@ -620,12 +633,9 @@ class UVLMOracle:
         if static:
             gamma_b_dot_n, f_unsteady = None, None
         else:
-            g = self.gamma_dot_relaxation
-            gamma_b_dot_n = [
-                (gn - gp) / self.dt * g + gdp * (1.0 - g)
-                for gn, gp, gdp in zip(gamma_b_n, q_nm1["gamma_b"], q_nm1["gamma_b_dot"])
-            ]
-            f_unsteady = [split_to_vertex(ff.rho * gd[..., None] * nc, (0, 1)) for gd, nc in zip(gamma_b_dot_n, nc_n)]
+            gamma_b_dot_n = compute_gamma_dot(gamma_b_n, q_nm1["gamma_b"], q_nm1["gamma_b_dot"], self.dt,
+                                              self.gamma_dot_relaxation)
+            f_unsteady = unsteady_forcing(gamma_b_dot_n, nc_n, ff.rho)
         return dict(c=c_n, nc=nc_n, gamma_b=gamma_b_n, gamma_w=gamma_w_n, gamma_b_dot=gamma_b_dot_n,
                     zeta_b=zeta_b_n, zeta_w=zeta_w_n, zeta_b_dot=zeta_b_dot_n, f_steady=f_steady,
                     f_unsteady=f_unsteady, alpha=alpha, aic=aic)
@@ -716,24 +726,35 @@ def rigid_wing_hg(n, span, alpha_rad=0.0, z=0.0):
     return hg
 
 
-def pitch_plunge_motion(n, span, n_tstep, dt, alpha0_rad, pitch_amp_rad, plunge_amp, omega):
+def pitch_plunge_motion(n, span, n_tstep, dt, alpha0_rad, pitch_amp_rad, plunge_amp, omega, skew_rate=False,
+                        t0=0.0):
     """Prescribed pitch-plunge history about the beam axis (y):
     theta(t) = alpha0 + A_p sin(w t), h(t) = A_h sin(w t).
 
-    hg_dot follows the convention of the reference's prescribed-motion tests
-    (tests/aero/test_linear_planar_wing.py:168-170 and
-    tests/aero/test_variable_wake_disc.py:54-55): a skew rotation-rate block
-    [0,2] = theta_dot, [2,0] = -theta_dot and the translation rate in [:3, 3],
-    which aero/uvlm.py:562-621 applies to the local grid as hg_dot @ [zeta0; 1].
+    hg_dot is the exact time derivative of hg (aero/uvlm.py:562-621 maps it
+    onto the local grid as hg_dot @ [zeta0; 1], the global grid velocity):
+    dR/dt = theta_dot dR/dtheta for the rotation block and h_dot in [2, 3].
+    skew_rate=True gives the small-angle form of the reference's
+    prescribed-motion tests instead (tests/aero/test_linear_planar_wing.py:168-170):
+    a skew rate block [0,2] = theta_dot, [2,0] = -theta_dot, which equals the
+    exact derivative only at theta = 0.  Times are t0 + i dt.
     """
-    t = np.arange(n_tstep) * dt
+    t = t0 + np.arange(n_tstep) * dt
     th = alpha0_rad + pitch_amp_rad * np.sin(omega * t)
     thd = pitch_amp_rad * omega * np.cos(omega * t)
     h = plunge_amp * np.sin(omega * t)
     hd = plunge_amp * omega * np.cos(omega * t)
     hg_t = np.stack([rigid_wing_hg(n, span, th[i], h[i]) for i in range(n_tstep)])
     hg_dot_t = np.zeros_like(hg_t)
-    hg_dot_t[:, :, 0, 2] = thd[:, None]
-    hg_dot_t[:, :, 2, 0] = -thd[:, None]
+    if skew_rate:
+        hg_dot_t[:, :, 0, 2] = thd[:, None]
+        hg_dot_t[:, :, 2, 0] = -thd[:, None]
+    else:
+        # R = [[c, 0, s], [0, 1, 0], [-s, 0, c]] -> dR/dtheta = [[-s, 0, c], [0, 0, 0], [-c, 0, -s]]
+        s_, c_ = np.sin(th), np.cos(th)
+        hg_dot_t[:, :, 0, 0] = (-s_ * thd)[:, None]
+        hg_dot_t[:, :, 0, 2] = (c_ * thd)[:, None]
+        hg_dot_t[:, :, 2, 0] = (-c_ * thd)[:, None]
+        hg_dot_t[:, :, 2, 2] = (-s_ * thd)[:, None]
     hg_dot_t[:, :, 2, 3] = hd[:, None]
     return hg_t, hg_dot_t

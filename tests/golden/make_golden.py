@@ -45,8 +45,83 @@ def small_free_wake():
             "f_steady": d.f_steady[0], "f_unsteady": d.f_unsteady[0], "zeta_w": d.zeta_w[0][-1]}
 
 
+CFG1_STEPS_KEPT = (1, 10, 25, 40, 50, 60, 80, 100, 150, 199)
+CFG1_STATE_STEPS = (50, 199)
+
+
+def _cfg1_run(reverse):
+    """One oracle run of the full unsteady free-wake workload; ``reverse``
+    sums every pair sum over the filaments in reverse order (the same
+    arithmetic with a different rounding)."""
+    from oracle import c_oracle
+
+    def ps(t, y0, y1, g, kernel="epsilon", chunk=256):
+        assert kernel == "epsilon"
+        if reverse:
+            return c_oracle.pair_sum_eps(t, y0[::-1], y1[::-1], g[::-1])
+        return c_oracle.pair_sum_eps(t, y0, y1, g)
+
+    m, n, m_star, steps, span = 16, 64, 200, 200, 10.0
+    dt = 1.0 / m
+    zeta0 = O.make_rectangular_grid(m, n, 1.0, 0.25)
+    hg_t, hgd_t = O.pitch_plunge_motion(n, span, steps, dt, np.deg2rad(5.0), np.deg2rad(3.0), 0.05, 0.4)
+    O.set_pair_sum(ps)
+    try:
+        o = O.UVLMOracle([(m, n, m_star)], np.arange(n + 1), free_wake=True)
+        o.set_design_variables(dt, O.Constant(np.array([1.0, 0.0, 0.0]), 1.225), zeta0, hg_t[0])
+        d = o.solve_prescribed_dynamic(o.solve_static(hg=hg_t[0]), hg_t, hgd_t)
+        res = {"gamma_b": d.gamma_b[0][list(CFG1_STEPS_KEPT)]}
+        for s in CFG1_STATE_STEPS:
+            v_c = O.compute_v_ind(d.c[0][s], [d.zeta_b[0][s], d.zeta_w[0][s]], [d.gamma_b[0][s], d.gamma_w[0][s]],
+                                  "epsilon")
+            for key in ("gamma_w", "zeta_w", "f_steady", "f_unsteady", "alpha"):
+                res[f"{key}_{s}"] = getattr(d, key)[0][s]
+            res[f"v_ind_c_{s}"] = v_c
+    finally:
+        O.set_pair_sum(None)
+    return res
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+def cfg1_free_wake():
+    """The full unsteady free-wake workload: 16 x 64 rings, chord 1, span 10,
+    200 wake rows from the static solve, free wake, dt = 1/16, 200 steps of
+    pitch 5 + 3 sin(0.4 t) deg and plunge 0.05 sin(0.4 t) (k = 0.2) with the
+    exact hg_dot.  The epsilon-kernel pair sums run in the oracle's C
+    restatement (same formula, OpenMP), the rest in NumPy.
+
+    The run is made twice, the second time with every pair sum summed over
+    the filaments in reverse order.  The two differ only by rounding, and
+    their difference ("floor_*": per kept step for gamma_b, per state step
+    for the rest) is what two correct FP64 evaluations can agree to: the
+    free-wake rollup amplifies rounding (wake vertices approach each other
+    in the tip roll-up, and the epsilon kernel's core is 1e-7), from ~1e-16
+    at step 10 to ~1e-10 at step 50 and ~1e-2 at step 199 on the wake
+    vertices.  Kept: gamma_b at CFG1_STEPS_KEPT, and gamma_w, zeta_w,
+    f_steady, f_unsteady, alpha and the induced velocity at the collocation
+    points (bound + wake, compute_v_ind) at CFG1_STATE_STEPS.  About 20
+    minutes on 8 cores."""
+    ref = _cfg1_run(False)
+    alt = _cfg1_run(True)
+    out = dict(ref)
+    out["steps_kept"] = np.array(CFG1_STEPS_KEPT)
+    out["state_steps"] = np.array(CFG1_STATE_STEPS)
+    out["floor_gamma_b"] = np.array([_rel(alt["gamma_b"][k], ref["gamma_b"][k]) for k in range(len(CFG1_STEPS_KEPT))])
+    for key in ref:
+        if key != "gamma_b":
+            out[f"floor_{key}"] = np.array(_rel(alt[key], ref[key]))
+    return out
+
+
 if __name__ == "__main__":
-    np.savez_compressed(HERE / "cfg0_static.npz", **cfg0_static())
-    np.savez_compressed(HERE / "free_wake_8x16.npz", **small_free_wake())
+    which = sys.argv[1:] or ["small", "cfg1"]
+    if "small" in which:
+        np.savez_compressed(HERE / "cfg0_static.npz", **cfg0_static())
+        np.savez_compressed(HERE / "free_wake_8x16.npz", **small_free_wake())
+    if "cfg1" in which:
+        np.savez_compressed(HERE / "cfg1_free_wake_16x64.npz", **cfg1_free_wake())
     for f in sorted(HERE.glob("*.npz")):
         print(f.name, f.stat().st_size)

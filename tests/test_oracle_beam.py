@@ -237,3 +237,69 @@ def test_static_aeroelastic_equilibrium_and_persistence():
     assert np.abs(g - g[[0]]).max() < 3e-7
     vp = dyn["varphi"]
     assert np.abs(vp - vp[[0]]).max() < 3e-7
+
+
+def _cantilever(nn, ei_flap=1.0, ei_lag=4.0, gj=1.0, mu=1.0, jx=1e-2):
+    """Uniform cantilever of length 1 along x (clamped at node 0), Euler-Bernoulli
+    proportions: axial and shear stiffness 1e6, negligible rotary inertia of
+    bending.  With y_vector = y the element's local z is global z, so k_cs[4]
+    bends about y (flap, deflection along z) and k_cs[5] about z (lag)."""
+    b = B.BeamOracle(nn, B.chain(nn), np.array((0.0, 1.0, 0.0)))
+    x = np.zeros((nn, 3))
+    x[:, 0] = np.linspace(0.0, 1.0, nn)
+    b.set_design_variables(x, np.diag((1e6, 1e6, 1e6, gj, ei_flap, ei_lag)), np.diag((mu, mu, mu, jx, 1e-6, 1e-6)))
+    return b
+
+
+def _modes(b):
+    import scipy.linalg
+
+    d = b.make_d(b.hg0)
+    z = np.zeros((b.n_nodes, 6))
+    els = [b.element(e, d[e], z, z) for e in range(b.n_elem)]
+    k = b.assemble_mat([el["k_t"] for el in els])[6:, 6:]
+    m = b.assemble_mat([el["m_t"] for el in els])[6:, 6:]
+    lam, vec = scipy.linalg.eigh(k, m)
+    return np.sqrt(lam), vec
+
+
+def test_cantilever_natural_frequencies():
+    """Mass and stiffness of the element (structure/utils.py, structure/beam.py)
+    against the Euler-Bernoulli cantilever: omega_k = (beta_k L)^2 sqrt(EI / (mu L^4))
+    with beta_1 L = 1.8751, beta_2 L = 4.6941, for flap (EI = 1) and lag (EI = 4),
+    and the clamped-free torsion omega = (pi / 2L) sqrt(GJ / J); the error of the
+    constant-strain element falls as h^2."""
+    b1, b2 = 1.8751040687, 4.6940911330
+    w, _ = _modes(_cantilever(21))
+    expect = np.sort([b1 ** 2, 2.0 * b1 ** 2, b2 ** 2, 2.0 * b2 ** 2, 0.5 * np.pi * np.sqrt(1.0 / 1e-2)])
+    got = np.sort(w[:5])
+    assert np.allclose(got, expect, rtol=5e-3), (got, expect)
+    assert abs(w[0] / b1 ** 2 - 1.0) < 1e-3
+    e11 = _modes(_cantilever(11))[0][0] / b1 ** 2 - 1.0
+    e21 = w[0] / b1 ** 2 - 1.0
+    assert 3.0 < e11 / e21 < 5.0
+
+
+def test_cantilever_free_vibration_period():
+    """Generalised-alpha dynamics (structure/time_integration.py) of the first
+    flap mode, started by a small velocity in its discrete mode shape: the tip
+    moves along z only, with the discrete period 2 pi / omega_1 (to the scheme's
+    O((omega dt)^2) elongation) and an amplitude that the spectral radius 0.9
+    barely damps at 100 steps per period."""
+    b = _cantilever(11)
+    w, vec = _modes(b)
+    period = 2.0 * np.pi / w[0]
+    v0 = np.zeros((b.n_nodes, 6))
+    v0[1:] = 1e-4 * vec[:, 0].reshape(-1, 6) / np.abs(vec[:, 0]).max()
+    dt = period / 100
+    out = b.dynamic_solve(dict(hg=b.hg0, v=v0), np.arange(6), 112, dt)
+    tip = out["hg"][:, -1, :3, 3] - out["hg"][0, -1, :3, 3]
+    assert np.abs(tip[:, 1]).max() < 1e-6 * np.abs(tip[:, 2]).max()
+    s = tip[:, 2]
+    t = np.arange(s.size) * dt
+    zc = [t[i] - s[i] * dt / (s[i + 1] - s[i]) for i in range(1, s.size - 1) if s[i] * s[i + 1] < 0]
+    assert len(zc) == 2
+    assert abs(2.0 * (zc[1] - zc[0]) / period - 1.0) < 1e-3
+    assert abs(zc[0] / (0.5 * period) - 1.0) < 1e-3
+    a1, a2 = np.abs(s[: s.size // 2]).max(), np.abs(s[s.size // 2:]).max()
+    assert 0.97 < a2 / a1 <= 1.0
