@@ -255,6 +255,26 @@ def pair_sum_lattices(seg, lattices, targets, kernel_kind, eps, rcut, mode=L.OUT
     return out
 
 
+def pair_sum_lattices_noalloc(seg, lattices, targets, kernel_kind, eps, rcut, mode, out, flow=None, time=0.0,
+                              dt=0.0):
+    """pair_sum_lattices into a caller-given `out`, with no device allocation
+    at all (fj_pair_sum_lattices_noalloc: the library's fixed scratch array of
+    NOALLOC_SCRATCH_BYTES per GPU).  Calls overlapping on one GPU race."""
+    if len(lattices) > MAX_LATTICES:
+        raise ValueError(f"at most {MAX_LATTICES} lattices per call")
+    flat = _c64(targets, (3,), "targets").reshape(-1, 3)
+    if out is None or out.dtype != F64 or not out.is_contiguous() or out.numel() != flat.numel():
+        raise ValueError("a contiguous float64 output of the targets' shape is required")
+    refs = lattice_refs(lattices)
+    fd = flow_desc(flow) if flow is not None else None
+    rc = L.lib().fj_pair_sum_lattices_noalloc(L.ptr(seg) if seg is not None else None, len(lattices), refs,
+                                              L.ptr(flat), flat.shape[0], L.ptr(out), int(kernel_kind), float(eps),
+                                              float(rcut), int(mode), ctypes.byref(fd) if fd is not None else None,
+                                              float(time), float(dt), L.stream())
+    L.check(rc, "fj_pair_sum_lattices_noalloc")
+    return out
+
+
 # ---------------------------------------------------------------------------
 # K2
 # ---------------------------------------------------------------------------

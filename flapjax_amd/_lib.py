@@ -23,6 +23,8 @@ FJ_ERR_LAUNCH = -2
 FJ_ERR_ALLOC = -3
 FJ_ERR_SINGULAR = -4
 FJ_ERR_NOCONV = -5
+FJ_ERR_SCRATCH = -6
+NOALLOC_SCRATCH_BYTES = 32 << 20
 
 KERNEL_EPSILON, KERNEL_CUTOFF, KERNEL_PLAIN = 0, 1, 2
 PREC_FP64, PREC_MIXED = 0, 1
@@ -36,6 +38,7 @@ _ERRS = {
     FJ_ERR_ALLOC: "workspace allocation failed",
     FJ_ERR_SINGULAR: "singular matrix",
     FJ_ERR_NOCONV: "iterative refinement did not converge",
+    FJ_ERR_SCRATCH: f"the call needs more than the fixed {NOALLOC_SCRATCH_BYTES >> 20} MiB device scratch",
 }
 
 
@@ -65,6 +68,8 @@ _SIGS = {
     "fj_pair_sum_lattices_workspace_bytes": ([c_int, c_int, c_void_p], c_size_t),
     "fj_pair_sum_lattices": ([c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p, c_int, c_double, c_double,
                               c_int, POINTER(FjFlowfield), c_double, c_double, c_void_p, c_void_p], c_int),
+    "fj_pair_sum_lattices_noalloc": ([c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p, c_int, c_double,
+                                      c_double, c_int, POINTER(FjFlowfield), c_double, c_double, c_void_p], c_int),
     "fj_aic_assemble": ([c_void_p, c_void_p, c_int, c_void_p, c_int, c_int, c_void_p, c_int, c_void_p, c_void_p,
                          c_int, c_double, c_double, c_void_p, c_int, c_int, c_int, c_void_p], c_int),
     "fj_lu_aux_bytes": ([c_int], c_size_t),

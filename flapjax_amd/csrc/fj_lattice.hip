@@ -30,6 +30,8 @@
 // tasks of a chunk (one per wave, strided over the chunk), sums its waves in
 // LDS in a fixed order and writes one partial per chunk; the partials are
 // summed in a fixed order by the common reduction (deterministic).
+#include <mutex>
+
 #include "fj_common.hpp"
 #include "fj_lattice_dev.hpp"
 
@@ -202,10 +204,12 @@ extern "C" size_t fj_pair_sum_lattices_workspace_bytes(int n_tgt, int n_lat,
   return fj_lat_part_bytes(p) + (size_t)p.list.n_recs * FJ_LAT_REC * sizeof(double) + 256;
 }
 
-extern "C" int fj_pair_sum_lattices(const double* seg, int n_lat, const fj_lattice_ref* lats,
-                                    const double* targets, int n_tgt, double* out, int kernel,
-                                    double eps, double rcut, int mode, const fj_flowfield* flow,
-                                    double time, double dt, void* workspace, void* stream) {
+// The body of fj_pair_sum_lattices on a given workspace (partials, then the
+// strip-column records), shared with the allocation-free entry point below.
+static int fj_lat_run(const double* seg, int n_lat, const fj_lattice_ref* lats,
+                      const double* targets, int n_tgt, double* out, int kernel, double eps,
+                      double rcut, int mode, const fj_flowfield* flow, double time, double dt,
+                      void* workspace, size_t ws_limit, void* stream) {
   if (n_tgt < 0) return FJ_ERR_ARG;
   if (kernel < FJ_KERNEL_EPSILON || kernel > FJ_KERNEL_PLAIN) return FJ_ERR_ARG;
   if (mode < FJ_OUT_STORE || mode > FJ_OUT_CONVECT) return FJ_ERR_ARG;
@@ -217,7 +221,10 @@ extern "C" int fj_pair_sum_lattices(const double* seg, int n_lat, const fj_latti
   hipStream_t st = (hipStream_t)stream;
   const FjFlow fl = fj_flow_from_desc(flow);
   if (p.list.n_tasks == 0) return fj_pair_epilogue(nullptr, 0, 0, n_tgt, targets, out, mode, fl, time, dt, st);
-  if (!seg || !workspace) return FJ_ERR_ARG;
+  if (!seg) return FJ_ERR_ARG;
+  if (fj_lat_part_bytes(p) + (size_t)p.list.n_recs * FJ_LAT_REC * sizeof(double) > ws_limit)
+    return FJ_ERR_SCRATCH;
+  if (!workspace) return FJ_ERR_ARG;
   FjKernelParams kp;
   kp.eps = eps; kp.eps2 = eps * eps; kp.rcut = rcut;
   double* part = (double*)workspace;
@@ -235,4 +242,62 @@ extern "C" int fj_pair_sum_lattices(const double* seg, int n_lat, const fj_latti
     hipLaunchKernelGGL(k_lattice_partial<FJ_KERNEL_PLAIN>, grid, blk, 0, st, tab, p.list, p.chunk_len, targets, n_tgt, p.ld, part, kp);
   FJ_CHECK_LAUNCH();
   return fj_pair_epilogue(part, p.chunks, p.ld, n_tgt, targets, out, mode, fl, time, dt, st);
+}
+
+extern "C" int fj_pair_sum_lattices(const double* seg, int n_lat, const fj_lattice_ref* lats,
+                                    const double* targets, int n_tgt, double* out, int kernel,
+                                    double eps, double rcut, int mode, const fj_flowfield* flow,
+                                    double time, double dt, void* workspace, void* stream) {
+  return fj_lat_run(seg, n_lat, lats, targets, n_tgt, out, kernel, eps, rcut, mode, flow, time, dt,
+                    workspace, (size_t)-1, stream);
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free variant: the workspace is a file-scope device array of
+// FJ_NOALLOC_SCRATCH_BYTES (one copy per GPU, held until the process ends).
+// In host code `scratch` names a host-side stand-in, never the device
+// memory: its device address is taken once per device with
+// hipGetSymbolAddress (device current) and only that address is used.
+// Nothing in it needs to start at zero (the records kernel writes every
+// record the partial kernel reads, and the partial kernel writes every
+// partial the reduction reads), so no clearing launch is needed.  Calls that
+// overlap on one GPU share this memory and race.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[FJ_NOALLOC_SCRATCH_BYTES];
+
+#define FJ_NOALLOC_MAX_DEVICES 64
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[FJ_NOALLOC_MAX_DEVICES];
+
+static int fj_scratch_ptr(void** out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= FJ_NOALLOC_MAX_DEVICES) return FJ_ERR_LAUNCH;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return FJ_ERR_LAUNCH;
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return FJ_OK;
+}
+
+extern "C" int fj_pair_sum_lattices_noalloc(const double* seg, int n_lat, const fj_lattice_ref* lats,
+                                            const double* targets, int n_tgt, double* out,
+                                            int kernel, double eps, double rcut, int mode,
+                                            const fj_flowfield* flow, double time, double dt,
+                                            void* stream) {
+  // sizes first (host-only), so an oversized call fails before any device call
+  if (n_tgt > 0) {
+    FjLatPlan p;
+    if (n_lat >= 0 && fj_lat_plan(n_tgt, n_lat, lats, p) == FJ_OK && p.list.n_tasks > 0 &&
+        fj_lat_part_bytes(p) + (size_t)p.list.n_recs * FJ_LAT_REC * sizeof(double) >
+            (size_t)FJ_NOALLOC_SCRATCH_BYTES)
+      return FJ_ERR_SCRATCH;
+  }
+  void* ws = nullptr;
+  const int rc = fj_scratch_ptr(&ws);
+  if (rc != FJ_OK) return rc;
+  return fj_lat_run(seg, n_lat, lats, targets, n_tgt, out, kernel, eps, rcut, mode, flow, time, dt, ws,
+                    (size_t)FJ_NOALLOC_SCRATCH_BYTES, stream);
 }

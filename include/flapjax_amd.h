@@ -32,6 +32,11 @@ extern "C" {
 #define FJ_ERR_ALLOC (-3)    /* workspace allocation failed */
 #define FJ_ERR_SINGULAR (-4) /* zero pivot met in the LU factorisation */
 #define FJ_ERR_NOCONV (-5)   /* iterative refinement did not converge */
+#define FJ_ERR_SCRATCH (-6)  /* the call needs more than the library's fixed scratch memory */
+
+/* Size of the fixed device scratch array of the allocation-free entry point
+ * fj_pair_sum_lattices_noalloc (one copy per GPU, held until the process ends). */
+#define FJ_NOALLOC_SCRATCH_BYTES (32u << 20)
 
 /* Biot-Savart kernel kinds: the three KernelFunction implementations the
  * reference ships (aero/utils.py:19 type, :580-648 implementations). */
@@ -148,6 +153,20 @@ int fj_pair_sum_lattices(const double* seg, int n_lat, const fj_lattice_ref* lat
                          const double* targets, int n_tgt, double* out, int kernel, double eps,
                          double rcut, int mode, const fj_flowfield* flow, double time, double dt,
                          void* workspace, void* stream);
+
+/* fj_pair_sum_lattices without a workspace argument, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  It
+ * runs the same kernels and plan on a fixed file-scope device array of
+ * FJ_NOALLOC_SCRATCH_BYTES and calls no allocating routine; a call whose
+ * fj_pair_sum_lattices_workspace_bytes exceeds that size returns
+ * FJ_ERR_SCRATCH before any device work.  Every call on one GPU shares that
+ * GPU's array: calls that overlap on one GPU (two streams, two host threads)
+ * race.  Results equal fj_pair_sum_lattices' bit for bit. */
+int fj_pair_sum_lattices_noalloc(const double* seg, int n_lat, const fj_lattice_ref* lats,
+                                 const double* targets, int n_tgt, double* out, int kernel,
+                                 double eps, double rcut, int mode, const fj_flowfield* flow,
+                                 double time, double dt, void* stream);
 
 /* Reverse mode of fj_pair_sum (FJ_OUT_STORE) for hand-written adjoints:
  * with upstream weights grad_v (n_tgt, 3),
