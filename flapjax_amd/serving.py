@@ -81,7 +81,8 @@ class CaseResult:
 
 
 def motion(req: CaseRequest):
-    """hg (steps, n+1, 4, 4), hg_dot for theta = a0 + A sin(w t)."""
+    """hg (steps, n+1, 4, 4) and its exact time derivative hg_dot for the
+    pitch theta = a0 + A sin(w t) about the beam axis."""
     dt = 1.0 / req.m
     omega = 2.0 * req.k_red
     t = np.arange(req.steps) * dt
@@ -95,8 +96,8 @@ def motion(req: CaseRequest):
     hg[:, :, 1, 3] = np.linspace(0.0, req.span, req.n + 1)[None, :]
     hg[:, :, 3, 3] = 1.0
     hgd = np.zeros_like(hg)
-    hgd[:, :, 0, 2] = thd[:, None]
-    hgd[:, :, 2, 0] = -thd[:, None]
+    hgd[:, :, 0, 0], hgd[:, :, 0, 2] = (-s * thd)[:, None], (c * thd)[:, None]
+    hgd[:, :, 2, 0], hgd[:, :, 2, 2] = (-c * thd)[:, None], (-s * thd)[:, None]
     return hg, hgd
 
 

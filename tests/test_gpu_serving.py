@@ -131,3 +131,49 @@ def test_admission_by_footprint(dev):
     # a generous budget runs cases concurrently
     res2, stats2 = ServingEngine().run(reqs)
     assert stats2["peak_bytes"] > fps[-1]
+
+
+def test_config_size_stream(dev):
+    """Served stream at the sizes of the served workload: a 32 x 128 case
+    (4096 bound rings) with a 210-row wake beside 16 x 64, 8 x 32 and 4 x 16
+    cases with 120-300 wake rows.  Each case in the mix equals the case
+    served alone (1e-12).  Against the plain drop-in: with a prescribed wake
+    the whole run within 1e-10; with the free wake the lift over the first
+    40 steps within 1e-10 (the rollup amplifies rounding later: the served
+    and drop-in pair sums round differently), and the final circulation
+    within 1e-3 and wake within 0.3 -- the size of the divergence the free
+    wake gives two correct FP64 runs after 200-300 steps (the cfg1 oracle
+    against itself: 7.7e-6 and 2.5e-2 at step 199; measured here up to
+    6.7e-6 and 3.1e-2 on the 32 x 128 case)."""
+    from flapjax_amd import Constant
+    from flapjax_amd.serving import CaseRequest, ServingEngine, motion
+    from flapjax_amd.uvlm import UVLM
+
+    specs = [(32, 128, 210), (16, 64, 250), (8, 32, 300), (4, 16, 120)]
+    for free in (False, True):
+        reqs = [CaseRequest(i, m, n, st, st, 3.0, 0.2, arrival=0.001 * i) for i, (m, n, st) in enumerate(specs)]
+        mix, stats = ServingEngine(free_wake=free).run(reqs)
+        assert stats["cases"] == len(reqs)
+        by_id = {r.case_id: r for r in mix}
+        for req in reqs:
+            b = by_id[req.case_id]
+            alone, _ = ServingEngine(free_wake=free).run([req])
+            assert rel_err(b.lift, alone[0].lift) < 1e-12
+            assert rel_err(b.gamma_b_final, alone[0].gamma_b_final) < 1e-12
+            assert rel_err(b.zeta_w_final, alone[0].zeta_w_final) < 1e-12
+            hg, hgd = motion(req)
+            uv = UVLM([(req.m, req.n, req.m_star)], np.arange(req.n + 1), free_wake=free)
+            z0 = np.zeros((req.m + 1, req.n + 1, 3))
+            z0[..., 0] = (np.linspace(0.0, 1.0, req.m + 1) - 0.25)[:, None]
+            uv.set_design_variables(1.0 / req.m, Constant(np.array([1.0, 0, 0]), 1.225, True), z0, hg[0])
+            d = uv.solve_prescribed_dynamic(uv.solve_static(hg=hg[0]), hg, hgd)
+            lift = d.f_steady[0][..., 2].sum(dim=(1, 2)).cpu().numpy()
+            e_g = rel_err(b.gamma_b_final, d.gamma_b[0][-1].cpu().numpy())
+            e_w = rel_err(b.zeta_w_final, d.zeta_w[0][-1].cpu().numpy())
+            print(f"free={free} case {req.m}x{req.n} m*={req.m_star}: lift(1..40) "
+                  f"{rel_err(b.lift[1:41], lift[1:41]):.2e}, final gamma {e_g:.2e}, wake {e_w:.2e}")
+            assert rel_err(b.lift[1:41], lift[1:41]) < 1e-10
+            if not free:
+                assert rel_err(b.lift[1:], lift[1:]) < 1e-10 and e_g < 1e-10 and e_w < 1e-10
+            else:
+                assert e_g < 1e-3 and e_w < 0.3
