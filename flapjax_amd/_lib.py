@@ -131,12 +131,22 @@ _SIGS = {
     "fj_beam_update": ([c_int, c_int, c_void_p, c_int, c_void_p, c_double, c_double, c_double] + [c_void_p] * 7,
                        c_int),
     "fj_beam_subnorm": ([c_void_p, c_void_p, c_int, c_void_p, c_void_p], c_int),
+    "fj_beam_newton_dynamic": ([c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                c_double, c_double, c_double, c_void_p, c_int, c_int, c_int, c_void_p]
+                               + [c_void_p] * 11, c_int),
     "fj_beam_nodes": ([c_int, c_int] + [c_void_p] * 7, c_int),
     "fj_beam_project_forces": ([c_int, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
                                 c_void_p, c_void_p], c_int),
 }
 
 BEAM_NODE_UPDATE, BEAM_NODE_REL, BEAM_NODE_RATE = 0, 1, 2
+
+
+class FjConvergence(ctypes.Structure):
+    """fj_convergence: a Newton loop's tolerances (NaN = unset)."""
+
+    _fields_ = [("max_n_iter", c_int), ("rel_disp_tol", c_double), ("abs_disp_tol", c_double),
+                ("rel_force_tol", c_double), ("abs_force_tol", c_double)]
 
 
 class FjBeamModel(ctypes.Structure):

@@ -256,6 +256,10 @@ class BeamStructure:
         self.relaxation_factor = float(relaxation_factor)
         self.spectral_radius = float(spectral_radius)
         self.struct_convergence_settings = struct_convergence_settings
+        # the dynamic Newton loop's stop test on the device (one status read
+        # per batch of iterations); False: the host reads the norms after
+        # every iteration
+        self.device_newton = True
         self.orientation = np.eye(3)
         self.time_integrator = None
         self._model = None
@@ -423,17 +427,21 @@ class BeamStructure:
         self._nodes(L.BEAM_NODE_RATE, hg.contiguous(), x=v.contiguous(), hg_out=out)
         return out
 
-    def _system(self, dynamic, hg, v, v_dot, t_phi, f_fol, f_dead, sd, n_s, ti=None):
-        # one set of system buffers per size, reused by every Newton iteration
-        # (the previous iteration's system is consumed before the next is built)
+    def _sys_buffers(self, n_s, device):
+        """(A, r, f_abs, x) of one Newton system size, reused by every Newton
+        iteration (the previous iteration's system is consumed before the
+        next is built)."""
         buf = self._sys_bufs.get(n_s) if hasattr(self, "_sys_bufs") else None
-        if buf is None or buf[0].device != hg.device:
-            buf = (torch.empty((n_s, n_s), dtype=F64, device=hg.device), torch.empty(n_s, dtype=F64, device=hg.device),
-                   torch.empty(n_s, dtype=F64, device=hg.device), torch.empty(n_s, dtype=F64, device=hg.device))
+        if buf is None or buf[0].device != device:
+            buf = (torch.empty((n_s, n_s), dtype=F64, device=device), torch.empty(n_s, dtype=F64, device=device),
+                   torch.empty(n_s, dtype=F64, device=device), torch.empty(n_s, dtype=F64, device=device))
             if not hasattr(self, "_sys_bufs"):
                 self._sys_bufs = {}
             self._sys_bufs[n_s] = buf
-        A, r, fa = buf[0], buf[1], buf[2]
+        return buf
+
+    def _system(self, dynamic, hg, v, v_dot, t_phi, f_fol, f_dead, sd, n_s, ti=None):
+        A, r, fa, _ = self._sys_buffers(n_s, hg.device)
         bp = ti.beta_prime if ti is not None else 0.0
         gp = ti.gamma_prime if ti is not None else 0.0
         rc = L.lib().fj_beam_system(ctypes.c_void_p(ctypes.addressof(self._model)), int(dynamic), L.ptr(hg), L.ptr(v), L.ptr(v_dot),
@@ -518,10 +526,52 @@ class BeamStructure:
                           torch.einsum("...ij,...j->...i", r, f[..., 3:])), -1).contiguous()
 
     # ----------------------------------------------------------------- dynamic
+    #: Newton iterations enqueued before the first status read of a loop
+    newton_batch = 3
+
+    def _conv_struct(self):
+        s = self.struct_convergence_settings
+        nan = float("nan")
+        f = lambda v: nan if v is None else float(v)  # noqa: E731
+        return L.FjConvergence(0 if s.max_n_iter is None else int(s.max_n_iter), f(s.rel_disp_tol),
+                               f(s.abs_disp_tol), f(s.rel_force_tol), f(s.abs_force_tol))
+
+    def _newton_dynamic_device(self, hg_prev, phi_a, v_a, vd_a, f_fol, f_dead, sd, n_s, ti, bufs):
+        """The Newton loop with the stop test on the device
+        (fj_beam_newton_dynamic): iterations are enqueued in batches and the
+        host reads one status word per batch; the iterates are those of the
+        host-driven loop."""
+        hg_u, t_phi = bufs
+        A, r, fa, x = self._sys_buffers(n_s, hg_prev.device)
+        if not hasattr(self, "_nstatus"):
+            self._nstatus = torch.zeros(4, dtype=torch.int32, device=hg_prev.device)
+            self._nstatus_host = torch.zeros(4, dtype=torch.int32).pin_memory()
+        conv = self._conv_struct()
+        bw = self._bandwidth
+        lib = L.lib()
+        batch, reset = self.newton_batch, 1
+        while True:
+            rc = lib.fj_beam_newton_dynamic(
+                ctypes.c_void_p(ctypes.addressof(self._model)), batch, reset, L.ptr(hg_prev), L.ptr(phi_a),
+                L.ptr(v_a), L.ptr(vd_a), L.ptr(f_fol), L.ptr(f_dead), ti.beta_prime, ti.gamma_prime,
+                self.relaxation_factor, L.ptr(sd), n_s, bw, bw, ctypes.byref(conv), L.ptr(hg_u), L.ptr(t_phi),
+                L.ptr(A), L.ptr(r), L.ptr(fa), L.ptr(x), L.ptr(self._norms), L.ptr(self._nstatus), None,
+                L.ptr(self._ws), L.stream())
+            L.check(rc, "fj_beam_newton_dynamic")
+            reset = 0
+            self._nstatus_host.copy_(self._nstatus, non_blocking=True)
+            torch.cuda.current_stream().synchronize()
+            it, done = int(self._nstatus_host[0]), int(self._nstatus_host[1])
+            if done:
+                return it
+            batch = 2
+
     def _newton_dynamic(self, hg_prev, phi_a, v_a, vd_a, f_fol, f_dead, sd, n_s, ti, bufs):
         """The Newton loop of one load step at the alpha point
         (structure/beam.py:2170-2316, 2727-2792); phi_a, v_a, vd_a are updated
         in place."""
+        if n_s and self.device_newton and K.band_fits(n_s, self._bandwidth, self._bandwidth):
+            return self._newton_dynamic_device(hg_prev, phi_a, v_a, vd_a, f_fol, f_dead, sd, n_s, ti, bufs)
         hg_u, t_phi = bufs
         lib = L.lib()
         st = ConvergenceStatus(self.struct_convergence_settings)

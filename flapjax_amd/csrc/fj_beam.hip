@@ -55,6 +55,11 @@ struct FjBeamDev {
   const int32_t* node_elem;
 };
 
+// A guarded launch returns at once when the Newton status word says the
+// loop has finished (status[1] != 0): the device-driven Newton loop enqueues
+// iterations ahead and lets the ones after convergence fall through.
+__device__ __forceinline__ bool fj_guard_done(const int32_t* guard) { return guard && guard[1] != 0; }
+
 static FjBeamDev fj_beam_dev(const fj_beam_model* m) {
   FjBeamDev d;
   d.n_nodes = m->n_nodes; d.n_elem = m->n_elem;
@@ -93,7 +98,8 @@ __device__ __forceinline__ void fj_beam_elem_vec(const FjBeamDev& bm, const doub
 
 template <int DYN>
 __global__ __launch_bounds__(64) void k_beam_points(FjBeamDev bm, const double* __restrict__ hg, const double* __restrict__ v,
-                              double* __restrict__ ws) {
+                              double* __restrict__ ws, const int32_t* __restrict__ guard) {
+  if (fj_guard_done(guard)) return;
   const int slots = DYN ? 5 : 1;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= bm.n_elem * slots) return;
@@ -169,7 +175,9 @@ __global__ __launch_bounds__(64) void k_beam_points(FjBeamDev bm, const double* 
 template <int DYN>
 __global__ __launch_bounds__(FJ_BEAM_MAT_THREADS) void k_beam_matrices(
     FjBeamDev bm, const double* __restrict__ v, const double* __restrict__ v_dot,
-    const double* __restrict__ t_phi, double beta_p, double gamma_p, double* __restrict__ ws) {
+    const double* __restrict__ t_phi, double beta_p, double gamma_p, double* __restrict__ ws,
+    const int32_t* __restrict__ guard) {
+  if (fj_guard_done(guard)) return;
   const int e = blockIdx.x;
   const int t = threadIdx.x;
   double* w = ws + (size_t)FJ_BW_STRIDE * e;
@@ -319,7 +327,9 @@ __device__ __forceinline__ double fj_dead_local(const double* hg, const double* 
 __global__ void k_beam_assemble(FjBeamDev bm, const int32_t* __restrict__ sd, int n_s,
                                 const double* __restrict__ hg, const double* __restrict__ f_fol,
                                 const double* __restrict__ f_dead, const double* __restrict__ ws,
-                                double* __restrict__ A, double* __restrict__ rv, double* __restrict__ fav) {
+                                double* __restrict__ A, double* __restrict__ rv, double* __restrict__ fav,
+                                const int32_t* __restrict__ guard) {
+  if (fj_guard_done(guard)) return;
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long nn = (long long)n_s * n_s;
   if (t < nn) {
@@ -391,7 +401,8 @@ __global__ __launch_bounds__(FJ_BEAM_UPD_THREADS) void k_beam_update(
     int dynamic, int n_dof, const int32_t* __restrict__ sd, int n_s, const double* __restrict__ x,
     double relax, double gamma_p, double beta_p, double* __restrict__ phi, double* __restrict__ v,
     double* __restrict__ v_dot, const double* __restrict__ rv, const double* __restrict__ fav,
-    double* __restrict__ norms) {
+    double* __restrict__ norms, const int32_t* __restrict__ guard) {
+  if (fj_guard_done(guard)) return;
   __shared__ double red[4 * FJ_BEAM_UPD_THREADS];
   __shared__ int redn[FJ_BEAM_UPD_THREADS];
   const int t = threadIdx.x;
@@ -445,7 +456,9 @@ __global__ __launch_bounds__(FJ_BEAM_UPD_THREADS) void k_beam_subnorm(const doub
 // configuration log(H^-1 H_ref), or rate H tilde(v)
 __global__ void k_beam_nodes(int op, int n, const double* __restrict__ hg, const double* __restrict__ x,
                              const double* __restrict__ hg_ref, double* __restrict__ hg_out,
-                             double* __restrict__ t_out, double* __restrict__ h_out) {
+                             double* __restrict__ t_out, double* __restrict__ h_out,
+                             const int32_t* __restrict__ guard) {
+  if (fj_guard_done(guard)) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double* g = hg + 16 * i;
@@ -529,10 +542,11 @@ static int fj_beam_model_ok(const fj_beam_model* m) {
   return 1;
 }
 
-extern "C" int fj_beam_system(const fj_beam_model* model, int dynamic, const double* hg, const double* v,
-                              const double* v_dot, const double* t_phi, const double* f_follower,
-                              const double* f_dead, double beta_p, double gamma_p, const int32_t* solve_dofs,
-                              int n_solve, double* A, double* r, double* f_abs, void* workspace, void* stream) {
+static int fj_beam_system_g(const fj_beam_model* model, int dynamic, const double* hg, const double* v,
+                            const double* v_dot, const double* t_phi, const double* f_follower,
+                            const double* f_dead, double beta_p, double gamma_p, const int32_t* solve_dofs,
+                            int n_solve, double* A, double* r, double* f_abs, void* workspace, void* stream,
+                            const int32_t* guard) {
   if (!fj_beam_model_ok(model) || n_solve < 0 || n_solve > 6 * model->n_nodes) return FJ_ERR_ARG;
   if (n_solve == 0) return FJ_OK;
   if (!hg || !solve_dofs || !A || !r || !f_abs || (model->n_elem > 0 && !workspace)) return FJ_ERR_ARG;
@@ -544,21 +558,29 @@ extern "C" int fj_beam_system(const fj_beam_model* model, int dynamic, const dou
   if (bm.n_elem > 0) {
     const int pts = bm.n_elem * (dynamic ? 5 : 1);
     if (dynamic) {
-      hipLaunchKernelGGL(k_beam_points<1>, dim3(fj_cdiv(pts, 64)), dim3(64), 0, st, bm, hg, v, ws);
+      hipLaunchKernelGGL(k_beam_points<1>, dim3(fj_cdiv(pts, 64)), dim3(64), 0, st, bm, hg, v, ws, guard);
       hipLaunchKernelGGL(k_beam_matrices<1>, dim3(bm.n_elem), dim3(FJ_BEAM_MAT_THREADS), 0, st, bm, v, v_dot,
-                         t_phi, beta_p, gamma_p, ws);
+                         t_phi, beta_p, gamma_p, ws, guard);
     } else {
-      hipLaunchKernelGGL(k_beam_points<0>, dim3(fj_cdiv(pts, 64)), dim3(64), 0, st, bm, hg, v, ws);
+      hipLaunchKernelGGL(k_beam_points<0>, dim3(fj_cdiv(pts, 64)), dim3(64), 0, st, bm, hg, v, ws, guard);
       hipLaunchKernelGGL(k_beam_matrices<0>, dim3(bm.n_elem), dim3(FJ_BEAM_MAT_THREADS), 0, st, bm, v, v_dot,
-                         t_phi, beta_p, gamma_p, ws);
+                         t_phi, beta_p, gamma_p, ws, guard);
     }
     FJ_CHECK_LAUNCH();
   }
   const long long tot = (long long)n_solve * n_solve + n_solve;
   hipLaunchKernelGGL(k_beam_assemble, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, bm, solve_dofs,
-                     n_solve, hg, f_follower, f_dead, ws, A, r, f_abs);
+                     n_solve, hg, f_follower, f_dead, ws, A, r, f_abs, guard);
   FJ_CHECK_LAUNCH();
   return FJ_OK;
+}
+
+extern "C" int fj_beam_system(const fj_beam_model* model, int dynamic, const double* hg, const double* v,
+                              const double* v_dot, const double* t_phi, const double* f_follower,
+                              const double* f_dead, double beta_p, double gamma_p, const int32_t* solve_dofs,
+                              int n_solve, double* A, double* r, double* f_abs, void* workspace, void* stream) {
+  return fj_beam_system_g(model, dynamic, hg, v, v_dot, t_phi, f_follower, f_dead, beta_p, gamma_p, solve_dofs,
+                          n_solve, A, r, f_abs, workspace, stream, nullptr);
 }
 
 extern "C" int fj_beam_update(int dynamic, int n_nodes, const int32_t* solve_dofs, int n_solve, const double* x,
@@ -568,7 +590,8 @@ extern "C" int fj_beam_update(int dynamic, int n_nodes, const int32_t* solve_dof
   if (n_solve > 0 && (!solve_dofs || !x || !r || !f_abs)) return FJ_ERR_ARG;
   if (dynamic && (!v || !v_dot)) return FJ_ERR_ARG;
   hipLaunchKernelGGL(k_beam_update, dim3(1), dim3(FJ_BEAM_UPD_THREADS), 0, (hipStream_t)stream, dynamic,
-                     6 * n_nodes, solve_dofs, n_solve, x, relax, gamma_p, beta_p, phi, v, v_dot, r, f_abs, norms);
+                     6 * n_nodes, solve_dofs, n_solve, x, relax, gamma_p, beta_p, phi, v, v_dot, r, f_abs, norms,
+                     nullptr);
   FJ_CHECK_LAUNCH();
   return FJ_OK;
 }
@@ -594,7 +617,7 @@ extern "C" int fj_beam_nodes(int op, int n_nodes, const double* hg, const double
   }
   if (n_nodes == 0) return FJ_OK;
   hipLaunchKernelGGL(k_beam_nodes, dim3(fj_cdiv(n_nodes, 64)), dim3(64), 0, (hipStream_t)stream, op, n_nodes,
-                     hg, x, hg_ref, hg_out, t_out, h_out);
+                     hg, x, hg_ref, hg_out, t_out, h_out, nullptr);
   FJ_CHECK_LAUNCH();
   return FJ_OK;
 }
@@ -610,5 +633,83 @@ extern "C" int fj_beam_project_forces(int n_nodes, const double* rmat, int m1, i
     hipLaunchKernelGGL(k_beam_project, dim3(fj_cdiv(n1, 64)), dim3(64), 0, st, m1, n1, rmat, f_a, f_b, x0,
                        dof_mapping, out);
   FJ_CHECK_LAUNCH();
+  return FJ_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Device-driven dynamic Newton loop
+// ---------------------------------------------------------------------------
+__global__ void k_newton_reset(int32_t* __restrict__ status) {
+  if (threadIdx.x < 4) status[threadIdx.x] = 0;
+}
+
+// ConvergenceStatus.update and get_status (utils/data_structures.py:81-170)
+// on the device: norms = [|dx|, |total disp|, |r|, max |f_abs|, NaN flag];
+// a NaN tolerance is an unset one (every comparison with it is false).
+__device__ __forceinline__ double fj_ratio(double num, double den) {
+  if (den == 0.0) return num != 0.0 ? __builtin_huge_val() : __builtin_nan("");
+  return num / den;
+}
+__device__ __forceinline__ double fj_finite_or_inf(double x) {
+  if (x != x) return __builtin_huge_val();
+  if (x == __builtin_huge_val()) return 1.7976931348623157e308;
+  return x;
+}
+
+__global__ void k_newton_status(const double* __restrict__ norms, fj_convergence conv,
+                                int32_t* __restrict__ status) {
+  if (threadIdx.x != 0 || status[1] != 0) return;
+  const int it = status[0] + 1;
+  const bool has_nan = norms[4] != 0.0;
+  const bool c_ad = norms[0] < conv.abs_disp_tol;
+  const bool c_rd = fj_finite_or_inf(fj_ratio(norms[0], norms[1])) < conv.rel_disp_tol;
+  const bool c_af = norms[2] < conv.abs_force_tol;
+  const bool c_rf = fj_finite_or_inf(fj_ratio(norms[2], norms[3])) < conv.rel_force_tol;
+  const bool converged = (c_rd || c_ad || c_rf || c_af) && it > 0;
+  const bool final_iter = conv.max_n_iter > 0 && it >= conv.max_n_iter;
+  status[0] = it;
+  status[2] = converged ? 1 : 0;
+  status[3] = has_nan ? 1 : 0;
+  status[1] = (converged || has_nan || final_iter) ? 1 : 0;
+}
+
+int fj_band_launch(const double* A, int n, int lda, int kl, int ku, const double* b, double* x, int32_t* info,
+                   const int32_t* guard, hipStream_t st);
+
+extern "C" int fj_beam_newton_dynamic(const fj_beam_model* model, int n_iter, int reset, const double* hg_prev,
+                                      double* phi_a, double* v_a, double* v_dot_a, const double* f_follower,
+                                      const double* f_dead, double beta_p, double gamma_p, double relax,
+                                      const int32_t* solve_dofs, int n_solve, int kl, int ku,
+                                      const fj_convergence* conv, double* hg_u, double* t_phi, double* A,
+                                      double* r, double* f_abs, double* x, double* norms, int32_t* status,
+                                      int32_t* band_info, void* workspace, void* stream) {
+  if (!fj_beam_model_ok(model) || n_iter < 0 || !conv || !status || !hg_prev || !phi_a || !v_a || !v_dot_a ||
+      !hg_u || !t_phi || !norms || n_solve < 0 || n_solve > 6 * model->n_nodes)
+    return FJ_ERR_ARG;
+  if (n_solve > 0 && (!A || !r || !f_abs || !x || !solve_dofs)) return FJ_ERR_ARG;
+  if (n_solve > 0 && (kl < 0 || ku < 0 || kl > 63)) return FJ_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  if (reset) {
+    hipLaunchKernelGGL(k_newton_reset, dim3(1), dim3(64), 0, st, status);
+    FJ_CHECK_LAUNCH();
+  }
+  const int nn = model->n_nodes;
+  for (int it = 0; it < n_iter; ++it) {
+    hipLaunchKernelGGL(k_beam_nodes, dim3(fj_cdiv(nn, 64)), dim3(64), 0, st, FJ_BEAM_NODE_UPDATE, nn, hg_prev,
+                       (const double*)phi_a, (const double*)nullptr, hg_u, t_phi, (double*)nullptr,
+                       (const int32_t*)status);
+    int rc = fj_beam_system_g(model, 1, hg_u, v_a, v_dot_a, t_phi, f_follower, f_dead, beta_p, gamma_p,
+                              solve_dofs, n_solve, A, r, f_abs, workspace, stream, status);
+    if (rc != FJ_OK) return rc;
+    if (n_solve > 0) {
+      rc = fj_band_launch(A, n_solve, n_solve, kl, ku, r, x, band_info, status, st);
+      if (rc != FJ_OK) return rc;
+    }
+    hipLaunchKernelGGL(k_beam_update, dim3(1), dim3(FJ_BEAM_UPD_THREADS), 0, st, 1, 6 * nn, solve_dofs, n_solve,
+                       (const double*)x, relax, gamma_p, beta_p, phi_a, v_a, v_dot_a, (const double*)r,
+                       (const double*)f_abs, norms, (const int32_t*)status);
+    hipLaunchKernelGGL(k_newton_status, dim3(1), dim3(64), 0, st, (const double*)norms, *conv, status);
+    FJ_CHECK_LAUNCH();
+  }
   return FJ_OK;
 }

@@ -950,7 +950,9 @@ template <int KLMAX>
 __global__ __launch_bounds__(64) void k_band_solve(const double* __restrict__ A, int n, int lda,
                                                    int kl, int ku, const double* __restrict__ b,
                                                    double* __restrict__ x,
-                                                   int32_t* __restrict__ info) {
+                                                   int32_t* __restrict__ info,
+                                                   const int32_t* __restrict__ guard) {
+  if (guard && guard[1] != 0) return;
   __shared__ double s[FJ_BAND_LDS_DOUBLES];
   const int lane = threadIdx.x;
   const int W = 2 * kl + ku + 1;
@@ -1091,7 +1093,9 @@ template <int KL, int KU>
 __global__ __launch_bounds__(64) void k_band_solve_fixed(const double* __restrict__ A, int n,
                                                          int lda, const double* __restrict__ b,
                                                          double* __restrict__ x,
-                                                         int32_t* __restrict__ info) {
+                                                         int32_t* __restrict__ info,
+                                                         const int32_t* __restrict__ guard) {
+  if (guard && guard[1] != 0) return;
   constexpr int W = 2 * KL + KU + 1;
   constexpr int D = W - 1;  // from (row i, column c) to (row i + 1, column c)
   __shared__ double s[FJ_BAND_LDS_DOUBLES];
@@ -1195,19 +1199,23 @@ __global__ __launch_bounds__(64) void k_band_solve_fixed(const double* __restric
 
 extern "C" size_t fj_band_solve_capacity(void) { return FJ_BAND_LDS_DOUBLES; }
 
-extern "C" int fj_band_solve(const double* A, int n, int lda, int kl, int ku, const double* b,
-                             double* x, int32_t* info, void* stream) {
+// The band solve launch; a non-NULL guard (a Newton status word) skips it
+// once the loop it belongs to has finished.
+int fj_band_launch(const double* A, int n, int lda, int kl, int ku, const double* b, double* x, int32_t* info,
+                   const int32_t* guard, hipStream_t st) {
   if (n <= 0 || lda < n || kl < 0 || ku < 0 || kl > 63 || !A || !b || !x) return FJ_ERR_ARG;
   if ((size_t)n * (size_t)(2 * kl + ku + 2) > (size_t)FJ_BAND_LDS_DOUBLES) return FJ_ERR_ARG;
   if (kl == 11 && ku == 11 && (size_t)(n + 11) * 36 <= (size_t)FJ_BAND_LDS_DOUBLES)
-    hipLaunchKernelGGL((k_band_solve_fixed<11, 11>), dim3(1), dim3(64), 0, (hipStream_t)stream, A,
-                       n, lda, b, x, info);
+    hipLaunchKernelGGL((k_band_solve_fixed<11, 11>), dim3(1), dim3(64), 0, st, A, n, lda, b, x, info, guard);
   else if (kl <= 15)
-    hipLaunchKernelGGL(k_band_solve<15>, dim3(1), dim3(64), 0, (hipStream_t)stream, A, n, lda, kl, ku,
-                       b, x, info);
+    hipLaunchKernelGGL(k_band_solve<15>, dim3(1), dim3(64), 0, st, A, n, lda, kl, ku, b, x, info, guard);
   else
-    hipLaunchKernelGGL(k_band_solve<63>, dim3(1), dim3(64), 0, (hipStream_t)stream, A, n, lda, kl, ku,
-                       b, x, info);
+    hipLaunchKernelGGL(k_band_solve<63>, dim3(1), dim3(64), 0, st, A, n, lda, kl, ku, b, x, info, guard);
   FJ_CHECK_LAUNCH();
   return FJ_OK;
+}
+
+extern "C" int fj_band_solve(const double* A, int n, int lda, int kl, int ku, const double* b,
+                             double* x, int32_t* info, void* stream) {
+  return fj_band_launch(A, n, lda, kl, ku, b, x, info, nullptr, (hipStream_t)stream);
 }

@@ -508,6 +508,34 @@ int fj_beam_update(int dynamic, int n_nodes, const int32_t* solve_dofs, int n_so
                    double relax, double gamma_p, double beta_p, double* phi, double* v, double* v_dot,
                    const double* r, const double* f_abs, double* norms, void* stream);
 
+/* Convergence settings of a Newton loop (ConvergenceSettings,
+ * utils/data_structures.py:16-30): a NaN tolerance is unset (it never
+ * passes); max_n_iter <= 0 sets no limit. */
+typedef struct fj_convergence {
+  int max_n_iter;
+  double rel_disp_tol, abs_disp_tol, rel_force_tol, abs_force_tol;
+} fj_convergence;
+
+/* Up to n_iter iterations of the dynamic Newton loop at the alpha point
+ * (structure/beam.py:2170-2316, 2727-2792) driven by a device status word,
+ * so the host waits once per batch instead of once per iteration.  Each
+ * iteration: hg_u = hg_prev exp(phi_a) and t_phi = T(phi_a); the Newton
+ * system (fj_beam_system, dynamic); x = A^-1 r by the banded solve
+ * (bandwidths kl, ku, as fj_band_solve); the update of phi_a, v_a, v_dot_a
+ * and the norms (fj_beam_update); then ConvergenceStatus.update on the
+ * device.  status (4 int32): [0] iterations run, [1] done (converged, NaN or
+ * the iteration limit), [2] converged, [3] NaN met; every launch of an
+ * iteration returns at once after done, so iterations may be enqueued past
+ * convergence.  reset != 0 zeroes status first (a new loop).  No counterpart
+ * entry in the reference (its loop is a Python while around jnp.linalg.solve). */
+int fj_beam_newton_dynamic(const fj_beam_model* model, int n_iter, int reset, const double* hg_prev,
+                           double* phi_a, double* v_a, double* v_dot_a, const double* f_follower,
+                           const double* f_dead, double beta_p, double gamma_p, double relax,
+                           const int32_t* solve_dofs, int n_solve, int kl, int ku,
+                           const fj_convergence* conv, double* hg_u, double* t_phi, double* A, double* r,
+                           double* f_abs, double* x, double* norms, int32_t* status, int32_t* band_info,
+                           void* workspace, void* stream);
+
 /* out = |x[idx]| (the static solve's total-displacement norm). */
 int fj_beam_subnorm(const double* x, const int32_t* idx, int n, double* out, void* stream);
 

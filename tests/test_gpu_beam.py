@@ -243,3 +243,36 @@ def test_coupled_static_and_dynamic(dev):
     _held("gamma_b", dyn.aero.gamma_b[0].cpu().numpy(), ref["aero"].gamma_b[0], pdyn["aero"].gamma_b[0])
     _held("zeta_w", dyn.aero.zeta_w[0].cpu().numpy(), ref["aero"].zeta_w[0], pdyn["aero"].zeta_w[0])
     _held("f_ext_aero", dyn.structure.f_ext_aero.cpu().numpy(), ref["f_ext_aero"], pdyn["f_ext_aero"])
+
+
+def test_device_newton_loop_equals_host_loop(dev):
+    """The dynamic Newton loop with its stop test on the device
+    (fj_beam_newton_dynamic: batches of guarded iterations, one status read
+    per batch) runs the iterations the host-driven loop runs, to the bit:
+    same trajectory and the same iteration count at every step, including a
+    loop that stops at the iteration limit."""
+    from flapjax_amd.beam import ConvergenceSettings
+
+    n = 6
+    ob = B.BeamOracle(n, B.chain(n), np.array((0.0, 0.0, 1.0)), spectral_radius=0.8)
+    coords = np.zeros((n, 3))
+    coords[:, 1] = np.linspace(0.0, 4.0, n)
+    ob.set_design_variables(coords, np.diag((1e5, 1e5, 1e5, 300.0, 400.0, 800.0)),
+                            np.diag((0.9, 0.9, 0.9, 0.05, 0.03, 0.07)))
+    n_t, dt = 20, 0.01
+    t = dt * np.arange(n_t)
+    ff = np.zeros((n_t, n, 6))
+    ff[:, -1, 2] = 40.0 * np.sin(7.0 * t)
+    outs = []
+    for conv in (None, ConvergenceSettings(2, None, None, 1e-14, None)):
+        for device in (True, False):
+            b = _beam_from_oracle(ob)
+            if conv is not None:
+                b.struct_convergence_settings = conv
+            b.device_newton = device
+            outs.append(b.dynamic_solve(None, np.arange(6), n_t, dt, f_ext_follower=ff))
+    for a, c in ((outs[0], outs[1]), (outs[2], outs[3])):
+        assert a.iters["struct"] == c.iters["struct"]
+        for name in ("hg", "v", "v_dot"):
+            assert torch.equal(getattr(a, name), getattr(c, name)), name
+    assert max(outs[0].iters["struct"]) >= 2 and set(outs[2].iters["struct"]) == {2}
