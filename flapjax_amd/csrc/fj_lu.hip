@@ -1,11 +1,11 @@
 // K3: dense circulation solve, replacing jnp.linalg.solve (aero/uvlm.py:985).
 //
 // Blocked right-looking LU with partial pivoting on a row-major matrix.
-//  * panel kernel (one 1024-thread workgroup): the (n-k) x jb panel is held in
+//  * panel kernel (one 512-thread workgroup): the (n-k) x jb panel is held in
 //    registers, each thread owning up to R rows; per column: register argmax,
-//    wave shuffles + LDS for the workgroup argmax, the pivot row broadcast
-//    through LDS, row exchange between the two owner threads, scale and
-//    rank-1 update in registers.  The composed row permutation is kept in
+//    a DPP wave maximum, each wave's candidate row and key published in LDS,
+//    one barrier, the workgroup argmax reduced by every thread, row exchange
+//    between the two owner threads, scale and rank-1 update in registers.  The composed row permutation is kept in
 //    ipiv[n .. 2n) so the solves gather Pb in one parallel step.
 //  * swap/trsm kernel, one thread per column outside the panel: the panel's
 //    row exchanges (laswp) and, right of the panel, the unit-lower triangular
@@ -21,122 +21,231 @@
 
 #define FJ_LU_THREADS 1024
 #define FJ_LU_WAVES (FJ_LU_THREADS / 64)
-#define FJ_PANEL_THREADS 512
-#define FJ_PANEL_WAVES (FJ_PANEL_THREADS / 64)
 
 template <typename T>
 __device__ __forceinline__ T fj_abs(T x) { return x < T(0) ? -x : x; }
 
 // Pivot search key: |a| as the high bits (non-negative doubles order like
-// their bit patterns) with the low 13 mantissa bits replaced by 8191 - row,
-// so one unsigned max picks the largest |a| and, among values equal to about
-// 2^-39 relative, the smallest row.  Rows < 8192.
+// their bit patterns; |a| clears the sign bit, so -0.0 keys like +0.0) with
+// the low 13 mantissa bits replaced by 8191 - row, so one unsigned max picks
+// the largest |a| and, among values equal to about 2^-39 relative, the
+// smallest row.  Rows < 8192.
 __device__ __forceinline__ unsigned long long fj_pivot_key(double a, int row) {
-  const unsigned long long b = (unsigned long long)__double_as_longlong(a < 0.0 ? -a : a);
+  const unsigned long long b = (unsigned long long)__double_as_longlong(a) & 0x7FFFFFFFFFFFFFFFull;
   return (b & ~0x1FFFull) | (unsigned long long)(8191 - row);
 }
 
-template <typename T, int R, int JB>
-__global__ __launch_bounds__(FJ_PANEL_THREADS) void k_lu_panel(T* __restrict__ A, int n, int lda,
-                                                               int k, int32_t* __restrict__ ipiv,
-                                                               int32_t* __restrict__ info) {
-  __shared__ unsigned long long s_key[FJ_PANEL_WAVES];
-  __shared__ T s_piv[2][JB];
+// One step of a wave-wide unsigned maximum through the DPP crossbar: each
+// lane takes the value of the lane CTRL names (a register move, no LDS round
+// trip; the move folds into v_max_u32's DPP operand).  Lanes whose source is
+// masked off or out of range read 0, the identity of the maximum.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ unsigned fj_dpp_max_u32(unsigned x) {
+  const unsigned y = (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, CTRL, ROW_MASK, 0xf, false);
+  return x > y ? x : y;
+}
+
+// Maximum over the 64 lanes of a (fully active) wave: row maxima by quad
+// permutations and row rotations, then the two row broadcasts carry them
+// into lane 63, whose value every lane reads back.
+__device__ __forceinline__ unsigned fj_wave_max_u32(unsigned x) {
+  x = fj_dpp_max_u32<0xb1, 0xf>(x);   // quad_perm [1,0,3,2]
+  x = fj_dpp_max_u32<0x4e, 0xf>(x);   // quad_perm [2,3,0,1]
+  x = fj_dpp_max_u32<0x124, 0xf>(x);  // row_ror:4
+  x = fj_dpp_max_u32<0x128, 0xf>(x);  // row_ror:8
+  x = fj_dpp_max_u32<0x142, 0xa>(x);  // row_bcast:15 into rows 1, 3
+  x = fj_dpp_max_u32<0x143, 0xc>(x);  // row_bcast:31 into rows 2, 3
+  return (unsigned)__builtin_amdgcn_readlane((int)x, 63);
+}
+
+// 64-bit maximum in two 32-bit passes: the high words, then the low words of
+// the lanes that hold the maximal high word.
+__device__ __forceinline__ unsigned long long fj_wave_max_u64(unsigned long long key) {
+  const unsigned hi = (unsigned)(key >> 32);
+  const unsigned mh = fj_wave_max_u32(hi);
+  const unsigned ml = fj_wave_max_u32(hi == mh ? (unsigned)key : 0u);
+  return ((unsigned long long)mh << 32) | ml;
+}
+
+// NT threads, each owning the panel rows tid + r * NT (r < R).  Per column,
+// one workgroup barrier: every wave finds its own pivot candidate (register
+// argmax, then the DPP wave maximum), its owner lane publishes the candidate
+// row and the wave writes its key; the owner of row jj publishes that row.
+// After the barrier every thread reduces the wave keys itself, takes the
+// winning row from LDS and applies the exchange, scaling and rank-1 update
+// in registers.  The LDS slots alternate between columns: a wave can only
+// write slot (jj + 2) & 1 after every wave has passed barrier jj + 1, i.e.
+// has finished reading column jj.
+//
+// The column loop is NOT unrolled (an unrolled loop is tens of KB of code,
+// more than the instruction cache, and then runs at the rate instructions
+// stream in from L2).  Register indices must stay compile-time constants, so
+// every row's registers rotate left by one position per column: position 0
+// always holds the current column, positions 1 .. JB-1-jj the columns still
+// to update, and the finished columns wrap round to the end; after jb
+// columns, position q holds column (q + jb) mod JB.
+template <typename T, int NT, int R, int JB>
+__global__ __launch_bounds__(NT) void k_lu_panel(T* __restrict__ A, int n, int lda, int k,
+                                                 int32_t* __restrict__ ipiv,
+                                                 int32_t* __restrict__ info) {
+  constexpr int NW = NT / 64;
+  static_assert(NT % 64 == 0 && 2 * JB <= NT, "panel shape");
+  __shared__ unsigned long long s_key[2][NW];
+  __shared__ T s_cand[2][NW][JB];
   __shared__ T s_swp[2][JB];
   __shared__ int s_ipiv[JB];
+  __shared__ T s_stage[NT][JB + 1];  // one row block, padded against bank conflicts
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int jb = min(JB, n - k);
   const int rows = n - k;
 
-  // load the panel: thread tid owns local rows tid + r * FJ_PANEL_THREADS
+  // Load through LDS: row-per-thread loads would touch one cache line per
+  // lane and instruction.  Block r (rows r*NT .. r*NT+NT-1) is read with
+  // consecutive lanes on consecutive columns of a row (all blocks in flight
+  // at once), then transposed through one padded LDS block.
+  // (A wide tile is staged one block at a time, keeping the register peak
+  // at the tile plus one block.)
   T a[R][JB];
+  constexpr int RB = (R * JB <= 64) ? R : 1;  // blocks loaded per round
 #pragma unroll
-  for (int r = 0; r < R; ++r) {
-    const int lr = tid + r * FJ_PANEL_THREADS;
+  for (int r0 = 0; r0 < R; r0 += RB) {
+    T t[RB][JB];
 #pragma unroll
-    for (int c = 0; c < JB; ++c)
-      a[r][c] = (lr < rows && c < jb) ? A[(size_t)(k + lr) * lda + k + c] : T(0);
+    for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+      for (int i = 0; i < JB; ++i) {
+        const int e = tid + i * NT, lr = (r0 + rb) * NT + e / JB, c = e % JB;
+        t[rb][i] = (lr < rows && c < jb) ? A[(size_t)(k + lr) * lda + k + c] : T(0);
+      }
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb) {
+#pragma unroll
+      for (int i = 0; i < JB; ++i) {
+        const int e = tid + i * NT;
+        s_stage[e / JB][e % JB] = t[rb][i];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int c = 0; c < JB; ++c) a[r0 + rb][c] = s_stage[tid][c];
+      __syncthreads();
+    }
   }
 
-  // fully unrolled so that every register index below is a compile-time constant
+#pragma nounroll
+  for (int jj = 0; jj < jb; ++jj) {
+    const int buf = jj & 1;
+    unsigned long long key = 0ull;
 #pragma unroll
-  for (int jj = 0; jj < JB; ++jj) {
-    if (jj < jb) {  // uniform: jb is the same for the whole workgroup
-      const int buf = jj & 1;
-      unsigned long long key = 0ull;
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        const int lr = tid + r * FJ_PANEL_THREADS;
-        if (lr >= jj && lr < rows) {
-          const unsigned long long kk = fj_pivot_key((double)a[r][jj], lr);
-          key = kk > key ? kk : key;
-        }
-      }
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) {
-        const unsigned long long o = __shfl_xor(key, off);
-        key = o > key ? o : key;
-      }
-      if (lane == 0) s_key[wave] = key;
-      __syncthreads();
-      key = s_key[0];
-#pragma unroll
-      for (int w = 1; w < FJ_PANEL_WAVES; ++w) key = s_key[w] > key ? s_key[w] : key;
-      const int p = 8191 - (int)(key & 0x1FFFull);  // local pivot row
-      // owners publish the pivot row and the current row jj
+    for (int r = 0; r < R; ++r) {
+      const int lr = tid + r * NT;
+      const unsigned long long kk = fj_pivot_key((double)a[r][0], lr);
+      key = (lr >= jj && lr < rows && kk > key) ? kk : key;
+    }
+    const unsigned long long wkey = fj_wave_max_u64(key);
+    // a wave with no row at or below jj has key 0, which never wins: row jj
+    // itself has a nonzero key (8191 - jj in its low bits)
+    if (lane == 0) s_key[buf][wave] = wkey;
+    if (wkey != 0ull) {
+      const int wp = 8191 - (int)(wkey & 0x1FFFull);
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        const int lr = tid + r * FJ_PANEL_THREADS;
-        if (lr == p) {
+        if (tid + r * NT == wp) {
 #pragma unroll
-          for (int c = 0; c < JB; ++c) s_piv[buf][c] = a[r][c];
-        }
-        if (lr == jj) {
-#pragma unroll
-          for (int c = 0; c < JB; ++c) s_swp[buf][c] = a[r][c];
+          for (int c = 0; c < JB; ++c) s_cand[buf][wave][c] = a[r][c];
         }
       }
-      if (tid == 0) {
-        s_ipiv[jj] = k + p;
-        if ((key >> 13) == 0ull && *info == 0) *info = k + jj + 1;  // zero column
+    }
+    if (tid == jj) {  // row jj < JB <= NT sits in a[0]
+#pragma unroll
+      for (int c = 0; c < JB; ++c) s_swp[buf][c] = a[0][c];
+    }
+    __syncthreads();
+    unsigned long long best = s_key[buf][0];
+#pragma unroll
+    for (int w = 1; w < NW; ++w) {
+      const unsigned long long kw = s_key[buf][w];
+      best = kw > best ? kw : best;
+    }
+    const int p = 8191 - (int)(best & 0x1FFFull);  // local pivot row
+    const int bw = (p % NT) >> 6;                    // the wave that published it
+    if (tid == 0) {
+      s_ipiv[jj] = k + p;
+      if ((best >> 13) == 0ull && *info == 0) *info = k + jj + 1;  // zero column
+    }
+    const T* prow = s_cand[buf][bw];
+    const T piv = prow[0];
+    const T inv = (piv != T(0)) ? T(1) / piv : T(0);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int lr = tid + r * NT;
+      if (lr == jj) {
+#pragma unroll
+        for (int c = 0; c < JB; ++c) a[r][c] = prow[c];
+      } else if (lr == p) {
+#pragma unroll
+        for (int c = 0; c < JB; ++c) a[r][c] = s_swp[buf][c];
       }
-      __syncthreads();
-      const T piv = s_piv[buf][jj];
-      const T inv = (piv != T(0)) ? T(1) / piv : T(0);
+    }
+    // multipliers of the rows below jj; the other rows get l = 0, so the
+    // update below needs no per-row predicate (it leaves them unchanged)
+    T l[R];
 #pragma unroll
-      for (int r = 0; r < R; ++r) {
-        const int lr = tid + r * FJ_PANEL_THREADS;
-        if (lr == jj) {
+    for (int r = 0; r < R; ++r) {
+      const int lr = tid + r * NT;
+      const bool below = lr > jj && lr < rows;
+      l[r] = below ? a[r][0] * inv : T(0);
+      a[r][0] = below ? l[r] : a[r][0];
+    }
+    // the pivot row in registers, read in batches of 8 ahead of their use:
+    // read inside the per-column branches, every read would wait out the LDS
+    // latency on its own
+    const int live = JB - jj;  // positions 0 .. live-1 are columns jj .. JB-1
+    constexpr int PB = JB < 8 ? JB : 8;
 #pragma unroll
-          for (int c = 0; c < JB; ++c) a[r][c] = s_piv[buf][c];
-        } else if (lr == p) {
+    for (int c0 = 0; c0 < JB; c0 += PB) {
+      if ((c0 < 1 ? 1 : c0) < live) {  // uniform: some column of the batch is live
+        T pv[PB];
 #pragma unroll
-          for (int c = 0; c < JB; ++c) a[r][c] = s_swp[buf][c];
-        }
-        if (lr > jj && lr < rows) {
-          const T l = a[r][jj] * inv;
-          a[r][jj] = l;
+        for (int i = 0; i < PB; ++i) pv[i] = prow[c0 + i];
 #pragma unroll
-          for (int c = 0; c < JB; ++c)
-            if (c > jj) a[r][c] -= l * s_piv[buf][c];
+        for (int i = 0; i < PB; ++i) {
+          const int c = c0 + i;
+          if (c >= 1 && c < live) {  // uniform
+#pragma unroll
+            for (int r = 0; r < R; ++r) a[r][c] -= l[r] * pv[i];
+          }
         }
       }
-      // the next column writes the other LDS buffers (s_key is protected by
-      // the second barrier of this column)
+    }
+    // rotate: position q takes position q + 1, position 0 wraps to the end
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const T t0 = a[r][0];
+#pragma unroll
+      for (int c = 0; c + 1 < JB; ++c) a[r][c] = a[r][c + 1];
+      a[r][JB - 1] = t0;
     }
   }
   __syncthreads();
 
-  // write the panel back
+  // write the panel back through the same LDS block: position q holds
+  // column (q + jb) mod JB
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    const int lr = tid + r * FJ_PANEL_THREADS;
-    if (lr < rows) {
 #pragma unroll
-      for (int c = 0; c < JB; ++c)
-        if (c < jb) A[(size_t)(k + lr) * lda + k + c] = a[r][c];
+    for (int q = 0; q < JB; ++q) {
+      int c = q + jb;
+      if (c >= JB) c -= JB;
+      s_stage[tid][c] = a[r][q];
     }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < JB; ++i) {
+      const int e = tid + i * NT, lr = r * NT + e / JB, c = e % JB;
+      if (lr < rows && c < jb) A[(size_t)(k + lr) * lda + k + c] = s_stage[e / JB][c];
+    }
+    __syncthreads();
   }
   if (tid < jb) ipiv[k + tid] = s_ipiv[tid];
   // composed permutation ipiv[n + q]: only positions k..k+jb-1 and the pivot
@@ -167,42 +276,77 @@ __global__ __launch_bounds__(FJ_PANEL_THREADS) void k_lu_panel(T* __restrict__ A
 // column while its U12 is only formed inside the current outer block
 // (trsm_end); the outer block's own U12 is then formed once, without the
 // exchanges (do_swap = 0).
+//
+// The exchanges are applied as one gather, not one after another: only rows
+// k .. k+jb-1 and the pivot rows change, and the final content of each such
+// row q is the original content of the row src(q) found by undoing the
+// exchanges from the last to the first.  Each thread loads all its sources
+// at once (independent loads in flight together, where a sequence of
+// exchanges would wait out one memory round trip per exchange), solves the
+// top jb rows in registers and stores.
+#define FJ_SWAP_THREADS 128
 template <typename T, int JB>
-__global__ __launch_bounds__(256) void k_lu_swap_trsm(T* __restrict__ A, int n, int lda, int k,
-                                                      int jb, const int32_t* __restrict__ ipiv,
-                                                      int trsm_end, int do_swap) {
-  __shared__ T s_L11[JB][JB + 1];
-  __shared__ T s_u[JB][256];
-  __shared__ int s_ipiv[JB];
+__global__ __launch_bounds__(FJ_SWAP_THREADS) void k_lu_swap_trsm(T* __restrict__ A, int n, int lda,
+                                                                   int k, int jb,
+                                                                   const int32_t* __restrict__ ipiv,
+                                                                   int trsm_end, int do_swap) {
+  __shared__ T s_L11[JB][JB];
+  __shared__ int s_piv[JB], s_q[2 * JB], s_src[2 * JB];
   const int tid = threadIdx.x;
-  for (int e = tid; e < JB * JB; e += blockDim.x) {
+  for (int e = tid; e < JB * JB; e += FJ_SWAP_THREADS) {
     const int r = e / JB, q = e % JB;
-    s_L11[r][q] = (r < jb && q < jb) ? A[(size_t)(k + r) * lda + k + q] : T(0);
+    s_L11[r][q] = (r < jb && q < jb && q < r) ? A[(size_t)(k + r) * lda + k + q] : T(0);
   }
-  if (tid < JB) s_ipiv[tid] = (tid < jb) ? ipiv[k + tid] : k + tid;
+  if (tid < JB) s_piv[tid] = (tid < jb) ? ipiv[k + tid] : 0;
   __syncthreads();
-  int c = blockIdx.x * blockDim.x + tid;
-  if (c >= n - jb) return;
-  if (c >= k) c += jb;  // skip the panel columns
-  if (do_swap) {
-    for (int jj = 0; jj < jb; ++jj) {
-      const int p = s_ipiv[jj];
-      if (p != k + jj) {
-        const T t0 = A[(size_t)(k + jj) * lda + c];
-        A[(size_t)(k + jj) * lda + c] = A[(size_t)p * lda + c];
-        A[(size_t)p * lda + c] = t0;
-      }
+  for (int t = tid; t < 2 * JB; t += FJ_SWAP_THREADS) {
+    // target rows: k + t (t < jb), then the pivot rows; -1 marks "none"
+    int q = -1, src = -1;
+    if (t < jb) q = k + t;
+    else if (do_swap && t >= JB && t - JB < jb) q = s_piv[t - JB];
+    if (q >= 0) {
+      src = q;
+      if (do_swap)
+        for (int jj = jb - 1; jj >= 0; --jj) {
+          const int a0 = k + jj, b0 = s_piv[jj];
+          if (src == a0) src = b0;
+          else if (src == b0) src = a0;
+        }
     }
+    // a pivot row inside k .. k+jb-1 is written as a top row already
+    if (t >= JB && q >= k && q < k + jb) q = -1;
+    s_q[t] = q;
+    s_src[t] = src;
+  }
+  __syncthreads();
+  int c = blockIdx.x * FJ_SWAP_THREADS + tid;
+  if (do_swap) {  // every column but the panel's
+    if (c >= n - jb) return;
+    if (c >= k) c += jb;
+  } else {  // the trsm columns only
+    c += k + jb;
+    if (c >= trsm_end) return;
+  }
+  T v[2 * JB];
+#pragma unroll
+  for (int t = 0; t < 2 * JB; ++t) {
+    const int src = s_src[t];
+    v[t] = (s_q[t] >= 0) ? A[(size_t)src * lda + c] : T(0);
   }
   if (c >= k + jb && c < trsm_end) {
-    // forward substitution with the column staged in this thread's LDS slot
-    for (int r = 0; r < jb; ++r) s_u[r][tid] = A[(size_t)(k + r) * lda + c];
-    for (int r = 1; r < jb; ++r) {
-      T acc = s_u[r][tid];
-      for (int q = 0; q < r; ++q) acc -= s_L11[r][q] * s_u[q][tid];
-      s_u[r][tid] = acc;
-      A[(size_t)(k + r) * lda + c] = acc;
+    // forward substitution with the unit lower L11 (rows past jb are zero)
+#pragma unroll
+    for (int r = 1; r < JB; ++r) {
+      T acc = v[r];
+#pragma unroll
+      for (int q = 0; q < r; ++q) acc -= s_L11[r][q] * v[q];
+      v[r] = acc;
     }
+  }
+#pragma unroll
+  for (int t = 0; t < 2 * JB; ++t) {
+    const int q = s_q[t];
+    if (q >= 0) A[(size_t)q * lda + c] = v[t];
   }
 }
 
@@ -298,7 +442,7 @@ __global__ __launch_bounds__(64) void k_diag_inverses(const T* __restrict__ LU, 
                                                       T* __restrict__ aux);
 __host__ __device__ static inline int fj_nblk(int n);
 
-template <typename T, int R, int JB>
+template <typename T, int NT, int R, int JB>
 static int fj_lu_factor_t(T* A, int n, int lda, int32_t* ipiv, int32_t* info, T* aux,
                           hipStream_t st) {
   hipLaunchKernelGGL(k_perm_init, dim3(fj_cdiv(n, 256)), dim3(256), 0, st, ipiv, n, info);
@@ -314,11 +458,12 @@ static int fj_lu_factor_t(T* A, int n, int lda, int32_t* ipiv, int32_t* info, T*
     const int kend = k0 + nb;
     for (int k = k0; k < kend; k += JB) {
       const int jb = (n - k < JB) ? n - k : JB;
-      hipLaunchKernelGGL((k_lu_panel<T, R, JB>), dim3(1), dim3(FJ_PANEL_THREADS), 0, st, A, n, lda,
-                         k, ipiv, info);
+      hipLaunchKernelGGL((k_lu_panel<T, NT, R, JB>), dim3(1), dim3(NT), 0, st, A, n, lda, k, ipiv,
+                         info);
       FJ_CHECK_LAUNCH();
       if (n - jb > 0) {
-        hipLaunchKernelGGL((k_lu_swap_trsm<T, JB>), dim3(fj_cdiv(n - jb, 256)), dim3(256), 0, st, A,
+        hipLaunchKernelGGL((k_lu_swap_trsm<T, JB>), dim3(fj_cdiv(n - jb, FJ_SWAP_THREADS)),
+                           dim3(FJ_SWAP_THREADS), 0, st, A,
                            n, lda, k, jb, ipiv, JB == NB ? n : kend, 1);
         FJ_CHECK_LAUNCH();
       }
@@ -333,8 +478,8 @@ static int fj_lu_factor_t(T* A, int n, int lda, int32_t* ipiv, int32_t* info, T*
     if (JB < NB && kend < n) {
       // the outer block's U12 (its exchanges are already applied) and the
       // rank-nb trailing update
-      hipLaunchKernelGGL((k_lu_swap_trsm<T, NB>), dim3(fj_cdiv(n - nb, 256)), dim3(256), 0, st, A, n,
-                         lda, k0, nb, ipiv, n, 0);
+      hipLaunchKernelGGL((k_lu_swap_trsm<T, NB>), dim3(fj_cdiv(n - kend, FJ_SWAP_THREADS)),
+                         dim3(FJ_SWAP_THREADS), 0, st, A, n, lda, k0, nb, ipiv, n, 0);
       FJ_CHECK_LAUNCH();
       const int rest = n - kend;
       const dim3 g(fj_cdiv(rest, FJ_GT), fj_cdiv(rest, FJ_GT));
@@ -351,11 +496,12 @@ template <typename T>
 static int fj_lu_dispatch(T* A, int n, int lda, int32_t* ipiv, int32_t* info, T* aux,
                           hipStream_t st) {
   if (n <= 0 || n > 8192 || lda < n || !A || !ipiv || !info || !aux) return FJ_ERR_ARG;
-  if (n <= 512) return fj_lu_factor_t<T, 1, 32>(A, n, lda, ipiv, info, aux, st);
-  if (n <= 1024) return fj_lu_factor_t<T, 2, 32>(A, n, lda, ipiv, info, aux, st);
-  if (n <= 2048) return fj_lu_factor_t<T, 4, 16>(A, n, lda, ipiv, info, aux, st);
-  if (n <= 4096) return fj_lu_factor_t<T, 8, 8>(A, n, lda, ipiv, info, aux, st);
-  return fj_lu_factor_t<T, 16, 4>(A, n, lda, ipiv, info, aux, st);
+  // panel shapes (threads, rows per thread, columns held in registers;
+  // threads x rows >= n), the fastest measured at each size
+  if (n <= 1024) return fj_lu_factor_t<T, 512, 2, 32>(A, n, lda, ipiv, info, aux, st);
+  if (n <= 2048) return fj_lu_factor_t<T, 512, 4, 16>(A, n, lda, ipiv, info, aux, st);
+  if (n <= 4096) return fj_lu_factor_t<T, 512, 8, 8>(A, n, lda, ipiv, info, aux, st);
+  return fj_lu_factor_t<T, 512, 16, 4>(A, n, lda, ipiv, info, aux, st);
 }
 
 extern "C" int fj_lu_factor(double* A, int n, int lda, int32_t* ipiv, int32_t* info, void* aux,

@@ -372,7 +372,7 @@ class PrescribedAdjoint:
                 z0 = twisted_grid(zeta0, tw)
                 j_i = self._objective(q_i, z0, uu, hg_t[i], hgd_t[i], t[i])
                 gj = torch.autograd.grad(j_i, (q_i, tw, uu))
-            J += float(j_i)
+            J += float(j_i.detach())
             rhs = -gj[0]
             g_tw += gj[1]
             g_u += gj[2]

@@ -57,7 +57,7 @@ def test_aic_times_gamma_equals_normal_velocity(dev):
     assert rel_err(lhs, rhs) < 1e-12
 
 
-@pytest.mark.parametrize("n", [1, 7, 33, 64, 100, 257, 1024, 1500, 2100])
+@pytest.mark.parametrize("n", [1, 7, 33, 64, 100, 257, 1024, 1500, 2100, 4200])
 def test_lu_solve_random(dev, n):
     from flapjax_amd import _kernels as K
 
@@ -102,6 +102,23 @@ def test_lu_pivots_match_lapack_order(dev):
     lu_ref, piv_ref = scipy.linalg.lu_factor(A)
     assert np.array_equal(piv.cpu().numpy()[:96], piv_ref)
     assert rel_err(LU.cpu().numpy(), lu_ref) < 1e-12
+
+
+def test_lu_negative_zero_is_not_a_pivot(dev):
+    """-0.0 entries are |a| = 0 to the pivot search (a sign bit left in the
+    search key would make them the largest), so LAPACK's rows come out."""
+    from flapjax_amd import _kernels as K
+    import scipy.linalg
+
+    rng = np.random.default_rng(11)
+    A = rng.normal(size=(80, 80))
+    A[0, 0] = -0.0
+    A[5:40, 3] = -0.0
+    fac = K.lu_factor(_t(A, dev))
+    assert int(fac.info.item()) == 0
+    lu_ref, piv_ref = scipy.linalg.lu_factor(A)
+    assert np.array_equal(fac.piv.cpu().numpy()[:80], piv_ref)
+    assert rel_err(fac.lu.cpu().numpy(), lu_ref) < 1e-12
 
 
 def test_singular_reports_info(dev):
