@@ -307,7 +307,9 @@ def coupled_bench(precision, steps=CPL_STEPS):
             "lift_max": round(float(lift.max()), 6)}
 
 
-ADJ_STEPS, ADJ_TIP_WEIGHT, ADJ_CHECKPOINT = 100, 1.0, 10
+# checkpoint 0: the whole 100-step history stays in HBM (no recomputed
+# forward; peak reported in the line)
+ADJ_STEPS, ADJ_TIP_WEIGHT, ADJ_CHECKPOINT = 100, 1.0, 0
 ADJ_CHECK_STEPS, ADJ_CHECK_NEWTON, ADJ_CHECK_FSI = 10, 8, 10
 
 

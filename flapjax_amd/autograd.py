@@ -18,6 +18,7 @@ These replace what the reference obtains from jax.vjp of its residuals
 
 from __future__ import annotations
 
+import numpy as np
 import torch
 
 from . import _kernels as K
@@ -25,6 +26,24 @@ from . import _lib as L
 from .utils import SegmentKernel, as_segment_kernel
 
 F64 = torch.float64
+_CONSTS: dict = {}
+
+
+def device_const(x, device):
+    """A small FP64 constant (mirror plane, flow direction, ...) on the
+    device, copied there once: a copy from pageable host memory on every use
+    waits for the stream to drain, a stall per call in the reverse passes."""
+    if isinstance(x, torch.Tensor):
+        return x.to(device=device, dtype=F64)
+    a = np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+    key = (a.tobytes(), a.shape, str(torch.device(device)))
+    t = _CONSTS.get(key)
+    if t is None:
+        if len(_CONSTS) > 4096:
+            _CONSTS.clear()
+        t = torch.as_tensor(a, device=device)
+        _CONSTS[key] = t
+    return t
 
 
 class PairSum(torch.autograd.Function):
@@ -79,8 +98,8 @@ def grid_filaments(zeta, gamma):
 
 
 def mirror_grid(zeta, mirror):
-    p = torch.as_tensor(mirror[0], dtype=F64, device=zeta.device)
-    nrm = torch.as_tensor(mirror[1], dtype=F64, device=zeta.device)
+    p = device_const(mirror[0], zeta.device)
+    nrm = device_const(mirror[1], zeta.device)
     return zeta - 2.0 * ((zeta - p) * nrm).sum(-1, keepdim=True) * nrm
 
 
@@ -141,10 +160,19 @@ def _aic_action(c, nc, zetas_b, zetas_w, gamma_vec, shapes, kernel, mirror):
 
 class AICSolve(torch.autograd.Function):
     """Gamma = A^{-1} b where A = AIC(c, nc, zeta_b[, zeta_w]) is assembled on
-    the device; differentiable in b and in the geometry."""
+    the device; differentiable in b and in the geometry.
+
+    With a ``solver`` (a ``linalg.RefinedSolver``), the solve is the
+    production one: the solver's kept inverse M of an earlier matrix and FP64
+    refinement against the current A (refactoring when refinement fails), and
+    the reverse pass solves A^T lambda = gbar the same way with M^T, instead
+    of a fresh LU and two single-workgroup triangular sweeps per call.  A
+    transposed solve whose refinement never passes its stop test is counted
+    in ``solver.adjoint_failures`` (a device counter, read by the caller
+    after the reverse pass)."""
 
     @staticmethod
-    def forward(ctx, b, c, nc, kernel, mirror, shapes, has_wake, *grids):
+    def forward(ctx, b, c, nc, kernel, mirror, shapes, has_wake, solver, *grids):
         from .aic import compute_aic_solve
 
         nb = len(shapes)
@@ -162,20 +190,33 @@ class AICSolve(torch.autograd.Function):
             mp, mn = mirror
         A = compute_aic_solve(cs, ns, zb, zw, [kern] * nb, [kern] * nb if zw is not None else None,
                               mirror_point=mp, mirror_normal=mn)
+        ctx.meta = (kern, mirror, shapes, has_wake, nb)
+        ctx.solver = solver
+        if solver is not None:
+            x = solver.solve(A, b.detach().contiguous())
+            ctx.save_for_backward(A, solver.inverse, x, c, nc, *grids)
+            return x
         fac = K.lu_factor(A, overwrite=True)
         x = K.lu_solve(fac, b.detach().contiguous())
         ctx.save_for_backward(fac.lu, fac.piv, fac.aux, x, c, nc, *grids)
-        ctx.meta = (kern, mirror, shapes, has_wake, nb)
         return x
 
     @staticmethod
     def backward(ctx, gx):
-        LU, piv, aux, x, c, nc, *grids = ctx.saved_tensors
         kern, mirror, shapes, has_wake, nb = ctx.meta
-        lam = K.lu_solve((LU, piv, aux), gx.contiguous().to(F64), trans=True)
+        gx = gx.contiguous().to(F64)
+        if ctx.solver is not None:
+            A, M, x, c, nc, *grids = ctx.saved_tensors
+            lam, it = K.solve_refined_inv(A.t().contiguous(), M.t().contiguous(), gx, max_iter=8)
+            sv = ctx.solver
+            fails = (it < 0).to(torch.int32)
+            sv.adjoint_failures = fails if getattr(sv, "adjoint_failures", None) is None else sv.adjoint_failures + fails
+        else:
+            LU, piv, aux, x, c, nc, *grids = ctx.saved_tensors
+            lam = K.lu_solve((LU, piv, aux), gx, trans=True)
         need = ctx.needs_input_grad
         geo_inputs = [c, nc] + list(grids)
-        geo_need = [need[1], need[2]] + list(need[7:])
+        geo_need = [need[1], need[2]] + list(need[8:])
         geo_grads = [None] * len(geo_inputs)
         if any(geo_need):
             with torch.enable_grad():
@@ -187,11 +228,13 @@ class AICSolve(torch.autograd.Function):
                 gr = torch.autograd.grad(-s, req, allow_unused=True)
             it = iter(gr)
             geo_grads = [next(it) if nd else None for nd in geo_need]
-        return (lam, geo_grads[0], geo_grads[1], None, None, None, None, *geo_grads[2:])
+        return (lam, geo_grads[0], geo_grads[1], None, None, None, None, None, *geo_grads[2:])
 
 
-def aic_solve(b, c, nc, zetas_b, zetas_w=None, kernel=None, mirror=None):
-    """Differentiable circulation solve; c, nc packed (N_b, 3)."""
+def aic_solve(b, c, nc, zetas_b, zetas_w=None, kernel=None, mirror=None, solver=None):
+    """Differentiable circulation solve; c, nc packed (N_b, 3).  ``solver``:
+    a kept-factor ``linalg.RefinedSolver`` (see AICSolve), or None for an LU
+    per call."""
     shapes = tuple((z.shape[0] - 1, z.shape[1] - 1) for z in zetas_b)
     grids = list(zetas_b) + (list(zetas_w) if zetas_w is not None else [])
-    return AICSolve.apply(b, c, nc, kernel, mirror, shapes, zetas_w is not None, *grids)
+    return AICSolve.apply(b, c, nc, kernel, mirror, shapes, zetas_w is not None, solver, *grids)

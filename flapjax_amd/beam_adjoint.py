@@ -267,6 +267,11 @@ class BeamTables:
         idx = torch.cat((6 * self.conn[:, :1] + torch.arange(6, device=dev), 6 * self.conn[:, 1:] + torch.arange(6, device=dev)),
                         1)
         self.elem_dofs = idx  # (n_elem, 12)
+        # quadrature constants on the device once (a per-call host-to-device
+        # copy would also stop the backward passes from being captured)
+        self.s_quad = torch.tensor((LOBATTO_MID,) + tuple(LEGENDRE_S), dtype=F64, device=dev)[:, None, None]
+        self.w_quad = torch.tensor(LEGENDRE_W, dtype=F64, device=dev)[:, None, None, None]
+        self.eye6 = torch.eye(6, dtype=F64, device=dev)
 
     @classmethod
     def from_beam(cls, beam):
@@ -319,8 +324,7 @@ def beam_residual(tb: BeamTables, hg, k_scale=None, f_follower=None, f_dead=None
         f1 = torch.cat((torch.cat((z, z), -1), torch.cat((z, mad), -1)), -2)
         # Q at the Lobatto midpoint and the three Legendre points in one batch,
         # with its derivative along d_dot (the reference's q_dot JVP)
-        s_all = torch.tensor((LOBATTO_MID,) + tuple(LEGENDRE_S), dtype=F64, device=d.device)[:, None, None]
-        eye = torch.eye(6, dtype=F64, device=d.device)
+        s_all, eye = tb.s_quad, tb.eye6
 
         def qs(dd):
             ts = s_all[..., None] * (t_se3(s_all * dd[None]) @ t_inv_se3(dd)[None])
@@ -333,8 +337,7 @@ def beam_residual(tb: BeamTables, hg, k_scale=None, f_follower=None, f_dead=None
         ql, qdl = q[1:], qd[1:]
         u = (ql @ ve[None, ..., None])[..., 0]
         terms = ql.transpose(-1, -2) @ (me @ qdl - hat6(u).transpose(-1, -2) @ me @ ql)
-        w = torch.tensor(LEGENDRE_W, dtype=F64, device=d.device)[:, None, None, None]
-        c_l = 0.5 * l[:, None, None] * (w * terms).sum(0)
+        c_l = 0.5 * l[:, None, None] * (tb.w_quad * terms).sum(0)
         fe = fe - (m_t @ vde[..., None])[..., 0] - (c_l @ ve[..., None])[..., 0]
     f = tb.assemble(fe)
     if f_follower is not None:
@@ -356,9 +359,47 @@ def lie_gradient(hg, g_hg):
     return g
 
 
+class GraphedCall:
+    """fn(*inputs) replayed from one captured HIP graph.  The backward passes
+    of the implicit solves are thousands of small element-wise kernels on
+    tensors of fixed shapes, launch-bound when issued one by one from the
+    host; captured once per solver, a call is one copy of the inputs into the
+    graph's buffers, one replay and a copy of the outputs.  fn must issue the
+    same kernels whatever the values (no host reads of device data): the
+    backward passes below qualify (their jacobian colouring and band-solve
+    choice depend on shapes only and are settled in the eager warm-up)."""
+
+    def __init__(self, fn, inputs):
+        self.fn = fn
+        self.sig = self.signature(inputs)
+        self.static_in = [x.detach().clone() for x in inputs]
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            for _ in range(2):
+                fn(*self.static_in)
+        torch.cuda.current_stream().wait_stream(side)
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self.static_out = fn(*self.static_in)
+
+    @staticmethod
+    def signature(inputs):
+        return tuple((tuple(x.shape), x.dtype, x.device) for x in inputs)
+
+    def __call__(self, *inputs):
+        for buf, x in zip(self.static_in, inputs):
+            buf.copy_(x)
+        self.graph.replay()
+        return tuple(None if o is None else o.clone() for o in self.static_out)
+
+
 class BeamSolver:
     """What the implicit Functions need from a device beam: the element tables,
-    the solve DOFs and the two device Newton solves."""
+    the solve DOFs and the two device Newton solves.  ``use_graphs``: replay
+    the implicit backward passes from captured graphs (GPU tensors only)."""
+
+    use_graphs = True
 
     def __init__(self, beam, prescribed_dofs=(), time_integrator=None):
         self.beam = beam
@@ -372,6 +413,17 @@ class BeamSolver:
         dev = beam.hg0.device
         self._bufs = (torch.empty((n, 4, 4), dtype=F64, device=dev), torch.empty((n, 6, 6), dtype=F64, device=dev))
         self.iters = []
+        self._graphs = {}
+
+    def run_backward(self, key, fn, inputs):
+        """fn(*inputs), from a captured graph where the tensors are on the GPU."""
+        if not (self.use_graphs and inputs[0].is_cuda):
+            return fn(*inputs)
+        gc = self._graphs.get(key)
+        if gc is None or gc.sig != GraphedCall.signature(inputs):
+            gc = GraphedCall(fn, inputs)
+            self._graphs[key] = gc
+        return gc(*inputs)
 
     def static(self, f_dead, k_scale):
         self.beam.set_stiffness_scaling(k_scale)
@@ -427,10 +479,10 @@ class BeamSolver:
             seed = colour[node] * 6 + comp
             n_col = int(seed.max()) + 1 if len(sd) else 0
             dev = x.device
-            self._colour = (seed.to(dev), n_col, near[node][:, node].to(dev))
-        seed, n_col, mask = self._colour
-        v = torch.zeros((self.n_s, n_col), dtype=F64, device=x.device)
-        v[torch.arange(self.n_s, device=x.device), seed] = 1.0
+            v = torch.zeros((self.n_s, n_col), dtype=F64)
+            v[torch.arange(self.n_s), seed] = 1.0
+            self._colour = (seed.to(dev), v.to(dev), near[node][:, node].to(dev))
+        seed, v, mask = self._colour
         jv = torch.func.vmap(lambda t: torch.func.jvp(res, (x,), (t,))[1], in_dims=1, out_dims=1)(v)
         return torch.where(mask, jv[:, seed], torch.zeros((), dtype=F64, device=x.device))
 
@@ -462,9 +514,15 @@ class StaticBeamSolve(torch.autograd.Function):
     def backward(ctx, g_hg):
         hg, f_dead, k_scale = ctx.saved_tensors
         sv = ctx.solver
+        inputs = (hg.detach(), f_dead.detach(), k_scale.detach(), g_hg.contiguous())
+        gf, gk = sv.run_backward("static", lambda *x: StaticBeamSolve.pullback(sv, *x), inputs)
+        return gf, gk, None
+
+    @staticmethod
+    def pullback(sv, hg, fd, ks, g_hg):
+        """(dL/df_dead, dL/dk_scale) from dL/dhg at the solution hg."""
         tb = sv.tables
         g = sv.reduce(lie_gradient(hg, g_hg))
-        fd, ks = f_dead.detach(), k_scale.detach()
 
         def res(xs):
             xi = sv.expand(xs, fd)
@@ -476,7 +534,7 @@ class StaticBeamSolve(torch.autograd.Function):
             ks_ = ks.clone().requires_grad_(True)
             r = sv.reduce(beam_residual(tb, hg, ks_, None, fd_))
             gf, gk = torch.autograd.grad(r, (fd_, ks_), -mu)
-        return gf, gk, None
+        return gf, gk
 
 
 class DynamicBeamSolve(torch.autograd.Function):
@@ -514,7 +572,14 @@ class DynamicBeamSolve(torch.autograd.Function):
     def backward(ctx, g_phi_s):
         phi_s, *inputs = ctx.saved_tensors
         sv = ctx.solver
-        inputs = [x.detach() for x in inputs]
+        args = (phi_s.detach(), *[x.detach() for x in inputs], g_phi_s.contiguous())
+        grads = sv.run_backward("dynamic", lambda *x: DynamicBeamSolve.pullback(sv, *x), args)
+        return (*grads, None)
+
+    @staticmethod
+    def pullback(sv, phi_s, hg_prev, phi0, v0, vd0, f_dead, k_scale, g_phi_s):
+        """dL/d(hg_prev, phi0, v0, vd0, f_dead, k_scale) from dL/dphi_s."""
+        inputs = (hg_prev, phi0, v0, vd0, f_dead, k_scale)
 
         def res(xs):
             return DynamicBeamSolve.residual(sv, phi_s + xs, *inputs)
@@ -525,4 +590,4 @@ class DynamicBeamSolve(torch.autograd.Function):
             leaves = [x.clone().requires_grad_(True) for x in inputs]
             r = DynamicBeamSolve.residual(sv, phi_s, *leaves)
             grads = torch.autograd.grad(r, leaves, -mu, allow_unused=True)
-        return (*grads, None)
+        return grads

@@ -48,6 +48,7 @@ from . import beam_adjoint as BA
 from .adjoint import colloc, normals, split_to_vertex, twisted_grid
 from .beam import ConvergenceStatus, TimeIntegrator
 from .flowfields import OneMinusCosine
+from .linalg import RefinedSolver
 
 F64 = torch.float64
 
@@ -60,17 +61,17 @@ def flow_at(ff, x, t):
     """The flowfield velocity at points x (..., 3) and time t as a torch
     expression (aero/flowfields.py:88-181)."""
     dev = x.device
-    u = torch.as_tensor(ff.u_inf, dtype=F64, device=dev)
+    u = AG.device_const(ff.u_inf, dev)
     base = u.expand_as(x) if ff.relative_motion else torch.zeros_like(x)
     if isinstance(ff, OneMinusCosine):
-        rel = x - torch.as_tensor(ff.gust_x0, dtype=F64, device=dev)
+        rel = x - AG.device_const(ff.gust_x0, dev)
         if ff.relative_motion:
             rel = rel - u * float(t)
-        s = (rel * torch.as_tensor(ff.gust_travel_direction, dtype=F64, device=dev)).sum(-1)
+        s = (rel * AG.device_const(ff.gust_travel_direction, dev)).sum(-1)
         amp = ff.gust_amplitude * 0.5 * (1.0 - torch.cos(np.pi * s / ff.gust_length))
         inside = (s > 0.0) & (s < 2.0 * ff.gust_length)
         amp = torch.where(inside, amp, torch.zeros_like(amp))
-        base = base + amp[..., None] * torch.as_tensor(ff.gust_amplitude_direction, dtype=F64, device=dev)
+        base = base + amp[..., None] * AG.device_const(ff.gust_amplitude_direction, dev)
     elif type(ff).__name__ != "Constant":
         raise NotImplementedError(f"{type(ff).__name__} has no differentiable form")
     return base
@@ -79,7 +80,12 @@ def flow_at(ff, x, t):
 class DifferentiableAeroelastic:
     """A ``CoupledAeroelastic`` case with one surface, rerun differentiably."""
 
-    def __init__(self, coupled, prescribed_dofs, checkpoint_every: int = 10):
+    def __init__(self, coupled, prescribed_dofs, checkpoint_every: int = 10, aic_reuse: bool | None = None):
+        """checkpoint_every: steps per recomputed chunk of the time history
+        (0: keep the whole history, no recomputation; the history of a few
+        hundred steps of a 16 x 80 case is a few tens of GB).  aic_reuse: solve
+        the circulation systems with kept inverses and FP64 refinement, as
+        the production solve does (None: as the case's UVLM is set)."""
         aero, beam = coupled.aero, coupled.structure
         if aero.n_surf != 1:
             raise NotImplementedError("the coupled adjoint takes a single lifting surface")
@@ -105,6 +111,8 @@ class DifferentiableAeroelastic:
         self.ti = TimeIntegrator(beam.spectral_radius, self.dt)
         self.solver = BA.BeamSolver(beam, prescribed_dofs, self.ti)
         self.k = int(checkpoint_every)
+        reuse = aero.aic_reuse if aic_reuse is None else bool(aic_reuse)
+        self.aic_solvers = {True: RefinedSolver(), False: RefinedSolver()} if reuse else {True: None, False: None}
         free = torch.ones(beam.n_dof, dtype=torch.bool, device=beam.hg0.device)
         if self.solver.prescribed:
             free[torch.as_tensor(self.solver.prescribed, device=free.device)] = False
@@ -152,7 +160,8 @@ class DifferentiableAeroelastic:
         zw = zb[-1][None] + self.wake_s[:, None, :]
         c, nc = colloc(zb), normals(zb)
         b = -(flow_at(self.ff, c, t) * nc).sum(-1).reshape(-1)
-        gb = AG.aic_solve(b, c.reshape(-1, 3), nc.reshape(-1, 3), [zb], [zw], self.kernel, self.mirror).reshape(m, n)
+        gb = AG.aic_solve(b, c.reshape(-1, 3), nc.reshape(-1, 3), [zb], [zw], self.kernel, self.mirror,
+                          self.aic_solvers[True]).reshape(m, n)
         gw = gb[-1:].expand(ms, n)
         return (zw, gw, gb, torch.zeros_like(gb)), self.forces(zb, None, gb, gw, zw, t)
 
@@ -173,7 +182,7 @@ class DifferentiableAeroelastic:
                                                         self.mirror).reshape(c.shape)
         b = -(vbc * nc).sum(-1).reshape(-1)
         gb_n = AG.aic_solve(b, c.reshape(-1, 3), nc.reshape(-1, 3), [zb_n], None, self.kernel,
-                            self.mirror).reshape(m, n)
+                            self.mirror, self.aic_solvers[False]).reshape(m, n)
         gbd_n = (gb_n - gb) / self.dt * self.relax + gbd * (1.0 - self.relax)
         f_st = self.forces(zb_n, zbd_n, gb_n, gw_n, zw_n, t_n)
         f_un = split_to_vertex(split_to_vertex(self.rho * gbd_n[..., None] * nc, 0), 1)
@@ -292,6 +301,8 @@ class DifferentiableAeroelastic:
                 total = total + self.dt * (lift + tip_weight * (hg_[-1, 2, 3] - z_tip0) ** 2)
             return (hg_, v_, vd_, acc_, fg_, *a_, total)
 
+        if self.k <= 0:
+            return chunk(1, n_tstep, k_scale, z0, *state)[-1]
         J = torch.zeros((), dtype=F64, device=hg.device)
         for i0 in range(1, n_tstep, self.k):
             i1 = min(n_tstep, i0 + self.k)
@@ -304,9 +315,16 @@ class DifferentiableAeroelastic:
         ks = torch.as_tensor(k_scale, dtype=F64, device=self.beam.hg0.device).detach().clone().requires_grad_(True)
         tw = torch.as_tensor(twist, dtype=F64, device=self.beam.hg0.device).detach().clone().requires_grad_(True)
         before = getattr(self.beam, "stiffness_scaling", None)
+        for sv in self.aic_solvers.values():
+            if sv is not None:
+                sv.adjoint_failures = None
         try:
             J = self.objective(ks, tw, n_tstep, tip_weight)
             g_k, g_t = torch.autograd.grad(J, (ks, tw))
+            fails = [sv.adjoint_failures for sv in self.aic_solvers.values()
+                     if sv is not None and sv.adjoint_failures is not None]
+            if fails and int(sum(f.sum() for f in fails)) > 0:
+                raise RuntimeError("a transposed circulation solve did not pass its refinement stop test")
         finally:
             # the solves rescale the beam's device tables; leave them as found
             self.beam.set_stiffness_scaling(np.ones(self.beam.k_cs.shape[0]) if before is None else before)
