@@ -128,6 +128,7 @@ _SIGS = {
     "fj_beam_workspace_bytes": ([c_int], c_size_t),
     "fj_beam_system": ([c_void_p, c_int] + [c_void_p] * 6 + [c_double, c_double, c_void_p, c_int] + [c_void_p] * 5,
                        c_int),
+    "fj_beam_element_jacobians": ([c_void_p, c_int] + [c_void_p] * 10 + [c_void_p], c_int),
     "fj_beam_update": ([c_int, c_int, c_void_p, c_int, c_void_p, c_double, c_double, c_double] + [c_void_p] * 7,
                        c_int),
     "fj_beam_subnorm": ([c_void_p, c_void_p, c_int, c_void_p, c_void_p], c_int),

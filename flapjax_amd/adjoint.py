@@ -5,7 +5,7 @@ prescribed wake, bound-only dynamic AIC with the wake upwash on the right-hand
 side, Kutta-Joukowski forces, filtered circulation rate and unsteady force)
 written with differentiable pieces only: the pair sums are
 ``autograd.PairSum`` (hand-written adjoint Biot-Savart kernels), the
-circulation solve is ``autograd.AICSolve`` (transposed LU solve + adjoint pair
+circulation solve is ``autograd.aic_solve`` (transposed LU solve + adjoint pair
 sums for the geometry), and the O(panels) stencils are torch expressions.
 The time history is checkpointed every ``checkpoint_every`` steps and the
 steps in between are recomputed in the reverse pass

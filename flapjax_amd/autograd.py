@@ -6,11 +6,12 @@
 * ``LinearSolve`` - x = A^{-1} b by the device LU; backward solves the
                     transposed system with the same factors (fj_lu_solve,
                     trans=1): dL/db = A^{-T} xbar, dL/dA = -(A^{-T} xbar) x^T.
-* ``AICSolve``    - Gamma = A(geometry)^{-1} b without ever differentiating
+* ``aic_solve``   - Gamma = A(geometry)^{-1} b without ever differentiating
                     the assembled matrix: in reverse mode the geometry
                     gradient of -lambda^T A Gamma is the adjoint of the pair
                     sum v_ind(c; zeta, Gamma) . n weighted by lambda, so it
-                    reuses PairSum's adjoint kernels.
+                    reuses PairSum's adjoint kernels (``AdjointSolve``
+                    supplies lambda = A^-T xbar).
 
 These replace what the reference obtains from jax.vjp of its residuals
 (aero/uvlm.py:1529-2047, coupled/gradients/coupled.py:540-629).
@@ -158,28 +159,71 @@ def _aic_action(c, nc, zetas_b, zetas_w, gamma_vec, shapes, kernel, mirror):
     return (v * nc).sum(-1)
 
 
-class AICSolve(torch.autograd.Function):
-    """Gamma = A^{-1} b where A = AIC(c, nc, zeta_b[, zeta_w]) is assembled on
-    the device; differentiable in b and in the geometry.
-
-    With a ``solver`` (a ``linalg.RefinedSolver``), the solve is the
-    production one: the solver's kept inverse M of an earlier matrix and FP64
-    refinement against the current A (refactoring when refinement fails), and
-    the reverse pass solves A^T lambda = gbar the same way with M^T, instead
-    of a fresh LU and two single-workgroup triangular sweeps per call.  A
-    transposed solve whose refinement never passes its stop test is counted
-    in ``solver.adjoint_failures`` (a device counter, read by the caller
+class _KeptInverseSolve:
+    """A^-T g by FP64 refinement around M^T, M the kept inverse the forward
+    solve used; a refinement that never passes its stop test is counted in
+    ``solver.adjoint_failures`` (a device counter the caller reads once
     after the reverse pass)."""
 
-    @staticmethod
-    def forward(ctx, b, c, nc, kernel, mirror, shapes, has_wake, solver, *grids):
-        from .aic import compute_aic_solve
+    def __init__(self, A, M, solver):
+        self.A, self.M, self.solver = A, M, solver
 
-        nb = len(shapes)
-        zb = [z.detach() for z in grids[:nb]]
-        zw = [z.detach() for z in grids[nb:]] if has_wake else None
-        kern = as_segment_kernel(kernel)
-        # one collocation block per surface (packed c / nc split by panel count)
+    def transposed(self, g):
+        lam, it = K.solve_refined_inv(self.A.t().contiguous(), self.M.t().contiguous(), g, max_iter=8)
+        sv = self.solver
+        fails = (it < 0).to(torch.int32)
+        prev = getattr(sv, "adjoint_failures", None)
+        sv.adjoint_failures = fails if prev is None else prev + fails
+        return lam
+
+
+class _LUSolve:
+    """A^-T g with the LU factor of A (fj_lu_solve, trans = 1)."""
+
+    def __init__(self, fac):
+        self.fac = fac
+
+    def transposed(self, g):
+        return K.lu_solve(self.fac, g, trans=True)
+
+
+class AdjointSolve(torch.autograd.Function):
+    """Zero in the forward pass; the backward returns A^-T of its incoming
+    gradient (``handle.transposed``).
+    ``aic_solve`` adds it, with its value cancelled, to carry the implicit
+    derivative of the circulation solve through the caller's own graph."""
+
+    @staticmethod
+    def forward(ctx, r, handle):
+        ctx.handle = handle
+        return torch.zeros_like(r)
+
+    @staticmethod
+    def backward(ctx, g):
+        return ctx.handle.transposed(g.contiguous().to(F64)), None
+
+
+def aic_solve(b, c, nc, zetas_b, zetas_w=None, kernel=None, mirror=None, solver=None):
+    """Differentiable circulation solve Gamma = A^-1 b, A = AIC(c, nc,
+    zeta_b[, zeta_w]) assembled on the device (c, nc packed (N_b, 3));
+    differentiable in b and in the geometry without ever differentiating the
+    assembled matrix.
+
+    The value is the device solve x (``solver``: a ``linalg.RefinedSolver``,
+    the production form: its kept inverse of an earlier matrix and FP64
+    refinement against the current A, refactoring when refinement fails; None:
+    an LU of A).  The derivative is the implicit one, dx = -A^-1 (dA x - db):
+    the returned tensor is x + (s - s.detach()) with s = -AdjointSolve(r),
+    r = A(geometry) x - b written with differentiable pair sums (so the
+    reverse pass gets lambda = A^-T xbar from AdjointSolve, then dL/db =
+    lambda and dL/dgeometry = -lambda^T d(A x)/dgeometry from the pair sums'
+    adjoint kernels), all inside the caller's graph."""
+    from .aic import compute_aic_solve
+
+    shapes = tuple((z.shape[0] - 1, z.shape[1] - 1) for z in zetas_b)
+    nb = len(shapes)
+    kern = as_segment_kernel(kernel)
+    with torch.no_grad():
         cs, ns, off = [], [], 0
         for (m, n) in shapes:
             cs.append(c.detach()[off:off + m * n].reshape(m, n, 3))
@@ -188,53 +232,21 @@ class AICSolve(torch.autograd.Function):
         mp = mn = None
         if mirror is not None:
             mp, mn = mirror
+        zb = [z.detach() for z in zetas_b]
+        zw = [z.detach() for z in zetas_w] if zetas_w is not None else None
         A = compute_aic_solve(cs, ns, zb, zw, [kern] * nb, [kern] * nb if zw is not None else None,
                               mirror_point=mp, mirror_normal=mn)
-        ctx.meta = (kern, mirror, shapes, has_wake, nb)
-        ctx.solver = solver
         if solver is not None:
             x = solver.solve(A, b.detach().contiguous())
-            ctx.save_for_backward(A, solver.inverse, x, c, nc, *grids)
-            return x
-        fac = K.lu_factor(A, overwrite=True)
-        x = K.lu_solve(fac, b.detach().contiguous())
-        ctx.save_for_backward(fac.lu, fac.piv, fac.aux, x, c, nc, *grids)
-        return x
-
-    @staticmethod
-    def backward(ctx, gx):
-        kern, mirror, shapes, has_wake, nb = ctx.meta
-        gx = gx.contiguous().to(F64)
-        if ctx.solver is not None:
-            A, M, x, c, nc, *grids = ctx.saved_tensors
-            lam, it = K.solve_refined_inv(A.t().contiguous(), M.t().contiguous(), gx, max_iter=8)
-            sv = ctx.solver
-            fails = (it < 0).to(torch.int32)
-            sv.adjoint_failures = fails if getattr(sv, "adjoint_failures", None) is None else sv.adjoint_failures + fails
+            handle = _KeptInverseSolve(A, solver.inverse, solver)
         else:
-            LU, piv, aux, x, c, nc, *grids = ctx.saved_tensors
-            lam = K.lu_solve((LU, piv, aux), gx, trans=True)
-        need = ctx.needs_input_grad
-        geo_inputs = [c, nc] + list(grids)
-        geo_need = [need[1], need[2]] + list(need[8:])
-        geo_grads = [None] * len(geo_inputs)
-        if any(geo_need):
-            with torch.enable_grad():
-                leaves = [t.detach().requires_grad_(True) for t in geo_inputs]
-                zb = leaves[2:2 + nb]
-                zw = leaves[2 + nb:] if has_wake else None
-                s = (lam * _aic_action(leaves[0], leaves[1], zb, zw, x, shapes, kern, mirror)).sum()
-                req = [lf for lf, nd in zip(leaves, geo_need) if nd]
-                gr = torch.autograd.grad(-s, req, allow_unused=True)
-            it = iter(gr)
-            geo_grads = [next(it) if nd else None for nd in geo_need]
-        return (lam, geo_grads[0], geo_grads[1], None, None, None, None, None, *geo_grads[2:])
-
-
-def aic_solve(b, c, nc, zetas_b, zetas_w=None, kernel=None, mirror=None, solver=None):
-    """Differentiable circulation solve; c, nc packed (N_b, 3).  ``solver``:
-    a kept-factor ``linalg.RefinedSolver`` (see AICSolve), or None for an LU
-    per call."""
-    shapes = tuple((z.shape[0] - 1, z.shape[1] - 1) for z in zetas_b)
-    grids = list(zetas_b) + (list(zetas_w) if zetas_w is not None else [])
-    return AICSolve.apply(b, c, nc, kernel, mirror, shapes, zetas_w is not None, solver, *grids)
+            fac = K.lu_factor(A, overwrite=True)
+            x = K.lu_solve(fac, b.detach().contiguous())
+            handle = _LUSolve(fac)
+    inputs = [b, c, nc] + list(zetas_b) + (list(zetas_w) if zetas_w is not None else [])
+    if not torch.is_grad_enabled() or not any(t.requires_grad for t in inputs):
+        return x
+    r = _aic_action(c, nc, list(zetas_b), list(zetas_w) if zetas_w is not None else None, x, shapes, kern,
+                    mirror) - b
+    s = -AdjointSolve.apply(r, handle)
+    return x + (s - s.detach())

@@ -359,6 +359,53 @@ def lie_gradient(hg, g_hg):
     return g
 
 
+def element_jacobians(beam, hg, k_elem, v=None, v_dot=None):
+    """Exact derivative blocks of the element forces at the configuration hg
+    (device kernel fj_beam_element_jacobians): (fe (E, 12), df/dd (E, 12, 6),
+    df/dv_e (E, 12, 12) or None, M_t (E, 12, 12) or None, P(d) (E, 6, 12),
+    eps (E, 6)); v, v_dot (n_nodes, 6) select the dynamic form."""
+    import ctypes
+
+    from . import _lib as L
+
+    n_e = beam.n_elem
+    dyn = v is not None
+    hg = hg.contiguous()
+    k_elem = k_elem.contiguous()
+    vv = v.contiguous() if dyn else None
+    vd = v_dot.contiguous() if dyn else None
+
+    def e(*sh):
+        return torch.empty(sh, dtype=F64, device=hg.device)
+
+    fe, jd, p, eps = e(n_e, 12), e(n_e, 12, 6), e(n_e, 6, 12), e(n_e, 6)
+    jv, mt = (e(n_e, 12, 12), e(n_e, 12, 12)) if dyn else (None, None)
+    rc = L.lib().fj_beam_element_jacobians(ctypes.c_void_p(ctypes.addressof(beam._model)), int(dyn), L.ptr(hg),
+                                           L.ptr(vv), L.ptr(vd), L.ptr(k_elem), L.ptr(fe), L.ptr(jd), L.ptr(jv),
+                                           L.ptr(mt), L.ptr(p), L.ptr(eps), L.stream())
+    L.check(rc, "fj_beam_element_jacobians")
+    return fe, jd, jv, mt, p, eps
+
+
+def _node_blocks_add(mat, blocks):
+    """mat (6N x 6N) plus blockdiag(blocks (N, 6, 6))."""
+    n = blocks.shape[0]
+    mv = mat.view(n, 6, n, 6)
+    ar = torch.arange(n, device=mat.device)
+    mv[ar, :, ar, :] = mv[ar, :, ar, :] + blocks
+    return mat
+
+
+def _dead_stiffness(hg, f_dead):
+    """d(R^T f)/d(right increment) per node: rows u and w, columns w:
+    skew(R^T f_u), skew(R^T f_w) (R^T f changes by (R^T f) x w)."""
+    rt = hg[:, :3, :3].transpose(-1, -2)
+    lu = (rt @ f_dead[:, :3, None])[..., 0]
+    lw = (rt @ f_dead[:, 3:, None])[..., 0]
+    z = torch.zeros((hg.shape[0], 3, 3), dtype=F64, device=hg.device)
+    return torch.cat((torch.cat((z, skew(lu)), -1), torch.cat((z, skew(lw)), -1)), -2)
+
+
 class GraphedCall:
     """fn(*inputs) replayed from one captured HIP graph.  The backward passes
     of the implicit solves are thousands of small element-wise kernels on
@@ -400,6 +447,7 @@ class BeamSolver:
     the implicit backward passes from captured graphs (GPU tensors only)."""
 
     use_graphs = True
+    device_pullback = True
 
     def __init__(self, beam, prescribed_dofs=(), time_integrator=None):
         self.beam = beam
@@ -441,6 +489,37 @@ class BeamSolver:
 
     def reduce(self, x):
         return x.reshape(-1)[self.sd_long]
+
+    def solve_selector(self, device):
+        """1 at the solve DOFs, 0 elsewhere (6 n_nodes)."""
+        if getattr(self, "_sel", None) is None:
+            sel = torch.zeros(6 * self.tables.n_nodes, dtype=F64)
+            sel[self.sd_long.cpu()] = 1.0
+            self._sel = sel.to(device)
+        return self._sel
+
+    def assemble_blocks(self, blocks):
+        """Dense 6N x 6N matrix from element blocks (E, 12, 12) over the
+        element DOFs (at most two contributions per entry: exact in any
+        order)."""
+        if getattr(self, "_asm", None) is None:
+            idx = self.tables.elem_dofs
+            n6 = 6 * self.tables.n_nodes
+            self._asm = ((idx[:, :, None] * n6 + idx[:, None, :]).reshape(-1), n6)
+        flat, n6 = self._asm
+        out = torch.zeros(n6 * n6, dtype=F64, device=blocks.device)
+        return out.index_add_(0, flat, blocks.reshape(-1)).view(n6, n6)
+
+    def k_gradient(self, mu_full, p, eps):
+        """sum over the elements of table entry s of mu_e . P^T K_s eps: the
+        stiffness-scaling part of -mu^T dR/dk (the internal force is
+        -P^T (k_s K_s) eps)."""
+        tb = self.tables
+        mu_e = mu_full[tb.elem_dofs]
+        kb = tb.k_cs[tb.k_index]
+        f = (p.transpose(-1, -2) @ (kb @ eps[..., None]))[..., 0]
+        contrib = (mu_e * f).sum(-1)
+        return torch.zeros(tb.k_cs.shape[0], dtype=F64, device=p.device).index_add_(0, tb.k_index, contrib)
 
     def expand(self, xs, like):
         out = torch.zeros(like.numel(), dtype=F64, device=like.device)
@@ -515,8 +594,27 @@ class StaticBeamSolve(torch.autograd.Function):
         hg, f_dead, k_scale = ctx.saved_tensors
         sv = ctx.solver
         inputs = (hg.detach(), f_dead.detach(), k_scale.detach(), g_hg.contiguous())
-        gf, gk = sv.run_backward("static", lambda *x: StaticBeamSolve.pullback(sv, *x), inputs)
+        fn = StaticBeamSolve.pullback_device if (sv.device_pullback and hg.is_cuda) else StaticBeamSolve.pullback
+        gf, gk = sv.run_backward("static", lambda *x: fn(sv, *x), inputs)
         return gf, gk, None
+
+    @staticmethod
+    def pullback_device(sv, hg, fd, ks, g_hg):
+        """pullback from the exact element derivative blocks of the device
+        kernel: J = dR/dxi is the assembled dfe/dd P plus the dead-load
+        blocks (xi = 0 at the solution, so no tangent factor)."""
+        tb = sv.tables
+        g = sv.reduce(lie_gradient(hg, g_hg))
+        k_elem = (tb.k_cs * ks[:, None, None])[tb.k_index]
+        _, jd, _, _, p, eps = element_jacobians(sv.beam, hg, k_elem)
+        jz = _node_blocks_add(sv.assemble_blocks(jd @ p), _dead_stiffness(hg, fd))
+        sd = sv.sd_long
+        mu = sv.solve_transposed(jz[sd][:, sd], g)
+        mu_f = sv.expand(mu, fd).reshape(-1)
+        r = hg[:, :3, :3]
+        mu_n = mu_f.view(-1, 6)
+        gf = -torch.cat(((r @ mu_n[:, :3, None])[..., 0], (r @ mu_n[:, 3:, None])[..., 0]), -1)
+        return gf, sv.k_gradient(mu_f, p, eps)
 
     @staticmethod
     def pullback(sv, hg, fd, ks, g_hg):
@@ -573,8 +671,56 @@ class DynamicBeamSolve(torch.autograd.Function):
         phi_s, *inputs = ctx.saved_tensors
         sv = ctx.solver
         args = (phi_s.detach(), *[x.detach() for x in inputs], g_phi_s.contiguous())
-        grads = sv.run_backward("dynamic", lambda *x: DynamicBeamSolve.pullback(sv, *x), args)
+        fn = DynamicBeamSolve.pullback_device if (sv.device_pullback and phi_s.is_cuda) else DynamicBeamSolve.pullback
+        grads = sv.run_backward("dynamic", lambda *x: fn(sv, *x), args)
         return (*grads, None)
+
+    @staticmethod
+    def pullback_device(sv, phi_s, hg_prev, phi0, v0, vd0, f_dead, k_scale, g_phi_s):
+        """pullback from the exact element derivative blocks of the device
+        kernel.  With hg = hg_prev exp(phi) and dR/dzeta the derivative in
+        right increments of hg (assembled dfe/dd P plus dead-load blocks):
+        J = dR/dzeta T(phi) + gamma' dR/dv - beta' M_t over the solve DOFs;
+        mu = J^-T g; then -mu^T dR/d(inputs) through the linear maps of
+        full_state, Ad_{exp(-phi)} for hg_prev (returned as a matrix gradient
+        exact on every tangent direction hg_prev X, X in se(3), the only
+        directions hg_prev moves in), R mu for the dead loads and P^T K eps
+        for the stiffness scalings."""
+        tb, ti = sv.tables, sv.ti
+        n = tb.n_nodes
+        phi, v, vd = DynamicBeamSolve.full_state(sv, phi_s, phi0, v0, vd0)
+        e_phi = exp_se3(phi)
+        hg = hg_prev @ e_phi
+        k_elem = (tb.k_cs * k_scale[:, None, None])[tb.k_index]
+        _, jd, jv, mt, p, eps = element_jacobians(sv.beam, hg, k_elem, v, vd)
+        jz = _node_blocks_add(sv.assemble_blocks(jd @ p), _dead_stiffness(hg, f_dead))
+        jv_f = sv.assemble_blocks(jv)
+        mt_f = sv.assemble_blocks(mt)
+        t = t_se3(phi)
+        n6 = 6 * n
+        jzt = (jz.view(n6, n, 6).transpose(0, 1).contiguous() @ t).transpose(0, 1).reshape(n6, n6)
+        jac = jzt + ti.gamma_prime * jv_f - ti.beta_prime * mt_f
+        sd = sv.sd_long
+        mu = sv.solve_transposed(jac[sd][:, sd], g_phi_s)
+        mu_f = sv.expand(mu, phi0).reshape(-1)
+        gz = -(jz.transpose(0, 1) @ mu_f)
+        gv = -(jv_f.transpose(0, 1) @ mu_f)
+        gvd = mt_f.transpose(0, 1) @ mu_f
+        gphi = (t.transpose(-1, -2) @ gz.view(n, 6, 1))[..., 0].reshape(-1)
+        sel = sv.solve_selector(phi_s.device)
+        g_phi0 = (1.0 - sel) * gphi - sel * (ti.gamma_prime * gv + ti.beta_prime * gvd)
+        # hg_prev exp(eta) exp(phi) = hg exp(Ad_{exp(-phi)} eta): g_eta = Ad^T g_zeta
+        rp, pp = e_phi[:, :3, :3], e_phi[:, :3, 3]
+        gzn = gz.view(n, 6)
+        eta_u = (rp @ gzn[:, :3, None])[..., 0]
+        pl = (rp.transpose(-1, -2) @ pp[..., None])[..., 0]
+        eta_w = (rp @ (gzn[:, 3:] + torch.linalg.cross(pl, gzn[:, :3], dim=-1))[..., None])[..., 0]
+        h = torch.nn.functional.pad(torch.cat((0.5 * skew(eta_w), eta_u[..., None]), -1), (0, 0, 0, 1))
+        g_hg_prev = hg_inv(hg_prev).transpose(-1, -2) @ h
+        r = hg[:, :3, :3]
+        mu_n = mu_f.view(n, 6)
+        gf = -torch.cat(((r @ mu_n[:, :3, None])[..., 0], (r @ mu_n[:, 3:, None])[..., 0]), -1)
+        return (g_hg_prev, g_phi0.view(n, 6), gv.view(n, 6), gvd.view(n, 6), gf, sv.k_gradient(mu_f, p, eps))
 
     @staticmethod
     def pullback(sv, phi_s, hg_prev, phi0, v0, vd0, f_dead, k_scale, g_phi_s):

@@ -11,7 +11,7 @@ discrete adjoint:
   convection, the right-hand side and the Kutta-Joukowski and unsteady forces
   as torch expressions of the vertex grids; every pair sum is
   ``autograd.PairSum`` (hand-written adjoint Biot-Savart kernels) and every
-  circulation solve ``autograd.AICSolve`` (device AIC and LU, transposed LU
+  circulation solve ``autograd.aic_solve`` (device AIC and LU, transposed LU
   solve in the reverse pass);
 * structure: every Newton solve is ``beam_adjoint.StaticBeamSolve`` /
   ``DynamicBeamSolve`` (device Newton forward, implicit derivative backward);

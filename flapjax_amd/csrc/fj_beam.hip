@@ -530,6 +530,199 @@ __global__ void k_beam_zero(double* __restrict__ x, int n) {
 }
 
 // ---- C-ABI ----------------------------------------------------------------------
+// ---- exact element derivatives for the implicit adjoint -----------------
+// The element force f_e(d, v_e, v_dot_e) of the residual (internal
+// -P^T K eps, inertial -M_t v_dot, gyroscopic -C_L v; structure/beam.py:
+// 1749-1843 with structure/utils.py:_integrate_m_l/_integrate_c_t), written
+// once for any scalar S.  The gyroscopic term needs Q_dot = dQ/dd along
+// d_dot = P(d) v_e (the reference's JVP), taken on FjDualT<S>; with S a dual
+// number this differentiates Q_dot itself.  C_L v is formed from vectors:
+// 0.5 l sum_k w_k Q_k^T (M (Q_dot_k v) - hat(u_k)^T M u_k), u_k = Q_k v.
+template <typename S>
+__device__ void fj_elem_force(const FjBeamDev& bm, int e, const S* d, const S* ve, const double* vde,
+                              const double* kmat, int dyn, S fe[12]) {
+  const double* o = bm.o0 + 9 * e;
+  const double l = bm.l0[e];
+  S p[6][12], ke[6];
+  fj_p_mat(d, o, p);
+  {
+    S eps[6];
+    for (int i = 0; i < 6; ++i) eps[i] = (d[i] - bm.d0[6 * e + i]) / l;
+    for (int i = 0; i < 6; ++i) {
+      S acc(0.0);
+      for (int j = 0; j < 6; ++j) acc = acc + kmat[6 * i + j] * eps[j];
+      ke[i] = acc;
+    }
+  }
+  for (int c = 0; c < 12; ++c) {
+    S acc(0.0);
+    for (int i = 0; i < 6; ++i) acc = acc + p[i][c] * ke[i];
+    fe[c] = -acc;
+  }
+  if (!dyn) return;
+  const double* m = bm.m_cs + 36 * e;
+  // inertial: M_t v_dot = l (1/6 blockdiag(A0, A0) v_dot + 2/3 Qm^T M Qm v_dot),
+  // A0 = Ad^T M Ad with Ad = blockdiag(O^T, O^T): (Ad x)[3B + p] = sum_j O[j][p] x[3B + j]
+  {
+    double a0v[12];
+    for (int blk = 0; blk < 2; ++blk) {
+      double x[6], y[6];
+      for (int bb = 0; bb < 2; ++bb)
+        for (int pp = 0; pp < 3; ++pp)
+          x[3 * bb + pp] = o[pp] * vde[6 * blk + 3 * bb] + o[3 + pp] * vde[6 * blk + 3 * bb + 1] +
+                           o[6 + pp] * vde[6 * blk + 3 * bb + 2];
+      for (int i = 0; i < 6; ++i) {
+        double acc = 0.0;
+        for (int j = 0; j < 6; ++j) acc += m[6 * i + j] * x[j];
+        y[i] = acc;
+      }
+      for (int bb = 0; bb < 2; ++bb)
+        for (int q = 0; q < 3; ++q)
+          a0v[6 * blk + 3 * bb + q] = o[3 * q] * y[3 * bb] + o[3 * q + 1] * y[3 * bb + 1] + o[3 * q + 2] * y[3 * bb + 2];
+    }
+    S qm[6][12], z[6], mz[6];
+    fj_q_mat<S>(0.5, d, o, qm);
+    for (int i = 0; i < 6; ++i) {
+      S acc(0.0);
+      for (int c = 0; c < 12; ++c) acc = acc + qm[i][c] * vde[c];
+      z[i] = acc;
+    }
+    for (int i = 0; i < 6; ++i) {
+      S acc(0.0);
+      for (int j = 0; j < 6; ++j) acc = acc + m[6 * i + j] * z[j];
+      mz[i] = acc;
+    }
+    for (int c = 0; c < 12; ++c) {
+      S acc(0.0);
+      for (int i = 0; i < 6; ++i) acc = acc + qm[i][c] * mz[i];
+      fe[c] = fe[c] - l * (a0v[c] / 6.0 + (2.0 / 3.0) * acc);
+    }
+  }
+  // gyroscopic
+  S dd[6];
+  for (int i = 0; i < 6; ++i) {
+    S acc(0.0);
+    for (int c = 0; c < 12; ++c) acc = acc + p[i][c] * ve[c];
+    dd[i] = acc;
+  }
+  for (int k = 0; k < 3; ++k) {
+    FjDualT<S> dk[6], q2[6][12];
+    for (int i = 0; i < 6; ++i) dk[i] = FjDualT<S>(d[i], dd[i]);
+    fj_q_mat<FjDualT<S>>(fj_leg_s[k], dk, o, q2);
+    S u[6], a[6], mu[6], b[6], hh[6][6];
+    for (int i = 0; i < 6; ++i) {
+      S su(0.0), sa(0.0);
+      for (int c = 0; c < 12; ++c) {
+        su = su + q2[i][c].v * ve[c];
+        sa = sa + q2[i][c].d * ve[c];
+      }
+      u[i] = su;
+      a[i] = sa;
+    }
+    for (int i = 0; i < 6; ++i) {
+      S acc(0.0);
+      for (int j = 0; j < 6; ++j) acc = acc + m[6 * i + j] * u[j];
+      mu[i] = acc;
+    }
+    fj_hat6(u, hh);
+    for (int i = 0; i < 6; ++i) {
+      S acc(0.0);
+      for (int j = 0; j < 6; ++j) acc = acc + m[6 * i + j] * a[j] - hh[j][i] * mu[j];
+      b[i] = acc;
+    }
+    const double wl = 0.5 * l * fj_leg_w[k];
+    for (int c = 0; c < 12; ++c) {
+      S acc(0.0);
+      for (int i = 0; i < 6; ++i) acc = acc + q2[i][c].v * b[i];
+      fe[c] = fe[c] - wl * acc;
+    }
+  }
+}
+
+// A thread per (element, direction): directions 0-5 seed d, 6-17 the element
+// velocities (dynamic only).  Outputs per element: f_e (12), df/dd (12 x 6),
+// df/dv_e (12 x 12), M_t (12 x 12; df/dv_dot_e = -M_t), P(d) (6 x 12) and the
+// strain eps (6); direction 0 also writes the values.
+__global__ __launch_bounds__(64) void k_beam_elem_jac(FjBeamDev bm, int dyn, const double* __restrict__ hg,
+                                                      const double* __restrict__ v, const double* __restrict__ v_dot,
+                                                      const double* __restrict__ k_elem, double* __restrict__ fe_out,
+                                                      double* __restrict__ jd, double* __restrict__ jv,
+                                                      double* __restrict__ mt, double* __restrict__ p_out,
+                                                      double* __restrict__ eps_out) {
+  const int nd = dyn ? 18 : 6;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= bm.n_elem * nd) return;
+  const int e = t / nd, k = t - e * nd;
+  double dv[6], vev[12], vdev[12];
+  fj_beam_d(bm, hg, e, dv);
+  if (dyn) {
+    fj_beam_elem_vec(bm, v, e, vev);
+    fj_beam_elem_vec(bm, v_dot, e, vdev);
+  } else {
+    for (int c = 0; c < 12; ++c) vev[c] = vdev[c] = 0.0;
+  }
+  FjDual d[6], ve[12], fe[12];
+  for (int i = 0; i < 6; ++i) d[i] = FjDual(dv[i], k == i ? 1.0 : 0.0);
+  for (int c = 0; c < 12; ++c) ve[c] = FjDual(vev[c], k == 6 + c ? 1.0 : 0.0);
+  const double* kmat = k_elem + 36 * e;
+  fj_elem_force<FjDual>(bm, e, d, ve, vdev, kmat, dyn, fe);
+  if (k < 6) {
+    for (int r = 0; r < 12; ++r) jd[(12 * e + r) * 6 + k] = fe[r].d;
+  } else {
+    for (int r = 0; r < 12; ++r) jv[(12 * e + r) * 12 + k - 6] = fe[r].d;
+  }
+  if (k != 0) return;
+  const double* o = bm.o0 + 9 * e;
+  const double l = bm.l0[e];
+  for (int r = 0; r < 12; ++r) fe_out[12 * e + r] = fe[r].v;
+  double p[6][12];
+  fj_p_mat(dv, o, p);
+  for (int i = 0; i < 6; ++i) {
+    for (int c = 0; c < 12; ++c) p_out[72 * e + 12 * i + c] = p[i][c];
+    eps_out[6 * e + i] = (dv[i] - bm.d0[6 * e + i]) / l;
+  }
+  if (!dyn) return;
+  // M_t = l (1/6 blockdiag(A0, A0) + 2/3 Qm^T M Qm)
+  const double* m = bm.m_cs + 36 * e;
+  double qm[6][12], a0[6][6];
+  fj_q_mat<double>(0.5, dv, o, qm);
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j < 6; ++j) {
+      const int bi = (i / 3) * 3, bj = (j / 3) * 3, ii = i % 3, jj = j % 3;
+      double acc = 0.0;
+      for (int pp = 0; pp < 3; ++pp)
+        for (int q = 0; q < 3; ++q) acc += o[3 * ii + pp] * m[6 * (bi + pp) + bj + q] * o[3 * jj + q];
+      a0[i][j] = acc;
+    }
+  for (int r = 0; r < 12; ++r)
+    for (int c = 0; c < 12; ++c) {
+      double mq = 0.0;
+      for (int i = 0; i < 6; ++i) {
+        double x = 0.0;
+        for (int j = 0; j < 6; ++j) x += m[6 * i + j] * qm[j][c];
+        mq += qm[i][r] * x;
+      }
+      double f01 = 0.0;
+      if (r < 6 && c < 6) f01 = a0[r][c];
+      else if (r >= 6 && c >= 6) f01 = a0[r - 6][c - 6];
+      mt[144 * e + 12 * r + c] = l * (f01 / 6.0 + (2.0 / 3.0) * mq);
+    }
+}
+
+extern "C" int fj_beam_element_jacobians(const fj_beam_model* model, int dynamic, const double* hg, const double* v,
+                                         const double* v_dot, const double* k_elem, double* fe, double* jd,
+                                         double* jv, double* mt, double* p, double* eps, void* stream) {
+  if (!model || !hg || !k_elem || !fe || !jd || !p || !eps) return FJ_ERR_ARG;
+  if (dynamic && (!v || !v_dot || !jv || !mt)) return FJ_ERR_ARG;
+  if (model->n_elem <= 0) return FJ_ERR_ARG;
+  const FjBeamDev bm = fj_beam_dev(model);
+  const int nt = model->n_elem * (dynamic ? 18 : 6);
+  hipLaunchKernelGGL(k_beam_elem_jac, dim3(fj_cdiv(nt, 64)), dim3(64), 0, (hipStream_t)stream, bm, dynamic ? 1 : 0,
+                     hg, v, v_dot, k_elem, fe, jd, jv, mt, p, eps);
+  FJ_CHECK_LAUNCH();
+  return FJ_OK;
+}
+
 extern "C" size_t fj_beam_workspace_bytes(int n_elem) {
   return n_elem <= 0 ? 0 : (size_t)n_elem * FJ_BW_STRIDE * sizeof(double);
 }

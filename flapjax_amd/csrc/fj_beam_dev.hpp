@@ -12,42 +12,67 @@
 
 #define FJ_SMALL_ANG 1e-10  // utils/constants.py:13
 
-struct FjDual {
-  double v, d;
-  __device__ FjDual() : v(0.0), d(0.0) {}
-  __device__ FjDual(double x) : v(x), d(0.0) {}
-  __device__ FjDual(double x, double dx) : v(x), d(dx) {}
+// A value and one directional derivative.  T = double for first
+// derivatives; T = FjDualT<double> nests them (the derivative of a JVP, as
+// the exact Jacobian of the gyroscopic forces needs).
+template <typename T>
+struct FjDualT {
+  T v, d;
+  __device__ FjDualT() : v(0.0), d(0.0) {}
+  __device__ FjDualT(double x) : v(x), d(0.0) {}
+  __device__ FjDualT(T x, T dx) : v(x), d(dx) {}
 };
-__device__ __forceinline__ FjDual operator+(FjDual a, FjDual b) { return FjDual(a.v + b.v, a.d + b.d); }
-__device__ __forceinline__ FjDual operator-(FjDual a, FjDual b) { return FjDual(a.v - b.v, a.d - b.d); }
-__device__ __forceinline__ FjDual operator-(FjDual a) { return FjDual(-a.v, -a.d); }
-__device__ __forceinline__ FjDual operator*(FjDual a, FjDual b) { return FjDual(a.v * b.v, a.d * b.v + a.v * b.d); }
-__device__ __forceinline__ FjDual operator/(FjDual a, FjDual b) {
-  const double q = a.v / b.v;
-  return FjDual(q, (a.d - q * b.d) / b.v);
+typedef FjDualT<double> FjDual;
+
+template <typename T>
+__device__ __forceinline__ FjDualT<T> operator+(FjDualT<T> a, FjDualT<T> b) { return FjDualT<T>(a.v + b.v, a.d + b.d); }
+template <typename T>
+__device__ __forceinline__ FjDualT<T> operator-(FjDualT<T> a, FjDualT<T> b) { return FjDualT<T>(a.v - b.v, a.d - b.d); }
+template <typename T>
+__device__ __forceinline__ FjDualT<T> operator-(FjDualT<T> a) { return FjDualT<T>(-a.v, -a.d); }
+template <typename T>
+__device__ __forceinline__ FjDualT<T> operator*(FjDualT<T> a, FjDualT<T> b) {
+  return FjDualT<T>(a.v * b.v, a.d * b.v + a.v * b.d);
 }
-__device__ __forceinline__ FjDual operator+(FjDual a, double b) { return FjDual(a.v + b, a.d); }
-__device__ __forceinline__ FjDual operator+(double a, FjDual b) { return FjDual(a + b.v, b.d); }
-__device__ __forceinline__ FjDual operator-(FjDual a, double b) { return FjDual(a.v - b, a.d); }
-__device__ __forceinline__ FjDual operator-(double a, FjDual b) { return FjDual(a - b.v, -b.d); }
-__device__ __forceinline__ FjDual operator*(FjDual a, double b) { return FjDual(a.v * b, a.d * b); }
-__device__ __forceinline__ FjDual operator*(double a, FjDual b) { return FjDual(a * b.v, a * b.d); }
-__device__ __forceinline__ FjDual operator/(FjDual a, double b) { return FjDual(a.v / b, a.d / b); }
-__device__ __forceinline__ FjDual operator/(double a, FjDual b) {
-  const double q = a / b.v;
-  return FjDual(q, -q * b.d / b.v);
+template <typename T>
+__device__ __forceinline__ FjDualT<T> operator/(FjDualT<T> a, FjDualT<T> b) {
+  const T q = a.v / b.v;
+  return FjDualT<T>(q, (a.d - q * b.d) / b.v);
 }
-__device__ __forceinline__ FjDual fj_sqrt(FjDual a) {
-  const double s = sqrt(a.v);
-  return FjDual(s, 0.5 * a.d / s);
+template <typename T>
+__device__ __forceinline__ FjDualT<T> operator+(FjDualT<T> a, double b) { return FjDualT<T>(a.v + b, a.d); }
+template <typename T>
+__device__ __forceinline__ FjDualT<T> operator+(double a, FjDualT<T> b) { return FjDualT<T>(a + b.v, b.d); }
+template <typename T>
+__device__ __forceinline__ FjDualT<T> operator-(FjDualT<T> a, double b) { return FjDualT<T>(a.v - b, a.d); }
+template <typename T>
+__device__ __forceinline__ FjDualT<T> operator-(double a, FjDualT<T> b) { return FjDualT<T>(a - b.v, -b.d); }
+template <typename T>
+__device__ __forceinline__ FjDualT<T> operator*(FjDualT<T> a, double b) { return FjDualT<T>(a.v * b, a.d * b); }
+template <typename T>
+__device__ __forceinline__ FjDualT<T> operator*(double a, FjDualT<T> b) { return FjDualT<T>(a * b.v, a * b.d); }
+template <typename T>
+__device__ __forceinline__ FjDualT<T> operator/(FjDualT<T> a, double b) { return FjDualT<T>(a.v / b, a.d / b); }
+template <typename T>
+__device__ __forceinline__ FjDualT<T> operator/(double a, FjDualT<T> b) {
+  const T q = a / b.v;
+  return FjDualT<T>(q, -q * b.d / b.v);
 }
-__device__ __forceinline__ FjDual fj_sin(FjDual a) { return FjDual(sin(a.v), cos(a.v) * a.d); }
-__device__ __forceinline__ FjDual fj_cos(FjDual a) { return FjDual(cos(a.v), -sin(a.v) * a.d); }
 __device__ __forceinline__ double fj_sqrt(double a) { return sqrt(a); }
 __device__ __forceinline__ double fj_sin(double a) { return sin(a); }
 __device__ __forceinline__ double fj_cos(double a) { return cos(a); }
 __device__ __forceinline__ double fj_val(double a) { return a; }
-__device__ __forceinline__ double fj_val(FjDual a) { return a.v; }
+template <typename T>
+__device__ __forceinline__ FjDualT<T> fj_sqrt(FjDualT<T> a) {
+  const T s = fj_sqrt(a.v);
+  return FjDualT<T>(s, 0.5 * a.d / s);
+}
+template <typename T>
+__device__ __forceinline__ FjDualT<T> fj_sin(FjDualT<T> a) { return FjDualT<T>(fj_sin(a.v), fj_cos(a.v) * a.d); }
+template <typename T>
+__device__ __forceinline__ FjDualT<T> fj_cos(FjDualT<T> a) { return FjDualT<T>(fj_cos(a.v), -fj_sin(a.v) * a.d); }
+template <typename T>
+__device__ __forceinline__ double fj_val(FjDualT<T> a) { return fj_val(a.v); }
 
 // ---- 3x3 / 6x6 helpers (row-major) -----------------------------------------
 template <typename S>
@@ -349,8 +374,9 @@ __device__ __forceinline__ S fj_times_ad(const S x[6][6], const double* o, int i
 }
 
 // P(d) = [-T^-1(-d) Ad, T^-1(d) Ad], T^-1(-d) = T^-1(d) - hat(d)
-__device__ void fj_p_mat(const double* d, const double* o, double p[6][12]) {
-  double tp[6][6], tn[6][6], a[6][6];
+template <typename S>
+__device__ void fj_p_mat(const S* d, const double* o, S p[6][12]) {
+  S tp[6][6], tn[6][6], a[6][6];
   fj_t_inv_se3(d, tp);
   fj_hat6(d, a);
   for (int i = 0; i < 6; ++i)

@@ -6,7 +6,7 @@ y_n = C x_n + D u_{n+1} is the Jacobian of one UVLM step about the
 reference state, exactly as the reference builds it with jacrev of its
 ``step``: the step here is written with the differentiable device pieces
 (``autograd.PairSum`` -- the adjoint Biot-Savart kernels -- and
-``autograd.AICSolve`` -- the device LU and its transposed solve), and every
+``autograd.aic_solve`` -- the device LU and its transposed solve), and every
 row of [A B; C D] is one reverse pass through it.
 
 Inputs u (total values at step n+1): bound grid zeta_b, its velocity

@@ -3,7 +3,7 @@ matrix-free adjoint driver with GMRES and the frozen-wake preconditioner
 (coupled/gradients/coupled.py:848-990, 1234-1249), on the device.
 
 The residuals are the reference's, written with the differentiable device
-pieces (``autograd.PairSum`` for every Biot-Savart sum, ``autograd.AICSolve``
+pieces (``autograd.PairSum`` for every Biot-Savart sum, ``autograd.aic_solve``
 for the circulation solve, torch stencils), so reverse passes through them
 run the adjoint Biot-Savart kernels and the transposed LU solve:
 

@@ -498,6 +498,25 @@ int fj_beam_system(const fj_beam_model* model, int dynamic, const double* hg, co
                    const double* f_dead, double beta_p, double gamma_p, const int32_t* solve_dofs,
                    int n_solve, double* A, double* r, double* f_abs, void* workspace, void* stream);
 
+/* Exact derivatives of the element forces of the beam residual, for the
+ * implicit adjoint of the Newton solves: the reference obtains them with
+ * jax.vjp / jax.jacrev of the same residual (coupled/gradients/coupled.py:
+ * 195-353 static, :540-629 timestep residual; the residual is
+ * structure/beam.py:1749-1843).  Forward-mode dual numbers on the device,
+ * nested for the gyroscopic term's Q_dot (algebra/se3.py:q_dot).  Per element
+ * e, with d = log(H_A^-1 H_B) at the configuration hg (n_nodes, 4, 4) and the
+ * element velocities v and accelerations v_dot (n_nodes, 6, local; dynamic
+ * only): fe (n_elem, 12) the element force -P^T K eps [- M_t v_dot - C_L v],
+ * jd (n_elem, 12, 6) d fe / d d, jv (n_elem, 12, 12) d fe / d v_e, mt
+ * (n_elem, 12, 12) M_t (d fe / d v_dot_e = -M_t), p (n_elem, 6, 12) P(d) =
+ * d d / d(right increments of the two node frames) and eps (n_elem, 6) the
+ * strain.  k_elem (n_elem, 6, 6): the stiffness each element uses (the
+ * scaled table entry).  jv, mt, v, v_dot: dynamic only (may be NULL when
+ * dynamic = 0).  Returns FJ_ERR_ARG on a missing pointer. */
+int fj_beam_element_jacobians(const fj_beam_model* model, int dynamic, const double* hg, const double* v,
+                              const double* v_dot, const double* k_elem, double* fe, double* jd, double* jv,
+                              double* mt, double* p, double* eps, void* stream);
+
 /* Newton update with dx = relax x (structure/beam.py:1974-1990, 2274-2296):
  * dynamic = 1 adds dx, gamma' dx and beta' dx to phi, v and v_dot at the solve
  * DOFs; dynamic = 0 writes the full increment (zero at prescribed DOFs) to

@@ -1,7 +1,7 @@
 """Differentiation form: adjoint kernels vs central finite differences.
 
 * PairSum backward (dL/dx, dL/dy0, dL/dy1, dL/dg) vs FD of the FP64 forward;
-* LinearSolve / AICSolve backward (transposed LU solve + adjoint pair sums);
+* LinearSolve / aic_solve backward (transposed LU solve + adjoint pair sums);
 * the checkpointed differentiable UVLM: its forward equals the FP64 oracle's
   run, and dJ/d(twist, u_inf) matches central FD (step 1e-6) on 8 seeded
   random directions.
