@@ -348,7 +348,7 @@ def _adjoint_case(steps, fixed_iterations):
     return model, production
 
 
-def adjoint_bench(steps=ADJ_STEPS, n_dir=2, seed=5):
+def adjoint_bench(steps=ADJ_STEPS, n_dir=2, seed=5, check=True):
     """Gradient of J = dt sum_n [lift_n + w (z_tip,n - z_tip,0)^2] over the
     coupled gust case shortened to ``steps`` steps, with respect to the 40
     stiffness scalings and 80 spanwise twist angles: one reverse pass
@@ -377,6 +377,9 @@ def adjoint_bench(steps=ADJ_STEPS, n_dir=2, seed=5):
            "J_production_rel_diff": float("%.3g" % (abs(float(J) - J_prod) / abs(J_prod))),
            "grad_norm_k": float(g_k.norm()), "grad_norm_twist": float(g_t.norm()),
            "checkpoint_every": ADJ_CHECKPOINT, "fsi_iters_mean": round(float(np.mean(model.fsi_iters)), 3)}
+    if not check:
+        out["peak_mem_gb"] = round(torch.cuda.max_memory_allocated() / 1e9, 3)
+        return out
     model, production = _adjoint_case(ADJ_CHECK_STEPS, True)
     t2 = time.perf_counter()
     J, g_k, g_t = model.gradient(ks, tw, ADJ_CHECK_STEPS, ADJ_TIP_WEIGHT)
@@ -664,6 +667,8 @@ def main():
     ap.add_argument("--adjoint-only", action="store_true",
                     help="only the coupled adjoint gradient (shortened gust case)")
     ap.add_argument("--adjoint-steps", type=int, default=ADJ_STEPS)
+    ap.add_argument("--adjoint-no-check", action="store_true",
+                    help="with --adjoint-only: the gradient and the production forward only (for profiling)")
     ap.add_argument("--profile-kernels", action="store_true",
                     help="rollup, LU, inverse, convection and copy kernels a fixed number of times (for counter runs)")
     args = ap.parse_args()
@@ -686,7 +691,7 @@ def main():
         print(json.dumps({"coupled_gust": {p: coupled_bench(p, args.coupled_steps) for p in ("fp64", "mixed")}}))
         return
     if args.adjoint_only:
-        print(json.dumps({"coupled_adjoint": adjoint_bench(args.adjoint_steps)}))
+        print(json.dumps({"coupled_adjoint": adjoint_bench(args.adjoint_steps, check=not args.adjoint_no_check)}))
         return
     pairs, rollup_pairs = pairs_per_step()
     progress("cfg1 step, fp64")

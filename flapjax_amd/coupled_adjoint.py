@@ -112,7 +112,12 @@ class DifferentiableAeroelastic:
         self.solver = BA.BeamSolver(beam, prescribed_dofs, self.ti)
         self.k = int(checkpoint_every)
         reuse = aero.aic_reuse if aic_reuse is None else bool(aic_reuse)
-        self.aic_solvers = {True: RefinedSolver(), False: RefinedSolver()} if reuse else {True: None, False: None}
+        # up to 8 corrections per solve, the limit the production solve moves
+        # to once a deforming wing outgrows 3 (uvlm.UVLM._run_checked): one
+        # kept inverse then serves the whole run instead of ~1 refactorisation
+        # per step
+        self.aic_solvers = ({True: RefinedSolver(max_iter=8), False: RefinedSolver(max_iter=8)} if reuse
+                            else {True: None, False: None})
         free = torch.ones(beam.n_dof, dtype=torch.bool, device=beam.hg0.device)
         if self.solver.prescribed:
             free[torch.as_tensor(self.solver.prescribed, device=free.device)] = False
