@@ -124,6 +124,9 @@ _SIGS = {
     "fj_step_batched_table_bytes": ([c_int], c_size_t),
     "fj_step_batched": ([c_int, c_void_p, POINTER(FjFlowfield), c_double, c_double, c_int, c_double, c_double,
                          c_int, c_int, c_int, c_double, c_void_p, c_void_p, c_void_p, c_void_p], c_int),
+    "fj_step_batched_restage": ([c_int, c_void_p, POINTER(FjFlowfield), c_double, c_double, c_int, c_double,
+                                 c_double, c_int, c_int, c_int, c_double, c_void_p, c_void_p, c_void_p, c_void_p],
+                                c_int),
     "fj_flow_eval": ([POINTER(FjFlowfield), c_void_p, c_int, c_double, c_void_p, c_void_p], c_int),
     "fj_beam_workspace_bytes": ([c_int], c_size_t),
     "fj_beam_system": ([c_void_p, c_int] + [c_void_p] * 6 + [c_double, c_double, c_void_p, c_int] + [c_void_p] * 5,

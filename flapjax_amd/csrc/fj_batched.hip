@@ -377,7 +377,7 @@ static void fj_batched_partial_launch(int precision, int grid, const FjTask* dta
 
 int fj_batched_launch(const FjTaskDesc* d, int n_tasks, const double* seg, int kernel,
                       int precision, const FjKernelParams& kp, void* task_table,
-                      void* host_staging, void* workspace, hipStream_t st) {
+                      void* host_staging, void* workspace, hipStream_t st, bool launch) {
   if (n_tasks <= 0) return FJ_OK;
   if (!d || !task_table || !host_staging || !workspace) return FJ_ERR_ARG;
   if (kernel < FJ_KERNEL_EPSILON || kernel > FJ_KERNEL_PLAIN) return FJ_ERR_ARG;
@@ -442,6 +442,7 @@ int fj_batched_launch(const FjTaskDesc* d, int n_tasks, const double* seg, int k
   counts[2] = n_tgt_total;
   counts[3] = recs;
   const int part_total = part;
+  if (!launch) return FJ_OK;
   if (hipMemcpyAsync(task_table, h, fj_batched_task_bytes(n_tasks), hipMemcpyHostToDevice, st) !=
       hipSuccess)
     return FJ_ERR_LAUNCH;

@@ -230,9 +230,11 @@ struct FjTaskDesc {
 };
 size_t fj_batched_ws_bytes(const FjTaskDesc* d, int n, int precision);
 size_t fj_batched_desc_bytes(int n_tasks);
+// launch = false: fill host_staging only (no device call), exactly as a
+// launching call would, for replaying a captured launch with new tasks
 int fj_batched_launch(const FjTaskDesc* d, int n_tasks, const double* seg, int kernel,
                       int precision, const FjKernelParams& kp, void* task_table,
-                      void* host_staging, void* workspace, hipStream_t st);
+                      void* host_staging, void* workspace, hipStream_t st, bool launch = true);
 
 // pair-sum epilogue (fj_biot_savart.hip): chunk partials -> out by mode
 int fj_pair_epilogue(const double* part, int chunks, int ld, int n_tgt, const double* targets,

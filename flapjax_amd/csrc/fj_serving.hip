@@ -551,11 +551,11 @@ extern "C" size_t fj_step_batched_table_bytes(int n_cases) {
          3 * fj_align256(fj_batched_task_bytes(n_cases));
 }
 
-extern "C" int fj_step_batched(int n_cases, const fj_step_case* cases, const fj_flowfield* flow,
+static int fj_step_batched_run(int n_cases, const fj_step_case* cases, const fj_flowfield* flow,
                                double rho, double relax, int kernel, double eps, double rcut,
                                int precision, int free_wake, int max_iter, double tol,
                                void* device_table, void* host_staging, void* workspace,
-                               void* stream) {
+                               void* stream, bool launch) {
   if (n_cases <= 0) return FJ_OK;
   if (!cases || !flow || !device_table || !host_staging || !workspace) return FJ_ERR_ARG;
   if (kernel < FJ_KERNEL_EPSILON || kernel > FJ_KERNEL_PLAIN) return FJ_ERR_ARG;
@@ -601,6 +601,21 @@ extern "C" int fj_step_batched(int n_cases, const fj_step_case* cases, const fj_
   char* dt_base = (char*)device_table;
   char* hs_base = (char*)host_staging;
   memcpy(hs_base, recs.data(), (size_t)n_cases * sizeof(FjCaseDev));
+  if (!launch) {
+    // host staging only: the three task tables as the launches below write them
+    if (free_wake) {
+      rc = fj_batched_launch(roll.data(), n_cases, seg_roll, kernel, precision, kp, dt_base + rec_bytes,
+                             hs_base + rec_bytes, ws + L.off_ws_roll, st, false);
+      if (rc != FJ_OK) return rc;
+    }
+    rc = fj_batched_launch(up.data(), n_cases, seg_frc, kernel, FJ_PREC_FP64, kp,
+                           dt_base + rec_bytes + task_bytes, hs_base + rec_bytes + task_bytes,
+                           ws + L.off_ws_up, st, false);
+    if (rc != FJ_OK) return rc;
+    return fj_batched_launch(frc.data(), n_cases, seg_frc, kernel, precision, kp,
+                             dt_base + rec_bytes + 2 * task_bytes, hs_base + rec_bytes + 2 * task_bytes,
+                             ws + L.off_ws_frc, st, false);
+  }
   if (hipMemcpyAsync(dt_base, hs_base, (size_t)n_cases * sizeof(FjCaseDev),
                      hipMemcpyHostToDevice, st) != hipSuccess)
     return FJ_ERR_LAUNCH;
@@ -678,4 +693,22 @@ extern "C" int fj_step_batched(int n_cases, const fj_step_case* cases, const fj_
   hipLaunchKernelGGL(k_bt_lift, dim3(n_cases), dim3(256), 0, st, dcs);
   FJ_CHECK_LAUNCH();
   return FJ_OK;
+}
+
+extern "C" int fj_step_batched(int n_cases, const fj_step_case* cases, const fj_flowfield* flow,
+                               double rho, double relax, int kernel, double eps, double rcut,
+                               int precision, int free_wake, int max_iter, double tol,
+                               void* device_table, void* host_staging, void* workspace,
+                               void* stream) {
+  return fj_step_batched_run(n_cases, cases, flow, rho, relax, kernel, eps, rcut, precision, free_wake,
+                             max_iter, tol, device_table, host_staging, workspace, stream, true);
+}
+
+extern "C" int fj_step_batched_restage(int n_cases, const fj_step_case* cases, const fj_flowfield* flow,
+                                       double rho, double relax, int kernel, double eps, double rcut,
+                                       int precision, int free_wake, int max_iter, double tol,
+                                       void* device_table, void* host_staging, void* workspace,
+                                       void* stream) {
+  return fj_step_batched_run(n_cases, cases, flow, rho, relax, kernel, eps, rcut, precision, free_wake,
+                             max_iter, tol, device_table, host_staging, workspace, stream, false);
 }

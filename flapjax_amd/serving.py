@@ -260,7 +260,12 @@ class _Slot:
 
 class ServingEngine:
     def __init__(self, precision: str = "fp64", max_slots: int = 32, hbm_budget_bytes: int = 64 << 30,
-                 free_wake: bool = True, device=None, inverse_cache_bytes: int = 8 << 30):
+                 free_wake: bool = True, device=None, inverse_cache_bytes: int = 8 << 30,
+                 graph_ticks: bool = False):
+        """graph_ticks: replay each tick from a captured HIP graph (one per
+        staging parity, captured again whenever the active set changes)
+        instead of issuing its launches; the host then only restages the
+        step's records (fj_step_batched_restage)."""
         if precision not in ("fp64", "mixed"):
             raise ValueError("precision must be 'fp64' or 'mixed'")
         self.precision = precision
@@ -275,6 +280,10 @@ class ServingEngine:
         self._adm = None
         self._plan = None
         self._cache = _InverseCache(inverse_cache_bytes) if inverse_cache_bytes > 0 else None
+        self.graph_ticks = bool(graph_ticks)
+        self._graphs = {}
+        self._graph_set = None
+        self.graph_captures = 0
 
     # ---- fused pair sums ---------------------------------------------------
     def _batched(self, tasks, seg, targets, out, prec):
@@ -361,10 +370,29 @@ class ServingEngine:
         staging = self._grow(f"staging{parity}", plan["table"], pinned=True)
         fd = self.flow.descriptor()
         prec = L.PREC_MIXED if self.precision == "mixed" else L.PREC_FP64
-        rc = lib.fj_step_batched(nc, ctypes.c_void_p(rec.ctypes.data), ctypes.byref(fd), self.flow.rho, 0.7,
-                                 self.kind, EPSILON, R_CUTOFF, prec, int(self.free_wake), 3, 0.0, L.ptr(table),
-                                 ctypes.c_void_p(staging.data_ptr()), L.ptr(ws), L.stream())
-        L.check(rc, "fj_step_batched")
+        args = (nc, ctypes.c_void_p(rec.ctypes.data), ctypes.byref(fd), self.flow.rho, 0.7, self.kind, EPSILON,
+                R_CUTOFF, prec, int(self.free_wake), 3, 0.0, L.ptr(table), ctypes.c_void_p(staging.data_ptr()),
+                L.ptr(ws))
+        if self.graph_ticks:
+            # the graph holds this set's buffers, so a new set drops the
+            # graphs of the previous one (its buffers may have been regrown)
+            if self._graph_set != plan["key"]:
+                self._graphs.clear()
+                self._graph_set = plan["key"]
+            g = self._graphs.get(parity)
+            if g is None:
+                g = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(g):
+                    rc = lib.fj_step_batched(*args, L.stream())
+                L.check(rc, "fj_step_batched")
+                self._graphs[parity] = g
+                self.graph_captures += 1
+            else:
+                L.check(lib.fj_step_batched_restage(*args, L.stream()), "fj_step_batched_restage")
+            g.replay()
+        else:
+            rc = lib.fj_step_batched(*args, L.stream())
+            L.check(rc, "fj_step_batched")
         for sl in slots:
             sl.i += 1
         status_host = self._grow(f"status{parity}", 4 * nc, pinned=True)[:4 * nc].view(torch.int32)

@@ -459,6 +459,19 @@ int fj_step_batched(int n_cases, const fj_step_case* cases, const fj_flowfield* 
                     int free_wake, int max_iter, double tol, void* device_table,
                     void* host_staging, void* workspace, void* stream);
 
+/* The host half of fj_step_batched with the same arguments: writes the case
+ * records and pair-sum task tables of this step into host_staging exactly as
+ * fj_step_batched does, and makes no device call.  For replaying a captured
+ * fj_step_batched (its copies read host_staging, its kernels read the
+ * device table) with the next step's records while the set of cases and
+ * every buffer stay the same.  No counterpart in the reference (its step is
+ * a jitted function); it serves a serving loop that replays one captured
+ * graph per tick. */
+int fj_step_batched_restage(int n_cases, const fj_step_case* cases, const fj_flowfield* flow,
+                            double rho, double relax, int kernel, double eps, double rcut, int precision,
+                            int free_wake, int max_iter, double tol, void* device_table,
+                            void* host_staging, void* workspace, void* stream);
+
 /* ---- Structure side of the coupling: the SE(3) two-node beam -------------
  * The Newton iterations of BaseBeamStructure.static_solve and
  * base_dynamic_solve (structure/beam.py:1878-2092, 2129-2861) with the

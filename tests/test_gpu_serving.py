@@ -177,3 +177,27 @@ def test_config_size_stream(dev):
                 assert rel_err(b.lift[1:], lift[1:]) < 1e-10 and e_g < 1e-10 and e_w < 1e-10
             else:
                 assert e_g < 1e-3 and e_w < 0.3
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_graph_ticks_equal_launched_ticks(dev, precision):
+    """Ticks replayed from captured HIP graphs (restaged host records, one
+    graph per staging parity, recaptured at every change of the active set)
+    give the results of ticks issued launch by launch: the same kernels on
+    the same data, every case's lift history, circulation and wake equal."""
+    from flapjax_amd.serving import ServingEngine
+
+    reqs = _stream()
+    ref, _ = ServingEngine(precision=precision).run(reqs)
+    eng = ServingEngine(precision=precision, graph_ticks=True)
+    got, stats = eng.run(reqs)
+    assert stats["cases"] == len(reqs)
+    ticks = stats.get("ticks")
+    print(f"graph captures {eng.graph_captures}, ticks {ticks}")
+    assert eng.graph_captures > 0 and (ticks is None or eng.graph_captures < ticks)
+    by_id = {r.case_id: r for r in ref}
+    for r in got:
+        a = by_id[r.case_id]
+        assert np.array_equal(r.lift, a.lift)
+        assert np.array_equal(r.gamma_b_final, a.gamma_b_final)
+        assert np.array_equal(r.zeta_w_final, a.zeta_w_final)
