@@ -67,187 +67,220 @@ __device__ __forceinline__ unsigned long long fj_wave_max_u64(unsigned long long
   return ((unsigned long long)mh << 32) | ml;
 }
 
-// NT threads, each owning the panel rows tid + r * NT (r < R).  Per column,
-// one workgroup barrier: every wave finds its own pivot candidate (register
-// argmax, then the DPP wave maximum), its owner lane publishes the candidate
-// row and the wave writes its key; the owner of row jj publishes that row.
-// After the barrier every thread reduces the wave keys itself, takes the
-// winning row from LDS and applies the exchange, scaling and rank-1 update
-// in registers.  The LDS slots alternate between columns: a wave can only
-// write slot (jj + 2) & 1 after every wave has passed barrier jj + 1, i.e.
-// has finished reading column jj.
+// NT threads, each owning the panel slots tid + r * NT (r < R), all JB
+// columns of a slot's row in registers.  Rows never move between threads:
+// each slot carries the logical row it currently holds (lg), a row exchange
+// only swaps two logical indices, and the write-back stores each slot to its
+// logical row.  Pivot keys use the logical row, so ties go to the smallest
+// logical row as in LAPACK.  The column loop is unrolled at compile time
+// (fj_panel_col<JJ> calls fj_panel_col<JJ + 1>), so every register index is
+// a constant.  (Measured on the panel shapes below: a loop unrolled in groups
+// of 2, 4 or 8 columns with the registers rotated between groups, which
+// keeps the code inside the instruction cache, ran 3-20 % slower.)
 //
-// The column loop is NOT unrolled (an unrolled loop is tens of KB of code,
-// more than the instruction cache, and then runs at the rate instructions
-// stream in from L2).  Register indices must stay compile-time constants, so
-// every row's registers rotate left by one position per column: position 0
-// always holds the current column, positions 1 .. JB-1-jj the columns still
-// to update, and the finished columns wrap round to the end; after jb
-// columns, position q holds column (q + jb) mod JB.
+// A column step is bound by the instructions every wave issues (~400 per
+// column and wave, two waves per SIMD: ~3000 cycles whatever the panel
+// height), so only the chain from one pivot to the next sits between the
+// barriers:
+//   keys of column jj -> DPP wave maximum -> LDS            | barrier A
+//   (while waiting: the rank-1 update of column jj-1 on the columns
+//    jj+1.., deferred from the previous step)
+//   key reduction -> the global winner publishes its row   | barrier B
+//   pivot, logical exchange, multipliers, and the update of column jj+1
+//   only (the rest of column jj's update is deferred to the next step).
+// Every element still receives its updates in column order, each as the
+// same FMA, so the arithmetic is that of the plain rank-1 sequence.
+template <typename T, int NT, int JB>
+struct FjPanelLds {
+  unsigned long long key[2][NT / 64];
+  T prow[2][JB];
+  int ipiv[JB];
+  int keep[NT];
+};
+
+// a copy the optimiser cannot see through (an empty asm on the register)
+template <typename T>
+__device__ __forceinline__ T fj_opaque(T x) {
+  asm("" : "+v"(x));
+  return x;
+}
+
+// Workgroup barrier ordering LDS accesses only: global stores may stay in
+// flight across it (__syncthreads() would wait for every outstanding global
+// access of the wave first).
+__device__ __forceinline__ void fj_lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+template <typename T, int NT, int R, int JB, int JJ>
+__device__ __forceinline__ void fj_panel_col(T (&a)[R][JB], int (&lg)[R], T (&l)[R],
+                                             FjPanelLds<T, NT, JB>& sh, int tid, int rows, int k,
+                                             int32_t* __restrict__ info) {
+  if constexpr (JJ < JB) {
+    constexpr int jj = JJ;
+    if (jj >= rows) return;  // uniform: the panel is narrower than JB
+    constexpr int NW = NT / 64;
+    constexpr int buf = JJ & 1;
+    const int lane = tid & 63, wave = tid >> 6;
+    // this thread's best candidate among the logical rows jj .. (padding
+    // slots hold zeros and larger logical rows than every real row, so they
+    // never win)
+    unsigned long long key = 0ull;
+    int kr = 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const unsigned long long kk = (lg[r] >= jj) ? fj_pivot_key((double)a[r][JJ], lg[r]) : 0ull;
+      kr = kk > key ? r : kr;
+      key = kk > key ? kk : key;
+    }
+    const unsigned long long wkey = fj_wave_max_u64(key);
+    if (lane == 0) sh.key[buf][wave] = wkey;
+    // the deferred rank-1 update of column jj-1 on columns jj+1 ..
+    if constexpr (JJ > 0) {
+#pragma unroll
+      for (int c = JJ + 1; c < JB; ++c) {
+        const T pc = sh.prow[buf ^ 1][c];
+#pragma unroll
+        for (int r = 0; r < R; ++r) a[r][c] -= l[r] * pc;
+      }
+    }
+    fj_lds_barrier();  // A
+    unsigned long long kv[NW];
+#pragma unroll
+    for (int w = 0; w < NW; ++w) kv[w] = sh.key[buf][w];
+#pragma unroll
+    for (int s = 1; s < NW; s *= 2)
+#pragma unroll
+      for (int w = 0; w + s < NW; w += 2 * s) kv[w] = kv[w + s] > kv[w] ? kv[w + s] : kv[w];
+    const unsigned long long best = kv[0];
+    const int p = 8191 - (int)(best & 0x1FFFull);  // logical pivot row
+    // keys are distinct (the row sits in the low bits): exactly one lane
+    // holds the winning key, and it publishes its row's columns JJ ..  A
+    // select chain picks the slot; it starts from an opaque copy, because a
+    // select between two loads of `a` would be folded into a load from a
+    // selected address, which keeps `a` in scratch memory.
+    if (key == best) {
+#pragma unroll
+      for (int c = JJ; c < JB; ++c) {
+        T v = fj_opaque(a[0][c]);
+#pragma unroll
+        for (int r = 1; r < R; ++r) v = (kr == r) ? a[r][c] : v;
+        sh.prow[buf][c] = v;
+      }
+    }
+    if (tid == 0) {
+      sh.ipiv[jj] = k + p;
+      if ((best >> 13) == 0ull && *info == 0) *info = k + jj + 1;  // zero column
+    }
+    fj_lds_barrier();  // B
+    // exchange logical rows jj and p; multipliers of the rows below jj (the
+    // other slots get l = 0, so the rank-1 updates need no per-slot
+    // predicate: they leave them unchanged)
+    const T piv = sh.prow[buf][JJ];
+    const T inv = (piv != T(0)) ? T(1) / piv : T(0);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      lg[r] = (lg[r] == jj) ? p : ((lg[r] == p) ? jj : lg[r]);
+      const bool below = lg[r] > jj && lg[r] < rows;
+      const T lv = below ? a[r][JJ] * inv : T(0);
+      l[r] = lv;
+      if (below) a[r][JJ] = lv;
+    }
+    if constexpr (JJ + 1 < JB) {
+      const T pc = sh.prow[buf][JJ + 1];
+#pragma unroll
+      for (int r = 0; r < R; ++r) a[r][JJ + 1] -= l[r] * pc;
+    }
+    fj_panel_col<T, NT, R, JB, JJ + 1>(a, lg, l, sh, tid, rows, k, info);
+  }
+}
+
 template <typename T, int NT, int R, int JB>
 __global__ __launch_bounds__(NT) void k_lu_panel(T* __restrict__ A, int n, int lda, int k,
                                                  int32_t* __restrict__ ipiv,
                                                  int32_t* __restrict__ info) {
-  constexpr int NW = NT / 64;
-  static_assert(NT % 64 == 0 && 2 * JB <= NT, "panel shape");
-  __shared__ unsigned long long s_key[2][NW];
-  __shared__ T s_cand[2][NW][JB];
-  __shared__ T s_swp[2][JB];
-  __shared__ int s_ipiv[JB];
-  __shared__ T s_stage[NT][JB + 1];  // one row block, padded against bank conflicts
+  static_assert(NT % 64 == 0 && JB <= NT && NT * R <= 8192 && JB % 2 == 0, "panel shape");
+  // one row block of the panel staged through LDS on load and store
+  // (padded against bank conflicts)
+  __shared__ T s_stage[NT][JB + 1];
+  __shared__ FjPanelLds<T, NT, JB> sh;
   const int tid = threadIdx.x;
-  const int lane = tid & 63, wave = tid >> 6;
   const int jb = min(JB, n - k);
   const int rows = n - k;
 
-  // Load through LDS: row-per-thread loads would touch one cache line per
-  // lane and instruction.  Block r (rows r*NT .. r*NT+NT-1) is read with
-  // consecutive lanes on consecutive columns of a row (all blocks in flight
-  // at once), then transposed through one padded LDS block.
-  // (A wide tile is staged one block at a time, keeping the register peak
-  // at the tile plus one block.)
+  // Load: block r (rows r*NT .. r*NT+NT-1) is read with consecutive lanes on
+  // consecutive columns of a row (one cache line per 64 / JB rows instead of
+  // one per lane) -- every block's loads issued at once, into the registers
+  // that later hold the block -- then each block is transposed through the
+  // LDS block.
+  // (the write-back moves pairs of columns, JB being even, as one 16-byte
+  // store for FP64 when the pair is aligned in memory)
+  constexpr int JH = JB / 2;
   T a[R][JB];
-  constexpr int RB = (R * JB <= 64) ? R : 1;  // blocks loaded per round
-#pragma unroll
-  for (int r0 = 0; r0 < R; r0 += RB) {
-    T t[RB][JB];
-#pragma unroll
-    for (int rb = 0; rb < RB; ++rb)
-#pragma unroll
-      for (int i = 0; i < JB; ++i) {
-        const int e = tid + i * NT, lr = (r0 + rb) * NT + e / JB, c = e % JB;
-        t[rb][i] = (lr < rows && c < jb) ? A[(size_t)(k + lr) * lda + k + c] : T(0);
-      }
-#pragma unroll
-    for (int rb = 0; rb < RB; ++rb) {
-#pragma unroll
-      for (int i = 0; i < JB; ++i) {
-        const int e = tid + i * NT;
-        s_stage[e / JB][e % JB] = t[rb][i];
-      }
-      __syncthreads();
-#pragma unroll
-      for (int c = 0; c < JB; ++c) a[r0 + rb][c] = s_stage[tid][c];
-      __syncthreads();
-    }
-  }
-
-#pragma nounroll
-  for (int jj = 0; jj < jb; ++jj) {
-    const int buf = jj & 1;
-    unsigned long long key = 0ull;
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const int lr = tid + r * NT;
-      const unsigned long long kk = fj_pivot_key((double)a[r][0], lr);
-      key = (lr >= jj && lr < rows && kk > key) ? kk : key;
-    }
-    const unsigned long long wkey = fj_wave_max_u64(key);
-    // a wave with no row at or below jj has key 0, which never wins: row jj
-    // itself has a nonzero key (8191 - jj in its low bits)
-    if (lane == 0) s_key[buf][wave] = wkey;
-    if (wkey != 0ull) {
-      const int wp = 8191 - (int)(wkey & 0x1FFFull);
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        if (tid + r * NT == wp) {
-#pragma unroll
-          for (int c = 0; c < JB; ++c) s_cand[buf][wave][c] = a[r][c];
-        }
-      }
-    }
-    if (tid == jj) {  // row jj < JB <= NT sits in a[0]
-#pragma unroll
-      for (int c = 0; c < JB; ++c) s_swp[buf][c] = a[0][c];
-    }
-    __syncthreads();
-    unsigned long long best = s_key[buf][0];
-#pragma unroll
-    for (int w = 1; w < NW; ++w) {
-      const unsigned long long kw = s_key[buf][w];
-      best = kw > best ? kw : best;
-    }
-    const int p = 8191 - (int)(best & 0x1FFFull);  // local pivot row
-    const int bw = (p % NT) >> 6;                    // the wave that published it
-    if (tid == 0) {
-      s_ipiv[jj] = k + p;
-      if ((best >> 13) == 0ull && *info == 0) *info = k + jj + 1;  // zero column
-    }
-    const T* prow = s_cand[buf][bw];
-    const T piv = prow[0];
-    const T inv = (piv != T(0)) ? T(1) / piv : T(0);
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const int lr = tid + r * NT;
-      if (lr == jj) {
-#pragma unroll
-        for (int c = 0; c < JB; ++c) a[r][c] = prow[c];
-      } else if (lr == p) {
-#pragma unroll
-        for (int c = 0; c < JB; ++c) a[r][c] = s_swp[buf][c];
-      }
-    }
-    // multipliers of the rows below jj; the other rows get l = 0, so the
-    // update below needs no per-row predicate (it leaves them unchanged)
-    T l[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const int lr = tid + r * NT;
-      const bool below = lr > jj && lr < rows;
-      l[r] = below ? a[r][0] * inv : T(0);
-      a[r][0] = below ? l[r] : a[r][0];
-    }
-    // the pivot row in registers, read in batches of 8 ahead of their use:
-    // read inside the per-column branches, every read would wait out the LDS
-    // latency on its own
-    const int live = JB - jj;  // positions 0 .. live-1 are columns jj .. JB-1
-    constexpr int PB = JB < 8 ? JB : 8;
-#pragma unroll
-    for (int c0 = 0; c0 < JB; c0 += PB) {
-      if ((c0 < 1 ? 1 : c0) < live) {  // uniform: some column of the batch is live
-        T pv[PB];
-#pragma unroll
-        for (int i = 0; i < PB; ++i) pv[i] = prow[c0 + i];
-#pragma unroll
-        for (int i = 0; i < PB; ++i) {
-          const int c = c0 + i;
-          if (c >= 1 && c < live) {  // uniform
-#pragma unroll
-            for (int r = 0; r < R; ++r) a[r][c] -= l[r] * pv[i];
-          }
-        }
-      }
-    }
-    // rotate: position q takes position q + 1, position 0 wraps to the end
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const T t0 = a[r][0];
-#pragma unroll
-      for (int c = 0; c + 1 < JB; ++c) a[r][c] = a[r][c + 1];
-      a[r][JB - 1] = t0;
-    }
-  }
-  __syncthreads();
-
-  // write the panel back through the same LDS block: position q holds
-  // column (q + jb) mod JB
+  int lg[R];
+  const bool al = (sizeof(T) == 8) && ((((size_t)lda | (size_t)k) & 1) == 0);
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-#pragma unroll
-    for (int q = 0; q < JB; ++q) {
-      int c = q + jb;
-      if (c >= JB) c -= JB;
-      s_stage[tid][c] = a[r][q];
-    }
-    __syncthreads();
+    lg[r] = tid + r * NT;
 #pragma unroll
     for (int i = 0; i < JB; ++i) {
       const int e = tid + i * NT, lr = r * NT + e / JB, c = e % JB;
-      if (lr < rows && c < jb) A[(size_t)(k + lr) * lda + k + c] = s_stage[e / JB][c];
+      a[r][i] = (lr < rows && c < jb) ? A[(size_t)(k + lr) * lda + k + c] : T(0);
     }
-    __syncthreads();
   }
-  if (tid < jb) ipiv[k + tid] = s_ipiv[tid];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+#pragma unroll
+    for (int i = 0; i < JB; ++i) {
+      const int e = tid + i * NT;
+      s_stage[e / JB][e % JB] = a[r][i];
+    }
+    fj_lds_barrier();
+#pragma unroll
+    for (int c = 0; c < JB; ++c) a[r][c] = s_stage[tid][c];
+    fj_lds_barrier();
+  }
+
+  T l[R];
+  fj_panel_col<T, NT, R, JB, 0>(a, lg, l, sh, tid, rows, k, info);
+  fj_lds_barrier();
+
+  // write-back: slots still holding their own row go through the LDS block
+  // (coalesced); the at most 2 jb slots whose logical row changed store
+  // their row directly (the exchange is a bijection, so no position is
+  // written by both paths)
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const bool keep = lg[r] == tid + r * NT;
+    if (keep) {
+#pragma unroll
+      for (int c = 0; c < JB; ++c) s_stage[tid][c] = a[r][c];
+    }
+    sh.keep[tid] = keep ? 1 : 0;
+    fj_lds_barrier();
+#pragma unroll
+    for (int i = 0; i < JH; ++i) {
+      const int e = tid + i * NT, q = e / JH, lr = r * NT + q, c = 2 * (e % JH);
+      if (lr < rows && sh.keep[q]) {
+        T* dst = A + (size_t)(k + lr) * lda + k + c;
+        if (c + 1 < jb && al) {
+          *reinterpret_cast<double2*>(dst) = double2{(double)s_stage[q][c], (double)s_stage[q][c + 1]};
+        } else {
+          if (c < jb) dst[0] = s_stage[q][c];
+          if (c + 1 < jb) dst[1] = s_stage[q][c + 1];
+        }
+      }
+    }
+    fj_lds_barrier();
+    if (!keep) {
+#pragma unroll
+      for (int c = 0; c < JB; ++c)
+        if (c < jb) A[(size_t)(k + lg[r]) * lda + k + c] = a[r][c];
+    }
+  }
+  if (tid < jb) ipiv[k + tid] = sh.ipiv[tid];
   // composed permutation ipiv[n + q]: only positions k..k+jb-1 and the pivot
   // rows change; each is traced back through this panel's exchanges (last
   // exchange first) to the position its content came from.
@@ -255,11 +288,11 @@ __global__ __launch_bounds__(NT) void k_lu_panel(T* __restrict__ A, int n, int l
     int32_t* perm = ipiv + n;
     int q = -1, src = -1, val = 0;
     if (tid < jb) q = k + tid;
-    else if (tid < 2 * jb) q = s_ipiv[tid - jb];
+    else if (tid < 2 * jb) q = sh.ipiv[tid - jb];
     if (q >= 0) {
       src = q;
       for (int jj = jb - 1; jj >= 0; --jj) {
-        const int a0 = k + jj, b0 = s_ipiv[jj];
+        const int a0 = k + jj, b0 = sh.ipiv[jj];
         if (src == a0) src = b0;
         else if (src == b0) src = a0;
       }
@@ -501,7 +534,7 @@ static int fj_lu_dispatch(T* A, int n, int lda, int32_t* ipiv, int32_t* info, T*
   if (n <= 1024) return fj_lu_factor_t<T, 512, 2, 32>(A, n, lda, ipiv, info, aux, st);
   if (n <= 2048) return fj_lu_factor_t<T, 512, 4, 16>(A, n, lda, ipiv, info, aux, st);
   if (n <= 4096) return fj_lu_factor_t<T, 512, 8, 8>(A, n, lda, ipiv, info, aux, st);
-  return fj_lu_factor_t<T, 512, 16, 4>(A, n, lda, ipiv, info, aux, st);
+  return fj_lu_factor_t<T, 1024, 8, 4>(A, n, lda, ipiv, info, aux, st);
 }
 
 extern "C" int fj_lu_factor(double* A, int n, int lda, int32_t* ipiv, int32_t* info, void* aux,
@@ -1231,10 +1264,58 @@ __global__ __launch_bounds__(64) void k_band_solve(const double* __restrict__ A,
 // any chain of two-node beam elements).  KL zero rows below the matrix let
 // every column step read and update rows k .. k + KL unconditionally, and
 // the constant row stride turns every band address into an immediate
-// offset, so a column step is about a sixth of the generic kernel's
-// instructions (a single wave is bound by its instruction latency here).
-// Zero padding rows are never chosen as pivots (a zero is never strictly
-// larger than the current maximum) and never reach the solution.
+// offset.  Zero padding rows are never chosen as pivots (a zero is never
+// strictly larger than the current maximum) and never reach the solution.
+//
+// One wave runs the column steps, so a step costs its chain of dependent
+// instructions: every LDS read of the step (column k for all lanes, each
+// update lane's column k + j, x[k .. k + KL]) is issued first, since none
+// depends on the pivot; the pivot is found by a four-level tree over the
+// KL + 1 candidates (the left operand always holds the lower rows and the
+// right one wins only when strictly larger, which keeps idamax's first
+// index); and everything that depends on the pivot row is a select on the
+// (uniform) pivot offset, except the pivot row's entry of each update
+// lane's column, one LDS read once the pivot is known; a step without an
+// exchange (most of the beam's columns) takes a branch with none of them.
+// The
+// arithmetic is dgbtf2's to the bit: the same pivots, the same reciprocal
+// scaling, the same FMA per updated entry.
+template <int KL>
+struct FjBandPick {
+  // first maximal |a[lo .. lo + cnt)|: signed value and offset
+  template <int LO, int CNT>
+  static __device__ __forceinline__ void run(const double (&a)[KL + 1], double& v, int& ix) {
+    if constexpr (CNT == 1) {
+      v = a[LO];
+      ix = LO;
+    } else {
+      constexpr int H = (CNT + 1) / 2;
+      double v1, v2;
+      int i1, i2;
+      run<LO, H>(a, v1, i1);
+      run<LO + H, CNT - H>(a, v2, i2);
+      const bool t = fabs(v2) > fabs(v1);
+      v = t ? v2 : v1;
+      ix = t ? i2 : i1;
+    }
+  }
+};
+
+__device__ __forceinline__ double fj_readlane_f64(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Ordering of LDS accesses between the lanes of the one wave that runs a
+// kernel: a wave's LDS instructions execute in issue order, so a later read
+// sees an earlier write of any lane once the compiler keeps the order (a
+// wavefront-scope fence; no wait and no barrier instruction is needed).
+__device__ __forceinline__ void fj_wave_lds_order() {
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+}
+
 template <int KL, int KU>
 __global__ __launch_bounds__(64) void k_band_solve_fixed(const double* __restrict__ A, int n,
                                                          int lda, const double* __restrict__ b,
@@ -1244,6 +1325,8 @@ __global__ __launch_bounds__(64) void k_band_solve_fixed(const double* __restric
   if (guard && guard[1] != 0) return;
   constexpr int W = 2 * KL + KU + 1;
   constexpr int D = W - 1;  // from (row i, column c) to (row i + 1, column c)
+  constexpr int JM = KL + KU;  // the farthest column an update reaches
+  static_assert(JM < 64, "band shape");
   __shared__ double s[FJ_BAND_LDS_DOUBLES];
   const int lane = threadIdx.x;
   const int rows = n + KL;
@@ -1267,77 +1350,90 @@ __global__ __launch_bounds__(64) void k_band_solve_fixed(const double* __restric
   for (int i = lane; i < rows; i += 64) xs[i] = i < n ? b[i] : 0.0;
   int first_zero = 0;
   int ju = 0;
+  // update lanes 1 .. JM own column k + lane; lanes 1 .. KL also own row
+  // k + lane of column k and of b (addresses of the other lanes are clamped
+  // into range and their results discarded)
+  const bool upd = lane >= 1 && lane <= JM;
+  const int jl = upd ? lane : 1;
+  const bool ml = lane >= 1 && lane <= KL;
+  const int il = ml ? lane : 0;
   __syncthreads();
   for (int k = 0; k < n; ++k) {
     double* colk = s + (size_t)k * W + KL;  // (row k, column k)
-    double a[KL + 1];
+    double a[KL + 1], r[KL + 1];
 #pragma unroll
     for (int i = 0; i <= KL; ++i) a[i] = colk[i * D];
-    double amax = fabs(a[0]);
-    int ip = 0;
 #pragma unroll
-    for (int i = 1; i <= KL; ++i) {
-      const double t = fabs(a[i]);
-      const bool take = t > amax;
-      amax = take ? t : amax;
-      ip = take ? i : ip;
-    }
-    if (amax != 0.0) {
+    for (int i = 0; i <= KL; ++i) r[i] = colk[jl + i * D];  // (row k + i, column k + jl)
+    const double al = colk[il * D];                         // (row k + il, column k)
+    const double xl = xs[k + il];
+    double piv;
+    int ipv;
+    FjBandPick<KL>::template run<0, KL + 1>(a, piv, ipv);
+    const int ip = __builtin_amdgcn_readfirstlane(ipv);
+    if (piv != 0.0) {
       const int p = k + ip;
       ju = max(ju, min(p + KU, n - 1));
       const int nj = ju - k;
-      double piv = a[0];
-#pragma unroll
-      for (int i = 1; i <= KL; ++i) piv = i == ip ? a[i] : piv;
       const double rp = 1.0 / piv;
-      double l[KL + 1];
-      double lme = 0.0;
+      if (ip == 0) {
+        // no exchange (the common case on the beam's systems): the rank-1
+        // update of rows k + 1 .. k + KL of column k + jl, and of b
+        if (upd && lane <= nj) {
+          const double u = r[0];
 #pragma unroll
-      for (int i = 1; i <= KL; ++i) {
-        l[i] = (i == ip ? a[0] : a[i]) * rp;
-        lme = i == lane ? l[i] : lme;
-      }
-      if (lane >= 1 && lane <= nj) {
-        double* uk = colk + lane;        // (row k, column k + j)
-        double* up = uk + ip * D;        // (row p, column k + j)
-        const double u = *up;
-        const double t = *uk;
-        *uk = u;
-        *up = t;
-        double r[KL + 1];
+          for (int i = 1; i <= KL; ++i) colk[jl + i * D] = r[i] - (a[i] * rp) * u;
+        }
+        const double bk = fj_readlane_f64(xl, 0);
+        if (ml) {
+          const double lme = al * rp;
+          colk[lane * D] = lme;
+          xs[k + lane] = xl - lme * bk;
+        }
+      } else {
+        if (upd && lane <= nj) {
+          // exchange rows k and p of column k + jl, then the rank-1 update
+          // of rows k + 1 .. k + KL (row p now holds row k's entry)
+          const double u = colk[jl + ip * D];  // (row p, column k + jl)
+          const double t = r[0];
+          colk[jl] = u;
 #pragma unroll
-        for (int i = 1; i <= KL; ++i) r[i] = uk[i * D];
-#pragma unroll
-        for (int i = 1; i <= KL; ++i) uk[i * D] = r[i] - l[i] * u;
-      }
-      const double bk = xs[p];
-      const bool mine = lane >= 1 && lane <= KL;
-      const int rl = mine ? k + lane : k;
-      const double bi = (mine && lane == ip) ? xs[k] : xs[rl];
-      if (lane == 0) {
-        colk[0] = piv;
-        xs[k] = bk;
-      }
-      if (mine) {
-        colk[lane * D] = lme;
-        xs[rl] = bi - lme * bk;
+          for (int i = 1; i <= KL; ++i) {
+            const double li = ((i == ip) ? a[0] : a[i]) * rp;
+            const double ri = (i == ip) ? t : r[i];
+            colk[jl + i * D] = ri - li * u;
+          }
+        }
+        // column k: pivot and multipliers; b: exchange and update
+        const double xk = fj_readlane_f64(xl, 0);
+        const double bk = fj_readlane_f64(xl, ip);
+        if (lane == 0) {
+          colk[0] = piv;
+          xs[k] = bk;
+        }
+        if (ml) {
+          const double lme = ((lane == ip) ? a[0] : al) * rp;
+          colk[lane * D] = lme;
+          const double bi = (lane == ip) ? xk : xl;
+          xs[k + lane] = bi - lme * bk;
+        }
       }
     } else if (first_zero == 0) {
       first_zero = k + 1;
     }
-    __syncthreads();
+    fj_wave_lds_order();
   }
   for (int j = n - 1; j >= 0; --j) {
     const double xj = xs[j];
     double v = xj;
     if (xj != 0.0) v = xj / s[(size_t)j * W + KL];
-    __syncthreads();
+    fj_wave_lds_order();
     if (xj != 0.0) {
       const int kk = min(KL + KU, j);
       if (lane >= 1 && lane <= kk) xs[j - lane] -= v * s[(size_t)(j - lane) * W + (lane + KL)];
       if (lane == 0) xs[j] = v;
     }
-    __syncthreads();
+    fj_wave_lds_order();
   }
   for (int i = lane; i < n; i += 64) x[i] = xs[i];
   if (lane == 0 && info) *info = first_zero;
