@@ -109,6 +109,9 @@ class AeroCase:
     free_wake: bool = False
     gamma_dot_relaxation: float = 0.7
     batch_size: object = None
+    # control inputs (aero/data_structures.py: cs_ang, cs_vel): {name: (n_tstep,)}
+    cs_ang: dict = field(default_factory=dict)
+    cs_vel: dict = field(default_factory=dict)
 
     def __post_init__(self):
         # aero/uvlm.py:330-331 naming
@@ -276,7 +279,11 @@ class AeroCase:
             for dst, src in zip(getattr(out, name), getattr(self, name)):
                 dst[i_ts].copy_(src[i_ts if self.n_tstep > 1 else 0])
         out.t[i_ts] = self.t[i_ts if self.n_tstep > 1 else 0]
-        return out.settings_from(self)
+        out = out.settings_from(self)
+        # the control inputs held over the run (aero/data_structures.py:570-610)
+        out.cs_ang = {k: v[i_ts if v.shape[0] > 1 else 0].expand(n_tstep).clone() for k, v in self.cs_ang.items()}
+        out.cs_vel = {k: v[i_ts if v.shape[0] > 1 else 0].expand(n_tstep).clone() for k, v in self.cs_vel.items()}
+        return out
 
     def plot(self, directory, plot_bound: bool = True, plot_wake: bool = True, index=None):
         """VTK output (aero/data_structures.py:401-461): per-surface ``.vts``

@@ -116,6 +116,67 @@ def mirror_grid(zeta: torch.Tensor, mirror_point, mirror_normal) -> torch.Tensor
 
 
 # ---------------------------------------------------------------------------
+# control surfaces: building blocks of a UVLM ``grid_func``
+# ---------------------------------------------------------------------------
+HINGE_AXIS_DEFAULT = (0.0, 1.0, 0.0)  # aero/utils.py:39
+
+
+def _index_rows(index, n_entries: int) -> torch.Tensor:
+    """utils/utils.py:93-113 (index_to_arr) on the host."""
+    if isinstance(index, slice):
+        return torch.arange(n_entries)[index]
+    if isinstance(index, torch.Tensor):
+        return index.detach().cpu().long()
+    if isinstance(index, np.ndarray):
+        return torch.as_tensor(index, dtype=torch.long)
+    if isinstance(index, Sequence):
+        return torch.as_tensor(list(index), dtype=torch.long)
+    if isinstance(index, (int, np.integer)):
+        return torch.tensor([int(index)])
+    if index is None:
+        return torch.arange(n_entries)
+    raise TypeError("index must be a slices, sequence of ints, or Array")
+
+
+def add_control_surface(grid: torch.Tensor, angle, m_slice, n_slice, hinge_axis=HINGE_AXIS_DEFAULT) -> torch.Tensor:
+    """aero/utils.py:42-84: rotate the vertices ``grid[m_slice, n_slice]``
+    by ``angle`` about ``hinge_axis`` through each spanwise column's hinge
+    point (its first chordwise vertex of the slice).  Differentiable in
+    ``angle`` (torch autograd and torch.func.jvp), so a grid function built
+    from it gives the control-surface grid velocities of
+    ``UVLM.hg_dot_to_zeta_b_dot``."""
+    from .beam_adjoint import exp_so3
+
+    grid = to_device(grid)
+    ms = _index_rows(m_slice, grid.shape[0]).to(grid.device)
+    ns = _index_rows(n_slice, grid.shape[1]).to(grid.device)
+    ang = angle if isinstance(angle, torch.Tensor) else torch.as_tensor(float(angle), dtype=grid.dtype)
+    ang = ang.to(device=grid.device, dtype=grid.dtype).reshape(())
+    axis = torch.as_tensor(hinge_axis, dtype=grid.dtype, device=grid.device)
+    rmat = exp_so3(axis * ang)
+    sub = grid.index_select(0, ms).index_select(1, ns)  # (ms, ns, 3)
+    hinge = sub[:1]  # (1, ns, 3)
+    rot = torch.einsum("ij,hnj->hni", rmat, sub - hinge) + hinge
+    out = grid.clone()
+    out[ms[:, None], ns[None, :]] = rot
+    return out
+
+
+def cs_ang_to_cs_vel(cs_ang_t: dict, dt: float) -> dict:
+    """aero/utils.py:700-720: control velocities from the angle histories by
+    first-order backward differences (zero at the first step, which has no
+    predecessor)."""
+    out = {}
+    for k, v in cs_ang_t.items():
+        v = to_device(v)
+        d = torch.zeros_like(v)
+        if v.shape[0] > 1:
+            d[1:] = (v[1:] - v[:-1]) / float(dt)
+        out[k] = d
+    return out
+
+
+# ---------------------------------------------------------------------------
 # velocity closures the fused device path recognises
 # ---------------------------------------------------------------------------
 class FlowAt:

@@ -88,8 +88,13 @@ class UVLM:
             if d.shape != (self.grid_disc[i].n + 1,):
                 raise ValueError(f"dof_mapping of surface {i} must have shape ({self.grid_disc[i].n + 1},)")
             self._dof_host.append(d.astype(np.int32))
-        if grid_func is not None:
-            raise NotImplementedError("control-surface grid functions are not on the device path")
+        if grid_func is not None and not callable(grid_func):
+            raise TypeError("grid_func must be callable")
+        # control surfaces (aero/uvlm.py:333-339): grid_func(zeta_b0, **angles)
+        # returns the deflected local grids; None is the reference's identity
+        self.grid_func = grid_func
+        self.cs_ang0 = {}
+        self.cs_vel0 = {}
         if polars is not None and any(p is not None for p in polars):
             raise NotImplementedError("polar corrections are not on the device path")
         if not 0.0 <= polar_circulation_scale <= 1.0:
@@ -188,8 +193,9 @@ class UVLM:
             if dw.shape != (self.grid_disc[i].m_star,):
                 raise ValueError(f"delta_w must have shape ({self.grid_disc[i].m_star},)")
             self.delta_w.append(dw)
-        if reference_cs_angles:
-            raise NotImplementedError("control surfaces are not on the device path")
+        if reference_cs_angles is not None:
+            self.cs_ang0 = {k: to_device(v).reshape(()) for k, v in reference_cs_angles.items()}
+            self.cs_vel0 = {k: torch.zeros_like(v) for k, v in self.cs_ang0.items()}
         hg0 = to_device(hg0)
         if hg0.dim() != 3 or tuple(hg0.shape[1:]) != (4, 4):
             raise ValueError("hg0 must have shape (n_nodes, 4, 4)")
@@ -202,24 +208,65 @@ class UVLM:
                 raise TypeError("dt must be either a float or an Array scalar")
         self._dt = float(dt)
         self.flowfield = flowfield
-        self.zeta_b_ref = self.hg_to_zeta_b(hg0)
+        self.zeta_b_ref = self.hg_to_zeta_b(hg0, self.cs_ang0)
         self.zeta_w_ref = self.initialise_wake(self.zeta_b_ref)
 
-    def hg_to_zeta_b(self, hg_n, out=None):
-        """aero/uvlm.py:537-560."""
+    def _cs_grids(self, cs_ang_n):
+        """The local grids with the control surfaces deflected: zeta_b0, or
+        grid_func(zeta_b0, **cs_ang_n) (aero/uvlm.py:545-547)."""
+        if self.grid_func is None:
+            return self.zeta_b0
+        z = self.grid_func(self.zeta_b0, **(cs_ang_n or {}))
+        if len(z) != self.n_surf:
+            raise ValueError(f"grid_func must return {self.n_surf} grids, got {len(z)}")
+        out = []
+        for zi, z0 in zip(z, self.zeta_b0):
+            zi = to_device(zi)
+            if zi.shape != z0.shape:
+                raise ValueError(f"grid_func must keep the grid shape {tuple(z0.shape)}, got {tuple(zi.shape)}")
+            out.append(zi)
+        return out
+
+    def hg_to_zeta_b(self, hg_n, cs_ang_n=None, out=None):
+        """aero/uvlm.py:537-560: every vertex of the (deflected) local grid
+        moved by its beam node's frame.  cs_ang_n: {name: angle} (the
+        reference angles set by set_design_variables when None)."""
         hg_n = to_device(hg_n)
+        z0 = self._cs_grids(self.cs_ang0 if cs_ang_n is None else cs_ang_n)
         res = []
         for i in range(self.n_surf):
-            z, _ = K.grid_from_hg(hg_n, None, self._dof[i], self.zeta_b0[i], zeta=None if out is None else out[i])
+            z, _ = K.grid_from_hg(hg_n, None, self._dof[i], z0[i], zeta=None if out is None else out[i])
             res.append(z)
         return res
 
-    def hg_dot_to_zeta_b_dot(self, hg_n, hg_dot_n, out=None):
-        """aero/uvlm.py:562-621 (no control surfaces): hg_dot @ [zeta0; 1]."""
+    def hg_dot_to_zeta_b_dot(self, hg_n, hg_dot_n, cs_ang_n=None, cs_vel_n=None, out=None):
+        """aero/uvlm.py:562-621: hg_dot @ [zeta0_cs; 1] plus, with control
+        surfaces, R times the forward derivative of the grid function along
+        the control velocities (torch.func.jvp where the reference takes a
+        jax.jvp)."""
         hg_dot_n = to_device(hg_dot_n)
+        if cs_ang_n is None and cs_vel_n is None:
+            cs_ang_n, cs_vel_n = self.cs_ang0, self.cs_vel0
+        cs_ang_n = cs_ang_n or {}
+        cs_vel_n = cs_vel_n or {}
+        for key in set(cs_ang_n) | set(cs_vel_n):
+            if key not in cs_vel_n or key not in cs_ang_n:
+                raise ValueError(f"Missing pair of control angles and velocities for control surface key '{key}'")
+        angs = {k: to_device(v).reshape(()) for k, v in cs_ang_n.items()}
+        vels = {k: to_device(cs_vel_n[k]).reshape(()) for k in angs}
+        z0 = self._cs_grids(angs)
+        dz = None
+        if self.grid_func is not None and angs:
+            _, dz = torch.func.jvp(lambda a: self.grid_func(self.zeta_b0, **a), (angs,), (vels,))
         res = []
         for i in range(self.n_surf):
-            z, _ = K.grid_from_hg(hg_dot_n, None, self._dof[i], self.zeta_b0[i], zeta=None if out is None else out[i])
+            z, _ = K.grid_from_hg(hg_dot_n, None, self._dof[i], z0[i], zeta=None if out is None else out[i])
+            if dz is not None:
+                # the rotation of each node frame applied to the local grid velocity
+                hg_rot = to_device(hg_n).clone()
+                hg_rot[:, :3, 3] = 0.0
+                zr, _ = K.grid_from_hg(hg_rot, None, self._dof[i], to_device(dz[i]))
+                z += zr
             res.append(z)
         return res
 
@@ -596,22 +643,24 @@ class UVLM:
 
     def case_solve(self, case, i_ts, hg_n, hg_nm1, hg_dot_n, static, horseshoe, cs_ang_n=None, cs_ang_nm1=None,
                    cs_vel_n=None):
-        """aero/uvlm.py:1119-1215 (in place in ``case``)."""
-        if cs_ang_n or cs_ang_nm1 or cs_vel_n:
-            raise NotImplementedError("control surfaces are not on the device path")
+        """aero/uvlm.py:1119-1215 (in place in ``case``).  With a grid
+        function the step takes the general path (the fused one maps a single
+        undeflected local grid)."""
         q_nm1 = None
         if not static:
             q_nm1 = {k: [a[i_ts - 1] for a in getattr(case, k)] for k in ("gamma_b", "gamma_w", "gamma_b_dot", "zeta_w")}
             case.t[i_ts] = case.t[i_ts - 1] + self.dt if i_ts else 0.0
-        if i_ts > 0 and self._fast_step_ok(static, hg_nm1, hg_dot_n):
+        if i_ts > 0 and self.grid_func is None and self._fast_step_ok(static, hg_nm1, hg_dot_n):
             self._run_checked(lambda: self._dynamic_step(case, i_ts, hg_n, hg_nm1, hg_dot_n), False, i_ts)
             return case
         snap = case.snapshot(i_ts)
-        zeta_b_n = self.hg_to_zeta_b(hg_n if hg_n is not None else self.hg_ref, out=snap["zeta_b"])
+        cs_n = self.cs_ang0 if cs_ang_n is None else cs_ang_n
+        zeta_b_n = self.hg_to_zeta_b(hg_n if hg_n is not None else self.hg_ref, cs_n, out=snap["zeta_b"])
         zeta_b_dot_n = zeta_b_nm1 = None
         if hg_dot_n is not None:
-            zeta_b_dot_n = self.hg_dot_to_zeta_b_dot(hg_n, hg_dot_n, out=snap["zeta_b_dot"])
-            zeta_b_nm1 = self.hg_to_zeta_b(hg_nm1)
+            zeta_b_dot_n = self.hg_dot_to_zeta_b_dot(hg_n, hg_dot_n, cs_n, {} if cs_vel_n is None else cs_vel_n,
+                                                     out=snap["zeta_b_dot"])
+            zeta_b_nm1 = self.hg_to_zeta_b(hg_nm1, self.cs_ang0 if cs_ang_nm1 is None else cs_ang_nm1)
         out = None if static else snap
         res = self._run_checked(
             lambda: self.base_solve_from_grid(q_nm1, case.t[i_ts], zeta_b_n, zeta_b_nm1, zeta_b_dot_n, static,
@@ -623,8 +672,6 @@ class UVLM:
 
     def solve_static(self, hg=None, t=0.0, horseshoe=False, cs_ang=None):
         """aero/uvlm.py:1286-1333."""
-        if cs_ang:
-            raise NotImplementedError("control surfaces are not on the device path")
         if horseshoe and any(gd.m_star == 0 for gd in self.grid_disc):
             warn("Horseshoe wake requested but m_star == 0 for at least one surface, skipping horseshoe wake.")
         case = AeroCase.zeros(self.grid_disc, 1, self.hg_ref.device)
@@ -638,12 +685,15 @@ class UVLM:
         case.free_wake = self.free_wake
         case.gamma_dot_relaxation = self.gamma_dot_relaxation
         case.batch_size = self.batch_size
-        return self.case_solve(case, 0, hg, None, None, True, horseshoe)
+        cs = self.cs_ang0 if cs_ang is None else cs_ang
+        case.cs_ang = {k: to_device(v).reshape(1) for k, v in cs.items()}
+        case.cs_vel = {k: torch.zeros_like(v) for k, v in case.cs_ang.items()}
+        return self.case_solve(case, 0, hg, None, None, True, horseshoe, cs_ang_n=cs)
 
     def solve_prescribed_dynamic(self, init_case, hg_t, hg_dot_t, cs_ang_t=None, cs_vel_t=None, callback=None):
-        """aero/uvlm.py:1335-1406: one device step per time step."""
-        if cs_ang_t or cs_vel_t:
-            raise NotImplementedError("control surfaces are not on the device path")
+        """aero/uvlm.py:1335-1406: one device step per time step.
+        cs_ang_t, cs_vel_t: {name: (n_tstep,)} control angle and velocity
+        histories (empty when None, as in the reference)."""
         hg_t = to_device(hg_t)
         hg_dot_t = to_device(hg_dot_t)
         if hg_t.dim() != 4 or tuple(hg_t.shape[2:]) != (4, 4):
@@ -653,8 +703,21 @@ class UVLM:
                              f"{tuple(hg_t.shape)}")
         n_tstep = hg_t.shape[0]
         case = init_case.to_dynamic(0, n_tstep)
+        ang = {k: to_device(v) for k, v in (cs_ang_t or {}).items()}
+        vel = {k: to_device(v) for k, v in (cs_vel_t or {}).items()}
+        for k, v in list(ang.items()) + list(vel.items()):
+            if v.shape[0] != n_tstep:
+                raise ValueError(f"Inconsistent number of time steps for control surface input '{k}'")
+        if ang or vel:
+            case.cs_ang, case.cs_vel = ang, vel
         for i in range(1, n_tstep):
-            self.case_solve(case, i, hg_t[i], hg_t[i - 1], hg_dot_t[i], False, False)
+            if ang or vel or self.grid_func is not None:
+                self.case_solve(case, i, hg_t[i], hg_t[i - 1], hg_dot_t[i], False, False,
+                                cs_ang_n={k: v[i] for k, v in ang.items()},
+                                cs_ang_nm1={k: v[i - 1] for k, v in ang.items()},
+                                cs_vel_n={k: v[i] for k, v in vel.items()})
+            else:
+                self.case_solve(case, i, hg_t[i], hg_t[i - 1], hg_dot_t[i], False, False)
             if callback is not None:
                 callback(i, case)
         return case

@@ -493,6 +493,44 @@ def vect_product(hg, x):
     return np.einsum("jab,ijb->ija", hg[:, :3, :3], x) + hg[None, :, :3, 3]
 
 
+SMALL_ANG_THRESH = 1e-10  # utils/constants.py:7
+
+
+def exp_so3(w):
+    """algebra/so3.py:168-189: Rodrigues above the switch on |w|^2, exp_sum of
+    order 2 below it.  Complex arguments are allowed (w.w without conjugate),
+    so complex-step derivatives pass through."""
+    s = np.array([[0.0, -w[2], w[1]], [w[2], 0.0, -w[0]], [-w[1], w[0], 0.0]])
+    n2 = w[0] * w[0] + w[1] * w[1] + w[2] * w[2]
+    if np.real(n2) > SMALL_ANG_THRESH:
+        a = np.sqrt(n2)
+        return np.eye(3) + np.sin(a) / a * s + (1.0 - np.cos(a)) / (a * a) * (s @ s)
+    return np.eye(3) + s + (s @ s) / 2.0
+
+
+def add_control_surface(grid, angle, m_slice, n_slice, hinge_axis=(0.0, 1.0, 0.0)):
+    """aero/utils.py:42-84: the vertices grid[m_slice, n_slice] rotated by
+    angle about hinge_axis through each column's hinge point (the column's
+    first vertex of the slice)."""
+    grid = np.asarray(grid)
+    ms = np.arange(grid.shape[0])[m_slice] if isinstance(m_slice, slice) else np.atleast_1d(m_slice)
+    ns = np.arange(grid.shape[1])[n_slice] if isinstance(n_slice, slice) else np.atleast_1d(n_slice)
+    rmat = exp_so3(np.asarray(hinge_axis, float) * angle)
+    out = grid.astype(np.result_type(grid, angle, float)).copy()
+    for j in ns:
+        hinge = grid[ms[0], j]
+        out[ms, j] = (grid[ms, j] - hinge) @ rmat.T + hinge
+    return out
+
+
+def complex_step_jvp(fn, angles, velocities, h=1e-30):
+    """Forward derivative of fn(**angles) (a list of arrays) along
+    velocities, by one complex-step evaluation: Im f(a + i h v) / h, exact to
+    rounding for a function analytic in the angles."""
+    pert = {k: angles[k] + 1j * h * velocities[k] for k in angles}
+    return [np.imag(z) / h for z in fn(**pert)]
+
+
 @dataclass
 class OracleCase:
     """Snapshot history, the fields of AeroCase (aero/data_structures.py:43-615)."""
@@ -514,11 +552,13 @@ class OracleCase:
 
 class UVLMOracle:
     """Restatement of UVLM (aero/uvlm.py:166-1406) for the settings the
-    hot path covers: no control surfaces, no polars, identity grid_func."""
+    hot path covers: no polars; control surfaces through grid_func, whose
+    forward derivative is taken by complex step (the grid function must then
+    accept complex angles, as add_control_surface does)."""
 
     def __init__(self, grid_shapes, dof_mapping, kernel="epsilon", free_wake=False,
                  gamma_dot_relaxation=0.7, mirror_point=None, mirror_normal=None,
-                 variable_wake_disc=False):
+                 variable_wake_disc=False, grid_func=None):
         self.grid_disc = [tuple(int(v) for v in g) for g in grid_shapes]
         self.n_surf = len(self.grid_disc)
         dm = dof_mapping if isinstance(dof_mapping, (list, tuple)) else [dof_mapping]
@@ -531,24 +571,47 @@ class UVLMOracle:
         self.mirror_point = None if mirror_point is None else np.asarray(mirror_point, float)
         self.mirror_normal = None if mirror_normal is None else np.asarray(mirror_normal, float) / np.linalg.norm(mirror_normal)
         self.keep_aic = False
+        self.grid_func = grid_func
+        self.cs_ang0 = {}
 
     # aero/uvlm.py:387-472
-    def set_design_variables(self, dt, flowfield, zeta_b0, hg0, delta_w=None):
+    def set_design_variables(self, dt, flowfield, zeta_b0, hg0, delta_w=None, reference_cs_angles=None):
         self.dt = float(dt)
         self.flowfield = flowfield
         self.zeta_b0 = [np.asarray(z, float) for z in (zeta_b0 if isinstance(zeta_b0, (list, tuple)) else [zeta_b0])]
         self.hg_ref = np.asarray(hg0, float)
         self.delta_w = list(delta_w) if isinstance(delta_w, (list, tuple)) else [delta_w] * self.n_surf
-        self.zeta_b_ref = self.hg_to_zeta_b(self.hg_ref)
+        if reference_cs_angles is not None:
+            self.cs_ang0 = {k: float(v) for k, v in reference_cs_angles.items()}
+        self.zeta_b_ref = self.hg_to_zeta_b(self.hg_ref, self.cs_ang0)
         self.zeta_w_ref = self.initialise_wake(self.zeta_b_ref)
 
-    # aero/uvlm.py:537-560
-    def hg_to_zeta_b(self, hg):
-        return [vect_product(hg[self.dof_mapping[i]], self.zeta_b0[i]) for i in range(self.n_surf)]
+    def _cs_grids(self, cs_ang):
+        if self.grid_func is None:
+            return self.zeta_b0
+        return list(self.grid_func(self.zeta_b0, **(cs_ang or {})))
 
-    # aero/uvlm.py:562-621 (no control surfaces)
-    def hg_dot_to_zeta_b_dot(self, hg_dot):
-        return [vect_product(hg_dot[self.dof_mapping[i]], self.zeta_b0[i]) for i in range(self.n_surf)]
+    # aero/uvlm.py:537-560
+    def hg_to_zeta_b(self, hg, cs_ang=None):
+        z0 = self._cs_grids(self.cs_ang0 if cs_ang is None else cs_ang)
+        return [vect_product(hg[self.dof_mapping[i]], z0[i]) for i in range(self.n_surf)]
+
+    # aero/uvlm.py:562-621: hg_dot [zeta0_cs; 1] + R d(zeta0_cs)/dt, the last
+    # term the forward derivative of grid_func along the control velocities
+    def hg_dot_to_zeta_b_dot(self, hg_dot, hg=None, cs_ang=None, cs_vel=None):
+        cs_ang = self.cs_ang0 if cs_ang is None else cs_ang
+        cs_vel = {} if cs_vel is None else cs_vel
+        for key in set(cs_ang) | set(cs_vel):
+            if key not in cs_vel or key not in cs_ang:
+                raise ValueError(f"Missing pair of control angles and velocities for control surface key '{key}'")
+        z0 = self._cs_grids(cs_ang)
+        out = [vect_product(hg_dot[self.dof_mapping[i]], z0[i]) for i in range(self.n_surf)]
+        if self.grid_func is not None and cs_ang:
+            dz = complex_step_jvp(lambda **a: self.grid_func(self.zeta_b0, **a), cs_ang, cs_vel)
+            for i in range(self.n_surf):
+                rot = hg[self.dof_mapping[i], :3, :3]
+                out[i] = out[i] + np.einsum("njk,mnk->mnj", rot, dz[i])
+        return out
 
     # aero/uvlm.py:662-689
     def initialise_wake(self, zeta_b):
@@ -670,10 +733,10 @@ class UVLMOracle:
             case.aic.append(out["aic"])
 
     # aero/uvlm.py:1286-1333
-    def solve_static(self, hg=None, t=0.0, horseshoe=False):
+    def solve_static(self, hg=None, t=0.0, horseshoe=False, cs_ang=None):
         case = self._empty_case(1)
         case.t[0] = t
-        zeta_b = self.hg_to_zeta_b(self.hg_ref if hg is None else np.asarray(hg, float))
+        zeta_b = self.hg_to_zeta_b(self.hg_ref if hg is None else np.asarray(hg, float), cs_ang)
         out = self.base_solve_from_grid(None, t, zeta_b, None, None, True, horseshoe)
         if horseshoe:
             # the stored wake is the regular one (aero/uvlm.py:1329-1331)
@@ -682,7 +745,7 @@ class UVLMOracle:
         return case
 
     # aero/uvlm.py:1335-1406 with case_solve :1119-1215
-    def solve_prescribed_dynamic(self, init_case, hg_t, hg_dot_t, callback=None):
+    def solve_prescribed_dynamic(self, init_case, hg_t, hg_dot_t, callback=None, cs_ang_t=None, cs_vel_t=None):
         hg_t = np.asarray(hg_t, float)
         hg_dot_t = np.asarray(hg_dot_t, float)
         n_tstep = hg_t.shape[0]
@@ -700,9 +763,17 @@ class UVLMOracle:
                 gamma_b_dot=[g[i - 1] for g in case.gamma_b_dot],
                 zeta_w=[z[i - 1] for z in case.zeta_w],
             )
-            zb = self.hg_to_zeta_b(hg_t[i])
-            zbd = self.hg_dot_to_zeta_b_dot(hg_dot_t[i])
-            zbm1 = self.hg_to_zeta_b(hg_t[i - 1])
+            if cs_ang_t is None and cs_vel_t is None:
+                zb = self.hg_to_zeta_b(hg_t[i])
+                zbd = self.hg_dot_to_zeta_b_dot(hg_dot_t[i], hg_t[i])
+                zbm1 = self.hg_to_zeta_b(hg_t[i - 1])
+            else:
+                a_n = {k: float(v[i]) for k, v in (cs_ang_t or {}).items()}
+                a_nm1 = {k: float(v[i - 1]) for k, v in (cs_ang_t or {}).items()}
+                v_n = {k: float(v[i]) for k, v in (cs_vel_t or {}).items()}
+                zb = self.hg_to_zeta_b(hg_t[i], a_n)
+                zbd = self.hg_dot_to_zeta_b_dot(hg_dot_t[i], hg_t[i], a_n, v_n)
+                zbm1 = self.hg_to_zeta_b(hg_t[i - 1], a_nm1)
             out = self.base_solve_from_grid(q, case.t[i], zb, zbm1, zbd, False, False)
             self._store(case, i, out, False)
             if callback is not None:
