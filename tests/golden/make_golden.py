@@ -116,6 +116,95 @@ def cfg1_free_wake():
     return out
 
 
+CFG2_STEPS = 21  # the initial state and 20 coupled time steps
+CFG2_STEPS_KEPT = (5, 10, 20)
+CFG2_NEWTON, CFG2_FSI = 5, 3  # fixed iteration counts (every tolerance 0)
+
+
+def _cfg2_model():
+    """The coupled gust workload in the oracle: 81-node cantilever along y
+    (80 two-node elements, two per stiffness station, 40 seeded stations),
+    16 x 80 lattice mirrored about y = 0 with 300 prescribed wake rows,
+    U = 10 at 5 deg, 1-cos gust of 0.05 U over 2 x 5 chords starting 11
+    chords upstream, dt = c / (16 U) -- the model bench.py times -- with
+    CFG2_NEWTON Newton and CFG2_FSI FSI iterations per solve."""
+    sys.path.insert(0, str(HERE.parent.parent))
+    import bench
+    from oracle import beam_oracle as B
+
+    m, n, ms, stations = bench.CPL_M, bench.CPL_N, bench.CPL_M_STAR, bench.CPL_STATIONS
+    nn = n + 1
+    beam = B.BeamOracle(nn, B.chain(nn), np.array((0.0, 0.0, 1.0)), k_cs_index=np.arange(n) * stations // n,
+                        settings=B.ConvergenceSettings(CFG2_NEWTON, 0.0, 0.0, 0.0, 0.0))
+    coords = np.zeros((nn, 3))
+    coords[:, 1] = np.linspace(0.0, 10.0, nn)
+    k, mm = bench.coupled_design(stations=stations)
+    beam.set_design_variables(coords, k, mm)
+    uv = O.UVLMOracle([(m, n, ms)], np.arange(nn), mirror_point=np.zeros(3),
+                      mirror_normal=np.array((0.0, 1.0, 0.0)))
+    a = np.deg2rad(5.0)
+    u_inf = bench.CPL_U * np.array((np.cos(a), 0.0, np.sin(a)))
+    gust = O.OneMinusCosine(u_inf, 1.225, True, gust_length=5.0, gust_amplitude=0.05 * bench.CPL_U,
+                            gust_x0=np.array((-11.0, 0.0, 0.0)))
+    uv.set_design_variables(1.0 / (m * bench.CPL_U), gust, O.make_rectangular_grid(m, n, 1.0, 0.25), beam.hg0)
+    return B.CoupledOracle(beam, uv, fsi_settings=B.ConvergenceSettings(CFG2_FSI, 0.0, 0.0, 0.0, 0.0))
+
+
+def _cfg2_run(alt):
+    """One oracle run; ``alt`` moves the beam's sin and cos by one ulp and
+    sums every pair sum over the filaments in reverse order."""
+    from oracle import beam_oracle as B
+    from oracle import c_oracle
+
+    def ps(t, y0, y1, g, kernel="epsilon", chunk=256):
+        assert kernel == "epsilon"
+        if alt:
+            return c_oracle.pair_sum_eps(t, y0[::-1], y1[::-1], g[::-1])
+        return c_oracle.pair_sum_eps(t, y0, y1, g)
+
+    O.set_pair_sum(ps)
+    try:
+        if alt:
+            with B.ulp_shifted_trig():
+                cw = _cfg2_model()
+                st = cw.static_solve(np.arange(6))
+                init = cw.uvlm.solve_static(st["hg"])
+                d = cw.dynamic_solve(st["hg"], init, np.arange(6), CFG2_STEPS)
+        else:
+            cw = _cfg2_model()
+            st = cw.static_solve(np.arange(6))
+            init = cw.uvlm.solve_static(st["hg"])
+            d = cw.dynamic_solve(st["hg"], init, np.arange(6), CFG2_STEPS)
+    finally:
+        O.set_pair_sum(None)
+    keep = list(CFG2_STEPS_KEPT)
+    return {"static_hg": st["hg"], "static_gamma_b": st["aero"]["gamma_b"][0],
+            "hg": d["hg"][keep], "gamma_b": d["aero"].gamma_b[0][keep], "f_ext_aero": d["f_ext_aero"][keep],
+            "zeta_w": d["aero"].zeta_w[0][CFG2_STEPS - 1][:CFG2_STEPS + 1]}
+
+
+def cfg2_coupled():
+    """The coupled gust workload at its full size for 20 steps (static
+    Aitken FSI, then generalised-alpha FSI steps), with fixed iteration
+    counts so that both sides take the same iterations.  Run twice, the
+    second time with the beam's sin and cos moved by one ulp and the pair
+    sums summed in reverse order: the difference ("floor_*") is how closely
+    two correct FP64 evaluations of the reference's formulas agree on this
+    case.  Kept: the static equilibrium (frames, circulation), and at
+    CFG2_STEPS_KEPT the node frames, circulation and the beam's aero loads,
+    and the wake rows that have been shed by step 20.  About 15 minutes on 8
+    cores."""
+    ref = _cfg2_run(False)
+    alt = _cfg2_run(True)
+    out = dict(ref)
+    out["steps_kept"] = np.array(CFG2_STEPS_KEPT)
+    out["iterations"] = np.array((CFG2_NEWTON, CFG2_FSI))
+    for key in ("hg", "gamma_b", "f_ext_aero"):
+        out[f"floor_{key}"] = np.array([_rel(alt[key][k], ref[key][k]) for k in range(len(CFG2_STEPS_KEPT))])
+    for key in ("static_hg", "static_gamma_b", "zeta_w"):
+        out[f"floor_{key}"] = np.array(_rel(alt[key], ref[key]))
+    return out
+
 if __name__ == "__main__":
     which = sys.argv[1:] or ["small", "cfg1"]
     if "small" in which:
@@ -123,5 +212,7 @@ if __name__ == "__main__":
         np.savez_compressed(HERE / "free_wake_8x16.npz", **small_free_wake())
     if "cfg1" in which:
         np.savez_compressed(HERE / "cfg1_free_wake_16x64.npz", **cfg1_free_wake())
+    if "cfg2" in which:
+        np.savez_compressed(HERE / "cfg2_coupled_16x80.npz", **cfg2_coupled())
     for f in sorted(HERE.glob("*.npz")):
         print(f.name, f.stat().st_size)
