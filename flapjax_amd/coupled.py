@@ -22,7 +22,7 @@ import torch
 from . import _lib as L
 from .beam import BeamStructure, ConvergenceSettings, ConvergenceStatus, StructureCase
 from .data_structures import AeroCase
-from .utils import to_device
+from .utils import cs_ang_to_cs_vel, to_device
 
 F64 = torch.float64
 
@@ -151,17 +151,36 @@ class CoupledAeroelastic:
                       cs_vel_t=None) -> AeroelasticCase:
         if thrust_t:
             raise NotImplementedError("thrust forces are not on the device path")
-        if cs_ang_t or cs_vel_t:
-            raise NotImplementedError("control surfaces are not on the device path")
         st_ = self.structure
         pre = st_.make_prescribed_dofs_tuple(prescribed_dofs)
         dt = self.aero.dt
         t = np.arange(n_tstep) * dt + t_init
+        # control inputs (coupled/coupled.py:322-342): checked, else the
+        # initial case's angles held over the run; velocities by backward
+        # differences when not given
+        if cs_ang_t is not None:
+            for key in cs_ang_t:
+                if np.shape(cs_ang_t[key])[0] != n_tstep:
+                    raise ValueError(f"Inconsistent number of time steps for control surface input cs_ang_t['{key}']")
+                if cs_vel_t is not None and key not in cs_vel_t:
+                    raise ValueError(f"Missing control velocity for control surface input cs_vel_t['{key}']")
+            cs_ang_t = {k: to_device(v) for k, v in cs_ang_t.items()}
+        elif init_case is not None:
+            cs_ang_t = {k: to_device(v).reshape(-1)[0].expand(n_tstep).clone() for k, v in init_case.aero.cs_ang.items()}
+        if cs_vel_t is None and cs_ang_t is not None:
+            cs_vel_t = cs_ang_to_cs_vel(cs_ang_t, dt)
+        elif cs_vel_t is not None:
+            cs_vel_t = {k: to_device(v) for k, v in cs_vel_t.items()}
+        cs_ang_t = cs_ang_t or {}
+        cs_vel_t = cs_vel_t or {}
+        controls = bool(cs_ang_t or cs_vel_t) or self.aero.grid_func is not None
         if init_case is None:
             init_case = self.reference_configuration(pre, use_f_ext_follower=f_ext_follower is not None,
                                                      use_f_ext_dead=f_ext_dead is not None, t_init=t_init)
         struct0 = init_case.structure.to_dynamic()
         aero_case = init_case.aero.to_dynamic(0, n_tstep)
+        if cs_ang_t or cs_vel_t:
+            aero_case.cs_ang, aero_case.cs_vel = dict(cs_ang_t), dict(cs_vel_t)
         shp = (n_tstep, st_.n_nodes, 6)
 
         def series(f, name):
@@ -184,7 +203,13 @@ class CoupledAeroelastic:
 
             @staticmethod
             def solve(i, hg_n, hg_nm1, hg_dot):
-                outer.aero.case_solve(aero_case, i, hg_n, hg_nm1, hg_dot, False, False)
+                if controls:
+                    outer.aero.case_solve(aero_case, i, hg_n, hg_nm1, hg_dot, False, False,
+                                          cs_ang_n={k: v[i] for k, v in cs_ang_t.items()},
+                                          cs_ang_nm1={k: v[i - 1] for k, v in cs_ang_t.items()},
+                                          cs_vel_n={k: v[i] for k, v in cs_vel_t.items()})
+                else:
+                    outer.aero.case_solve(aero_case, i, hg_n, hg_nm1, hg_dot, False, False)
                 return outer._project(aero_case, i, hg_n, unsteady)
 
             @staticmethod

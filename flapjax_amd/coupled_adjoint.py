@@ -91,6 +91,8 @@ class DifferentiableAeroelastic:
             raise NotImplementedError("the coupled adjoint takes a single lifting surface")
         if any(d is not None for d in aero.delta_w):
             raise NotImplementedError("the coupled adjoint takes a uniform wake discretisation")
+        if aero.grid_func is not None:
+            raise NotImplementedError("the coupled adjoint takes no control surfaces")
         gd = aero.grid_disc[0]
         if gd.m_star < 1:
             raise NotImplementedError("the coupled adjoint needs a wake")

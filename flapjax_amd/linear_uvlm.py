@@ -190,6 +190,8 @@ class LinearUVLM:
         self.wake_upwash = bool(wake_upwash)
         if any(dw is not None for dw in getattr(case, "delta_w", [None])):
             raise NotImplementedError("variable wake discretisation is not on the linear device path")
+        if getattr(case, "grid_func", None) is not None:
+            raise NotImplementedError("control-surface inputs are not on the linear device path")
         self.case = case
         self.dt = case.dt
         self._reference = reference

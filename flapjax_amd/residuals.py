@@ -81,6 +81,8 @@ class UVLMResiduals:
     def __init__(self, uvlm):
         if any(dw is not None for dw in getattr(uvlm, "delta_w", [None])):
             raise NotImplementedError("variable wake discretisation is not on the residual device path")
+        if getattr(uvlm, "grid_func", None) is not None:
+            raise NotImplementedError("control surfaces are not on the residual device path")
         self.uvlm = uvlm
         self.kernel = uvlm.kernel
         self.mirror = uvlm.mirror

@@ -716,9 +716,17 @@ class CoupledOracle:
         return dict(hg=hg, aero=aero, iters=st.i_iter)
 
     # coupled/coupled.py:307-433 + structure/beam.py:2319-2861
-    def dynamic_solve(self, init_hg, init_aero_case, prescribed_dofs, n_tstep, load_steps=1):
-        """init_aero_case: an OracleCase snapshot at t0 (e.g. from solve_static)."""
+    def dynamic_solve(self, init_hg, init_aero_case, prescribed_dofs, n_tstep, load_steps=1, cs_ang_t=None,
+                      cs_vel_t=None):
+        """init_aero_case: an OracleCase snapshot at t0 (e.g. from solve_static).
+        cs_ang_t, cs_vel_t: control angle and velocity histories {name:
+        (n_tstep,)} (coupled/coupled.py:322-342: the reference angles held and
+        backward-difference velocities when not given)."""
         uv, beam = self.uvlm, self.beam
+        if cs_ang_t is None:
+            cs_ang_t = {k: np.full(n_tstep, v) for k, v in uv.cs_ang0.items()}
+        if cs_vel_t is None:
+            cs_vel_t = {k: np.concatenate(([0.0], np.diff(np.asarray(v, float)) / uv.dt)) for k, v in cs_ang_t.items()}
         case = uv._empty_case(n_tstep)
         for key in ("zeta_b", "zeta_b_dot", "zeta_w", "gamma_b", "gamma_b_dot", "gamma_w", "f_steady",
                     "f_unsteady", "alpha", "c", "nc"):
@@ -741,8 +749,11 @@ class CoupledOracle:
                 case.t[i] = case.t[i - 1] + uv.dt
                 q = dict(gamma_b=[g[i - 1] for g in case.gamma_b], gamma_w=[g[i - 1] for g in case.gamma_w],
                          gamma_b_dot=[g[i - 1] for g in case.gamma_b_dot], zeta_w=[z[i - 1] for z in case.zeta_w])
-                out = uv.base_solve_from_grid(q, case.t[i], uv.hg_to_zeta_b(hg_n), uv.hg_to_zeta_b(hg_nm1),
-                                              uv.hg_dot_to_zeta_b_dot(hg_dot), False, False)
+                a_n = {k: float(v[i]) for k, v in cs_ang_t.items()}
+                a_nm1 = {k: float(v[i - 1]) for k, v in cs_ang_t.items()}
+                v_n = {k: float(v[i]) for k, v in cs_vel_t.items()}
+                out = uv.base_solve_from_grid(q, case.t[i], uv.hg_to_zeta_b(hg_n, a_n), uv.hg_to_zeta_b(hg_nm1, a_nm1),
+                                              uv.hg_dot_to_zeta_b_dot(hg_dot, hg_n, a_n, v_n), False, False)
                 uv._store(case, i, out, False)
                 return outer._forces(out, hg_n, outer.include_unsteady)
 

@@ -116,3 +116,56 @@ def test_control_velocities_and_pairs(dev):
                             O.make_rectangular_grid(M, N, 1.0, 0.25), hg_t[0], reference_cs_angles={"aileron": 0.0})
     with pytest.raises(ValueError, match="Missing pair of control angles and velocities"):
         uv.hg_dot_to_zeta_b_dot(hg_t[1], hgd_t[1], {"aileron": 0.1}, {})
+
+
+def test_coupled_aileron_vs_oracle(dev):
+    """Aitken static FSI at the reference aileron angle, then FSI time steps
+    with a moving aileron (velocities by backward differences on both sides,
+    coupled/coupled.py:322-342): the device beam + UVLM follow the oracle's
+    iterates, held to 10x the oracle's one-ulp-trig floor."""
+    from oracle import beam_oracle as B
+    from tests.test_gpu_beam import _beam_from_oracle, _held
+
+    import flapjax_amd as fj
+    from flapjax_amd import UVLM, GridDiscretisation
+    from flapjax_amd.beam import ConvergenceSettings
+    from flapjax_amd.coupled import CoupledAeroelastic
+    from flapjax_amd.utils import add_control_surface
+
+    fm, fn = slice(1, None), slice(0, 2)
+    n_t = 8
+    ocw = B.cantilever_wing(m=2, n_nodes=5, m_star=6, k_diag=(1e6, 1e6, 1e6, 1e3, 5e3, 1e5), u_inf=(10.0, 0.0, 1.0))
+    ou = ocw.uvlm
+    ang = 0.05 + 0.1 * np.sin(3.0 * ou.dt * np.arange(n_t))
+
+    def oracle_run():
+        ou.grid_func = lambda x0, *, aileron: [O.add_control_surface(x0[0], aileron, fm, fn)]
+        ou.set_design_variables(ou.dt, ou.flowfield, ou.zeta_b0, ocw.beam.hg0, reference_cs_angles={"aileron": ang[0]})
+        st_ = ocw.static_solve(np.arange(6))
+        init = ou.solve_static(st_["hg"])
+        return st_, ocw.dynamic_solve(st_["hg"], init, np.arange(6), n_t, cs_ang_t={"aileron": ang})
+
+    ref_st, ref = oracle_run()
+    with B.ulp_shifted_trig():
+        pst, pdyn = oracle_run()
+    ob = ocw.beam
+    beam = _beam_from_oracle(ob)
+    m, n, ms = ou.grid_disc[0]
+    uv = UVLM([GridDiscretisation(m, n, ms)], np.arange(ob.n_nodes), mirror_point=np.zeros(3),
+              mirror_normal=np.array((0.0, 1.0, 0.0)),
+              grid_func=lambda x0, *, aileron: [add_control_surface(x0[0], aileron, fm, fn)])
+    s = ocw.fsi_settings
+    cw = CoupledAeroelastic(beam, uv, ConvergenceSettings(s.max_n_iter, s.rel_disp_tol, s.abs_disp_tol,
+                                                          s.rel_force_tol, s.abs_force_tol))
+    ff = ou.flowfield
+    cw.set_design_variables(ob.x0, ob.k_cs, ob.m_cs, None, ou.dt, fj.Constant(ff.u_inf, ff.rho, True), ou.zeta_b0[0],
+                            cs_angles_reference={"aileron": ang[0]})
+    st = cw.static_solve(np.arange(6))
+    assert st.structure.iters["fsi"] == ref_st["iters"]
+    _held("static hg", st.structure.hg.cpu().numpy(), ref_st["hg"], pst["hg"])
+    dyn = cw.dynamic_solve(st, np.arange(6), n_t, cs_ang_t={"aileron": ang})
+    assert dyn.structure.iters["fsi"] == ref["info"]["fsi_iters"]
+    _held("hg", dyn.structure.hg.cpu().numpy(), ref["hg"], pdyn["hg"])
+    _held("gamma_b", dyn.aero.gamma_b[0].cpu().numpy(), ref["aero"].gamma_b[0], pdyn["aero"].gamma_b[0])
+    _held("zeta_b_dot", dyn.aero.zeta_b_dot[0].cpu().numpy(), ref["aero"].zeta_b_dot[0], pdyn["aero"].zeta_b_dot[0])
+    _held("f_ext_aero", dyn.structure.f_ext_aero.cpu().numpy(), ref["f_ext_aero"], pdyn["f_ext_aero"])
