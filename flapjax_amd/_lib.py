@@ -135,6 +135,8 @@ _SIGS = {
     "fj_beam_update": ([c_int, c_int, c_void_p, c_int, c_void_p, c_double, c_double, c_double] + [c_void_p] * 7,
                        c_int),
     "fj_beam_subnorm": ([c_void_p, c_void_p, c_int, c_void_p, c_void_p], c_int),
+    "fj_beam_ga": ([c_int, c_int] + [c_double] * 5 + [c_void_p] * 11, c_int),
+    "fj_beam_fsi_norms": ([c_int] + [c_void_p] * 7, c_int),
     "fj_beam_newton_dynamic": ([c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                 c_double, c_double, c_double, c_void_p, c_int, c_int, c_int, c_void_p]
                                + [c_void_p] * 11, c_int),
@@ -144,6 +146,7 @@ _SIGS = {
 }
 
 BEAM_NODE_UPDATE, BEAM_NODE_REL, BEAM_NODE_RATE = 0, 1, 2
+GA_PREDICT, GA_ALPHA, GA_FINISH, GA_FORCE = 0, 1, 2, 3
 
 
 class FjConvergence(ctypes.Structure):

@@ -667,8 +667,26 @@ class BeamStructure:
                 struct_iters.append(self._newton_dynamic(hg[i - 1], phi_a, v_a, vd_a, ffol, fdead, sd, n_s, ti,
                                                          bufs))
 
+        lib = L.lib()
+        n6 = 6 * nn
+        am, g_, b_ = ti.alpha_m, ti.gamma, ti.beta
+
+        def ga(op, ins, outs):
+            # the generalised-alpha bookkeeping in one launch (fj_beam_ga)
+            p = [L.ptr(x) for x in ins] + [None] * (6 - len(ins))
+            o = [L.ptr(x) for x in outs] + [None] * (4 - len(outs))
+            L.check(lib.fj_beam_ga(op, n6, am, af, g_, b_, ti.dt, *p, *o, L.stream()), "fj_beam_ga")
+
+        phi_n = torch.empty((nn, 6), dtype=F64, device=dev)
+        v_n = torch.empty_like(phi_n)
+        phi_old = torch.empty_like(phi_n)
+        phi = torch.empty_like(phi_n)
+        nrm_dev = torch.empty(5, dtype=F64, device=dev)
+        nrm_host = torch.empty(5, dtype=F64).pin_memory()
         for i in range(1, n_tstep):
-            phi_a, v_a, vd_a = ti.predict_alpha(v[i - 1], vd[i - 1], acc[i - 1])
+            # predict_q + calculate_q_alpha (structure/time_integration.py:86-150)
+            phi_a, v_a, vd_a = (torch.empty((nn, 6), dtype=F64, device=dev) for _ in range(3))
+            ga(L.GA_PREDICT, (v[i - 1], vd[i - 1], acc[i - 1]), (phi_a, v_a, vd_a))
             if aero is None:
                 load_step_loop(i, phi_a, v_a, vd_a, None)
             else:
@@ -677,29 +695,25 @@ class BeamStructure:
                 f_prev = f_nm1
                 fst = ConvergenceStatus(fsi_settings)
                 while not fst.get_status():
-                    phi_n = (phi_a / (1.0 - af)).contiguous()
-                    v_n = ((v_a - af * v[i - 1]) / (1.0 - af)).contiguous()
+                    ga(L.GA_ALPHA, (phi_a, v_a, v[i - 1]), (phi_n, v_n, phi_old))
                     hg_n = torch.empty((nn, 4, 4), dtype=F64, device=dev)
                     self._nodes(L.BEAM_NODE_UPDATE, hg[i - 1], x=phi_n, hg_out=hg_n)
                     hg_dot = self.make_hg_dot(hg_n, v_n)
-                    f_n = aero.solve(i, hg_n, hg[i - 1], hg_dot)
-                    f_al = ti.f_alpha(f_nm1, f_n)
-                    steps = [(wk * f_al).contiguous() for wk in w]
-                    phi_old = phi_a.clone()
+                    f_n = aero.solve(i, hg_n, hg[i - 1], hg_dot).contiguous()
+                    f_al = torch.empty((nn, 6), dtype=F64, device=dev)
+                    ga(L.GA_FORCE, (f_nm1, f_n), (f_al,))
+                    steps = [f_al] if load_steps == 1 else [(wk * f_al).contiguous() for wk in w]
                     load_step_loop(i, phi_a, v_a, vd_a, steps)
-                    dd = (phi_old - phi_a).reshape(-1)[sd]
-                    df = (f_al - f_prev).reshape(-1)[sd]
-                    nrm = torch.stack((dd.norm(), phi_a.reshape(-1)[sd].norm(), df.norm(),
-                                       f_al.reshape(-1)[sd].abs().max() if n_s else dd.new_zeros(()),
-                                       torch.isnan(dd).any().to(F64))).cpu().numpy()
+                    L.check(lib.fj_beam_fsi_norms(n_s, L.ptr(sd), L.ptr(phi_old), L.ptr(phi_a), L.ptr(f_al),
+                                                  L.ptr(f_prev), L.ptr(nrm_dev), L.stream()), "fj_beam_fsi_norms")
+                    nrm_host.copy_(nrm_dev, non_blocking=True)
+                    torch.cuda.current_stream().synchronize()
+                    nrm = nrm_host.numpy()
                     fst.update_norms(*nrm[:4], nrm[4] != 0.0)
                     f_prev = f_al
                 fsi_iters.append(fst.i_iter)
             # calculate_q_n_from_q_alpha + calculate_a_n (structure/time_integration.py:53-84)
-            phi = (phi_a / (1.0 - af)).contiguous()
-            v[i] = (v_a - af * v[i - 1]) / (1.0 - af)
-            vd[i] = (vd_a - af * vd[i - 1]) / (1.0 - af)
-            acc[i] = 1.0 / (1.0 - ti.alpha_m) * ((1.0 - af) * vd[i] + af * vd[i - 1] - ti.alpha_m * acc[i - 1])
+            ga(L.GA_FINISH, (phi_a, v_a, vd_a, v[i - 1], vd[i - 1], acc[i - 1]), (phi, v[i], vd[i], acc[i]))
             self._nodes(L.BEAM_NODE_UPDATE, hg[i - 1], x=phi, hg_out=hg[i])
             if aero is not None:
                 f_aero_g[i] = aero.project(i, hg[i])

@@ -906,3 +906,156 @@ extern "C" int fj_beam_newton_dynamic(const fj_beam_model* model, int n_iter, in
   }
   return FJ_OK;
 }
+
+// ---- generalised-alpha bookkeeping of the coupled time step ------------------
+// predict_q + calculate_q_alpha (structure/time_integration.py:86-150), the
+// alpha-point state of an FSI iteration and calculate_q_n_from_q_alpha +
+// calculate_a_n (:53-84), the alpha-point aero force and the FSI convergence
+// norms (structure/beam.py:2612-2725): one launch each where the host would
+// issue a dozen array operations.  Every product and sum is rounded on its
+// own, in the order of the reference's array expressions (no contraction).
+#define FJ_GA_THREADS 256
+
+__global__ void k_beam_ga_predict(int n, double am, double af, double g, double b, double dt,
+                                  const double* __restrict__ v, const double* __restrict__ vd,
+                                  const double* __restrict__ a, double* __restrict__ phi_a,
+                                  double* __restrict__ v_a, double* __restrict__ vd_a) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double a_n = (af * vd[i] - am * a[i]) / (1.0 - am);
+  const double phi_n = dt * v[i] + (dt * dt) * ((0.5 - b) * a[i] + b * a_n);
+  const double v_n = v[i] + ((1.0 - g) * dt) * a[i] + (g * dt) * a_n;
+  const double vd_n = ((1.0 - am) * a_n + am * a[i] - af * vd[i]) / (1.0 - af);
+  phi_a[i] = (1.0 - af) * phi_n;
+  v_a[i] = (1.0 - af) * v_n + af * v[i];
+  vd_a[i] = (1.0 - af) * vd_n + af * vd[i];
+}
+
+// the step-n state from the alpha point: phi_n, v_n (and a copy of phi_a)
+__global__ void k_beam_ga_alpha(int n, double af, const double* __restrict__ phi_a,
+                                const double* __restrict__ v_a, const double* __restrict__ v_prev,
+                                double* __restrict__ phi_n, double* __restrict__ v_n,
+                                double* __restrict__ phi_copy) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  phi_n[i] = phi_a[i] / (1.0 - af);
+  v_n[i] = (v_a[i] - af * v_prev[i]) / (1.0 - af);
+  if (phi_copy) phi_copy[i] = phi_a[i];
+}
+
+__global__ void k_beam_ga_finish(int n, double am, double af, const double* __restrict__ phi_a,
+                                 const double* __restrict__ v_a, const double* __restrict__ vd_a,
+                                 const double* __restrict__ v_prev, const double* __restrict__ vd_prev,
+                                 const double* __restrict__ a_prev, double* __restrict__ phi,
+                                 double* __restrict__ v, double* __restrict__ vd, double* __restrict__ a) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  phi[i] = phi_a[i] / (1.0 - af);
+  const double vi = (v_a[i] - af * v_prev[i]) / (1.0 - af);
+  const double vdi = (vd_a[i] - af * vd_prev[i]) / (1.0 - af);
+  v[i] = vi;
+  vd[i] = vdi;
+  a[i] = (1.0 / (1.0 - am)) * ((1.0 - af) * vdi + af * vd_prev[i] - am * a_prev[i]);
+}
+
+__global__ void k_beam_fsi_force(int n, double af, const double* __restrict__ f_nm1,
+                                 const double* __restrict__ f_n, double* __restrict__ f_al) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  f_al[i] = (1.0 - af) * f_n[i] + af * f_nm1[i];
+}
+
+// out = {|dphi|, |phi_a|, |df|, max |f_al|, any NaN in dphi} over the solve
+// DOFs, dphi = phi_old - phi_a, df = f_al - f_prev (one workgroup, a fixed
+// summation order)
+__global__ __launch_bounds__(FJ_GA_THREADS) void k_beam_fsi_norms(
+    int n_solve, const int32_t* __restrict__ sd, const double* __restrict__ phi_old,
+    const double* __restrict__ phi_a, const double* __restrict__ f_al, const double* __restrict__ f_prev,
+    double* __restrict__ out) {
+  __shared__ double red[4][FJ_GA_THREADS];
+  __shared__ int nan_any;
+  const int tid = threadIdx.x;
+  if (tid == 0) nan_any = 0;
+  double s_dd = 0.0, s_phi = 0.0, s_df = 0.0, mx = 0.0;
+  int nan_here = 0;
+  for (int k = tid; k < n_solve; k += FJ_GA_THREADS) {
+    const int j = sd[k];
+    const double dd = phi_old[j] - phi_a[j];
+    const double df = f_al[j] - f_prev[j];
+    s_dd += dd * dd;
+    s_phi += phi_a[j] * phi_a[j];
+    s_df += df * df;
+    mx = fmax(mx, fabs(f_al[j]));
+    nan_here |= (dd != dd) ? 1 : 0;
+  }
+  __syncthreads();
+  if (nan_here) atomicOr(&nan_any, 1);
+  red[0][tid] = s_dd;
+  red[1][tid] = s_phi;
+  red[2][tid] = s_df;
+  red[3][tid] = mx;
+  __syncthreads();
+  for (int s = FJ_GA_THREADS / 2; s > 0; s >>= 1) {
+    if (tid < s) {
+      red[0][tid] += red[0][tid + s];
+      red[1][tid] += red[1][tid + s];
+      red[2][tid] += red[2][tid + s];
+      red[3][tid] = fmax(red[3][tid], red[3][tid + s]);
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    out[0] = sqrt(red[0][0]);
+    out[1] = sqrt(red[1][0]);
+    out[2] = sqrt(red[2][0]);
+    out[3] = red[3][0];
+    out[4] = nan_any ? 1.0 : 0.0;
+  }
+}
+
+extern "C" int fj_beam_ga(int op, int n, double am, double af, double g, double b, double dt,
+                          const double* p0, const double* p1, const double* p2, const double* p3,
+                          const double* p4, const double* p5, double* o0, double* o1, double* o2,
+                          double* o3, void* stream) {
+  if (n < 0) return FJ_ERR_ARG;
+  if (n == 0) return FJ_OK;
+  const dim3 grid(fj_cdiv(n, FJ_GA_THREADS)), block(FJ_GA_THREADS);
+  hipStream_t st = (hipStream_t)stream;
+  switch (op) {
+    case FJ_GA_PREDICT:  // p0 v, p1 v_dot, p2 a -> o0 phi_a, o1 v_a, o2 v_dot_a
+      if (!p0 || !p1 || !p2 || !o0 || !o1 || !o2) return FJ_ERR_ARG;
+      hipLaunchKernelGGL(k_beam_ga_predict, grid, block, 0, st, n, am, af, g, b, dt, p0, p1, p2, o0, o1, o2);
+      break;
+    case FJ_GA_ALPHA:  // p0 phi_a, p1 v_a, p2 v_prev -> o0 phi_n, o1 v_n, o2 copy of phi_a (optional)
+      if (!p0 || !p1 || !p2 || !o0 || !o1) return FJ_ERR_ARG;
+      hipLaunchKernelGGL(k_beam_ga_alpha, grid, block, 0, st, n, af, p0, p1, p2, o0, o1, o2);
+      break;
+    case FJ_GA_FINISH:  // p0 phi_a, p1 v_a, p2 v_dot_a, p3 v_prev, p4 v_dot_prev, p5 a_prev -> o0 phi, o1 v, o2 v_dot, o3 a
+      if (!p0 || !p1 || !p2 || !p3 || !p4 || !p5 || !o0 || !o1 || !o2 || !o3) return FJ_ERR_ARG;
+      hipLaunchKernelGGL(k_beam_ga_finish, grid, block, 0, st, n, am, af, p0, p1, p2, p3, p4, p5, o0, o1, o2, o3);
+      break;
+    case FJ_GA_FORCE:  // p0 f_nm1, p1 f_n -> o0 f_alpha
+      if (!p0 || !p1 || !o0) return FJ_ERR_ARG;
+      hipLaunchKernelGGL(k_beam_fsi_force, grid, block, 0, st, n, af, p0, p1, o0);
+      break;
+    default:
+      return FJ_ERR_ARG;
+  }
+  FJ_CHECK_LAUNCH();
+  return FJ_OK;
+}
+
+extern "C" int fj_beam_fsi_norms(int n_solve, const int32_t* solve_dofs, const double* phi_old,
+                                 const double* phi_a, const double* f_al, const double* f_prev, double* out,
+                                 void* stream) {
+  if (n_solve < 0 || !out || (n_solve > 0 && (!solve_dofs || !phi_old || !phi_a || !f_al || !f_prev)))
+    return FJ_ERR_ARG;
+  hipLaunchKernelGGL(k_beam_fsi_norms, dim3(1), dim3(FJ_GA_THREADS), 0, (hipStream_t)stream, n_solve, solve_dofs,
+                     phi_old, phi_a, f_al, f_prev, out);
+  FJ_CHECK_LAUNCH();
+  return FJ_OK;
+}

@@ -571,6 +571,30 @@ int fj_beam_newton_dynamic(const fj_beam_model* model, int n_iter, int reset, co
 /* out = |x[idx]| (the static solve's total-displacement norm). */
 int fj_beam_subnorm(const double* x, const int32_t* idx, int n, double* out, void* stream);
 
+#define FJ_GA_PREDICT 0 /* p0 v, p1 v_dot, p2 a (step n-1) -> o0 phi_a, o1 v_a, o2 v_dot_a:
+                           predict_q + calculate_q_alpha, structure/time_integration.py:86-150 */
+#define FJ_GA_ALPHA 1   /* p0 phi_a, p1 v_a, p2 v_prev -> o0 phi_n, o1 v_n, o2 copy of phi_a (or NULL):
+                           the step-n state of an FSI iteration, structure/beam.py:2612-2725 */
+#define FJ_GA_FINISH 2  /* p0 phi_a, p1 v_a, p2 v_dot_a, p3 v_prev, p4 v_dot_prev, p5 a_prev ->
+                           o0 phi, o1 v, o2 v_dot, o3 a: calculate_q_n_from_q_alpha + calculate_a_n,
+                           structure/time_integration.py:53-84 */
+#define FJ_GA_FORCE 3   /* p0 f_nm1, p1 f_n -> o0 (1 - af) f_n + af f_nm1 (the alpha-point force) */
+
+/* Generalised-alpha bookkeeping of a time step over n values, one launch per
+ * op (am, af, gamma, beta, dt: the integrator's constants; unused ones are
+ * ignored).  Each product and sum is rounded on its own, in the order of the
+ * reference's array expressions (structure/time_integration.py:53-150). */
+int fj_beam_ga(int op, int n, double am, double af, double g, double b, double dt, const double* p0,
+               const double* p1, const double* p2, const double* p3, const double* p4, const double* p5,
+               double* o0, double* o1, double* o2, double* o3, void* stream);
+
+/* The FSI convergence norms of a coupled iteration (structure/beam.py:
+ * 2700-2720) over the solve DOFs, one workgroup in a fixed order:
+ * out = {|phi_old - phi_a|, |phi_a|, |f_al - f_prev|, max |f_al|, 1 if
+ * phi_old - phi_a holds a NaN else 0}. */
+int fj_beam_fsi_norms(int n_solve, const int32_t* solve_dofs, const double* phi_old, const double* phi_a,
+                      const double* f_al, const double* f_prev, double* out, void* stream);
+
 #define FJ_BEAM_NODE_UPDATE 0 /* hg_out = hg exp(x) (structure/beam.py:1845-1856); t_out = T(x)
                                  and h_out = log(hg_ref^-1 hg_out) when not NULL */
 #define FJ_BEAM_NODE_REL 1    /* h_out = log(hg^-1 hg_ref) (algebra/se3.py:hg_to_d) */
