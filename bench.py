@@ -299,7 +299,7 @@ def coupled_bench(precision, steps=CPL_STEPS):
     return {"static_s": round(t1 - t0, 3), "static_fsi_iters": int(st.structure.iters["fsi"]),
             "dynamic_s": round(dyn_s, 3), "ms_per_step": round(1e3 * dyn_s / (steps - 1), 3),
             "fsi_iters_mean": round(float(fsi.mean()), 3), "fsi_iters_max": int(fsi.max()),
-            "newton_iters_per_step": round(len(dyn.structure.iters["struct"]) / (steps - 1), 3),
+            "newton_iters_per_step": round(sum(dyn.structure.iters["struct"]) / (steps - 1), 3),
             "pairs_per_fsi_iter": int(pairs_it),
             "aero_gpairs_per_s": round(pairs_it * fsi.sum() / dyn_s / 1e9, 3),
             "cases_per_s": round(1.0 / (t2 - t0), 4), "tip_z_min_max": [round(float(tip.min()), 6),

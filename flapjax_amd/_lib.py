@@ -145,7 +145,7 @@ _SIGS = {
                                 c_void_p, c_void_p], c_int),
 }
 
-BEAM_NODE_UPDATE, BEAM_NODE_REL, BEAM_NODE_RATE = 0, 1, 2
+BEAM_NODE_UPDATE, BEAM_NODE_REL, BEAM_NODE_RATE, BEAM_NODE_LOCAL, BEAM_NODE_GLOBAL = 0, 1, 2, 3, 4
 GA_PREDICT, GA_ALPHA, GA_FINISH, GA_FORCE = 0, 1, 2, 3
 
 

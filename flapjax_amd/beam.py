@@ -513,17 +513,24 @@ class BeamStructure:
                              prescribed_dofs=pre, iters=iters)
 
     @staticmethod
+    def _frame_map(op, f, hg):
+        lead = torch.broadcast_shapes(f.shape[:-1], hg.shape[:-2])
+        f = f.expand(lead + (6,)).contiguous()
+        hg = hg.expand(lead + (4, 4)).contiguous()
+        out = torch.empty(lead + (6,), dtype=F64, device=f.device)
+        L.check(L.lib().fj_beam_nodes(op, hg.numel() // 16, L.ptr(hg), L.ptr(f), None, None, None, L.ptr(out),
+                                      L.stream()), "fj_beam_nodes")
+        return out
+
+    @staticmethod
     def _to_local(f, hg):
         """make_f_dead_ext (structure/beam.py:1387-1395): R^T f per node."""
-        r = hg[..., :3, :3]
-        return torch.cat((torch.einsum("...ji,...j->...i", r, f[..., :3]),
-                          torch.einsum("...ji,...j->...i", r, f[..., 3:])), -1).contiguous()
+        return BeamStructure._frame_map(L.BEAM_NODE_LOCAL, f, hg)
 
     @staticmethod
     def _to_global(f, hg):
-        r = hg[..., :3, :3]
-        return torch.cat((torch.einsum("...ij,...j->...i", r, f[..., :3]),
-                          torch.einsum("...ij,...j->...i", r, f[..., 3:])), -1).contiguous()
+        """R f per node (the inverse of ``_to_local``)."""
+        return BeamStructure._frame_map(L.BEAM_NODE_GLOBAL, f, hg)
 
     # ----------------------------------------------------------------- dynamic
     #: Newton iterations enqueued before the first status read of a loop
@@ -536,11 +543,12 @@ class BeamStructure:
         return L.FjConvergence(0 if s.max_n_iter is None else int(s.max_n_iter), f(s.rel_disp_tol),
                                f(s.abs_disp_tol), f(s.rel_force_tol), f(s.abs_force_tol))
 
-    def _newton_dynamic_device(self, hg_prev, phi_a, v_a, vd_a, f_fol, f_dead, sd, n_s, ti, bufs):
+    def _newton_dynamic_device(self, hg_prev, phi_a, v_a, vd_a, f_fol, f_dead, sd, n_s, ti, bufs, on_batch=None):
         """The Newton loop with the stop test on the device
         (fj_beam_newton_dynamic): iterations are enqueued in batches and the
         host reads one status word per batch; the iterates are those of the
-        host-driven loop."""
+        host-driven loop.  ``on_batch`` enqueues work behind every batch
+        that the same synchronisation waits for."""
         hg_u, t_phi = bufs
         A, r, fa, x = self._sys_buffers(n_s, hg_prev.device)
         if not hasattr(self, "_nstatus"):
@@ -560,18 +568,22 @@ class BeamStructure:
             L.check(rc, "fj_beam_newton_dynamic")
             reset = 0
             self._nstatus_host.copy_(self._nstatus, non_blocking=True)
+            if on_batch is not None:
+                on_batch()
             torch.cuda.current_stream().synchronize()
             it, done = int(self._nstatus_host[0]), int(self._nstatus_host[1])
             if done:
                 return it
             batch = 2
 
-    def _newton_dynamic(self, hg_prev, phi_a, v_a, vd_a, f_fol, f_dead, sd, n_s, ti, bufs):
+    def _newton_dynamic(self, hg_prev, phi_a, v_a, vd_a, f_fol, f_dead, sd, n_s, ti, bufs, on_batch=None):
         """The Newton loop of one load step at the alpha point
         (structure/beam.py:2170-2316, 2727-2792); phi_a, v_a, vd_a are updated
-        in place."""
+        in place.  ``on_batch`` (optional) enqueues work that reads the
+        final iterate: on return it has run behind the last iteration and
+        the stream has been synchronised."""
         if n_s and self.device_newton and K.band_fits(n_s, self._bandwidth, self._bandwidth):
-            return self._newton_dynamic_device(hg_prev, phi_a, v_a, vd_a, f_fol, f_dead, sd, n_s, ti, bufs)
+            return self._newton_dynamic_device(hg_prev, phi_a, v_a, vd_a, f_fol, f_dead, sd, n_s, ti, bufs, on_batch)
         hg_u, t_phi = bufs
         lib = L.lib()
         st = ConvergenceStatus(self.struct_convergence_settings)
@@ -584,6 +596,9 @@ class BeamStructure:
                                        L.ptr(fa), L.ptr(self._norms), L.stream()), "fj_beam_update")
             nv = self._read_norms()
             st.update_norms(nv[0], nv[1], nv[2], nv[3], nv[4] != 0.0)
+        if on_batch is not None:
+            on_batch()
+            torch.cuda.current_stream().synchronize()
         return st.i_iter
 
     def dynamic_solve(self, init_state, prescribed_dofs, n_tstep: int, dt, f_ext_follower=None, f_ext_dead=None,
@@ -654,7 +669,7 @@ class BeamStructure:
         struct_iters, fsi_iters = [], []
         af = ti.alpha_f
 
-        def load_step_loop(i, phi_a, v_a, vd_a, f_aero_steps):
+        def load_step_loop(i, phi_a, v_a, vd_a, f_aero_steps, on_last=None):
             for i_ls in range(load_steps):
                 ffol = None if ffa is None else (w[i_ls] * ffa[i]).contiguous()
                 fdead = None
@@ -665,7 +680,7 @@ class BeamStructure:
                 elif f_aero_steps is not None:
                     fdead = f_aero_steps[i_ls].contiguous()
                 struct_iters.append(self._newton_dynamic(hg[i - 1], phi_a, v_a, vd_a, ffol, fdead, sd, n_s, ti,
-                                                         bufs))
+                                                         bufs, on_last if i_ls == load_steps - 1 else None))
 
         lib = L.lib()
         n6 = 6 * nn
@@ -683,6 +698,15 @@ class BeamStructure:
         phi = torch.empty_like(phi_n)
         nrm_dev = torch.empty(5, dtype=F64, device=dev)
         nrm_host = torch.empty(5, dtype=F64).pin_memory()
+        fsi = {}
+
+        def fsi_norms():
+            # the FSI norms of this iteration, read at the Newton loop's last
+            # status synchronisation
+            L.check(lib.fj_beam_fsi_norms(n_s, L.ptr(sd), L.ptr(fsi["phi_old"]), L.ptr(fsi["phi_a"]),
+                                          L.ptr(fsi["f_al"]), L.ptr(fsi["f_prev"]), L.ptr(nrm_dev), L.stream()),
+                    "fj_beam_fsi_norms")
+            nrm_host.copy_(nrm_dev, non_blocking=True)
         for i in range(1, n_tstep):
             # predict_q + calculate_q_alpha (structure/time_integration.py:86-150)
             phi_a, v_a, vd_a = (torch.empty((nn, 6), dtype=F64, device=dev) for _ in range(3))
@@ -703,11 +727,8 @@ class BeamStructure:
                     f_al = torch.empty((nn, 6), dtype=F64, device=dev)
                     ga(L.GA_FORCE, (f_nm1, f_n), (f_al,))
                     steps = [f_al] if load_steps == 1 else [(wk * f_al).contiguous() for wk in w]
-                    load_step_loop(i, phi_a, v_a, vd_a, steps)
-                    L.check(lib.fj_beam_fsi_norms(n_s, L.ptr(sd), L.ptr(phi_old), L.ptr(phi_a), L.ptr(f_al),
-                                                  L.ptr(f_prev), L.ptr(nrm_dev), L.stream()), "fj_beam_fsi_norms")
-                    nrm_host.copy_(nrm_dev, non_blocking=True)
-                    torch.cuda.current_stream().synchronize()
+                    fsi.update(phi_old=phi_old, phi_a=phi_a, f_al=f_al, f_prev=f_prev)
+                    load_step_loop(i, phi_a, v_a, vd_a, steps, fsi_norms)
                     nrm = nrm_host.numpy()
                     fst.update_norms(*nrm[:4], nrm[4] != 0.0)
                     f_prev = f_al

@@ -482,6 +482,15 @@ __global__ void k_beam_nodes(int op, int n, const double* __restrict__ hg, const
     double r[16];
     fj_hg_inv_mul(g, hg_ref + 16 * i, r);
     fj_log_se3(r, h_out + 6 * i);
+  } else if (op == FJ_BEAM_NODE_LOCAL || op == FJ_BEAM_NODE_GLOBAL) {
+    // R^T f or R f on the force and the moment of a node's 6-vector
+    const double* f = x + 6 * i;
+    const bool tr = op == FJ_BEAM_NODE_LOCAL;
+    for (int h = 0; h < 6; h += 3)
+      for (int p = 0; p < 3; ++p) {
+        const double r0 = tr ? g[p] : g[4 * p], r1 = tr ? g[4 + p] : g[4 * p + 1], r2 = tr ? g[8 + p] : g[4 * p + 2];
+        h_out[6 * i + h + p] = r0 * f[h] + r1 * f[h + 1] + r2 * f[h + 2];
+      }
   } else {
     // H tilde(v): [[R S(w), R u], [0, 0]]
     const double* vv = x + 6 * i;
@@ -805,6 +814,8 @@ extern "C" int fj_beam_nodes(int op, int n_nodes, const double* hg, const double
     if (!hg_ref || !h_out) return FJ_ERR_ARG;
   } else if (op == FJ_BEAM_NODE_RATE) {
     if (!x || !hg_out) return FJ_ERR_ARG;
+  } else if (op == FJ_BEAM_NODE_LOCAL || op == FJ_BEAM_NODE_GLOBAL) {
+    if (!x || !h_out) return FJ_ERR_ARG;
   } else {
     return FJ_ERR_ARG;
   }

@@ -1260,26 +1260,8 @@ __global__ __launch_bounds__(64) void k_band_solve(const double* __restrict__ A,
   if (lane == 0 && info) *info = first_zero;
 }
 
-// The same solve with the bandwidths fixed at compile time (kl = ku = 11:
-// any chain of two-node beam elements).  KL zero rows below the matrix let
-// every column step read and update rows k .. k + KL unconditionally, and
-// the constant row stride turns every band address into an immediate
-// offset.  Zero padding rows are never chosen as pivots (a zero is never
-// strictly larger than the current maximum) and never reach the solution.
-//
-// One wave runs the column steps, so a step costs its chain of dependent
-// instructions: every LDS read of the step (column k for all lanes, each
-// update lane's column k + j, x[k .. k + KL]) is issued first, since none
-// depends on the pivot; the pivot is found by a four-level tree over the
-// KL + 1 candidates (the left operand always holds the lower rows and the
-// right one wins only when strictly larger, which keeps idamax's first
-// index); and everything that depends on the pivot row is a select on the
-// (uniform) pivot offset, except the pivot row's entry of each update
-// lane's column, one LDS read once the pivot is known; a step without an
-// exchange (most of the beam's columns) takes a branch with none of them.
-// The
-// arithmetic is dgbtf2's to the bit: the same pivots, the same reciprocal
-// scaling, the same FMA per updated entry.
+// First maximal |a| of a column (idamax) by a tree: the left operand always
+// holds the lower rows and the right one wins only when strictly larger.
 template <int KL>
 struct FjBandPick {
   // first maximal |a[lo .. lo + cnt)|: signed value and offset
@@ -1308,134 +1290,218 @@ __device__ __forceinline__ double fj_readlane_f64(double v, int l) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
-// Ordering of LDS accesses between the lanes of the one wave that runs a
-// kernel: a wave's LDS instructions execute in issue order, so a later read
-// sees an earlier write of any lane once the compiler keeps the order (a
+// Ordering of LDS accesses between the lanes of one wave: a wave's LDS
+// instructions execute in issue order, so a later read sees an earlier write
+// of any lane of the same wave once the compiler keeps the order (a
 // wavefront-scope fence; no wait and no barrier instruction is needed).
 __device__ __forceinline__ void fj_wave_lds_order() {
   __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
 }
 
+// The same solve with the bandwidths fixed at compile time (kl = ku = 11:
+// any chain of two-node beam elements), spread over three waves.  KL zero
+// rows below the matrix let every column step read and update rows
+// k .. k + KL unconditionally (a zero row is never chosen as a pivot and
+// never reaches the solution).
+//
+// Elimination: 16-lane groups, group g < KL owning columns k + 1 + g and
+// k + 12 + g of the step's rank-1 update (lane i: row k + i; lane 0 the
+// exchanged row k), group KL the right-hand side; one wave per SIMD, since a
+// wave sharing its SIMD reached each barrier 100-200 cycles late.  A column
+// step is one barrier: each lane reads its entry of column k, and a 16-lane
+// DPP maximum of the high words of |a| below the diagonal tells whether the
+// diagonal is idamax's choice (every entry below it smaller in magnitude);
+// then, with the reciprocal of the diagonal issued meanwhile, the rank-1
+// update is one FMA per lane and column and one store.  A tie or a larger
+// high word (NaN included, and a zero diagonal) takes the exact search (the
+// tree above over the whole column) and the general exchange.  A group lies
+// inside one wave and within a step only the group of a column reads or
+// writes that column, so the reads of the pivot row before its rows are
+// rewritten need no barrier.
+//
+// Substitution: one wave with x in registers (lane l, register q: row
+// 64 q + l); each row is two lane reads, a division and one FMA per lane,
+// with U's entries read one pointer step a row.  The arithmetic is dgbtf2's
+// and dgbtrs's to the bit: the same pivot rows, the same scaling by the
+// reciprocal pivot, the same FMA per entry in the same order, dtbsv's
+// division per row.
+#define FJ_BAND_FIXED_GROUPS 12
+#define FJ_BAND_FIXED_THREADS (16 * FJ_BAND_FIXED_GROUPS)
+
 template <int KL, int KU>
-__global__ __launch_bounds__(64) void k_band_solve_fixed(const double* __restrict__ A, int n,
-                                                         int lda, const double* __restrict__ b,
-                                                         double* __restrict__ x,
-                                                         int32_t* __restrict__ info,
-                                                         const int32_t* __restrict__ guard) {
+__global__ __launch_bounds__(FJ_BAND_FIXED_THREADS) void k_band_solve_fixed(const double* __restrict__ A, int n,
+                                                                      int lda, const double* __restrict__ b,
+                                                                      double* __restrict__ x,
+                                                                      int32_t* __restrict__ info,
+                                                                      const int32_t* __restrict__ guard) {
   if (guard && guard[1] != 0) return;
   constexpr int W = 2 * KL + KU + 1;
-  constexpr int D = W - 1;  // from (row i, column c) to (row i + 1, column c)
-  constexpr int JM = KL + KU;  // the farthest column an update reaches
-  static_assert(JM < 64, "band shape");
+  constexpr int D = W - 1;
+  constexpr int JM = KL + KU;
+  static_assert(KL == KU && KL + 1 == FJ_BAND_FIXED_GROUPS && KL < 16, "band shape");
+  constexpr int QMAX = (FJ_BAND_LDS_DOUBLES / (W + 2) + 63) / 64;  // x registers per lane
   __shared__ double s[FJ_BAND_LDS_DOUBLES];
-  const int lane = threadIdx.x;
+  const int tid = threadIdx.x;
   const int rows = n + KL;
-  double* xs = s + (size_t)rows * W;
+  double* xs = s + (size_t)rows * W;  // b, then y = L^-1 P b
+  double* dg = xs + rows;             // U's diagonal
   const int total = rows * W;
-  for (int base = 0; base < total; base += 64 * 32) {
-    double v[32];
+  for (int base = 0; base < total; base += FJ_BAND_FIXED_THREADS * 16) {
+    double v[16];
 #pragma unroll
-    for (int u = 0; u < 32; ++u) {
-      const int q = base + 64 * u + lane;
+    for (int u = 0; u < 16; ++u) {
+      const int q = base + FJ_BAND_FIXED_THREADS * u + tid;
       const int i = q / W, t = q - i * W;
       const int c = i - KL + t;
       v[u] = (q < total && i < n && c >= 0 && c < n && t <= KL + KU) ? A[(size_t)i * lda + c] : 0.0;
     }
 #pragma unroll
-    for (int u = 0; u < 32; ++u) {
-      const int q = base + 64 * u + lane;
+    for (int u = 0; u < 16; ++u) {
+      const int q = base + FJ_BAND_FIXED_THREADS * u + tid;
       if (q < total) s[q] = v[u];
     }
   }
-  for (int i = lane; i < rows; i += 64) xs[i] = i < n ? b[i] : 0.0;
+  for (int i = tid; i < rows; i += FJ_BAND_FIXED_THREADS) xs[i] = i < n ? b[i] : 0.0;
+  const int g = tid >> 4, i = tid & 15;
+  const bool act = i <= KL;        // lanes 0 .. KL of a group: rows k .. k + KL
+  const int ic = act ? i : 0;      // idle lanes read row k and write nothing
+  const bool updg = g < KL, bgrp = g == KL;
+  const int j1 = g + 1, j2 = g + 1 + KL;  // update groups: columns k + j1 and k + j2
+  const int c1 = updg ? j1 : 1, c2 = updg ? j2 : 1;
+  // Each lane reads r and t and, without an exchange, stores r - l t over
+  // r: update lanes (row k + i, column k + j) and (row k, column k + j) for
+  // their two columns, b lanes y[k + i] and y[k]; the pointers advance by
+  // rs doubles a column.
+  double* const rb1 = updg ? s + KL + ic * D + c1 : (bgrp ? xs + ic : s + KL);
+  double* const tb1 = updg ? s + KL + c1 : (bgrp ? xs : s + KL);
+  double* const rb2 = updg ? s + KL + ic * D + c2 : s + KL;
+  double* const tb2 = updg ? s + KL + c2 : s + KL;
+  const int rs = bgrp ? 1 : W;
+  const bool st1 = (updg && act && i != 0) || (bgrp && act), st2 = updg && act && i != 0;
+  const bool b0 = bgrp && i == 0;  // stores U's diagonal instead
+  const int jl1 = updg ? j1 : 0, jl2 = updg ? j2 : 0;
   int first_zero = 0;
   int ju = 0;
-  // update lanes 1 .. JM own column k + lane; lanes 1 .. KL also own row
-  // k + lane of column k and of b (addresses of the other lanes are clamped
-  // into range and their results discarded)
-  const bool upd = lane >= 1 && lane <= JM;
-  const int jl = upd ? lane : 1;
-  const bool ml = lane >= 1 && lane <= KL;
-  const int il = ml ? lane : 0;
   __syncthreads();
   for (int k = 0; k < n; ++k) {
     double* colk = s + (size_t)k * W + KL;  // (row k, column k)
-    double a[KL + 1], r[KL + 1];
-#pragma unroll
-    for (int i = 0; i <= KL; ++i) a[i] = colk[i * D];
-#pragma unroll
-    for (int i = 0; i <= KL; ++i) r[i] = colk[jl + i * D];  // (row k + i, column k + jl)
-    const double al = colk[il * D];                         // (row k + il, column k)
-    const double xl = xs[k + il];
-    double piv;
-    int ipv;
-    FjBandPick<KL>::template run<0, KL + 1>(a, piv, ipv);
-    const int ip = __builtin_amdgcn_readfirstlane(ipv);
-    if (piv != 0.0) {
-      const int p = k + ip;
-      ju = max(ju, min(p + KU, n - 1));
+    double* const pr1 = rb1 + (size_t)k * rs;
+    double* const pr2 = rb2 + (size_t)k * rs;
+    const double a0 = colk[0];              // (row k, column k)
+    const double al = colk[ic * D];         // (row k + i, column k)
+    const double r1 = *pr1, t1 = tb1[(size_t)k * rs];
+    const double r2 = *pr2, t2 = tb2[(size_t)k * rs];
+    // The diagonal is idamax's choice when every entry below it is smaller
+    // in magnitude, which holds when its high word (sign cleared) is
+    // smaller: lane i of every group holds entry i, so a 16-lane DPP
+    // maximum of those words (entries 1 .. KL; the rest count 0) decides
+    // it, and the same in every group.  A tie or a larger word (NaN
+    // included, and a zero diagonal) takes the exact search over the
+    // column.  The reciprocal of the diagonal is issued meanwhile.
+    const unsigned hw = (unsigned)((unsigned long long)__double_as_longlong(al) >> 32) & 0x7FFFFFFFu;
+    unsigned m = (i >= 1 && i <= KL) ? hw : 0u;
+    m = fj_dpp_max_u32<0x111, 0xf>(m);  // row_shr:1 .. 8: lane 15 holds the row maximum
+    m = fj_dpp_max_u32<0x112, 0xf>(m);
+    m = fj_dpp_max_u32<0x114, 0xf>(m);
+    m = fj_dpp_max_u32<0x118, 0xf>(m);
+    const unsigned below = __builtin_amdgcn_readlane(m, 15);
+    const unsigned h0 =
+        __builtin_amdgcn_readfirstlane((unsigned)((unsigned long long)__double_as_longlong(a0) >> 32) & 0x7FFFFFFFu);
+    double rp = 1.0 / a0;
+    if (below < h0) {
+      // no exchange: row k is the pivot row (t)
+      ju = max(ju, min(k + KU, n - 1));
       const int nj = ju - k;
-      const double rp = 1.0 / piv;
-      if (ip == 0) {
-        // no exchange (the common case on the beam's systems): the rank-1
-        // update of rows k + 1 .. k + KL of column k + jl, and of b
-        if (upd && lane <= nj) {
-          const double u = r[0];
+      const double li = al * rp;
+      const double v1 = r1 - li * t1, v2 = r2 - li * t2;
+      if (st1 && jl1 <= nj) *(b0 ? dg + k : pr1) = b0 ? a0 : v1;
+      if (st2 && jl2 <= nj) *pr2 = v2;
+    } else {
+      double a[KL + 1];
 #pragma unroll
-          for (int i = 1; i <= KL; ++i) colk[jl + i * D] = r[i] - (a[i] * rp) * u;
-        }
-        const double bk = fj_readlane_f64(xl, 0);
-        if (ml) {
-          const double lme = al * rp;
-          colk[lane * D] = lme;
-          xs[k + lane] = xl - lme * bk;
-        }
-      } else {
-        if (upd && lane <= nj) {
-          // exchange rows k and p of column k + jl, then the rank-1 update
-          // of rows k + 1 .. k + KL (row p now holds row k's entry)
-          const double u = colk[jl + ip * D];  // (row p, column k + jl)
-          const double t = r[0];
-          colk[jl] = u;
-#pragma unroll
-          for (int i = 1; i <= KL; ++i) {
-            const double li = ((i == ip) ? a[0] : a[i]) * rp;
-            const double ri = (i == ip) ? t : r[i];
-            colk[jl + i * D] = ri - li * u;
+      for (int q = 0; q <= KL; ++q) a[q] = colk[q * D];
+      double piv;
+      int ipv;
+      FjBandPick<KL>::template run<0, KL + 1>(a, piv, ipv);
+      const int ip = __builtin_amdgcn_readfirstlane(ipv);
+      if (piv != 0.0) {
+        if (ip != 0) rp = 1.0 / piv;
+        const int p = k + ip;
+        ju = max(ju, min(p + KU, n - 1));
+        const int nj = ju - k;
+        const double li = ((i == ip) ? a0 : al) * rp;
+        if (updg && act) {
+          // rows k and p exchanged in the lane's columns, then the update
+          const double u1 = colk[ip * D + j1], u2 = colk[ip * D + j2];  // (row p, column k + j)
+          fj_wave_lds_order();
+          if (i == 0) {
+            if (ip != 0) {
+              if (j1 <= nj) colk[j1] = u1;
+              if (j2 <= nj) colk[j2] = u2;
+            }
+          } else {
+            if (j1 <= nj) colk[i * D + j1] = ((i == ip) ? t1 : r1) - li * u1;
+            if (j2 <= nj) colk[i * D + j2] = ((i == ip) ? t2 : r2) - li * u2;
+          }
+        } else if (bgrp && act) {
+          const double bk = xs[p];
+          fj_wave_lds_order();
+          if (i == 0) {
+            xs[k] = bk;
+            dg[k] = piv;
+          } else {
+            xs[k + i] = ((i == ip) ? t1 : r1) - li * bk;
           }
         }
-        // column k: pivot and multipliers; b: exchange and update
-        const double xk = fj_readlane_f64(xl, 0);
-        const double bk = fj_readlane_f64(xl, ip);
-        if (lane == 0) {
-          colk[0] = piv;
-          xs[k] = bk;
-        }
-        if (ml) {
-          const double lme = ((lane == ip) ? a[0] : al) * rp;
-          colk[lane * D] = lme;
-          const double bi = (lane == ip) ? xk : xl;
-          xs[k + lane] = bi - lme * bk;
-        }
+      } else {
+        if (first_zero == 0) first_zero = k + 1;
+        if (b0) dg[k] = 0.0;
       }
-    } else if (first_zero == 0) {
-      first_zero = k + 1;
     }
-    fj_wave_lds_order();
+    __syncthreads();
   }
-  for (int j = n - 1; j >= 0; --j) {
-    const double xj = xs[j];
-    double v = xj;
-    if (xj != 0.0) v = xj / s[(size_t)j * W + KL];
-    fj_wave_lds_order();
-    if (xj != 0.0) {
-      const int kk = min(KL + KU, j);
-      if (lane >= 1 && lane <= kk) xs[j - lane] -= v * s[(size_t)(j - lane) * W + (lane + KL)];
-      if (lane == 0) xs[j] = v;
+  if (tid >= 64) return;
+  // U x = y (dtbsv, upper, no transpose), rows from the last
+  const int lane = tid;
+  double y[QMAX];
+#pragma unroll
+  for (int q = 0; q < QMAX; ++q) {
+    const int row = 64 * q + lane;
+    y[q] = row < n ? xs[row] : 0.0;
+  }
+#pragma unroll
+  for (int q = QMAX - 1; q >= 0; --q) {
+    if (64 * q >= n) continue;
+    const int rq = 64 * q + lane;  // this lane's row in register q (and rq - 64 in q - 1)
+    const double dq = dg[rq < n ? rq : 0];  // the block's diagonal, read by lane
+    // U's entries (row rq, column jj) and (row rq - 64, column jj) sit at
+    // rq D + KL + jj and rq D + KL + jj - 64 D: a pointer moving one double
+    // a row (every address stays inside the LDS array; an entry outside the
+    // band is read and not used)
+    const double* p0 = s + (size_t)rq * D + KL;
+    const double* p1 = s + (size_t)(q > 0 ? rq - 64 : 0) * D + KL;
+    for (int jj = min(n - 1, 64 * q + 63); jj >= 64 * q; --jj) {
+      const int d0 = jj - rq;  // column offset of jj from row rq
+      const int d1 = d0 + 64;  // ... and from row rq - 64
+      const bool u0 = d0 >= 1 && d0 <= JM, u1 = q > 0 && d1 >= 1 && d1 <= JM;
+      const double a0 = p0[jj];
+      const double a1 = p1[jj];
+      const double dj = fj_readlane_f64(dq, jj - 64 * q);
+      const double yj = fj_readlane_f64(y[q], jj - 64 * q);
+      // dtbsv leaves a zero y_j and its column alone; the step is written
+      // without a branch (selects) so that the reads above are issued
+      // before the division
+      const bool nz = yj != 0.0;
+      const double xj = yj / dj;
+      y[q] = (d0 == 0) ? (nz ? xj : yj) : ((nz && u0) ? y[q] - xj * a0 : y[q]);
+      if (q > 0) y[q - 1] = (nz && u1) ? y[q - 1] - xj * a1 : y[q - 1];
     }
-    fj_wave_lds_order();
   }
-  for (int i = lane; i < n; i += 64) x[i] = xs[i];
+#pragma unroll
+  for (int q = 0; q < QMAX; ++q) {
+    const int row = 64 * q + lane;
+    if (row < n) x[row] = y[q];
+  }
   if (lane == 0 && info) *info = first_zero;
 }
 
@@ -1448,7 +1514,8 @@ int fj_band_launch(const double* A, int n, int lda, int kl, int ku, const double
   if (n <= 0 || lda < n || kl < 0 || ku < 0 || kl > 63 || !A || !b || !x) return FJ_ERR_ARG;
   if ((size_t)n * (size_t)(2 * kl + ku + 2) > (size_t)FJ_BAND_LDS_DOUBLES) return FJ_ERR_ARG;
   if (kl == 11 && ku == 11 && (size_t)(n + 11) * 36 <= (size_t)FJ_BAND_LDS_DOUBLES)
-    hipLaunchKernelGGL((k_band_solve_fixed<11, 11>), dim3(1), dim3(64), 0, st, A, n, lda, b, x, info, guard);
+    hipLaunchKernelGGL((k_band_solve_fixed<11, 11>), dim3(1), dim3(FJ_BAND_FIXED_THREADS), 0, st, A, n, lda, b, x, info,
+                       guard);
   else if (kl <= 15)
     hipLaunchKernelGGL(k_band_solve<15>, dim3(1), dim3(64), 0, st, A, n, lda, kl, ku, b, x, info, guard);
   else

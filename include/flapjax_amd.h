@@ -599,6 +599,9 @@ int fj_beam_fsi_norms(int n_solve, const int32_t* solve_dofs, const double* phi_
                                  and h_out = log(hg_ref^-1 hg_out) when not NULL */
 #define FJ_BEAM_NODE_REL 1    /* h_out = log(hg^-1 hg_ref) (algebra/se3.py:hg_to_d) */
 #define FJ_BEAM_NODE_RATE 2   /* hg_out = hg tilde(x) (structure/beam.py:1521-1531) */
+#define FJ_BEAM_NODE_LOCAL 3  /* h_out = (R^T x[0:3], R^T x[3:6]), R = hg[0:3, 0:3]
+                                 (structure/beam.py:1387-1395, make_f_dead_ext) */
+#define FJ_BEAM_NODE_GLOBAL 4 /* h_out = (R x[0:3], R x[3:6]) */
 
 /* Per-node SE(3) maps over n_nodes frames. */
 int fj_beam_nodes(int op, int n_nodes, const double* hg, const double* x, const double* hg_ref,

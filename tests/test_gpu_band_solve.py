@@ -50,6 +50,10 @@ def test_band_solve_matches_lapack(dev, n, kl, ku, weak):
     cond = np.linalg.cond(a)
     print(f"n={n} kl={kl} ku={ku}: rel err {err:.2e}, cond {cond:.1e}")
     assert err <= 64 * np.finfo(float).eps * max(1.0, cond)
+    if (kl, ku) == (11, 11):
+        # the fixed-bandwidth kernel restates dgbtf2 / dgbtrs operation by
+        # operation: the same bits as LAPACK's dgbsv
+        assert np.array_equal(x, ref)
     dense = K.lu_solve(K.lu_factor(torch.as_tensor(a, device=dev)), torch.as_tensor(b, device=dev)).cpu().numpy()
     assert rel_err(x, dense) <= 64 * np.finfo(float).eps * max(1.0, cond)
     # A and b are left as they were; x may alias b
@@ -77,4 +81,36 @@ def test_band_solve_singular_column(dev):
     a = np.diag(np.arange(1.0, 9.0))
     a[3, 3] = 0.0
     x = K.band_solve(torch.as_tensor(a, device=dev), torch.ones(8, dtype=torch.float64, device=dev), 1, 1)
+    assert not torch.isfinite(x).all()
+
+
+def test_band_solve_beam_newton_system(dev):
+    """A Newton system of the coupled gust wing (80 two-node elements, 480
+    solve DOFs, kl = ku = 11; 37 of its columns pivot off the diagonal),
+    tests/golden/beam_newton_system_480.npz: the same bits as dgbsv."""
+    from pathlib import Path
+
+    from flapjax_amd import _kernels as K
+
+    d = np.load(Path(__file__).parent / "golden" / "beam_newton_system_480.npz")
+    ab, b = d["ab"], d["b"]
+    n = ab.shape[1]
+    a = np.zeros((n, n))
+    for j in range(n):
+        for i in range(max(0, j - 11), min(n, j + 12)):
+            a[i, j] = ab[11 + i - j, j]
+    ref = sl.solve_banded((11, 11), ab, b)
+    x = K.band_solve(torch.as_tensor(a, device=dev), torch.as_tensor(b, device=dev), 11, 11).cpu().numpy()
+    assert np.array_equal(x, ref)
+
+
+def test_band_solve_fixed_singular_column(dev):
+    """kl = ku = 11 with a zero column: the exact pivot search finds no
+    pivot and the solution is not finite (as the general kernel's)."""
+    from flapjax_amd import _kernels as K
+
+    rng = np.random.default_rng(11)
+    a = _banded(64, 11, 11, rng)
+    a[:, 20] = 0.0
+    x = K.band_solve(torch.as_tensor(a, device=dev), torch.ones(64, dtype=torch.float64, device=dev), 11, 11)
     assert not torch.isfinite(x).all()

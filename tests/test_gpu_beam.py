@@ -74,6 +74,15 @@ def test_node_maps(dev):
     v = rng.normal(size=(n, 6))
     rate = beam.make_hg_dot(hg_d, torch.tensor(v, device=dev)).cpu().numpy()
     assert rel_err(rate, np.stack([a @ B.tilde4(x) for a, x in zip(hg0, v)])) < 1e-15
+    # force and moment into and out of the node frames (make_f_dead_ext,
+    # structure/beam.py:1387-1395), with a time axis broadcast over hg
+    f = rng.normal(size=(3, n, 6))
+    r = hg0[:, :3, :3]
+    loc = BeamStructure._to_local(torch.tensor(f, device=dev), hg_d).cpu().numpy()
+    exp = np.concatenate((np.einsum("nji,tnj->tni", r, f[..., :3]), np.einsum("nji,tnj->tni", r, f[..., 3:])), -1)
+    assert loc.shape == f.shape and rel_err(loc, exp) < 1e-15
+    back = BeamStructure._to_global(torch.tensor(loc, device=dev), hg_d).cpu().numpy()
+    assert rel_err(back, f) < 1e-14
 
 
 def test_static_geradin_large_load(dev):
