@@ -137,6 +137,7 @@ _SIGS = {
     "fj_beam_subnorm": ([c_void_p, c_void_p, c_int, c_void_p, c_void_p], c_int),
     "fj_beam_ga": ([c_int, c_int] + [c_double] * 5 + [c_void_p] * 11, c_int),
     "fj_beam_fsi_norms": ([c_int] + [c_void_p] * 7, c_int),
+    "fj_beam_ga_nodes": ([c_int, c_int, c_double, c_double] + [c_void_p] * 15, c_int),
     "fj_beam_newton_dynamic": ([c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                 c_double, c_double, c_double, c_void_p, c_int, c_int, c_int, c_void_p]
                                + [c_void_p] * 11, c_int),

@@ -450,9 +450,18 @@ class BeamStructure:
         L.check(rc, "fj_beam_system")
         return A, r, fa
 
+    def _wait(self):
+        """Wait for the work enqueued so far on the current stream (an event
+        wait returns sooner after the device finishes than a stream wait)."""
+        ev = getattr(self, "_ev", None)
+        if ev is None:
+            ev = self._ev = torch.cuda.Event()
+        ev.record()
+        ev.synchronize()
+
     def _read_norms(self):
         self._norms_host.copy_(self._norms, non_blocking=True)
-        torch.cuda.current_stream().synchronize()
+        self._wait()
         return self._norms_host.numpy()
 
     # ------------------------------------------------------------------ static
@@ -570,7 +579,7 @@ class BeamStructure:
             self._nstatus_host.copy_(self._nstatus, non_blocking=True)
             if on_batch is not None:
                 on_batch()
-            torch.cuda.current_stream().synchronize()
+            self._wait()
             it, done = int(self._nstatus_host[0]), int(self._nstatus_host[1])
             if done:
                 return it
@@ -598,7 +607,7 @@ class BeamStructure:
             st.update_norms(nv[0], nv[1], nv[2], nv[3], nv[4] != 0.0)
         if on_batch is not None:
             on_batch()
-            torch.cuda.current_stream().synchronize()
+            self._wait()
         return st.i_iter
 
     def dynamic_solve(self, init_state, prescribed_dofs, n_tstep: int, dt, f_ext_follower=None, f_ext_dead=None,
@@ -719,10 +728,13 @@ class BeamStructure:
                 f_prev = f_nm1
                 fst = ConvergenceStatus(fsi_settings)
                 while not fst.get_status():
-                    ga(L.GA_ALPHA, (phi_a, v_a, v[i - 1]), (phi_n, v_n, phi_old))
+                    # phi_n, v_n, the frames hg_n and their rates in one launch
                     hg_n = torch.empty((nn, 4, 4), dtype=F64, device=dev)
-                    self._nodes(L.BEAM_NODE_UPDATE, hg[i - 1], x=phi_n, hg_out=hg_n)
-                    hg_dot = self.make_hg_dot(hg_n, v_n)
+                    hg_dot = torch.empty_like(hg_n)
+                    L.check(lib.fj_beam_ga_nodes(L.GA_ALPHA, nn, am, af, L.ptr(phi_a), L.ptr(v_a), None,
+                                                 L.ptr(v[i - 1]), None, None, L.ptr(hg[i - 1]), L.ptr(phi_n),
+                                                 L.ptr(v_n), None, None, L.ptr(phi_old), L.ptr(hg_n),
+                                                 L.ptr(hg_dot), L.stream()), "fj_beam_ga_nodes")
                     f_n = aero.solve(i, hg_n, hg[i - 1], hg_dot).contiguous()
                     f_al = torch.empty((nn, 6), dtype=F64, device=dev)
                     ga(L.GA_FORCE, (f_nm1, f_n), (f_al,))
@@ -734,8 +746,11 @@ class BeamStructure:
                     f_prev = f_al
                 fsi_iters.append(fst.i_iter)
             # calculate_q_n_from_q_alpha + calculate_a_n (structure/time_integration.py:53-84)
-            ga(L.GA_FINISH, (phi_a, v_a, vd_a, v[i - 1], vd[i - 1], acc[i - 1]), (phi, v[i], vd[i], acc[i]))
-            self._nodes(L.BEAM_NODE_UPDATE, hg[i - 1], x=phi, hg_out=hg[i])
+            # and the step's frames, in one launch
+            L.check(lib.fj_beam_ga_nodes(L.GA_FINISH, nn, am, af, L.ptr(phi_a), L.ptr(v_a), L.ptr(vd_a),
+                                         L.ptr(v[i - 1]), L.ptr(vd[i - 1]), L.ptr(acc[i - 1]), L.ptr(hg[i - 1]),
+                                         L.ptr(phi), L.ptr(v[i]), L.ptr(vd[i]), L.ptr(acc[i]), None, L.ptr(hg[i]),
+                                         None, L.stream()), "fj_beam_ga_nodes")
             if aero is not None:
                 f_aero_g[i] = aero.project(i, hg[i])
                 f_aero_l[i] = self._to_local(f_aero_g[i], hg[i])

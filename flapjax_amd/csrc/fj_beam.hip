@@ -454,6 +454,18 @@ __global__ __launch_bounds__(FJ_BEAM_UPD_THREADS) void k_beam_subnorm(const doub
 
 // node frames: update H exp(phi) (+ T(phi), + log(H_ref^-1 H)), relative
 // configuration log(H^-1 H_ref), or rate H tilde(v)
+// H tilde(v) = [[R S(w), R u], [0, 0]] for the frame g = [[R, p], [0, 1]]
+// and v = (u, w) (structure/beam.py:1521-1531)
+__device__ __forceinline__ void fj_hg_rate(const double* g, const double* vv, double* out) {
+  double s[3][3];
+  fj_skew(vv + 3, s);
+  for (int p = 0; p < 3; ++p) {
+    for (int q = 0; q < 3; ++q) out[4 * p + q] = g[4 * p] * s[0][q] + g[4 * p + 1] * s[1][q] + g[4 * p + 2] * s[2][q];
+    out[4 * p + 3] = g[4 * p] * vv[0] + g[4 * p + 1] * vv[1] + g[4 * p + 2] * vv[2];
+  }
+  for (int q = 12; q < 16; ++q) out[q] = 0.0;
+}
+
 __global__ void k_beam_nodes(int op, int n, const double* __restrict__ hg, const double* __restrict__ x,
                              const double* __restrict__ hg_ref, double* __restrict__ hg_out,
                              double* __restrict__ t_out, double* __restrict__ h_out,
@@ -492,16 +504,7 @@ __global__ void k_beam_nodes(int op, int n, const double* __restrict__ hg, const
         h_out[6 * i + h + p] = r0 * f[h] + r1 * f[h + 1] + r2 * f[h + 2];
       }
   } else {
-    // H tilde(v): [[R S(w), R u], [0, 0]]
-    const double* vv = x + 6 * i;
-    double s[3][3];
-    fj_skew(vv + 3, s);
-    for (int p = 0; p < 3; ++p) {
-      for (int q = 0; q < 3; ++q)
-        hg_out[16 * i + 4 * p + q] = g[4 * p] * s[0][q] + g[4 * p + 1] * s[1][q] + g[4 * p + 2] * s[2][q];
-      hg_out[16 * i + 4 * p + 3] = g[4 * p] * vv[0] + g[4 * p + 1] * vv[1] + g[4 * p + 2] * vv[2];
-    }
-    for (int q = 12; q < 16; ++q) hg_out[16 * i + q] = 0.0;
+    fj_hg_rate(g, x + 6 * i, hg_out + 16 * i);
   }
 }
 
@@ -1028,6 +1031,60 @@ __global__ __launch_bounds__(FJ_GA_THREADS) void k_beam_fsi_norms(
   }
 }
 
+// The per-DOF arithmetic of k_beam_ga_alpha and k_beam_ga_finish (each
+// product and sum rounded on its own), for the node-level fused kernel.
+__device__ __forceinline__ void fj_ga_alpha_dof(double af, double pa, double va, double vp, double& pn,
+                                                double& vn) {
+#pragma clang fp contract(off)
+  pn = pa / (1.0 - af);
+  vn = (va - af * vp) / (1.0 - af);
+}
+
+__device__ __forceinline__ void fj_ga_finish_dof(double am, double af, double pa, double va, double vda,
+                                                 double vp, double vdp, double ap, double& p, double& v,
+                                                 double& vd, double& a) {
+#pragma clang fp contract(off)
+  p = pa / (1.0 - af);
+  v = (va - af * vp) / (1.0 - af);
+  vd = (vda - af * vdp) / (1.0 - af);
+  a = (1.0 / (1.0 - am)) * ((1.0 - af) * vd + af * vdp - am * ap);
+}
+
+// One thread per node: the generalised-alpha state of the node's six DOFs
+// (as k_beam_ga_alpha / k_beam_ga_finish), then its frame hg_prev exp(phi)
+// and, for FJ_GA_ALPHA, the frame rate hg tilde(v) (as k_beam_nodes), in
+// one launch instead of three.
+__global__ void k_beam_ga_nodes(int op, int nn, double am, double af, const double* __restrict__ phi_a,
+                                const double* __restrict__ v_a, const double* __restrict__ vd_a,
+                                const double* __restrict__ v_prev, const double* __restrict__ vd_prev,
+                                const double* __restrict__ a_prev, const double* __restrict__ hg_prev,
+                                double* __restrict__ phi, double* __restrict__ v, double* __restrict__ vd,
+                                double* __restrict__ a, double* __restrict__ phi_copy, double* __restrict__ hg,
+                                double* __restrict__ hg_dot) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nn) return;
+  double ph[6], vv[6];
+  for (int d = 0; d < 6; ++d) {
+    const int k = 6 * i + d;
+    if (op == FJ_GA_ALPHA) {
+      fj_ga_alpha_dof(af, phi_a[k], v_a[k], v_prev[k], ph[d], vv[d]);
+      if (phi_copy) phi_copy[k] = phi_a[k];
+    } else {
+      double vdk, ak;
+      fj_ga_finish_dof(am, af, phi_a[k], v_a[k], vd_a[k], v_prev[k], vd_prev[k], a_prev[k], ph[d], vv[d], vdk, ak);
+      vd[k] = vdk;
+      a[k] = ak;
+    }
+    phi[k] = ph[d];
+    v[k] = vv[d];
+  }
+  double ex[16], gn[16];
+  fj_exp_se3(ph, ex);
+  fj_hg_mul(hg_prev + 16 * i, ex, gn);
+  for (int q = 0; q < 16; ++q) hg[16 * i + q] = gn[q];
+  if (hg_dot) fj_hg_rate(gn, vv, hg_dot + 16 * i);
+}
+
 extern "C" int fj_beam_ga(int op, int n, double am, double af, double g, double b, double dt,
                           const double* p0, const double* p1, const double* p2, const double* p3,
                           const double* p4, const double* p5, double* o0, double* o1, double* o2,
@@ -1056,6 +1113,26 @@ extern "C" int fj_beam_ga(int op, int n, double am, double af, double g, double 
     default:
       return FJ_ERR_ARG;
   }
+  FJ_CHECK_LAUNCH();
+  return FJ_OK;
+}
+
+extern "C" int fj_beam_ga_nodes(int op, int n_nodes, double am, double af, const double* phi_a, const double* v_a,
+                                const double* v_dot_a, const double* v_prev, const double* v_dot_prev,
+                                const double* a_prev, const double* hg_prev, double* phi, double* v,
+                                double* v_dot, double* a, double* phi_copy, double* hg, double* hg_dot,
+                                void* stream) {
+  if (n_nodes < 0) return FJ_ERR_ARG;
+  if (!phi_a || !v_a || !v_prev || !hg_prev || !phi || !v || !hg) return FJ_ERR_ARG;
+  if (op == FJ_GA_FINISH) {
+    if (!v_dot_a || !v_dot_prev || !a_prev || !v_dot || !a || hg_dot || phi_copy) return FJ_ERR_ARG;
+  } else if (op != FJ_GA_ALPHA) {
+    return FJ_ERR_ARG;
+  }
+  if (n_nodes == 0) return FJ_OK;
+  hipLaunchKernelGGL(k_beam_ga_nodes, dim3(fj_cdiv(n_nodes, 64)), dim3(64), 0, (hipStream_t)stream, op, n_nodes, am,
+                     af, phi_a, v_a, v_dot_a, v_prev, v_dot_prev, a_prev, hg_prev, phi, v, v_dot, a, phi_copy, hg,
+                     hg_dot);
   FJ_CHECK_LAUNCH();
   return FJ_OK;
 }

@@ -646,13 +646,14 @@ class UVLM:
         """aero/uvlm.py:1119-1215 (in place in ``case``).  With a grid
         function the step takes the general path (the fused one maps a single
         undeflected local grid)."""
-        q_nm1 = None
         if not static:
-            q_nm1 = {k: [a[i_ts - 1] for a in getattr(case, k)] for k in ("gamma_b", "gamma_w", "gamma_b_dot", "zeta_w")}
             case.t[i_ts] = case.t[i_ts - 1] + self.dt if i_ts else 0.0
         if i_ts > 0 and self.grid_func is None and self._fast_step_ok(static, hg_nm1, hg_dot_n):
             self._run_checked(lambda: self._dynamic_step(case, i_ts, hg_n, hg_nm1, hg_dot_n), False, i_ts)
             return case
+        q_nm1 = None
+        if not static:
+            q_nm1 = {k: [a[i_ts - 1] for a in getattr(case, k)] for k in ("gamma_b", "gamma_w", "gamma_b_dot", "zeta_w")}
         snap = case.snapshot(i_ts)
         cs_n = self.cs_ang0 if cs_ang_n is None else cs_ang_n
         zeta_b_n = self.hg_to_zeta_b(hg_n if hg_n is not None else self.hg_ref, cs_n, out=snap["zeta_b"])

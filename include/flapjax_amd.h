@@ -588,6 +588,18 @@ int fj_beam_ga(int op, int n, double am, double af, double g, double b, double d
                const double* p1, const double* p2, const double* p3, const double* p4, const double* p5,
                double* o0, double* o1, double* o2, double* o3, void* stream);
 
+/* fj_beam_ga's FJ_GA_ALPHA or FJ_GA_FINISH over the 6 n_nodes DOFs fused with
+ * the node update that follows it (structure/beam.py:2612-2725, 2794-2834):
+ * hg = hg_prev exp(phi) per node (FJ_BEAM_NODE_UPDATE) and, for FJ_GA_ALPHA
+ * with hg_dot not NULL, hg_dot = hg tilde(v) (FJ_BEAM_NODE_RATE).  ALPHA:
+ * phi_a, v_a, v_prev -> phi (phi_n), v (v_n), phi_copy (copy of phi_a, or
+ * NULL); FINISH: phi_a, v_a, v_dot_a, v_prev, v_dot_prev, a_prev -> phi, v,
+ * v_dot, a.  The same values as the unfused launches. */
+int fj_beam_ga_nodes(int op, int n_nodes, double am, double af, const double* phi_a, const double* v_a,
+                     const double* v_dot_a, const double* v_prev, const double* v_dot_prev, const double* a_prev,
+                     const double* hg_prev, double* phi, double* v, double* v_dot, double* a, double* phi_copy,
+                     double* hg, double* hg_dot, void* stream);
+
 /* The FSI convergence norms of a coupled iteration (structure/beam.py:
  * 2700-2720) over the solve DOFs, one workgroup in a fixed order:
  * out = {|phi_old - phi_a|, |phi_a|, |f_al - f_prev|, max |f_al|, 1 if

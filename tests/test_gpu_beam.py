@@ -285,3 +285,45 @@ def test_device_newton_loop_equals_host_loop(dev):
         for name in ("hg", "v", "v_dot"):
             assert torch.equal(getattr(a, name), getattr(c, name)), name
     assert max(outs[0].iters["struct"]) >= 2 and set(outs[2].iters["struct"]) == {2}
+
+
+def test_ga_nodes_equals_separate_launches(dev):
+    """fj_beam_ga_nodes (generalised-alpha state + node frames in one
+    launch) gives the bits of fj_beam_ga followed by fj_beam_nodes."""
+    from flapjax_amd import _lib as L
+    from flapjax_amd.beam import BeamStructure
+
+    rng = np.random.default_rng(9)
+    nn = 81
+    am, af = 0.3, 0.4
+    beam = BeamStructure.__new__(BeamStructure)
+    t = lambda *shape: torch.tensor(rng.normal(size=shape), device=dev)  # noqa: E731
+    phi_a, v_a, vd_a, v_p, vd_p, a_p = (0.1 * t(nn, 6) for _ in range(6))
+    hg_p = torch.tensor(np.stack([B.exp_se3(x) for x in rng.normal(size=(nn, 6))]), device=dev)
+    lib = L.lib()
+    e = lambda: torch.empty((nn, 6), dtype=torch.float64, device=dev)  # noqa: E731
+    # alpha point: phi_n, v_n, copy of phi_a, frames and rates
+    o = [e(), e(), e()]
+    L.check(lib.fj_beam_ga(L.GA_ALPHA, 6 * nn, am, af, 0.0, 0.0, 0.0, L.ptr(phi_a), L.ptr(v_a), L.ptr(v_p), None,
+                           None, None, *[L.ptr(x) for x in o], None, L.stream()), "fj_beam_ga")
+    hg_ref = torch.empty((nn, 4, 4), dtype=torch.float64, device=dev)
+    beam._nodes(L.BEAM_NODE_UPDATE, hg_p, x=o[0], hg_out=hg_ref)
+    rate_ref = beam.make_hg_dot(hg_ref, o[1])
+    f = [e(), e(), e()]
+    hg_f, rate_f = torch.empty_like(hg_ref), torch.empty_like(hg_ref)
+    L.check(lib.fj_beam_ga_nodes(L.GA_ALPHA, nn, am, af, L.ptr(phi_a), L.ptr(v_a), None, L.ptr(v_p), None, None,
+                                 L.ptr(hg_p), L.ptr(f[0]), L.ptr(f[1]), None, None, L.ptr(f[2]), L.ptr(hg_f),
+                                 L.ptr(rate_f), L.stream()), "fj_beam_ga_nodes")
+    for x, y in zip(o + [hg_ref, rate_ref], f + [hg_f, rate_f]):
+        assert torch.equal(x, y)
+    # step completion: phi, v, v_dot, a and the frames
+    o = [e(), e(), e(), e()]
+    L.check(lib.fj_beam_ga(L.GA_FINISH, 6 * nn, am, af, 0.0, 0.0, 0.0, L.ptr(phi_a), L.ptr(v_a), L.ptr(vd_a),
+                           L.ptr(v_p), L.ptr(vd_p), L.ptr(a_p), *[L.ptr(x) for x in o], L.stream()), "fj_beam_ga")
+    beam._nodes(L.BEAM_NODE_UPDATE, hg_p, x=o[0], hg_out=hg_ref)
+    f = [e(), e(), e(), e()]
+    L.check(lib.fj_beam_ga_nodes(L.GA_FINISH, nn, am, af, L.ptr(phi_a), L.ptr(v_a), L.ptr(vd_a), L.ptr(v_p),
+                                 L.ptr(vd_p), L.ptr(a_p), L.ptr(hg_p), *[L.ptr(x) for x in f], None, L.ptr(hg_f),
+                                 None, L.stream()), "fj_beam_ga_nodes")
+    for x, y in zip(o + [hg_ref], f + [hg_f]):
+        assert torch.equal(x, y)
