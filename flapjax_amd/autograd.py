@@ -70,10 +70,9 @@ class PairSum(torch.autograd.Function):
         need_s = any(ctx.needs_input_grad[1:4])
         gx = torch.empty_like(tflat) if need_x else None
         gseg = torch.empty((seg.shape[0], 7), dtype=F64, device=seg.device) if need_s else None
-        ws = None
-        if need_x:
-            nb = L.lib().fj_pair_sum_backward_workspace_bytes(tflat.shape[0], seg.shape[0])
-            ws = torch.empty(nb // 8 + 1, dtype=F64, device=seg.device)
+        # the workspace selects the one-pass adjoint kernel (both sides at once)
+        nb = L.lib().fj_pair_sum_backward_workspace_bytes(tflat.shape[0], seg.shape[0])
+        ws = torch.empty(nb // 8 + 1, dtype=F64, device=seg.device)
         rc = L.lib().fj_pair_sum_backward(L.ptr(seg), seg.shape[0], L.ptr(tflat), tflat.shape[0], L.ptr(gv),
                                           kern.kind, kern.eps, kern.rcut, L.ptr(gx) if need_x else None, 0,
                                           L.ptr(gseg) if need_s else None, L.ptr(ws), L.stream())
