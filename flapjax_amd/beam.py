@@ -522,19 +522,20 @@ class BeamStructure:
                              prescribed_dofs=pre, iters=iters)
 
     @staticmethod
-    def _frame_map(op, f, hg):
+    def _frame_map(op, f, hg, out=None):
         lead = torch.broadcast_shapes(f.shape[:-1], hg.shape[:-2])
         f = f.expand(lead + (6,)).contiguous()
         hg = hg.expand(lead + (4, 4)).contiguous()
-        out = torch.empty(lead + (6,), dtype=F64, device=f.device)
+        if out is None or not out.is_contiguous() or tuple(out.shape) != lead + (6,):
+            out = torch.empty(lead + (6,), dtype=F64, device=f.device)
         L.check(L.lib().fj_beam_nodes(op, hg.numel() // 16, L.ptr(hg), L.ptr(f), None, None, None, L.ptr(out),
                                       L.stream()), "fj_beam_nodes")
         return out
 
     @staticmethod
-    def _to_local(f, hg):
+    def _to_local(f, hg, out=None):
         """make_f_dead_ext (structure/beam.py:1387-1395): R^T f per node."""
-        return BeamStructure._frame_map(L.BEAM_NODE_LOCAL, f, hg)
+        return BeamStructure._frame_map(L.BEAM_NODE_LOCAL, f, hg, out)
 
     @staticmethod
     def _to_global(f, hg):
@@ -706,6 +707,7 @@ class BeamStructure:
         phi_old = torch.empty_like(phi_n)
         phi = torch.empty_like(phi_n)
         nrm_dev = torch.empty(5, dtype=F64, device=dev)
+        hg_fsi = (torch.empty((nn, 4, 4), dtype=F64, device=dev), torch.empty((nn, 4, 4), dtype=F64, device=dev))
         nrm_host = torch.empty(5, dtype=F64).pin_memory()
         fsi = {}
 
@@ -729,8 +731,8 @@ class BeamStructure:
                 fst = ConvergenceStatus(fsi_settings)
                 while not fst.get_status():
                     # phi_n, v_n, the frames hg_n and their rates in one launch
-                    hg_n = torch.empty((nn, 4, 4), dtype=F64, device=dev)
-                    hg_dot = torch.empty_like(hg_n)
+                    # (the aero step copies what it keeps of them)
+                    hg_n, hg_dot = hg_fsi
                     L.check(lib.fj_beam_ga_nodes(L.GA_ALPHA, nn, am, af, L.ptr(phi_a), L.ptr(v_a), None,
                                                  L.ptr(v[i - 1]), None, None, L.ptr(hg[i - 1]), L.ptr(phi_n),
                                                  L.ptr(v_n), None, None, L.ptr(phi_old), L.ptr(hg_n),
@@ -752,8 +754,12 @@ class BeamStructure:
                                          L.ptr(phi), L.ptr(v[i]), L.ptr(vd[i]), L.ptr(acc[i]), None, L.ptr(hg[i]),
                                          None, L.stream()), "fj_beam_ga_nodes")
             if aero is not None:
-                f_aero_g[i] = aero.project(i, hg[i])
-                f_aero_l[i] = self._to_local(f_aero_g[i], hg[i])
+                fg = aero.project(i, hg[i], out=f_aero_g[i])
+                if fg.data_ptr() != f_aero_g[i].data_ptr():
+                    f_aero_g[i] = fg
+                fl = self._to_local(f_aero_g[i], hg[i], out=f_aero_l[i])
+                if fl.data_ptr() != f_aero_l[i].data_ptr():
+                    f_aero_l[i] = fl
         out = StructureCase(hg=hg, varphi=self.varphi_from_hg(hg), t=np.asarray(t, float), v=v, v_dot=vd, a=acc,
                             f_ext_follower=f_fol_t,
                             f_ext_dead=None if f_dead_t is None else self._to_local(f_dead_t, hg),

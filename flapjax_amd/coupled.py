@@ -77,10 +77,19 @@ class CoupledAeroelastic:
         self.aero.set_design_variables(dt=dt, flowfield=flowfield, delta_w=delta_w, zeta_b0=x0_aero,
                                        hg0=self.structure.hg0, reference_cs_angles=cs_angles_reference)
 
-    def _project(self, case: AeroCase, i_ts: int, hg, include_unsteady: bool):
+    def _dof_i32(self):
+        # the node mappings as the projection kernel reads them (int32 on the
+        # device), converted once per mapping instead of once per call
+        dof = self.aero._dof
+        if getattr(self, "_dof_src", None) is not dof:
+            self._dof_src = dof
+            self._dof32 = [d.to(torch.int32) for d in dof]
+        return self._dof32
+
+    def _project(self, case: AeroCase, i_ts: int, hg, include_unsteady: bool, out=None):
         f_un = [f[i_ts] for f in case.f_unsteady] if include_unsteady else None
-        return project_forcing_to_beam([f[i_ts] for f in case.f_steady], hg[:, :3, :3], self.aero._dof,
-                                       self.aero.zeta_b0, f_extra=f_un)
+        return project_forcing_to_beam([f[i_ts] for f in case.f_steady], hg[:, :3, :3], self._dof_i32(),
+                                       self.aero.zeta_b0, f_extra=f_un, out=out)
 
     # coupled/coupled.py:122-152
     def reference_configuration(self, prescribed_dofs, horseshoe: bool = False, use_f_ext_follower: bool = False,
@@ -213,8 +222,8 @@ class CoupledAeroelastic:
                 return outer._project(aero_case, i, hg_n, unsteady)
 
             @staticmethod
-            def project(i, hg_n):
-                return outer._project(aero_case, i, hg_n, unsteady)
+            def project(i, hg_n, out=None):
+                return outer._project(aero_case, i, hg_n, unsteady, out=out)
 
         struct = st_.base_dynamic_solve(struct0, t, dt, pre, load_steps, series(f_ext_follower, "f_ext_follower"),
                                         series(f_ext_dead, "f_ext_dead"), aero=_Hooks,

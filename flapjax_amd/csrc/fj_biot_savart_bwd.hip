@@ -273,15 +273,29 @@ __global__ __launch_bounds__(64 * FJ_BWDF_WAVES, 2) void k_bwd_fused(
   }
 }
 
-// dL/dx from the (filament tile, target) partials, tiles summed in order
-__global__ void k_bwd_fused_reduce_tgt(const double* __restrict__ part, int n_tile, int n_tgt,
-                                       double* __restrict__ gx, int accumulate) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;  // (target, component)
-  if (i >= 3 * n_tgt) return;
+// dL/dx from the (filament tile, target) partials: a workgroup per 64
+// outputs, its FJ_BWDR_WAVES waves summing interleaved tiles (wave w: tiles
+// w, w + W, ...) in order, then the wave sums added in wave order (a fixed
+// order, so the result does not depend on scheduling)
+#define FJ_BWDR_WAVES 8
+__global__ __launch_bounds__(64 * FJ_BWDR_WAVES) void k_bwd_fused_reduce_tgt(
+    const double* __restrict__ part, int n_tile, int n_tgt, double* __restrict__ gx, int accumulate) {
+  __shared__ double red[FJ_BWDR_WAVES][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int i = blockIdx.x * 64 + lane;  // (target, component)
+  const int n_out = 3 * n_tgt;
   double a = 0.0;
-  for (int b = 0; b < n_tile; ++b) a += part[(size_t)b * 3 * n_tgt + i];
-  a *= FJ_INV_4PI;
-  gx[i] = accumulate ? gx[i] + a : a;
+  if (i < n_out)
+    for (int b = wave; b < n_tile; b += FJ_BWDR_WAVES) a += part[(size_t)b * n_out + i];
+  red[wave][lane] = a;
+  __syncthreads();
+  if (wave == 0 && i < n_out) {
+    double t = red[0][lane];
+#pragma unroll
+    for (int w = 1; w < FJ_BWDR_WAVES; ++w) t += red[w][lane];
+    t *= FJ_INV_4PI;
+    gx[i] = accumulate ? gx[i] + t : t;
+  }
 }
 
 // [dL/dy0, dL/dy1, dL/dg] from the (target chunk, filament) partials
@@ -372,7 +386,8 @@ extern "C" int fj_pair_sum_backward(const double* seg, int n_seg, const double* 
                          seg, n_seg, targets, grad_v, n_tgt, kp, p.tchunk,                          \
                          grad_seg ? part_seg : nullptr, grad_targets ? part_tgt : nullptr);         \
       if (grad_targets)                                                                            \
-        hipLaunchKernelGGL(k_bwd_fused_reduce_tgt, dim3(fj_cdiv(3 * n_tgt, 256)), dim3(256), 0, st, \
+        hipLaunchKernelGGL(k_bwd_fused_reduce_tgt, dim3(fj_cdiv(3 * n_tgt, 64)),                   \
+                           dim3(64 * FJ_BWDR_WAVES), 0, st,                                        \
                            (const double*)part_tgt, p.tiles, n_tgt, grad_targets, accumulate);     \
       if (grad_seg)                                                                                \
         hipLaunchKernelGGL(k_bwd_fused_reduce_seg, dim3(fj_cdiv(7 * (long long)n_seg, 256)),       \
