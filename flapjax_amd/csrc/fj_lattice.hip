@@ -146,7 +146,10 @@ struct FjLatPlan {
 static int fj_lat_plan(int n_tgt, int n_lat, const fj_lattice_ref* lats, FjLatPlan& p) {
   if (n_lat < 0 || n_lat > FJ_MAX_LAT || (n_lat > 0 && !lats)) return FJ_ERR_ARG;
   p.tiles = n_tgt > 0 ? fj_cdiv(n_tgt, 64) : 1;
-  p.ld = p.tiles * 64;
+  // one spare row of 64 past the tiles: a split epilogue (below) reduces its
+  // second target range from an offset that is not a multiple of 64, and its
+  // last wave's reads then stay inside the partial row
+  p.ld = (p.tiles + 1) * 64;
   int max_cols = 1;
   for (int k = 0; k < n_lat; ++k) {
     if (lats[k].m < 0 || lats[k].n < 0 || lats[k].seg_off < 0) return FJ_ERR_ARG;
@@ -205,22 +208,33 @@ extern "C" size_t fj_pair_sum_lattices_workspace_bytes(int n_tgt, int n_lat,
 }
 
 // The body of fj_pair_sum_lattices on a given workspace (partials, then the
-// strip-column records), shared with the allocation-free entry point below.
+// strip-column records), shared with the allocation-free and the split entry
+// points below: targets [0, n_a) get epilogue mode_a into out_a, targets
+// [n_a, n_tgt) mode_b into out_b (n_a = n_tgt: one range).
 static int fj_lat_run(const double* seg, int n_lat, const fj_lattice_ref* lats,
-                      const double* targets, int n_tgt, double* out, int kernel, double eps,
-                      double rcut, int mode, const fj_flowfield* flow, double time, double dt,
-                      void* workspace, size_t ws_limit, void* stream) {
-  if (n_tgt < 0) return FJ_ERR_ARG;
+                      const double* targets, int n_tgt, int n_a, double* out_a, int mode_a,
+                      double* out_b, int mode_b, int kernel, double eps, double rcut,
+                      const fj_flowfield* flow, double time, double dt, void* workspace,
+                      size_t ws_limit, void* stream) {
+  if (n_tgt < 0 || n_a < 0 || n_a > n_tgt) return FJ_ERR_ARG;
   if (kernel < FJ_KERNEL_EPSILON || kernel > FJ_KERNEL_PLAIN) return FJ_ERR_ARG;
-  if (mode < FJ_OUT_STORE || mode > FJ_OUT_CONVECT) return FJ_ERR_ARG;
+  if (mode_a < FJ_OUT_STORE || mode_a > FJ_OUT_CONVECT) return FJ_ERR_ARG;
+  if (n_a < n_tgt && (mode_b < FJ_OUT_STORE || mode_b > FJ_OUT_CONVECT || !out_b)) return FJ_ERR_ARG;
   FjLatPlan p;
   const int rc = fj_lat_plan(n_tgt, n_lat, lats, p);
   if (rc != FJ_OK) return rc;
   if (n_tgt == 0) return FJ_OK;
-  if (!targets || !out) return FJ_ERR_ARG;
+  if (!targets || (n_a > 0 && !out_a)) return FJ_ERR_ARG;
   hipStream_t st = (hipStream_t)stream;
   const FjFlow fl = fj_flow_from_desc(flow);
-  if (p.list.n_tasks == 0) return fj_pair_epilogue(nullptr, 0, 0, n_tgt, targets, out, mode, fl, time, dt, st);
+  const int n_b = n_tgt - n_a;
+  if (p.list.n_tasks == 0) {
+    int r = FJ_OK;
+    if (n_a > 0) r = fj_pair_epilogue(nullptr, 0, 0, n_a, targets, out_a, mode_a, fl, time, dt, st);
+    if (r == FJ_OK && n_b > 0)
+      r = fj_pair_epilogue(nullptr, 0, 0, n_b, targets + 3 * (size_t)n_a, out_b, mode_b, fl, time, dt, st);
+    return r;
+  }
   if (!seg) return FJ_ERR_ARG;
   if (fj_lat_part_bytes(p) + (size_t)p.list.n_recs * FJ_LAT_REC * sizeof(double) > ws_limit)
     return FJ_ERR_SCRATCH;
@@ -241,15 +255,29 @@ static int fj_lat_run(const double* seg, int n_lat, const fj_lattice_ref* lats,
   else
     hipLaunchKernelGGL(k_lattice_partial<FJ_KERNEL_PLAIN>, grid, blk, 0, st, tab, p.list, p.chunk_len, targets, n_tgt, p.ld, part, kp);
   FJ_CHECK_LAUNCH();
-  return fj_pair_epilogue(part, p.chunks, p.ld, n_tgt, targets, out, mode, fl, time, dt, st);
+  int r = FJ_OK;
+  if (n_a > 0) r = fj_pair_epilogue(part, p.chunks, p.ld, n_a, targets, out_a, mode_a, fl, time, dt, st);
+  if (r == FJ_OK && n_b > 0)
+    r = fj_pair_epilogue(part + n_a, p.chunks, p.ld, n_b, targets + 3 * (size_t)n_a, out_b, mode_b, fl, time,
+                         dt, st);
+  return r;
 }
 
 extern "C" int fj_pair_sum_lattices(const double* seg, int n_lat, const fj_lattice_ref* lats,
                                     const double* targets, int n_tgt, double* out, int kernel,
                                     double eps, double rcut, int mode, const fj_flowfield* flow,
                                     double time, double dt, void* workspace, void* stream) {
-  return fj_lat_run(seg, n_lat, lats, targets, n_tgt, out, kernel, eps, rcut, mode, flow, time, dt,
-                    workspace, (size_t)-1, stream);
+  return fj_lat_run(seg, n_lat, lats, targets, n_tgt, n_tgt, out, mode, nullptr, FJ_OUT_STORE, kernel, eps, rcut,
+                    flow, time, dt, workspace, (size_t)-1, stream);
+}
+
+extern "C" int fj_pair_sum_lattices_split(const double* seg, int n_lat, const fj_lattice_ref* lats,
+                                          const double* targets, int n_tgt, int n_tgt_a, double* out_a,
+                                          int mode_a, double* out_b, int mode_b, int kernel, double eps,
+                                          double rcut, const fj_flowfield* flow, double time, double dt,
+                                          void* workspace, void* stream) {
+  return fj_lat_run(seg, n_lat, lats, targets, n_tgt, n_tgt_a, out_a, mode_a, out_b, mode_b, kernel, eps, rcut,
+                    flow, time, dt, workspace, (size_t)-1, stream);
 }
 
 // ---------------------------------------------------------------------------
@@ -298,6 +326,6 @@ extern "C" int fj_pair_sum_lattices_noalloc(const double* seg, int n_lat, const 
   void* ws = nullptr;
   const int rc = fj_scratch_ptr(&ws);
   if (rc != FJ_OK) return rc;
-  return fj_lat_run(seg, n_lat, lats, targets, n_tgt, out, kernel, eps, rcut, mode, flow, time, dt, ws,
-                    (size_t)FJ_NOALLOC_SCRATCH_BYTES, stream);
+  return fj_lat_run(seg, n_lat, lats, targets, n_tgt, n_tgt, out, mode, nullptr, FJ_OUT_STORE, kernel, eps, rcut,
+                    flow, time, dt, ws, (size_t)FJ_NOALLOC_SCRATCH_BYTES, stream);
 }

@@ -109,6 +109,9 @@ class UVLM:
         self.kernels_w = [self.kernel] * self.n_surf
         self.variable_wake_disc = variable_wake_disc
         self.free_wake = bool(free_wake)
+        # one launch for the forces of step n and the rollup of step n + 1
+        # (free wake, one surface, FP64); False keeps separate launches
+        self.fuse_rollup = True
         self.gamma_dot_relaxation = float(gamma_dot_relaxation)
         self.include_unsteady_force = include_unsteady_force
         self.batch_size = batch_size
@@ -208,6 +211,8 @@ class UVLM:
                 raise TypeError("dt must be either a float or an Array scalar")
         self._dt = float(dt)
         self.flowfield = flowfield
+        # precomputed next-step rollups (see _dynamic_step) belong to one design
+        self._design_version = getattr(self, "_design_version", 0) + 1
         self.zeta_b_ref = self.hg_to_zeta_b(hg0, self.cs_ang0)
         self.zeta_w_ref = self.initialise_wake(self.zeta_b_ref)
 
@@ -550,10 +555,28 @@ class UVLM:
         single = self.n_surf == 1
         n_b = self.n_b
         n_mid = int(self._s_off[-1])
+        # Free wake, one surface, FP64: the forces' pair sum of this step and
+        # the rollup of the next one sum over the same lattices with the same
+        # circulations (bound and wake at n), so one launch serves both
+        # (fj_pair_sum_lattices_split): targets = force midpoints, then the
+        # wake vertices, whose convected positions go straight to the next
+        # step's wake rows.  The next step skips its rollup when it finds the
+        # mark of this step and the same pose as its hg_nm1.
+        fuse_ok = (self.fuse_rollup and self.free_wake and single and self._prec == L.PREC_FP64
+                   and self.grid_disc[0].m_star > 0)
+        fuse_next = fuse_ok and i_ts + 1 < len(case.t)
+        rolled = fuse_ok and getattr(case, "_fj_rollup_mark", None) == (
+            id(self), self._design_version, i_ts, hg_nm1.data_ptr())
         c_all = snap["c"][0].view(-1, 3) if single else self._buf("c_all", (n_b, 3))
         nc_all = snap["nc"][0].view(-1, 3) if single else self._buf("nc_all", (n_b, 3))
         cd_all = self._buf("cd_all", (n_b, 3))
-        mids = self._buf("mids", (n_mid, 3))
+        if fuse_next:
+            gd0 = self.grid_disc[0]
+            n_wv = gd0.m_star * (gd0.n + 1)
+            tgt_all = self._buf("tgt_fused", (n_mid + n_wv, 3))
+            mids = tgt_all[:n_mid]
+        else:
+            mids = self._buf("mids", (n_mid, 3))
         zb_nm1 = []
         for s, gd in enumerate(self.grid_disc):
             a, b = self._b_off[s], self._b_off[s + 1]
@@ -565,7 +588,7 @@ class UVLM:
         # ---- wake: circulation shift, TE row, convection (aero/utils.py:400-577)
         zw_n, gw_n = snap["zeta_w"], snap["gamma_w"]
         seg_nm1 = None
-        if self.free_wake:
+        if self.free_wake and not rolled:
             grids = self._grid_list(zb_nm1 + prev["zeta_w"], prev["gamma_b"] + prev["gamma_w"])
             seg_nm1 = K.segments_from_grids(grids, kern.kind, kern.eps, self.mirror,
                                             out=self._buf("seg_nm1", (self._table_len(grids), L.SEG_STRIDE)))
@@ -574,6 +597,8 @@ class UVLM:
             if gd.m_star == 0:
                 continue
             if self.free_wake:
+                if rolled:
+                    continue  # rows 1.. were convected by the previous step's force launch
                 base = prev["zeta_w"][s][:-1].reshape(-1, 3)
                 lats = K.lattices_of(grids)
                 K.pair_sum(seg_nm1, base, kern.kind, kern.eps, kern.rcut, self._prec, L.OUT_CONVECT,
@@ -615,9 +640,19 @@ class UVLM:
         if s_b:
             K.segments_from_grids(grids_b, kern.kind, kern.eps, self.mirror, out=seg_n[:s_b])
         lats_n = K.lattices_of(grids_b + grids_w)
-        v_mid = K.pair_sum(seg_n if s_b + s_w else None, mids, kern.kind, kern.eps, kern.rcut, self._prec,
-                           L.OUT_STORE, out=self._buf("v_mid", (n_mid, 3)),
-                           workspace=self._pair_ws("ws_mid", n_mid, s_b + s_w, lats_n), lattices=lats_n)
+        if fuse_next and len(lats_n) <= K.MAX_LATTICES:
+            tgt_all[n_mid:].copy_(zw_n[0][:-1].reshape(-1, 3))
+            v_mid = self._buf("v_mid", (n_mid, 3))
+            t_next = float(case.t[i_ts] + self.dt)  # the next step's time, as case_solve sets it
+            K.pair_sum_lattices_split(seg_n, lats_n, tgt_all, n_mid, kern.kind, kern.eps, kern.rcut, v_mid,
+                                      L.OUT_STORE, case.zeta_w[0][i_ts + 1][1:].reshape(-1, 3), L.OUT_CONVECT,
+                                      flow=ff, time=t_next, dt=dt,
+                                      workspace=self._pair_ws("ws_fused", tgt_all.shape[0], s_b + s_w, lats_n))
+            case._fj_rollup_mark = (id(self), self._design_version, i_ts + 1, hg_n.data_ptr())
+        else:
+            v_mid = K.pair_sum(seg_n if s_b + s_w else None, mids, kern.kind, kern.eps, kern.rcut, self._prec,
+                               L.OUT_STORE, out=self._buf("v_mid", (n_mid, 3)),
+                               workspace=self._pair_ws("ws_mid", n_mid, s_b + s_w, lats_n), lattices=lats_n)
         for s, gd in enumerate(self.grid_disc):
             sa, sb = self._s_off[s], self._s_off[s + 1]
             gw0 = gw_n[s][0] if gd.m_star > 0 else None

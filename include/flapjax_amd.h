@@ -154,6 +154,22 @@ int fj_pair_sum_lattices(const double* seg, int n_lat, const fj_lattice_ref* lat
                          double rcut, int mode, const fj_flowfield* flow, double time, double dt,
                          void* workspace, void* stream);
 
+/* fj_pair_sum_lattices over two target ranges that share one filament table:
+ * targets [0, n_tgt_a) get epilogue mode_a into out_a ((n_tgt_a, 3)),
+ * targets [n_tgt_a, n_tgt) mode_b into out_b ((n_tgt - n_tgt_a, 3)); flow,
+ * time and dt serve whichever range uses FJ_OUT_CONVECT.  One launch of the
+ * pair kernel for both.  No single counterpart in the reference: it fuses
+ * the forces' induced velocity at the bound filament midpoints of step n
+ * (aero/uvlm.py:1018-1050, compute_v_ind over bound and wake) with the
+ * free-wake velocity of step n + 1 at the wake vertices
+ * (aero/uvlm.py:912-941, propagate_wake), which sum over the same lattices
+ * with the same circulations.  Workspace as fj_pair_sum_lattices for n_tgt. */
+int fj_pair_sum_lattices_split(const double* seg, int n_lat, const fj_lattice_ref* lats,
+                               const double* targets, int n_tgt, int n_tgt_a, double* out_a, int mode_a,
+                               double* out_b, int mode_b, int kernel, double eps, double rcut,
+                               const fj_flowfield* flow, double time, double dt, void* workspace,
+                               void* stream);
+
 /* fj_pair_sum_lattices without a workspace argument, for callers that cannot
  * allocate device memory (code under HIP graph capture, tight serving loops,
  * harnesses that forbid hipMalloc).  No counterpart in the reference.  It

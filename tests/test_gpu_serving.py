@@ -2,7 +2,8 @@
 
 * each case served in a mix equals the same case served alone (1e-12);
 * a served case equals the plain UVLM drop-in run of that case (1e-10 on the
-  lift history, circulation and wake), free and prescribed wake;
+  lift history, circulation and wake), free and prescribed wake, and the
+  oracle's FP64 run of it (1e-10 lift and circulation, 1e-8 wake);
 * the mixed form served in a mix equals the same case served alone and stays
   within 1e-6 (circulation, wake) / 1e-5 (lift) of the FP64 drop-in;
 * the fused multi-case pair sum equals per-case pair sums.
@@ -85,6 +86,31 @@ def test_mix_equals_alone_and_equals_uvlm(dev, free):
         assert rel_err(b.gamma_b_final, d.gamma_b[0][-1].cpu().numpy()) < 1e-10
         assert rel_err(b.zeta_w_final, d.zeta_w[0][-1].cpu().numpy()) < 1e-10
     print("serving stats:", stats)
+
+
+@pytest.mark.parametrize("free", [True, False])
+def test_served_cases_equal_oracle(dev, free):
+    """Served in a mix, three cases of the stream (different lattices, wake
+    lengths and motions) against the oracle's FP64 run of each: lift
+    history, final circulation (1e-10) and final wake (1e-8), norm-wise."""
+    from flapjax_amd.serving import ServingEngine, motion
+    from oracle import uvlm_oracle as O
+
+    reqs = _stream()
+    mix, _ = ServingEngine(free_wake=free).run(reqs)
+    by_id = {r.case_id: r for r in mix}
+    for req in (reqs[0], reqs[2], reqs[6]):
+        hg, hgd = motion(req)
+        orc = O.UVLMOracle([(req.m, req.n, req.m_star)], np.arange(req.n + 1), free_wake=free)
+        z0 = np.zeros((req.m + 1, req.n + 1, 3))
+        z0[..., 0] = (np.linspace(0.0, 1.0, req.m + 1) - 0.25)[:, None]
+        orc.set_design_variables(1.0 / req.m, O.Constant(np.array([1.0, 0.0, 0.0]), 1.225), z0, hg[0])
+        d = orc.solve_prescribed_dynamic(orc.solve_static(hg=hg[0]), hg, hgd)
+        lift = np.array([d.f_steady[0][i][..., 2].sum() for i in range(req.steps)])
+        b = by_id[req.case_id]
+        assert rel_err(b.lift[1:], lift[1:]) < 1e-10, req
+        assert rel_err(b.gamma_b_final, d.gamma_b[0][-1]) < 1e-10, req
+        assert rel_err(b.zeta_w_final, d.zeta_w[0][-1]) < 1e-8, req
 
 
 def test_mixed_form_served(dev):
