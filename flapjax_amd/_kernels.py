@@ -255,6 +255,43 @@ def pair_sum_lattices(seg, lattices, targets, kernel_kind, eps, rcut, mode=L.OUT
     return out
 
 
+def pair_sum_split(seg, targets, n_a, kernel_kind, eps, rcut, out_a, mode_a, out_b, mode_b,
+                   precision=L.PREC_FP64, flow=None, time=0.0, dt=0.0, workspace=None, lattices=None):
+    """pair_sum over two target ranges in one pair-kernel launch: rows
+    [0, n_a) of ``targets`` get ``mode_a`` into ``out_a`` and the rest
+    ``mode_b`` into ``out_b``; the lattice kernel when the FP64 table holds
+    whole lattices (fj_pair_sum_lattices_split), else the filament-table
+    kernels of ``precision`` (fj_pair_sum_split)."""
+    lats = _lattice_path(seg, precision, lattices)
+    if lats is not None:
+        return pair_sum_lattices_split(seg, lats, targets, n_a, kernel_kind, eps, rcut, out_a, mode_a, out_b,
+                                       mode_b, flow, time, dt, workspace)
+    flat = _c64(targets, (3,), "targets").reshape(-1, 3)
+    nt = flat.shape[0]
+    _check_split_outputs(nt, n_a, out_a, out_b)
+    ns = 0 if seg is None else seg.shape[0]
+    ws_bytes = L.lib().fj_pair_sum_workspace_bytes(nt, ns, int(precision))
+    if workspace is not None and workspace.numel() * 8 >= ws_bytes:
+        ws = workspace
+    else:
+        ws = torch.empty(ws_bytes // 8 + 1, dtype=F64, device=flat.device)
+    fd = flow_desc(flow) if flow is not None else None
+    rc = L.lib().fj_pair_sum_split(L.ptr(seg) if ns else None, ns, L.ptr(flat), nt, int(n_a),
+                                   L.ptr(out_a) if n_a else None, int(mode_a), L.ptr(out_b) if nt - n_a else None,
+                                   int(mode_b), int(kernel_kind), float(eps), float(rcut), int(precision),
+                                   ctypes.byref(fd) if fd is not None else None, float(time), float(dt), L.ptr(ws),
+                                   L.stream())
+    L.check(rc, "fj_pair_sum_split")
+
+
+def _check_split_outputs(nt, n_a, out_a, out_b):
+    if not 0 <= n_a <= nt:
+        raise ValueError("n_a must lie in [0, number of targets]")
+    for o, k in ((out_a, n_a), (out_b, nt - n_a)):
+        if k and (o is None or not o.is_contiguous() or o.numel() != 3 * k or o.dtype != F64):
+            raise ValueError("each output must be a contiguous float64 tensor of its range's (T, 3)")
+
+
 def pair_sum_lattices_split(seg, lattices, targets, n_a, kernel_kind, eps, rcut, out_a, mode_a, out_b, mode_b,
                             flow=None, time=0.0, dt=0.0, workspace=None):
     """pair_sum_lattices over two target ranges in one pair-kernel launch:
@@ -264,11 +301,7 @@ def pair_sum_lattices_split(seg, lattices, targets, n_a, kernel_kind, eps, rcut,
         raise ValueError(f"at most {MAX_LATTICES} lattices per call")
     flat = _c64(targets, (3,), "targets").reshape(-1, 3)
     nt = flat.shape[0]
-    if not 0 <= n_a <= nt:
-        raise ValueError("n_a must lie in [0, number of targets]")
-    for o, k in ((out_a, n_a), (out_b, nt - n_a)):
-        if k and (o is None or not o.is_contiguous() or o.numel() != 3 * k or o.dtype != F64):
-            raise ValueError("each output must be a contiguous float64 tensor of its range's (T, 3)")
+    _check_split_outputs(nt, n_a, out_a, out_b)
     refs = lattice_refs(lattices)
     nl = len(lattices)
     ws_bytes = L.lib().fj_pair_sum_lattices_workspace_bytes(nt, nl, refs)

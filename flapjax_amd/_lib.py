@@ -68,6 +68,9 @@ _SIGS = {
     "fj_pair_sum_lattices_workspace_bytes": ([c_int, c_int, c_void_p], c_size_t),
     "fj_pair_sum_lattices": ([c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p, c_int, c_double, c_double,
                               c_int, POINTER(FjFlowfield), c_double, c_double, c_void_p, c_void_p], c_int),
+    "fj_pair_sum_split": ([c_void_p, c_int, c_void_p, c_int, c_int, c_void_p, c_int, c_void_p, c_int, c_int,
+                           c_double, c_double, c_int, POINTER(FjFlowfield), c_double, c_double, c_void_p, c_void_p],
+                          c_int),
     "fj_pair_sum_lattices_split": ([c_void_p, c_int, c_void_p, c_void_p, c_int, c_int, c_void_p, c_int, c_void_p,
                                     c_int, c_int, c_double, c_double, POINTER(FjFlowfield), c_double, c_double,
                                     c_void_p, c_void_p], c_int),

@@ -155,7 +155,10 @@ static FjPairPlan fj_pair_plan(int n_tgt, int n_seg) {
   if (chunks < 1) chunks = 1;
   p.chunk_len = fj_cdiv(n_seg > 0 ? n_seg : 1, chunks);
   p.chunks = fj_cdiv(n_seg > 0 ? n_seg : 1, p.chunk_len);
-  p.ld = p.tiles * 64 * p.tpl;
+  // one spare row of 64 past the tiles: a split epilogue reduces its second
+  // target range from an offset that is not a multiple of 64, and its last
+  // wave's reads then stay inside the partial row
+  p.ld = (p.tiles * p.tpl + 1) * 64;
   return p;
 }
 
@@ -335,19 +338,28 @@ int fj_pair_epilogue(const double* part, int chunks, int ld, int n_tgt, const do
   return FJ_OK;
 }
 
-extern "C" int fj_pair_sum(const double* seg, int n_seg, const double* targets, int n_tgt,
-                           double* out, int kernel, double eps, double rcut, int precision,
-                           int mode, const fj_flowfield* flow, double time, double dt,
-                           void* workspace, void* stream) {
-  if (n_tgt < 0 || n_seg < 0) return FJ_ERR_ARG;
+// fj_pair_sum over two target ranges: [0, n_a) with mode_a into out_a,
+// [n_a, n_tgt) with mode_b into out_b (n_a = n_tgt: one range)
+static int fj_pair_run(const double* seg, int n_seg, const double* targets, int n_tgt, int n_a,
+                       double* out_a, int mode_a, double* out_b, int mode_b, int kernel, double eps,
+                       double rcut, int precision, const fj_flowfield* flow, double time, double dt,
+                       void* workspace, hipStream_t st) {
+  if (n_tgt < 0 || n_seg < 0 || n_a < 0 || n_a > n_tgt) return FJ_ERR_ARG;
   if (kernel < FJ_KERNEL_EPSILON || kernel > FJ_KERNEL_PLAIN) return FJ_ERR_ARG;
-  if (mode < FJ_OUT_STORE || mode > FJ_OUT_CONVECT) return FJ_ERR_ARG;
+  if (mode_a < FJ_OUT_STORE || mode_a > FJ_OUT_CONVECT) return FJ_ERR_ARG;
+  if (n_a < n_tgt && (mode_b < FJ_OUT_STORE || mode_b > FJ_OUT_CONVECT || !out_b)) return FJ_ERR_ARG;
   if (precision != FJ_PREC_FP64 && precision != FJ_PREC_MIXED) return FJ_ERR_ARG;
   if (n_tgt == 0) return FJ_OK;
-  if (!targets || !out) return FJ_ERR_ARG;
-  hipStream_t st = (hipStream_t)stream;
+  if (!targets || (n_a > 0 && !out_a)) return FJ_ERR_ARG;
   const FjFlow fl = fj_flow_from_desc(flow);
-  if (n_seg == 0) return fj_pair_epilogue(nullptr, 0, 0, n_tgt, targets, out, mode, fl, time, dt, st);
+  const int n_b = n_tgt - n_a;
+  const double* tgt_b = targets + 3 * (size_t)n_a;
+  if (n_seg == 0) {
+    int r = FJ_OK;
+    if (n_a > 0) r = fj_pair_epilogue(nullptr, 0, 0, n_a, targets, out_a, mode_a, fl, time, dt, st);
+    if (r == FJ_OK && n_b > 0) r = fj_pair_epilogue(nullptr, 0, 0, n_b, tgt_b, out_b, mode_b, fl, time, dt, st);
+    return r;
+  }
   if (!seg || !workspace) return FJ_ERR_ARG;
   FjKernelParams kp;
   kp.eps = eps; kp.eps2 = eps * eps; kp.rcut = rcut;
@@ -366,5 +378,26 @@ extern "C" int fj_pair_sum(const double* seg, int n_seg, const double* targets, 
 #undef FJ_DISPATCH_TPL
   }
   FJ_CHECK_LAUNCH();
-  return fj_pair_epilogue(part, p.chunks, p.ld, n_tgt, targets, out, mode, fl, time, dt, st);
+  int r = FJ_OK;
+  if (n_a > 0) r = fj_pair_epilogue(part, p.chunks, p.ld, n_a, targets, out_a, mode_a, fl, time, dt, st);
+  if (r == FJ_OK && n_b > 0)
+    r = fj_pair_epilogue(part + n_a, p.chunks, p.ld, n_b, tgt_b, out_b, mode_b, fl, time, dt, st);
+  return r;
+}
+
+extern "C" int fj_pair_sum(const double* seg, int n_seg, const double* targets, int n_tgt,
+                           double* out, int kernel, double eps, double rcut, int precision,
+                           int mode, const fj_flowfield* flow, double time, double dt,
+                           void* workspace, void* stream) {
+  return fj_pair_run(seg, n_seg, targets, n_tgt, n_tgt, out, mode, nullptr, FJ_OUT_STORE, kernel, eps, rcut,
+                     precision, flow, time, dt, workspace, (hipStream_t)stream);
+}
+
+extern "C" int fj_pair_sum_split(const double* seg, int n_seg, const double* targets, int n_tgt,
+                                 int n_tgt_a, double* out_a, int mode_a, double* out_b, int mode_b,
+                                 int kernel, double eps, double rcut, int precision,
+                                 const fj_flowfield* flow, double time, double dt, void* workspace,
+                                 void* stream) {
+  return fj_pair_run(seg, n_seg, targets, n_tgt, n_tgt_a, out_a, mode_a, out_b, mode_b, kernel, eps, rcut,
+                     precision, flow, time, dt, workspace, (hipStream_t)stream);
 }

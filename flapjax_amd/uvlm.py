@@ -555,15 +555,15 @@ class UVLM:
         single = self.n_surf == 1
         n_b = self.n_b
         n_mid = int(self._s_off[-1])
-        # Free wake, one surface, FP64: the forces' pair sum of this step and
-        # the rollup of the next one sum over the same lattices with the same
+        # Free wake, one surface: the forces' pair sum of this step and the
+        # rollup of the next one sum over the same lattices with the same
         # circulations (bound and wake at n), so one launch serves both
-        # (fj_pair_sum_lattices_split): targets = force midpoints, then the
-        # wake vertices, whose convected positions go straight to the next
-        # step's wake rows.  The next step skips its rollup when it finds the
-        # mark of this step and the same pose as its hg_nm1.
-        fuse_ok = (self.fuse_rollup and self.free_wake and single and self._prec == L.PREC_FP64
-                   and self.grid_disc[0].m_star > 0)
+        # (fj_pair_sum_lattices_split in FP64, fj_pair_sum_split in the
+        # mixed form): targets = force midpoints, then the wake vertices,
+        # whose convected positions go straight to the next step's wake rows.
+        # The next step skips its rollup when it finds the mark of this step
+        # and the same pose as its hg_nm1.
+        fuse_ok = self.fuse_rollup and self.free_wake and single and self.grid_disc[0].m_star > 0
         fuse_next = fuse_ok and i_ts + 1 < len(case.t)
         rolled = fuse_ok and getattr(case, "_fj_rollup_mark", None) == (
             id(self), self._design_version, i_ts, hg_nm1.data_ptr())
@@ -640,14 +640,14 @@ class UVLM:
         if s_b:
             K.segments_from_grids(grids_b, kern.kind, kern.eps, self.mirror, out=seg_n[:s_b])
         lats_n = K.lattices_of(grids_b + grids_w)
-        if fuse_next and len(lats_n) <= K.MAX_LATTICES:
+        if fuse_next:
             tgt_all[n_mid:].copy_(zw_n[0][:-1].reshape(-1, 3))
             v_mid = self._buf("v_mid", (n_mid, 3))
             t_next = float(case.t[i_ts] + self.dt)  # the next step's time, as case_solve sets it
-            K.pair_sum_lattices_split(seg_n, lats_n, tgt_all, n_mid, kern.kind, kern.eps, kern.rcut, v_mid,
-                                      L.OUT_STORE, case.zeta_w[0][i_ts + 1][1:].reshape(-1, 3), L.OUT_CONVECT,
-                                      flow=ff, time=t_next, dt=dt,
-                                      workspace=self._pair_ws("ws_fused", tgt_all.shape[0], s_b + s_w, lats_n))
+            K.pair_sum_split(seg_n, tgt_all, n_mid, kern.kind, kern.eps, kern.rcut, v_mid, L.OUT_STORE,
+                             case.zeta_w[0][i_ts + 1][1:].reshape(-1, 3), L.OUT_CONVECT, self._prec, flow=ff,
+                             time=t_next, dt=dt, lattices=lats_n,
+                             workspace=self._pair_ws("ws_fused", tgt_all.shape[0], s_b + s_w, lats_n))
             case._fj_rollup_mark = (id(self), self._design_version, i_ts + 1, hg_n.data_ptr())
         else:
             v_mid = K.pair_sum(seg_n if s_b + s_w else None, mids, kern.kind, kern.eps, kern.rcut, self._prec,

@@ -131,6 +131,18 @@ int fj_pair_sum(const double* seg, int n_seg, const double* targets, int n_tgt, 
                 const fj_flowfield* flow, double time, double dt, void* workspace,
                 void* stream);
 
+/* fj_pair_sum over two target ranges in one pair-kernel launch: targets
+ * [0, n_tgt_a) get epilogue mode_a into out_a, targets [n_tgt_a, n_tgt)
+ * mode_b into out_b; flow, time and dt serve whichever range uses
+ * FJ_OUT_CONVECT.  The filament-table counterpart of
+ * fj_pair_sum_lattices_split (the UVLM step's forces at step n and rollup
+ * at step n + 1 in the mixed form: aero/uvlm.py:1018-1050 and 912-941).
+ * Workspace as fj_pair_sum for n_tgt. */
+int fj_pair_sum_split(const double* seg, int n_seg, const double* targets, int n_tgt, int n_tgt_a,
+                      double* out_a, int mode_a, double* out_b, int mode_b, int kernel, double eps,
+                      double rcut, int precision, const fj_flowfield* flow, double time, double dt,
+                      void* workspace, void* stream);
+
 /* One vortex lattice inside a filament table: the S(m, n) = m(n+1) + (m+1)n
  * records written for an (m+1, n+1) vertex grid by fj_segments_from_grid(s)
  * (chordwise filaments first, then spanwise), starting at row seg_off. */

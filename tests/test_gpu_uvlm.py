@@ -190,12 +190,16 @@ def test_fused_step_equals_general_path(mirror, two_surf, free):
             assert rel_err(x, y) <= 1e-15, (key, s, rel_err(x, y))
 
 
-def test_forces_and_next_rollup_in_one_launch():
-    """Free wake, one surface, FP64: the forces' pair sum of step n and the
-    rollup of step n + 1 run as one launch (fj_pair_sum_lattices_split); the
-    results equal separate launches, also when a step is solved twice (the
-    second solve of step n finds no mark for n and rolls up itself) and when
-    the next step is given another pose buffer as hg_nm1 (no fusion)."""
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_forces_and_next_rollup_in_one_launch(precision):
+    """Free wake, one surface: the forces' pair sum of step n and the rollup
+    of step n + 1 run as one launch (fj_pair_sum_lattices_split in FP64,
+    fj_pair_sum_split in the mixed form); the results equal separate
+    launches -- in FP64 to the last bit's order (the same partial plan), in
+    the mixed form within its FP32 far-field error (the fused launch groups
+    the FP32 partial sums differently) -- also when a step is solved twice
+    (the second solve of step n finds no mark for n and rolls up itself) and
+    when the next step is given another pose buffer as hg_nm1 (no fusion)."""
     import torch
 
     import flapjax_amd as fj
@@ -209,7 +213,8 @@ def test_forces_and_next_rollup_in_one_launch():
     mp, mn = np.zeros(3), np.array([0.0, 1.0, 0.0])
 
     def run(fuse, resolve=(), copy_at=()):
-        uv = UVLM([(m, n, m_star)], np.arange(n + 1), free_wake=True, mirror_point=mp, mirror_normal=mn)
+        uv = UVLM([(m, n, m_star)], np.arange(n + 1), free_wake=True, mirror_point=mp, mirror_normal=mn,
+                  precision=precision)
         uv.set_design_variables(0.2, fj.Constant(np.array([1.0, 0.0, 0.1]), 1.225, True), zeta0, hg_np[0])
         uv.fuse_rollup = fuse
         case = uv.solve_static(hg=hg[0]).to_dynamic(0, steps)
@@ -226,4 +231,6 @@ def test_forces_and_next_rollup_in_one_launch():
         got = run(True, **kw)
         for key in ("gamma_b", "gamma_w", "zeta_w", "f_steady", "f_unsteady", "gamma_b_dot", "alpha"):
             e = rel_err(getattr(got, key)[0].cpu().numpy(), getattr(ref, key)[0].cpu().numpy())
-            assert e <= 1e-15, (kw, key, e)
+            tol = 1e-15 if precision == "fp64" else (1e-4 if key in ("f_steady", "f_unsteady", "gamma_b_dot")
+                                                     else 1e-5)
+            assert e <= tol, (kw, key, e)
