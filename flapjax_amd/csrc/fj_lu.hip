@@ -1341,6 +1341,9 @@ __global__ __launch_bounds__(FJ_BAND_FIXED_THREADS) void k_band_solve_fixed(cons
   static_assert(KL == KU && KL + 1 == FJ_BAND_FIXED_GROUPS && KL < 16, "band shape");
   constexpr int QMAX = (FJ_BAND_LDS_DOUBLES / (W + 2) + 63) / 64;  // x registers per lane
   __shared__ double s[FJ_BAND_LDS_DOUBLES];
+  // a lane's slot for the stores it does not own in a column step, so that
+  // every lane stores unconditionally (no divergent branch around a store)
+  __shared__ double sink[FJ_BAND_FIXED_THREADS];
   const int tid = threadIdx.x;
   const int rows = n + KL;
   double* xs = s + (size_t)rows * W;  // b, then y = L^-1 P b
@@ -1408,14 +1411,16 @@ __global__ __launch_bounds__(FJ_BAND_FIXED_THREADS) void k_band_solve_fixed(cons
     const unsigned h0 =
         __builtin_amdgcn_readfirstlane((unsigned)((unsigned long long)__double_as_longlong(a0) >> 32) & 0x7FFFFFFFu);
     double rp = 1.0 / a0;
-    if (below < h0) {
+    if (__builtin_expect(below < h0, 1)) {
       // no exchange: row k is the pivot row (t)
       ju = max(ju, min(k + KU, n - 1));
       const int nj = ju - k;
       const double li = al * rp;
       const double v1 = r1 - li * t1, v2 = r2 - li * t2;
-      if (st1 && jl1 <= nj) *(b0 ? dg + k : pr1) = b0 ? a0 : v1;
-      if (st2 && jl2 <= nj) *pr2 = v2;
+      double* const d1 = (st1 && jl1 <= nj) ? (b0 ? dg + k : pr1) : sink + tid;
+      double* const d2 = (st2 && jl2 <= nj) ? pr2 : sink + tid;
+      *d1 = b0 ? a0 : v1;
+      *d2 = v2;
     } else {
       double a[KL + 1];
 #pragma unroll
