@@ -161,7 +161,7 @@ def pair_workspace_bytes(n_tgt, n_seg, precision=L.PREC_FP64, lattices=None):
     """Workspace bytes of the pair sum that ``pair_sum`` runs for this size,
     precision and lattice list."""
     if int(precision) == L.PREC_FP64 and lattices and len(lattices) <= MAX_LATTICES and n_tgt > 0:
-        return L.lib().fj_pair_sum_lattices_workspace_bytes(n_tgt, len(lattices), lattice_refs(lattices))
+        return lattice_workspace_bytes(n_tgt, lattices)
     return L.lib().fj_pair_sum_workspace_bytes(n_tgt, n_seg, int(precision))
 
 
@@ -215,12 +215,36 @@ def lattices_of(grids, base=0):
     return out
 
 
+_REFS: dict = {}
+_LAT_WS: dict = {}
+
+
 def lattice_refs(lattices):
-    """ctypes array of fj_lattice_ref from (seg_off, m, n) tuples."""
-    arr = (L.FjLatticeRef * max(1, len(lattices)))()
-    for k, (off, m, n) in enumerate(lattices):
-        arr[k].seg_off, arr[k].m, arr[k].n = int(off), int(m), int(n)
+    """ctypes array of fj_lattice_ref from (seg_off, m, n) tuples (kept per
+    list: a step builds the same few lists every time, and the callee only
+    reads the array)."""
+    key = tuple((int(o), int(m), int(n)) for o, m, n in lattices)
+    arr = _REFS.get(key)
+    if arr is None:
+        if len(_REFS) > 1024:
+            _REFS.clear()
+        arr = (L.FjLatticeRef * max(1, len(key)))()
+        for k, (off, m, n) in enumerate(key):
+            arr[k].seg_off, arr[k].m, arr[k].n = off, m, n
+        _REFS[key] = arr
     return arr
+
+
+def lattice_workspace_bytes(n_tgt, lattices):
+    """fj_pair_sum_lattices_workspace_bytes, kept per (targets, lattices)."""
+    key = (int(n_tgt), tuple((int(o), int(m), int(n)) for o, m, n in lattices))
+    nb = _LAT_WS.get(key)
+    if nb is None:
+        if len(_LAT_WS) > 1024:
+            _LAT_WS.clear()
+        nb = L.lib().fj_pair_sum_lattices_workspace_bytes(key[0], len(key[1]), lattice_refs(key[1]))
+        _LAT_WS[key] = nb
+    return nb
 
 
 def pair_sum_lattices(seg, lattices, targets, kernel_kind, eps, rcut, mode=L.OUT_STORE, out=None, flow=None,
@@ -241,7 +265,7 @@ def pair_sum_lattices(seg, lattices, targets, kernel_kind, eps, rcut, mode=L.OUT
         return out
     refs = lattice_refs(lattices)
     nl = len(lattices)
-    ws_bytes = L.lib().fj_pair_sum_lattices_workspace_bytes(nt, nl, refs)
+    ws_bytes = lattice_workspace_bytes(nt, lattices)
     if workspace is not None and workspace.numel() * 8 >= ws_bytes:
         ws = workspace
     else:
@@ -304,7 +328,7 @@ def pair_sum_lattices_split(seg, lattices, targets, n_a, kernel_kind, eps, rcut,
     _check_split_outputs(nt, n_a, out_a, out_b)
     refs = lattice_refs(lattices)
     nl = len(lattices)
-    ws_bytes = L.lib().fj_pair_sum_lattices_workspace_bytes(nt, nl, refs)
+    ws_bytes = lattice_workspace_bytes(nt, lattices)
     if workspace is not None and workspace.numel() * 8 >= ws_bytes:
         ws = workspace
     else:

@@ -209,6 +209,8 @@ class AeroCase:
         for name in FIELDS:
             for dst, src in zip(getattr(self, name), getattr(snapshot, name)):
                 dst[i].copy_(src[0])
+        # a wake the UVLM step convected ahead into a later step is replaced
+        self._fj_rollup_mark = None
 
     def __getitem__(self, idx: int) -> "AeroCase":
         """aero/data_structures.py:533-568: a one-step case (views) of step idx."""
