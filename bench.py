@@ -21,6 +21,7 @@ over all host threads, on a bounded sample).
 from __future__ import annotations
 
 import argparse
+import gc
 import json
 import os
 import sys
@@ -39,6 +40,16 @@ PEAK_FP64 = 78.6  # TFLOP/s, MI355X FP64 vector (datasheet)
 # 256 MiB device copy (FETCH_SIZE reads 0.5x the copied bytes on MI355X,
 # WRITE_SIZE 1.0x); re-measured whenever the pair-sum launch shape changes.
 ROLLUP_TRAFFIC_BYTES = 47_463_000
+
+
+def quiesce():
+    """Collect, then freeze every object alive so far (gc.freeze) before a
+    timed region: a full collection otherwise rescans the interpreter's ~1e6
+    long-lived objects (torch, numpy, scipy) and stalls a step every few
+    hundred by ~40 ms.  A serving process would freeze after start-up the
+    same way; the library itself allocates no more per step than before."""
+    gc.collect()
+    gc.freeze()
 
 
 def n_seg(m, n):
@@ -154,6 +165,7 @@ def time_form(precision, steps, warmup, aic_reuse=True, whole_case=True):
     for _ in range(warmup):
         step()
     torch.cuda.synchronize()
+    quiesce()
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
     t0 = time.perf_counter()
     ev[0].record()
@@ -168,6 +180,7 @@ def time_form(precision, steps, warmup, aic_reuse=True, whole_case=True):
                 "p99_ms": float(np.percentile(lat, 99)), "case_s": float("nan")}
     # whole case (static + 199 dynamic steps) end to end
     torch.cuda.synchronize()
+    quiesce()
     tc = time.perf_counter()
     s2 = uv.solve_static(hg=hg[0])
     c2 = s2.to_dynamic(0, N_TSTEP)
@@ -281,6 +294,7 @@ def coupled_bench(precision, steps=CPL_STEPS):
     del warm
     cw = make_coupled(precision)
     torch.cuda.synchronize()
+    quiesce()
     t0 = time.perf_counter()
     st = cw.static_solve(np.arange(6))
     torch.cuda.synchronize()
@@ -367,14 +381,17 @@ def adjoint_bench(steps=ADJ_STEPS, n_dir=2, seed=5, check=True):
     # caching allocator to the history's footprint: timed on its own, then
     # the gradient is timed warm, as the production forward is
     torch.cuda.synchronize()
+    quiesce()
     t0 = time.perf_counter()
     model.gradient(ks, tw, steps, ADJ_TIP_WEIGHT)
     torch.cuda.synchronize()
     t_first = time.perf_counter() - t0
+    quiesce()
     t0 = time.perf_counter()
     J, g_k, g_t = model.gradient(ks, tw, steps, ADJ_TIP_WEIGHT)
     torch.cuda.synchronize()
     t_grad = time.perf_counter() - t0
+    quiesce()
     t1 = time.perf_counter()
     J_prod = production(ks, tw)
     t_fwd = time.perf_counter() - t1
@@ -655,6 +672,7 @@ def serve(precision, n_cases, rate, m_star_cap, steps_range):
         poisson_stream(4, 1e6, seed=5, m_star=8, steps_range=(5, 8)))
     torch.cuda.synchronize()
     eng = ServingEngine(precision=precision, max_slots=64)
+    quiesce()
     _, stats = eng.run(reqs)
     return stats
 

@@ -263,7 +263,7 @@ def ptr(t):
         raise ValueError("flapjax_amd expects device tensors")
     if not t.is_contiguous():
         raise ValueError("flapjax_amd expects contiguous tensors")
-    return ctypes.c_void_p(t.data_ptr())
+    return t.data_ptr()  # ctypes passes an int as the c_void_p argument
 
 
 _raw_stream = getattr(torch._C, "_cuda_getCurrentRawStream", None)

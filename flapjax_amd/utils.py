@@ -66,14 +66,24 @@ def as_segment_kernel(kernel) -> SegmentKernel:
     )
 
 
+_gpu_seen = False
+
+
 def _device():
-    if not torch.cuda.is_available():
-        raise L.FlapjaxLibError("flapjax_amd needs a ROCm GPU; none is visible (there is no CPU fallback)")
+    global _gpu_seen
+    if not _gpu_seen:
+        if not torch.cuda.is_available():
+            raise L.FlapjaxLibError("flapjax_amd needs a ROCm GPU; none is visible (there is no CPU fallback)")
+        _gpu_seen = True
     return torch.device("cuda", torch.cuda.current_device())
 
 
 def to_device(x) -> torch.Tensor:
     if isinstance(x, torch.Tensor):
+        # already what the kernels take (the common case inside a time loop)
+        if x.is_cuda and x.dtype == F64 and _gpu_seen and x.get_device() == torch.cuda.current_device() \
+                and x.is_contiguous():
+            return x
         return x.to(device=_device(), dtype=F64).contiguous()
     return torch.as_tensor(np.asarray(x, dtype=np.float64), device=_device()).contiguous()
 

@@ -585,6 +585,10 @@ class UVLM:
             K.step_geometry(hg_n, hg_nm1, hg_dot_n, self._dof[s], self.zeta_b0[s], zeta_n=snap["zeta_b"][s],
                             zeta_nm1=zb_nm1[s], zeta_dot_n=snap["zeta_b_dot"][s], c=c_all[a:b], nc=nc_all[a:b],
                             c_dot=cd_all[a:b], mid=mids[sa:sb])
+        # ---- the bound-on-bound AIC needs only this step's geometry: issued
+        # first, so the device has a long kernel while the host prepares the
+        # wake launches (aero/uvlm.py:944-993)
+        A = self._assemble_aic(c_all, nc_all, snap["zeta_b"], None)
         # ---- wake: circulation shift, TE row, convection (aero/utils.py:400-577)
         zw_n, gw_n = snap["zeta_w"], snap["gamma_w"]
         seg_nm1 = None
@@ -608,7 +612,6 @@ class UVLM:
             else:
                 K.wake_convect(prev["zeta_w"][s], None, ff, t_n, dt, zw_n[s])
         # ---- system (aero/uvlm.py:944-993)
-        A = self._assemble_aic(c_all, nc_all, snap["zeta_b"], None)
         grids_b = self._grid_list(snap["zeta_b"], snap["gamma_b"])
         grids_w = self._grid_list(zw_n, gw_n)
         s_b, s_w = self._table_len(grids_b), self._table_len(grids_w)
