@@ -148,6 +148,8 @@ _SIGS = {
                                 c_double, c_double, c_double, c_void_p, c_int, c_int, c_int, c_void_p]
                                + [c_void_p] * 11, c_int),
     "fj_beam_nodes": ([c_int, c_int] + [c_void_p] * 7, c_int),
+    "fj_beam_step_end": ([c_int, c_double, c_double, c_double, c_double, c_double] + [c_void_p] * 7
+                         + [c_int, c_int] + [c_void_p] * 14 + [c_void_p], c_int),
     "fj_beam_project_forces": ([c_int, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
                                 c_void_p, c_void_p], c_int),
 }

@@ -545,6 +545,8 @@ class BeamStructure:
     # ----------------------------------------------------------------- dynamic
     #: Newton iterations enqueued before the first status read of a loop
     newton_batch = 3
+    #: end a coupled step (one aero surface) with fj_beam_step_end
+    fused_step_end = True
 
     def _conv_struct(self):
         s = self.struct_convergence_settings
@@ -718,10 +720,15 @@ class BeamStructure:
                                           L.ptr(fsi["f_al"]), L.ptr(fsi["f_prev"]), L.ptr(nrm_dev), L.stream()),
                     "fj_beam_fsi_norms")
             nrm_host.copy_(nrm_dev, non_blocking=True)
+        predicted = None
         for i in range(1, n_tstep):
             # predict_q + calculate_q_alpha (structure/time_integration.py:86-150)
-            phi_a, v_a, vd_a = (torch.empty((nn, 6), dtype=F64, device=dev) for _ in range(3))
-            ga(L.GA_PREDICT, (v[i - 1], vd[i - 1], acc[i - 1]), (phi_a, v_a, vd_a))
+            if predicted is not None:
+                phi_a, v_a, vd_a = predicted  # by the previous step's fj_beam_step_end
+                predicted = None
+            else:
+                phi_a, v_a, vd_a = (torch.empty((nn, 6), dtype=F64, device=dev) for _ in range(3))
+                ga(L.GA_PREDICT, (v[i - 1], vd[i - 1], acc[i - 1]), (phi_a, v_a, vd_a))
             if aero is None:
                 load_step_loop(i, phi_a, v_a, vd_a, None)
             else:
@@ -748,7 +755,28 @@ class BeamStructure:
                     f_prev = f_al
                 fsi_iters.append(fst.i_iter)
             # calculate_q_n_from_q_alpha + calculate_a_n (structure/time_integration.py:53-84)
-            # and the step's frames, in one launch
+            # and the step's frames, in one launch; with one aero surface
+            # also the force projection, its local copy and the next step's
+            # predictor (fj_beam_step_end: the same arithmetic, one launch
+            # instead of six)
+            se = None
+            if aero is not None and self.fused_step_end and nn <= 512 and hasattr(aero, "step_end_inputs"):
+                se = aero.step_end_inputs(i)
+            if se is not None:
+                fa_i, fb_i, x0_i, dof_i = se
+                nxt = None
+                if i + 1 < n_tstep:
+                    nxt = tuple(torch.empty((nn, 6), dtype=F64, device=dev) for _ in range(3))
+                m1, n1 = fa_i.shape[0], fa_i.shape[1]
+                L.check(lib.fj_beam_step_end(
+                    nn, am, af, g_, b_, ti.dt, L.ptr(phi_a), L.ptr(v_a), L.ptr(vd_a), L.ptr(v[i - 1]),
+                    L.ptr(vd[i - 1]), L.ptr(acc[i - 1]), L.ptr(hg[i - 1]), m1, n1, L.ptr(fa_i.contiguous()),
+                    L.ptr(None if fb_i is None else fb_i.contiguous()), L.ptr(x0_i.contiguous()), L.ptr(dof_i),
+                    L.ptr(phi), L.ptr(v[i]), L.ptr(vd[i]), L.ptr(acc[i]), L.ptr(hg[i]), L.ptr(f_aero_g[i]),
+                    L.ptr(f_aero_l[i]), *(L.ptr(x) for x in (nxt or (None, None, None))), L.stream()),
+                    "fj_beam_step_end")
+                predicted = nxt
+                continue
             L.check(lib.fj_beam_ga_nodes(L.GA_FINISH, nn, am, af, L.ptr(phi_a), L.ptr(v_a), L.ptr(vd_a),
                                          L.ptr(v[i - 1]), L.ptr(vd[i - 1]), L.ptr(acc[i - 1]), L.ptr(hg[i - 1]),
                                          L.ptr(phi), L.ptr(v[i]), L.ptr(vd[i]), L.ptr(acc[i]), None, L.ptr(hg[i]),

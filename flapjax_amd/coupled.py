@@ -225,6 +225,16 @@ class CoupledAeroelastic:
             def project(i, hg_n, out=None):
                 return outer._project(aero_case, i, hg_n, unsteady, out=out)
 
+            @staticmethod
+            def step_end_inputs(i):
+                # one surface: what fj_beam_step_end projects (vertex forces
+                # of step i, the reference grid, the station -> node map)
+                if outer.aero.n_surf != 1:
+                    return None
+                fa = aero_case.f_steady[0][i]
+                fb = aero_case.f_unsteady[0][i] if unsteady else None
+                return fa, fb, outer.aero.zeta_b0[0], outer._dof_i32()[0]
+
         struct = st_.base_dynamic_solve(struct0, t, dt, pre, load_steps, series(f_ext_follower, "f_ext_follower"),
                                         series(f_ext_dead, "f_ext_dead"), aero=_Hooks,
                                         fsi_settings=self.fsi_convergence_settings)

@@ -1147,3 +1147,117 @@ extern "C" int fj_beam_fsi_norms(int n_solve, const int32_t* solve_dofs, const d
   FJ_CHECK_LAUNCH();
   return FJ_OK;
 }
+
+// ---- the end of a coupled time step in one launch ----------------------------
+// calculate_q_n_from_q_alpha + calculate_a_n and the node frames (as
+// k_beam_ga_nodes, FJ_GA_FINISH), the aerodynamic vertex forces of one
+// surface projected onto the new frames (as k_beam_zero + k_beam_project),
+// their local-frame copy (as k_beam_nodes, FJ_BEAM_NODE_LOCAL) and the
+// predictor of the next step (as k_beam_ga_predict): one workgroup, the
+// phases separated by barriers, the frames' rotations and the projected
+// forces kept in LDS.  Every value is the same expression as in the
+// separate kernels, so the results are the same bits.
+#define FJ_STEP_END_MAX_NODES 512
+#define FJ_STEP_END_THREADS 256
+
+__device__ __forceinline__ void fj_ga_predict_dof(double am, double af, double g, double b, double dt, double v,
+                                                  double vd, double a, double& phi_a, double& v_a, double& vd_a) {
+#pragma clang fp contract(off)
+  const double a_n = (af * vd - am * a) / (1.0 - am);
+  const double phi_n = dt * v + (dt * dt) * ((0.5 - b) * a + b * a_n);
+  const double v_n = v + ((1.0 - g) * dt) * a + (g * dt) * a_n;
+  const double vd_n = ((1.0 - am) * a_n + am * a - af * vd) / (1.0 - af);
+  phi_a = (1.0 - af) * phi_n;
+  v_a = (1.0 - af) * v_n + af * v;
+  vd_a = (1.0 - af) * vd_n + af * vd;
+}
+
+__global__ __launch_bounds__(FJ_STEP_END_THREADS) void k_beam_step_end(
+    int nn, double am, double af, double g, double b, double dt, const double* __restrict__ phi_a,
+    const double* __restrict__ v_a, const double* __restrict__ vd_a, const double* __restrict__ v_prev,
+    const double* __restrict__ vd_prev, const double* __restrict__ a_prev, const double* __restrict__ hg_prev,
+    int m1, int n1, const double* __restrict__ fa, const double* __restrict__ fb, const double* __restrict__ x0,
+    const int32_t* __restrict__ dof, double* __restrict__ phi, double* __restrict__ v, double* __restrict__ vd,
+    double* __restrict__ a, double* __restrict__ hg, double* __restrict__ f_glob, double* __restrict__ f_loc,
+    double* __restrict__ phi_a_next, double* __restrict__ v_a_next, double* __restrict__ vd_a_next) {
+  __shared__ double rot[FJ_STEP_END_MAX_NODES][9];
+  __shared__ double fg[FJ_STEP_END_MAX_NODES][6];
+  const int tid = threadIdx.x;
+  for (int i = tid; i < nn; i += FJ_STEP_END_THREADS) {
+    double ph[6];
+    for (int d = 0; d < 6; ++d) {
+      const int k = 6 * i + d;
+      double pk, vk, vdk, ak;
+      fj_ga_finish_dof(am, af, phi_a[k], v_a[k], vd_a[k], v_prev[k], vd_prev[k], a_prev[k], pk, vk, vdk, ak);
+      phi[k] = pk;
+      v[k] = vk;
+      vd[k] = vdk;
+      a[k] = ak;
+      ph[d] = pk;
+      fg[i][d] = 0.0;
+    }
+    double ex[16], gn[16];
+    fj_exp_se3(ph, ex);
+    fj_hg_mul(hg_prev + 16 * i, ex, gn);
+    for (int q = 0; q < 16; ++q) hg[16 * i + q] = gn[q];
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) rot[i][3 * r + c] = gn[4 * r + c];
+  }
+  __syncthreads();
+  for (int j = tid; j < n1; j += FJ_STEP_END_THREADS) {
+    const int node = dof[j];
+    if (node < 0 || node >= nn) continue;
+    const double* gr = rot[node];
+    double f[3] = {0.0, 0.0, 0.0}, mo[3] = {0.0, 0.0, 0.0};
+    for (int i = 0; i < m1; ++i) {
+      const size_t o = 3 * ((size_t)i * n1 + j);
+      double fx = fa[o], fy = fa[o + 1], fz = fa[o + 2];
+      if (fb) { fx += fb[o]; fy += fb[o + 1]; fz += fb[o + 2]; }
+      const double ax = gr[0] * x0[o] + gr[1] * x0[o + 1] + gr[2] * x0[o + 2];
+      const double ay = gr[3] * x0[o] + gr[4] * x0[o + 1] + gr[5] * x0[o + 2];
+      const double az = gr[6] * x0[o] + gr[7] * x0[o + 1] + gr[8] * x0[o + 2];
+      f[0] += fx; f[1] += fy; f[2] += fz;
+      mo[0] += ay * fz - az * fy;
+      mo[1] += az * fx - ax * fz;
+      mo[2] += ax * fy - ay * fx;
+    }
+    for (int k = 0; k < 3; ++k) {
+      fg[node][k] = f[k];
+      fg[node][3 + k] = mo[k];
+    }
+  }
+  __syncthreads();
+  for (int i = tid; i < nn; i += FJ_STEP_END_THREADS) {
+    const double* gg = hg + 16 * i;  // written by this thread above
+    const double* f = fg[i];
+    for (int k = 0; k < 6; ++k) f_glob[6 * i + k] = f[k];
+    for (int h = 0; h < 6; h += 3)
+      for (int p = 0; p < 3; ++p) {
+        const double r0 = gg[p], r1 = gg[4 + p], r2 = gg[8 + p];
+        f_loc[6 * i + h + p] = r0 * f[h] + r1 * f[h + 1] + r2 * f[h + 2];
+      }
+    if (phi_a_next)
+      for (int d = 0; d < 6; ++d) {
+        const int k = 6 * i + d;
+        fj_ga_predict_dof(am, af, g, b, dt, v[k], vd[k], a[k], phi_a_next[k], v_a_next[k], vd_a_next[k]);
+      }
+  }
+}
+
+extern "C" int fj_beam_step_end(int n_nodes, double am, double af, double g, double b, double dt,
+                                const double* phi_a, const double* v_a, const double* v_dot_a, const double* v_prev,
+                                const double* v_dot_prev, const double* a_prev, const double* hg_prev, int m1,
+                                int n1, const double* f_a, const double* f_b, const double* x0, const int32_t* dof,
+                                double* phi, double* v, double* v_dot, double* a, double* hg, double* f_global,
+                                double* f_local, double* phi_a_next, double* v_a_next, double* v_dot_a_next,
+                                void* stream) {
+  if (n_nodes <= 0 || n_nodes > FJ_STEP_END_MAX_NODES || m1 < 1 || n1 < 1) return FJ_ERR_ARG;
+  if (!phi_a || !v_a || !v_dot_a || !v_prev || !v_dot_prev || !a_prev || !hg_prev || !f_a || !x0 || !dof) return FJ_ERR_ARG;
+  if (!phi || !v || !v_dot || !a || !hg || !f_global || !f_local) return FJ_ERR_ARG;
+  if ((phi_a_next || v_a_next || v_dot_a_next) && !(phi_a_next && v_a_next && v_dot_a_next)) return FJ_ERR_ARG;
+  hipLaunchKernelGGL(k_beam_step_end, dim3(1), dim3(FJ_STEP_END_THREADS), 0, (hipStream_t)stream, n_nodes, am, af,
+                     g, b, dt, phi_a, v_a, v_dot_a, v_prev, v_dot_prev, a_prev, hg_prev, m1, n1, f_a, f_b, x0, dof,
+                     phi, v, v_dot, a, hg, f_global, f_local, phi_a_next, v_a_next, v_dot_a_next);
+  FJ_CHECK_LAUNCH();
+  return FJ_OK;
+}

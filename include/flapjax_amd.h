@@ -656,6 +656,25 @@ int fj_beam_project_forces(int n_nodes, const double* rmat, int m1, int n1, cons
                            const double* f_b, const double* x0, const int32_t* dof_mapping, int clear,
                            double* out, void* stream);
 
+/* The end of a coupled time step in one launch: fj_beam_ga_nodes
+ * (FJ_GA_FINISH: phi, v, v_dot, a of the step from the alpha point, and the
+ * node frames hg = hg_prev exp(phi)), fj_beam_project_forces of one surface
+ * onto those frames (f_a + f_b, (m1, n1, 3); reference grid x0; node of
+ * each of the n1 stations in dof; f_global (n_nodes, 6) zeroed first), the
+ * local-frame copy R^T f (f_local, as fj_beam_nodes FJ_BEAM_NODE_LOCAL) and,
+ * when phi_a_next is given, the next step's predictor (as fj_beam_ga
+ * FJ_GA_PREDICT from v, v_dot, a).  The same arithmetic as those entry
+ * points, in one workgroup; n_nodes <= 512.  Restates structure/beam.py:
+ * 2727-2792 (step completion), aero/utils.py:666-697 (projection),
+ * structure/beam.py:1387-1395 (local frame) and
+ * structure/time_integration.py:86-150 (predictor). */
+int fj_beam_step_end(int n_nodes, double alpha_m, double alpha_f, double gamma, double beta, double dt,
+                     const double* phi_a, const double* v_a, const double* v_dot_a, const double* v_prev,
+                     const double* v_dot_prev, const double* a_prev, const double* hg_prev, int m1, int n1,
+                     const double* f_a, const double* f_b, const double* x0, const int32_t* dof, double* phi,
+                     double* v, double* v_dot, double* a, double* hg, double* f_global, double* f_local,
+                     double* phi_a_next, double* v_a_next, double* v_dot_a_next, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

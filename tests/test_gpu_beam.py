@@ -327,3 +327,29 @@ def test_ga_nodes_equals_separate_launches(dev):
                                  None, L.stream()), "fj_beam_ga_nodes")
     for x, y in zip(o + [hg_ref], f + [hg_f]):
         assert torch.equal(x, y)
+
+
+def test_fused_step_end_equals_separate_launches(dev):
+    """fj_beam_step_end (step completion, frames, force projection, local
+    copy and next predictor in one launch) gives the bits of the six
+    separate launches over coupled time steps with a gust."""
+    from flapjax_amd import OneMinusCosine
+    from flapjax_amd.beam import BeamStructure
+
+    gust = U.OneMinusCosine(np.array((10.0, 0.0, 1.0)), 1.225, True, gust_length=1.0, gust_amplitude=0.5)
+    ocw = B.cantilever_wing(m=2, n_nodes=5, m_star=6, k_diag=(1e6, 1e6, 1e6, 1e3, 5e3, 1e5), u_inf=(10.0, 0.0, 1.0),
+                            flow=gust)
+    runs = []
+    for fused in (True, False):
+        cw = _coupled(ocw, dev)
+        cw.aero.flowfield = OneMinusCosine((10.0, 0.0, 1.0), 1.225, True, 1.0, 0.5)
+        cw.structure.fused_step_end = fused
+        assert BeamStructure.fused_step_end  # the default
+        st = cw.static_solve(np.arange(6))
+        runs.append(cw.dynamic_solve(st, np.arange(6), 8))
+    a, b = runs
+    for name in ("hg", "v", "v_dot", "a", "f_ext_aero"):
+        assert torch.equal(getattr(a.structure, name), getattr(b.structure, name)), name
+    for name in ("gamma_b", "zeta_w", "f_steady", "f_unsteady"):
+        assert torch.equal(getattr(a.aero, name)[0], getattr(b.aero, name)[0]), name
+    assert a.structure.iters == b.structure.iters
