@@ -234,3 +234,35 @@ def test_lattice_pair_sum_modes_and_sizes(dev):
     x = _t(pts[:10], dev)
     z = K.pair_sum_lattices(seg, [(lat[2][0], 0, 13)], x, L.KERNEL_EPSILON, O.EPSILON, O.R_CUTOFF)
     assert (z == 0).all()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_split_pair_sums(dev, precision):
+    """fj_pair_sum_lattices_split / fj_pair_sum_split: two target ranges with
+    their own epilogues (store, add, convect) in one launch equal two
+    separate pair sums, for splits at 0, inside a 64-target tile and at the
+    end, in the lattice (FP64) and filament-table (FP64 and mixed) forms."""
+    import flapjax_amd as fj
+    from flapjax_amd import _kernels as K
+    from flapjax_amd import _lib as L
+
+    prec = L.PREC_FP64 if precision == "fp64" else L.PREC_MIXED
+    tol = 1e-13 if precision == "fp64" else 1e-6
+    rng = np.random.default_rng(23)
+    seg, lat, pts = _lattice_case(dev, L.KERNEL_EPSILON, rng)
+    flow = fj.Constant(np.array([1.0, 0.0, 0.2]), 1.225, True)
+    x = _t(pts[:300], dev)
+    ref_s = K.pair_sum(seg, x, L.KERNEL_EPSILON, O.EPSILON, O.R_CUTOFF, prec)
+    ref_c = K.pair_sum(seg, x, L.KERNEL_EPSILON, O.EPSILON, O.R_CUTOFF, prec, L.OUT_CONVECT, flow=flow, time=0.3,
+                       dt=0.05)
+    lattice_forms = [lat, None] if precision == "fp64" else [None]
+    for lats in lattice_forms:
+        for n_a in (0, 37, 200, 300):
+            out_a = torch.ones((n_a, 3), dtype=torch.float64, device=dev)
+            out_b = torch.empty((300 - n_a, 3), dtype=torch.float64, device=dev)
+            K.pair_sum_split(seg, x, n_a, L.KERNEL_EPSILON, O.EPSILON, O.R_CUTOFF, out_a, L.OUT_ADD, out_b,
+                             L.OUT_CONVECT, prec, flow=flow, time=0.3, dt=0.05, lattices=lats)
+            if n_a:
+                assert rel_err((out_a - 1.0).cpu().numpy(), ref_s[:n_a].cpu().numpy()) < tol, (lats is None, n_a)
+            if n_a < 300:
+                assert rel_err(out_b.cpu().numpy(), ref_c[n_a:].cpu().numpy()) < tol, (lats is None, n_a)
