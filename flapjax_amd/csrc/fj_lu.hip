@@ -437,16 +437,30 @@ __global__ __launch_bounds__(256) void k_lu_update(T* __restrict__ A, int n, int
     const int q = e / FJ_GT, j = e % FJ_GT;
     sU[q][j] = (c0 + j < c_end && q < kb) ? A[(size_t)(k + q) * lda + c0 + j] : T(0);
   }
-  __syncthreads();
   const int lane = tid & 63, wave = tid >> 6;
   const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+  const int li = lane & 15, lk = lane >> 4;
+  // the lane's 16 entries of C, loaded with the operand tiles so that their
+  // memory latency overlaps the staging and the matrix-core loop (C lies
+  // outside the L and U panels; the update is the same c - acc per entry)
+  T cv[2][2][4];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = r0 + wr + a * 16 + FjMfma<T>::row(lk, v);
+        const int j = c0 + wc + b * 16 + li;
+        cv[a][b][v] = (i < n && j < c_end) ? A[(size_t)i * lda + j] : T(0);
+      }
+  __syncthreads();
   typedef typename FjMfma<T>::acc_t acc_t;
   acc_t acc[2][2];
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int b = 0; b < 2; ++b) acc[a][b] = acc_t{0, 0, 0, 0};
-  const int li = lane & 15, lk = lane >> 4;
   for (int q = 0; q < kb; q += 4) {
 #pragma unroll
     for (int a = 0; a < 2; ++a) {
@@ -466,7 +480,7 @@ __global__ __launch_bounds__(256) void k_lu_update(T* __restrict__ A, int n, int
       for (int v = 0; v < 4; ++v) {
         const int i = r0 + wr + a * 16 + FjMfma<T>::row(lk, v);
         const int j = c0 + wc + b * 16 + li;
-        if (i < n && j < c_end) A[(size_t)i * lda + j] -= acc[a][b][v];
+        if (i < n && j < c_end) A[(size_t)i * lda + j] = cv[a][b][v] - acc[a][b][v];
       }
 }
 
