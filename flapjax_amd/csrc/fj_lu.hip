@@ -1048,11 +1048,12 @@ __global__ __launch_bounds__(256) void k_inv_update(const T* __restrict__ LU, in
   const int r0 = jblk * FJ_TS_B, rs = min(FJ_TS_B, n - r0);
   const int k0 = kb * FJ_TS_B, ks = min(FJ_TS_B, n - k0);
   const int c0 = blockIdx.x * FJ_GT, cb = min(FJ_GT, n - c0);
-  fj_tile_mma<T>(LU + (size_t)r0 * lda + k0, lda, rs, X + (size_t)k0 * ldx + c0, ldx, cb, ks, sA,
-                 sB, acc);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int li = lane & 15, lk = lane >> 4;
   const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+  // the lane's entries of X_j, loaded before the products (X_j is another
+  // block row than the operand X_kb; the update is the same x - acc)
+  T xv[2][2][4];
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -1060,7 +1061,18 @@ __global__ __launch_bounds__(256) void k_inv_update(const T* __restrict__ LU, in
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int i = wr + a * 16 + FjMfma<T>::row(lk, v), j = wc + c * 16 + li;
-        if (i < rs && j < cb) X[(size_t)(r0 + i) * ldx + c0 + j] -= acc[a][c][v];
+        xv[a][c][v] = (i < rs && j < cb) ? X[(size_t)(r0 + i) * ldx + c0 + j] : T(0);
+      }
+  fj_tile_mma<T>(LU + (size_t)r0 * lda + k0, lda, rs, X + (size_t)k0 * ldx + c0, ldx, cb, ks, sA,
+                 sB, acc);
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = wr + a * 16 + FjMfma<T>::row(lk, v), j = wc + c * 16 + li;
+        if (i < rs && j < cb) X[(size_t)(r0 + i) * ldx + c0 + j] = xv[a][c][v] - acc[a][c][v];
       }
 }
 
